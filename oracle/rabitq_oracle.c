@@ -1,0 +1,997 @@
+/*
+ * rabitq_oracle.c -- CPU restatement of the ExRaBitQ4 distance path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see rabitq_oracle.h). Compiled with
+ * -ffp-contract=off so that every multiply-add that the reference compiler
+ * contracts is spelled out explicitly through ro_fma_mask.
+ *
+ * Citations are to the reference checkout: Ours/core/hnswlib/space_rabitq.h
+ * unless another file is named.
+ */
+#include "rabitq_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Contraction pattern of the reference compiled with g++ 11 -O3 (pinned
+ * against oracle/_ref by tests/test_oracle_vs_ref.py). */
+unsigned ro_fma_mask = RO_FMA_REFERENCE;
+void ro_set_fma_mask(unsigned mask) { ro_fma_mask = mask; }
+unsigned ro_get_fma_mask(void) { return ro_fma_mask; }
+
+#define SITE(bit) ((ro_fma_mask & (bit)) != 0)
+
+static float madd_f(unsigned site, float a, float b, float c) {
+    if (ro_fma_mask & site) return fmaf(a, b, c);
+    float p = a * b;
+    return p + c;
+}
+
+static double madd_d(unsigned site, double a, double b, double c) {
+    if (ro_fma_mask & site) return fma(a, b, c);
+    double p = a * b;
+    return p + c;
+}
+
+/* _mm512_reduce_add_ps tree: halves 8, then 4, then 2, then 1. */
+static float reduce16(const float *v) {
+    float t8[8], t4[4], t2[2];
+    for (int i = 0; i < 8; ++i) t8[i] = v[8 + i] + v[i];
+    for (int i = 0; i < 4; ++i) t4[i] = t8[4 + i] + t8[i];
+    for (int i = 0; i < 2; ++i) t2[i] = t4[i] + t4[2 + i];
+    return t2[0] + t2[1];
+}
+
+/* ---------------------------------------------------------------- mt19937 */
+/* std::mt19937 / std::mt19937_64 as specified by [rand.eng.mers]; the
+ * reference draws rotation signs from the 32-bit engine (space_rabitq.h:
+ * 2900-2906) and the INT8 dither from the 64-bit one (:3559-3560). */
+typedef struct { uint32_t mt[624]; int idx; } mt32;
+static void mt32_seed(mt32 *g, uint32_t seed) {
+    g->mt[0] = seed;
+    for (int i = 1; i < 624; ++i)
+        g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->idx = 624;
+}
+static uint32_t mt32_next(mt32 *g) {
+    if (g->idx >= 624) {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (g->mt[i] & 0x80000000u) | (g->mt[(i + 1) % 624] & 0x7fffffffu);
+            uint32_t v = g->mt[(i + 397) % 624] ^ (y >> 1);
+            if (y & 1u) v ^= 0x9908b0dfu;
+            g->mt[i] = v;
+        }
+        g->idx = 0;
+    }
+    uint32_t y = g->mt[g->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+typedef struct { uint64_t mt[312]; int idx; } mt64;
+static void mt64_seed(mt64 *g, uint64_t seed) {
+    g->mt[0] = seed;
+    for (int i = 1; i < 312; ++i)
+        g->mt[i] = 6364136223846793005ull * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+    g->idx = 312;
+}
+static uint64_t mt64_next(mt64 *g) {
+    if (g->idx >= 312) {
+        for (int i = 0; i < 312; ++i) {
+            uint64_t y = (g->mt[i] & 0xFFFFFFFF80000000ull) | (g->mt[(i + 1) % 312] & 0x7FFFFFFFull);
+            uint64_t v = g->mt[(i + 156) % 312] ^ (y >> 1);
+            if (y & 1ull) v ^= 0xB5026F5AA96619E9ull;
+            g->mt[i] = v;
+        }
+        g->idx = 0;
+    }
+    uint64_t x = g->mt[g->idx++];
+    x ^= (x >> 29) & 0x5555555555555555ull;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ull;
+    x ^= (x << 37) & 0xFFF7EEE000000000ull;
+    x ^= x >> 43;
+    return x;
+}
+/* uniform_real_distribution<double>(0,1) over a 64-bit engine: one draw,
+ * converted to double and scaled by 2^-64, clamped below 1. */
+static double u01_from(uint64_t x) {
+    double r = (double)x * 0x1p-64;
+    if (r >= 1.0) r = nextafter(1.0, 0.0);
+    return r;
+}
+
+/* ------------------------------------------------------------------ space */
+size_t ro_code_dim(size_t dim) { /* roundUp64, space_rabitq.h:217-224 */
+    size_t rounded = ((dim + 63u) / 64u) * 64u;
+    size_t p = 1;
+    while (p < rounded) p <<= 1u;
+    return p;
+}
+
+int ro_space_init(ro_space *s, size_t dim, uint32_t seed, size_t block, int mse, int fp16) {
+    memset(s, 0, sizeof(*s));
+    if (dim == 0 || block == 0) return -1;
+    s->dim = dim;
+    s->code_dim = ro_code_dim(dim);
+    s->block_size = block;
+    s->block_count = (s->code_dim + block - 1) / block;
+    s->scale_bytes = s->block_count * (fp16 ? 2u : 4u);
+    s->compact_code_bytes = (s->code_dim + 1) / 2;
+    s->residual_code_bytes = (s->code_dim * 4 + 7) / 8;
+    s->compact_bytes = 8 + 8 + s->compact_code_bytes + 1;
+    s->residual_bytes = 8 + s->scale_bytes + s->residual_code_bytes;
+    s->full_bytes = 8 + 8 + 8 + s->scale_bytes + s->compact_code_bytes + 1 + s->residual_code_bytes;
+    s->seed = seed;
+    s->mse = mse;
+    s->fp16 = fp16;
+    s->inv_sqrt_code_dim = 1.0f / sqrtf((float)s->code_dim);
+    s->signs = (float *)calloc(s->code_dim, sizeof(float));
+    s->center = (float *)calloc(dim, sizeof(float));
+    s->rotated_center = (float *)calloc(s->code_dim, sizeof(float));
+    /* setRandomRotation, space_rabitq.h:2900-2906: uniform_int_distribution
+     * <int>(0,1) over the 32-bit engine keeps the top bit of each draw. */
+    if (seed != 0) {
+        mt32 g;
+        mt32_seed(&g, seed);
+        for (size_t i = 0; i < s->code_dim; ++i)
+            s->signs[i] = ((mt32_next(&g) >> 31) ? 1.0f : -1.0f) * s->inv_sqrt_code_dim;
+    } else {
+        for (size_t i = 0; i < s->code_dim; ++i) s->signs[i] = s->inv_sqrt_code_dim;
+    }
+    return 0;
+}
+
+void ro_space_free(ro_space *s) {
+    free(s->signs);
+    free(s->center);
+    free(s->rotated_center);
+    memset(s, 0, sizeof(*s));
+}
+
+/* hadamard + rotate, space_rabitq.h:944-1000. */
+void ro_rotate(const ro_space *s, const float *x, float *out) {
+    const size_t cd = s->code_dim;
+    memset(out, 0, cd * sizeof(float));
+    if (s->seed == 0) {
+        memcpy(out, x, s->dim * sizeof(float));
+        return;
+    }
+    for (size_t i = 0; i < s->dim; ++i) out[i] = x[i] * s->signs[i];
+    for (size_t step = 1; step < cd; step <<= 1u) {
+        for (size_t blk = 0; blk < cd; blk += step << 1u) {
+            for (size_t i = 0; i < step; ++i) {
+                float a = out[blk + i], b = out[blk + step + i];
+                out[blk + i] = a + b;
+                out[blk + step + i] = a - b;
+            }
+        }
+    }
+}
+
+/* setCentroids with K = 1, space_rabitq.h:2918-2936. */
+void ro_space_set_center(ro_space *s, const float *center) {
+    memcpy(s->center, center, s->dim * sizeof(float));
+    double norm = 0.0;
+    for (size_t i = 0; i < s->dim; ++i) norm += (double)center[i] * (double)center[i];
+    s->center_norm_sqr = (float)norm;
+    ro_rotate(s, s->center, s->rotated_center);
+}
+
+/* ------------------------------------------------------- fastQuantizeAbs */
+typedef struct { double t; size_t i; } ro_event;
+static int ev_less(ro_event a, ro_event b) { return a.t < b.t || (a.t == b.t && a.i < b.i); }
+static void heap_push(ro_event *h, size_t *n, ro_event e) {
+    size_t c = (*n)++;
+    h[c] = e;
+    while (c > 0) {
+        size_t p = (c - 1) / 2;
+        if (!ev_less(h[c], h[p])) break;
+        ro_event t = h[c]; h[c] = h[p]; h[p] = t;
+        c = p;
+    }
+}
+static ro_event heap_pop(ro_event *h, size_t *n) {
+    ro_event top = h[0];
+    h[0] = h[--(*n)];
+    size_t c = 0;
+    for (;;) {
+        size_t l = 2 * c + 1, r = l + 1, m = c;
+        if (l < *n && ev_less(h[l], h[m])) m = l;
+        if (r < *n && ev_less(h[r], h[m])) m = r;
+        if (m == c) break;
+        ro_event t = h[c]; h[c] = h[m]; h[m] = t;
+        c = m;
+    }
+    return top;
+}
+
+/* fastQuantizeAbs, space_rabitq.h:867-942. The pop order of the reference's
+ * std::priority_queue over (t, i) pairs is the (t, i) total order, which any
+ * binary min-heap on that key reproduces. */
+void ro_fast_quantize_abs(size_t cd, const float *a, uint8_t *code, float *ip_norm) {
+    const double eps = 1e-5;
+    double max_o = 0.0;
+    for (size_t i = 0; i < cd; ++i) if ((double)a[i] > max_o) max_o = (double)a[i];
+    if (max_o <= eps) {
+        memset(code, 0, cd);
+        *ip_norm = 1.0f;
+        return;
+    }
+    const double t_start = 2.0 / max_o;          /* kRemainingMax / 3 = 2 */
+    const double t_end = (7.0 + 10.0) / max_o;   /* (kRemainingMax + n_enum) */
+    int *cur = (int *)calloc(cd, sizeof(int));
+    ro_event *heap = (ro_event *)malloc(cd * sizeof(ro_event));
+    size_t hn = 0;
+    double sqr_den = (double)cd * 0.25;
+    double num = 0.0;
+    for (size_t i = 0; i < cd; ++i) {
+        int c = (int)madd_d(RO_FMA_FQA_T, t_start, (double)a[i], eps);
+        if (c > 7) c = 7;
+        cur[i] = c;
+        sqr_den += (double)(c * c + c);
+        num += ((double)c + 0.5) * (double)a[i];
+    }
+    for (size_t i = 0; i < cd; ++i) {
+        if ((double)a[i] > eps) {
+            ro_event e = {(double)(cur[i] + 1) / (double)a[i], i};
+            heap_push(heap, &hn, e);
+        }
+    }
+    double best_t = 0.0, max_ip = 0.0;
+    while (hn > 0) {
+        ro_event e = heap_pop(heap, &hn);
+        int uc = ++cur[e.i];
+        sqr_den += 2.0 * (double)uc;
+        num += (double)a[e.i];
+        double ip = num / sqrt(sqr_den);
+        if (ip > max_ip) { max_ip = ip; best_t = e.t; }
+        if (uc < 7) {
+            double ct = (double)(uc + 1) / (double)a[e.i];
+            if (ct < t_end) { ro_event n = {ct, e.i}; heap_push(heap, &hn, n); }
+        }
+    }
+    num = 0.0;
+    for (size_t i = 0; i < cd; ++i) {
+        int v = (int)madd_d(RO_FMA_FQA_T, best_t, (double)a[i], eps);
+        if (v > 7) v = 7;
+        code[i] = (uint8_t)v;
+        num += ((double)v + 0.5) * (double)a[i];
+    }
+    float ipn = num > eps ? (float)(1.0 / num) : 1.0f;
+    if (!isfinite(ipn)) ipn = 1.0f;
+    *ip_norm = ipn;
+    free(cur);
+    free(heap);
+}
+
+/* floatToHalfScalar / halfToFloatScalar, space_rabitq.h:392-438: truncating
+ * add-0x1000 rounding, not IEEE round-to-nearest-even. */
+uint16_t ro_float_to_half(float value) {
+    uint32_t bits;
+    memcpy(&bits, &value, 4);
+    uint32_t sign = (bits >> 16) & 0x8000u;
+    uint32_t mant = bits & 0x007FFFFFu;
+    int32_t e = (int32_t)((bits >> 23) & 0xFFu) - 127 + 15;
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        mant = (mant | 0x00800000u) >> (uint32_t)(1 - e);
+        return (uint16_t)(sign | ((mant + 0x00001000u) >> 13));
+    }
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    return (uint16_t)(sign | ((uint32_t)e << 10) | ((mant + 0x00001000u) >> 13));
+}
+
+float ro_half_to_float(uint16_t v) {
+    uint32_t sign = (uint32_t)(v & 0x8000u) << 16;
+    uint32_t e = (v >> 10) & 0x1Fu;
+    uint32_t m = v & 0x03FFu;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = sign;
+        } else {
+            e = 1;
+            while ((m & 0x0400u) == 0) { m <<= 1; --e; }
+            m &= 0x03FFu;
+            bits = sign | ((e + 127 - 15) << 23) | (m << 13);
+        }
+    } else if (e == 31) {
+        bits = sign | 0x7F800000u | (m << 13);
+    } else {
+        bits = sign | ((e + 127 - 15) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+static void store_scale(const ro_space *s, uint8_t *dst, size_t blk, float v) {
+    if (s->fp16) { uint16_t h = ro_float_to_half(v); memcpy(dst + 2 * blk, &h, 2); }
+    else memcpy(dst + 4 * blk, &v, 4);
+}
+static float load_scale(const ro_space *s, const uint8_t *src, size_t blk) {
+    if (s->fp16) { uint16_t h; memcpy(&h, src + 2 * blk, 2); return ro_half_to_float(h); }
+    float v; memcpy(&v, src + 4 * blk, 4); return v;
+}
+
+/* residualMaxAbsScale / residualMseOptimalScale, space_rabitq.h:729-786
+ * (4-bit: qmax = 7). */
+static float max_abs_scale(const float *e, size_t b, size_t n) {
+    float m = 0.0f;
+    for (size_t i = b; i < n; ++i) { float v = fabsf(e[i]); if (v > m) m = v; }
+    if (m <= 0.0f) return 0.0f;
+    return m / 7.0f;
+}
+static int32_t clamp7(double v) {
+    int32_t q = (int32_t)round(v);
+    if (q > 7) q = 7;
+    if (q < -7) q = -7;
+    return q;
+}
+static float mse_scale(const float *e, size_t b, size_t n) {
+    float scale = max_abs_scale(e, b, n);
+    if (scale == 0.0f || !isfinite(scale)) return scale;
+    for (int it = 0; it < 8; ++it) {
+        double num = 0.0, den = 0.0;
+        for (size_t i = b; i < n; ++i) {
+            int32_t q = clamp7((double)e[i] / (double)scale);
+            num += (double)e[i] * (double)q;
+            den += (double)q * (double)q;
+        }
+        if (den <= 0.0) break;
+        float next = (float)(num / den);
+        if (next <= 0.0f || !isfinite(next)) break;
+        float lim = 1e-6f * (scale > 1.0f ? scale : 1.0f);
+        if (fabsf(next - scale) <= lim) { scale = next; break; }
+        scale = next;
+    }
+    return scale;
+}
+
+static void set_nibble(uint8_t *code, size_t i, uint8_t v) {
+    v &= 0x0Fu;
+    if (i & 1u) code[i >> 1] = (uint8_t)((code[i >> 1] & 0x0Fu) | (v << 4));
+    else code[i >> 1] = (uint8_t)((code[i >> 1] & 0xF0u) | v);
+}
+static uint8_t get_nibble(const uint8_t *code, size_t i) {
+    uint8_t b = code[i >> 1];
+    return (uint8_t)((i & 1u) ? (b >> 4) : (b & 0x0Fu));
+}
+
+/* encodeVectorFullWithCentroid, space_rabitq.h:5845-5999 (K = 1, sequential
+ * nibble layout). Full record: header{norm_sqr, long_scale} | short factors
+ * {short_scale, error_scale} | residual factors | scales | codes | cid |
+ * residual codes (:261-266, 2857-2861). */
+void ro_encode_full(const ro_space *s, const float *x, uint8_t *out) {
+    const size_t D = s->dim, cd = s->code_dim;
+    float *res = (float *)calloc(D, sizeof(float));
+    /* The reference compiler vectorises this loop: products of the 16- and
+     * 8-wide blocks are rounded and then added in index order; only the
+     * scalar tail (D mod 8 elements) is contracted into FMAs. */
+    float rn2 = 0.0f;
+    const size_t vec_end = D >= 16 ? (D / 16) * 16 + ((D % 16) >= 8 ? 8 : 0) : 0;
+    for (size_t i = 0; i < D; ++i) {
+        res[i] = x[i] - s->center[i];
+        if (i < vec_end && !SITE(RO_FMA_ENC_NORM)) { const float p = res[i] * res[i]; rn2 = rn2 + p; }
+        else rn2 = fmaf(res[i], res[i], rn2);
+    }
+    const float rn = sqrtf(rn2);
+    if (rn > 0.0f) {
+        const float inv = 1.0f / rn;
+        for (size_t i = 0; i < D; ++i) res[i] *= inv;
+    }
+    float *rot = (float *)malloc(cd * sizeof(float));
+    ro_rotate(s, res, rot);
+
+    memset(out, 0, s->full_bytes);
+    uint8_t *scales = out + 24;
+    uint8_t *code = scales + s->scale_bytes;
+    uint8_t *cid = code + s->compact_code_bytes;
+    uint8_t *rcode = cid + 1;
+    float norm_sqr = rn * rn;
+    float long_scale = 0.0f, short_scale = 0.0f, error_scale = 0.0f;
+
+    float *a = (float *)malloc(cd * sizeof(float));
+    uint8_t *m = (uint8_t *)malloc(cd);
+    for (size_t i = 0; i < cd; ++i) a[i] = fabsf(rot[i]);
+    float ip_norm = 1.0f;
+    ro_fast_quantize_abs(cd, a, m, &ip_norm);
+    long_scale = 2.0f * rn * ip_norm;
+
+    double o_obar = 0.0, half_l1 = 0.0;
+    const double inv_sqrt_d = 1.0 / sqrt((double)cd);
+    for (size_t i = 0; i < cd; ++i) {
+        const int pos = rot[i] > 0.0f;
+        set_nibble(code, i, pos ? (uint8_t)(8 + m[i]) : (uint8_t)(7 - m[i]));
+        const double sign = pos ? 1.0 : -1.0;
+        o_obar = madd_d(RO_FMA_ENC_OOBAR, (double)rot[i] * sign, inv_sqrt_d, o_obar);
+        half_l1 += 0.5 * fabs((double)rot[i]);
+    }
+    if (rn > 0.0f && half_l1 > 1e-12) {
+        short_scale = (float)(2.0 * (double)rn / half_l1);
+        if (!isfinite(o_obar)) o_obar = 0.8;
+        if (o_obar < 1e-6) o_obar = 1e-6;
+        if (o_obar > 0.999999) o_obar = 0.999999;
+        const double o2 = o_obar * o_obar;
+        const double fac = ldexp(5.75, -4) / sqrt((double)cd);
+        double r = (1.0 - o2) / o2;
+        if (r < 0.0) r = 0.0;
+        error_scale = (float)(sqrt(r) * fac * 2.0 * (double)rn);
+    }
+
+    float *qerr = (float *)malloc(cd * sizeof(float));
+    for (size_t i = 0; i < cd; ++i) {
+        const double tv = (double)rn * (double)rot[i];
+        const double x4 = 0.5 * (double)long_scale * ((double)get_nibble(code, i) - 7.5);
+        qerr[i] = (float)(tv - x4);
+    }
+    double dec_norm = 0.0, x4_ip = 0.0;
+    int32_t *rc = (int32_t *)calloc(cd, sizeof(int32_t));
+    for (size_t blk = 0; blk < s->block_count; ++blk) {
+        const size_t b = blk * s->block_size;
+        const size_t e = (b + s->block_size < cd) ? b + s->block_size : cd;
+        const float sc = s->mse ? mse_scale(qerr, b, e) : max_abs_scale(qerr, b, e);
+        store_scale(s, scales, blk, sc);
+        const float st = load_scale(s, scales, blk);
+        if (st == 0.0f || !isfinite(st)) continue;
+        for (size_t i = b; i < e; ++i) {
+            const int32_t q = clamp7((double)qerr[i] / (double)st);
+            rc[i] = q;
+            const double dr = (double)st * (double)q;
+            const double x4 = 0.5 * (double)long_scale * ((double)get_nibble(code, i) - 7.5);
+            dec_norm = madd_d(RO_FMA_ENC_DECNORM, dr, dr, dec_norm);
+            x4_ip = madd_d(RO_FMA_ENC_X4IP, x4, dr, x4_ip);
+        }
+    }
+    /* pack_residual_codes 4-bit, space_rabitq.h:580-595 */
+    for (size_t i = 0; i < cd; ++i) set_nibble(rcode, i, (uint8_t)((uint32_t)rc[i] & 0x0Fu));
+    float rf[2] = {(float)dec_norm, (float)x4_ip};
+    float hdr[4] = {norm_sqr, long_scale, short_scale, error_scale};
+    memcpy(out, hdr, 16);
+    memcpy(out + 16, rf, 8);
+    *cid = 0;
+    free(res); free(rot); free(a); free(m); free(qerr); free(rc);
+}
+
+/* copyCompactPayloadFromFull / copyResidualRecordFromFull, :6005-6029. */
+void ro_copy_compact(const ro_space *s, const uint8_t *full, uint8_t *out) {
+    memcpy(out, full, 16);
+    memcpy(out + 16, full + 24 + s->scale_bytes, s->compact_code_bytes);
+    out[16 + s->compact_code_bytes] = full[24 + s->scale_bytes + s->compact_code_bytes];
+}
+void ro_copy_residual(const ro_space *s, const uint8_t *full, uint8_t *out) {
+    memcpy(out, full + 16, 8);
+    memcpy(out + 8, full + 24, s->scale_bytes);
+    memcpy(out + 8 + s->scale_bytes, full + 24 + s->scale_bytes + s->compact_code_bytes + 1,
+           s->residual_code_bytes);
+}
+
+/* extract_paper_prune_sidecar, space_rabitq.h:4364-4392. */
+void ro_extract_sidecar(const ro_space *s, const uint8_t *compact, uint8_t *msb, ro_factors *f) {
+    const size_t cd = s->code_dim;
+    const uint8_t *code = compact + 16;
+    memset(msb, 0, (cd + 7) / 8);
+    for (size_t i = 0; i < cd; ++i)
+        if (get_nibble(code, i) >> 3) msb[i >> 3] |= (uint8_t)(1u << (i & 7u));
+    float hdr[4];
+    memcpy(hdr, compact, 16);
+    memset(f, 0, sizeof(*f));
+    f->norm_sqr = hdr[0];
+    if (cd <= 1 || hdr[0] <= 0.0f || hdr[2] <= 0.0f) return;
+    const float dn = sqrtf(hdr[0]);
+    f->data_norm = dn;
+    float alpha = 4.0f * dn / (hdr[2] * sqrtf((float)cd));
+    if (alpha > 1.0f) alpha = 1.0f;
+    if (!(alpha > 0.0f) || !isfinite(alpha)) return;
+    const float a2 = alpha * alpha;
+    f->cross_scale = hdr[2];
+    float r = (1.0f - a2) / a2;
+    if (r < 0.0f) r = 0.0f;
+    f->error_cross_scale = 2.0f * dn * sqrtf(r) / sqrtf((float)(cd - 1));
+    f->valid = isfinite(f->error_cross_scale) ? 1 : 0;
+}
+
+/* ------------------------------------------------------------------ query */
+void ro_query_alloc(const ro_space *s, ro_query *q) {
+    memset(q, 0, sizeof(*q));
+    q->q = (float *)calloc(s->code_dim, sizeof(float));
+    q->x8 = (int8_t *)calloc(s->code_dim, 1);
+}
+void ro_query_free(ro_query *q) { free(q->q); free(q->x8); memset(q, 0, sizeof(*q)); }
+
+/* initializePreparedQuery + buildCentroidQuery, space_rabitq.h:3429-3698
+ * (K = 1; Full or Int8 codec). */
+void ro_prepare_query(const ro_space *s, const float *raw, int codec, ro_query *q) {
+    const size_t D = s->dim, cd = s->code_dim;
+    float *rq = (float *)malloc(cd * sizeof(float));
+    ro_rotate(s, raw, rq);
+    double raw_n2 = 0.0;
+    for (size_t i = 0; i < D; ++i) raw_n2 += (double)raw[i] * (double)raw[i];
+    for (size_t i = 0; i < cd; ++i) q->q[i] = rq[i] - s->rotated_center[i];
+    double dot = 0.0;
+    for (size_t i = 0; i < D; ++i) dot += (double)raw[i] * (double)s->center[i];
+    float qn2 = (float)(raw_n2 + (double)s->center_norm_sqr - 2.0 * dot);
+    if (qn2 < 0.0f) qn2 = 0.0f;
+    q->query_norm_sqr = qn2;
+    float half = 0.0f, sum = 0.0f, pos = 0.0f, mx = 0.0f;
+    for (size_t p = 0; p < cd / 2; ++p) {
+        const float ev = q->q[2 * p], od = q->q[2 * p + 1];
+        half += ev + od;
+        sum += ev + od;
+        pos += (ev > 0.0f ? ev : 0.0f) + (od > 0.0f ? od : 0.0f);
+        const float ae = fabsf(ev), ao = fabsf(od);
+        const float am = ae > ao ? ae : ao;
+        if (am > mx) mx = am;
+    }
+    q->half_sum = half * 0.5f;
+    q->sum = sum;
+    q->positive_sum = pos;
+    q->query_norm = sqrtf(q->query_norm_sqr);
+    q->int8_scale = 1.0f;
+    q->int8_error_norm = 0.0f;
+    q->int8_code_sum = 0;
+    memset(q->x8, 0, cd);
+    if (codec == 3 && mx > 0.0f) {
+        q->int8_scale = mx / 128.0f;
+        /* FNV-1a over the raw query bytes, mixed with the centroid id
+         * (:3549-3558), seeds mt19937_64; two uniform draws per dimension,
+         * the first for INT8 (:3585-3590, 3628-3631). */
+        uint64_t seed = 1469598103934665603ull;
+        const uint8_t *bytes = (const uint8_t *)raw;
+        for (size_t i = 0; i < D * 4; ++i) { seed ^= bytes[i]; seed *= 1099511628211ull; }
+        seed ^= 0ull + 0x9E3779B97F4A7C15ull + (seed << 6) + (seed >> 2);
+        mt64 g;
+        mt64_seed(&g, seed);
+        double err = 0.0;
+        for (size_t i = 0; i < cd; ++i) {
+            const double u8 = u01_from(mt64_next(&g));
+            (void)mt64_next(&g);
+            int32_t x = (int32_t)floor((double)q->q[i] / (double)q->int8_scale + u8);
+            if (x > 127) x = 127;
+            if (x < -128) x = -128;
+            const double e = (double)q->q[i] - (double)q->int8_scale * (double)x;
+            err += e * e;
+            q->x8[i] = (int8_t)x;
+            q->int8_code_sum += x;
+        }
+        q->int8_error_norm = (float)sqrt(err);
+    }
+    free(rq);
+}
+
+/* ------------------------------------------------------------- estimators */
+static int long_scale_ok(float ls) { return ls > 0.0f && isfinite(ls); }
+
+/* queryDistanceLongWithIps, space_rabitq.h:1621-1632. */
+static float finish_long(const ro_query *q, float norm_sqr, float ls, float short_ip, float rem_ip) {
+    const float t = 8.0f * short_ip + rem_ip;
+    const float signed_ip = madd_f(RO_FMA_C1_SUB7, -7.0f, q->half_sum, t);
+    const float base = norm_sqr + q->query_norm_sqr;
+    if (SITE(RO_FMA_C1_EST)) return fmaf(-ls, signed_ip, base);
+    const float est = ls * signed_ip;
+    return base - est;
+}
+
+/* queryDistanceLong + longCodeIpsAvx512 (space_rabitq.h:1551-1607): lane j
+ * of the 16-wide accumulators takes pairs j, j+16, j+32, ... */
+float ro_asym_distance(const ro_space *s, const ro_query *q, const uint8_t *compact) {
+    float hdr[2];
+    memcpy(hdr, compact, 8);
+    if (!long_scale_ok(hdr[1])) return hdr[0] + q->query_norm_sqr;
+    const uint8_t *code = compact + 16;
+    const size_t pairs = s->code_dim / 2;
+    float sh[16] = {0}, rm[16] = {0};
+    for (size_t p0 = 0; p0 < pairs; p0 += 16) {
+        for (int j = 0; j < 16; ++j) {
+            const uint8_t b = code[p0 + j];
+            const uint8_t lo = b & 0x0Fu, hi = b >> 4;
+            const float ev = q->q[2 * (p0 + j)], od = q->q[2 * (p0 + j) + 1];
+            rm[j] = madd_f(RO_FMA_C1_LANE, (float)(lo & 7u), ev, rm[j]);
+            rm[j] = madd_f(RO_FMA_C1_LANE, (float)(hi & 7u), od, rm[j]);
+            if (lo > 7) sh[j] = sh[j] + ev;
+            if (hi > 7) sh[j] = sh[j] + od;
+        }
+    }
+    const float short_ip = reduce16(sh) - q->half_sum;
+    const float rem_ip = reduce16(rm);
+    return finish_long(q, hdr[0], hdr[1], short_ip, rem_ip);
+}
+
+/* FP64 restatement of the same estimate: ||o||^2 + ||q||^2 - 2<x4, q>,
+ * with x4_i = 0.5 * long_scale * (c_i - 7.5) (space_rabitq.h:5896, 1621). */
+double ro_asym_distance_f64(const ro_space *s, const ro_query *q, const uint8_t *compact) {
+    float hdr[2];
+    memcpy(hdr, compact, 8);
+    if (!long_scale_ok(hdr[1])) return (double)hdr[0] + (double)q->query_norm_sqr;
+    const uint8_t *code = compact + 16;
+    double ip = 0.0;
+    for (size_t i = 0; i < s->code_dim; ++i)
+        ip += ((double)get_nibble(code, i) - 7.5) * (double)q->q[i];
+    return (double)hdr[0] + (double)q->query_norm_sqr - (double)hdr[1] * ip;
+}
+
+/* query_distance_with_paper_msb_batch_by_id (:3972-4073): remaining 3 bits
+ * in 16 lanes, MSB part supplied by the gate's short_ip. */
+float ro_remaining_distance_fp32(const ro_space *s, const ro_query *q, const uint8_t *compact, float short_ip) {
+    float hdr[2];
+    memcpy(hdr, compact, 8);
+    if (!long_scale_ok(hdr[1])) return hdr[0] + q->query_norm_sqr;
+    const uint8_t *code = compact + 16;
+    const size_t pairs = s->code_dim / 2;
+    float rm[16] = {0};
+    for (size_t p0 = 0; p0 < pairs; p0 += 16) {
+        for (int j = 0; j < 16; ++j) {
+            const uint8_t b = code[p0 + j];
+            rm[j] = madd_f(RO_FMA_C1_LANE, (float)(b & 7u), q->q[2 * (p0 + j)], rm[j]);
+            rm[j] = madd_f(RO_FMA_C1_LANE, (float)((b >> 4) & 7u), q->q[2 * (p0 + j) + 1], rm[j]);
+        }
+    }
+    return finish_long(q, hdr[0], hdr[1], short_ip, reduce16(rm));
+}
+
+/* finish_paper_prune_estimate_sidecar, space_rabitq.h:4701-4721. */
+static ro_estimate finish_gate(float short_ip, const ro_factors *f, const ro_query *q, float eps0) {
+    ro_estimate r;
+    memset(&r, 0, sizeof(r));
+    if (!f->valid) return r;
+    const float cross = short_ip * f->cross_scale;
+    const float max_cross = 2.0f * f->data_norm * q->query_norm;
+    const float ee = eps0 * f->error_cross_scale;
+    float up = madd_f(RO_FMA_GATE_UPPER, ee, q->query_norm, cross);
+    if (up > max_cross) up = max_cross;
+    if (up < -max_cross) up = -max_cross;
+    r.lower_bound = f->norm_sqr + q->query_norm_sqr - up;
+    r.short_ip = short_ip;
+    r.ip_hat = max_cross > 0.0f ? cross / max_cross : 0.0f;
+    r.error_bound = max_cross > 0.0f ? eps0 * f->error_cross_scale * q->query_norm / max_cross : 0.0f;
+    r.valid = isfinite(r.lower_bound) ? 1 : 0;
+    return r;
+}
+
+/* compute_full_paper_prune_estimate_batch_by_id (:4427-4508): FP32 query
+ * summed over the MSB plane in 16 lanes. */
+ro_estimate ro_gate_fp32(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0) {
+    float acc[16] = {0};
+    for (size_t d0 = 0; d0 < s->code_dim; d0 += 16) {
+        const uint16_t bits = (uint16_t)(msb[d0 >> 3] | ((uint16_t)msb[(d0 >> 3) + 1] << 8));
+        for (int j = 0; j < 16; ++j) acc[j] = acc[j] + (((bits >> j) & 1u) ? q->q[d0 + j] : 0.0f);
+    }
+    return finish_gate(reduce16(acc) - q->half_sum, f, q, eps0);
+}
+
+/* maskedQuantizedQuerySum (:5169-5202): exact integer sum. */
+int64_t ro_masked_sum(const ro_space *s, const ro_query *q, const uint8_t *msb) {
+    int64_t t = 0;
+    for (size_t i = 0; i < s->code_dim; ++i)
+        if ((msb[i >> 3] >> (i & 7u)) & 1u) t += q->x8[i];
+    return t;
+}
+
+/* compute_int_paper_prune_estimate_batch_by_id (:4991-5093). */
+ro_estimate ro_gate_int8(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0) {
+    ro_estimate r;
+    memset(&r, 0, sizeof(r));
+    if (!f->valid) return r;
+    if (q->query_norm <= 0.0f || f->cross_scale <= 0.0f || !isfinite(f->cross_scale)) return r;
+    const int64_t sel = ro_masked_sum(s, q, msb);
+    const float short_ip = 0.5f * q->int8_scale * (float)(2 * sel - q->int8_code_sum);
+    return finish_gate(short_ip, f, q, eps0);
+}
+
+/* quantizedCodeDotDispatch (:5099-5155): exact integer dot. */
+static int64_t qdot(const ro_space *s, const ro_query *q, const uint8_t *code, uint8_t mask) {
+    int64_t t = 0;
+    for (size_t i = 0; i < s->code_dim; ++i) t += (int64_t)(get_nibble(code, i) & mask) * (int64_t)q->x8[i];
+    return t;
+}
+int64_t ro_remaining_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact) {
+    return qdot(s, q, compact + 16, 7u);
+}
+int64_t ro_full_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact) {
+    return qdot(s, q, compact + 16, 15u);
+}
+
+/* finishQuantizedTraversalDistance (:4342-4358). */
+float ro_remaining_distance_int8(const ro_space *s, const ro_query *q, const uint8_t *compact, float short_ip) {
+    float hdr[2];
+    memcpy(hdr, compact, 8);
+    if (!long_scale_ok(hdr[1])) return hdr[0] + q->query_norm_sqr;
+    const int64_t rem = ro_remaining_dot_int8(s, q, compact);
+    const double centered = (double)q->int8_scale * ((double)rem - 3.5 * (double)q->int8_code_sum);
+    const double signed_ip = 8.0 * (double)short_ip + centered;
+    return hdr[0] + q->query_norm_sqr - (float)((double)hdr[1] * signed_ip);
+}
+
+/* compute_paper_prune_estimate_int_codec + finish_int_paper_prune_estimate
+ * (:5204-5260). */
+ro_estimate ro_int_codec_estimate(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0) {
+    ro_estimate r;
+    memset(&r, 0, sizeof(r));
+    float hdr[4];
+    memcpy(hdr, compact, 16);
+    const float scale = q->int8_scale;
+    if (!long_scale_ok(hdr[1]) || !(scale > 0.0f) || !isfinite(scale) || q->query_norm <= 0.0f) {
+        r.lower_bound = hdr[0] + q->query_norm_sqr;
+        r.valid = isfinite(r.lower_bound) ? 1 : 0;
+        return r;
+    }
+    const int64_t dot = ro_full_dot_int8(s, q, compact);
+    const double centered = (double)dot - 7.5 * (double)q->int8_code_sum;
+    const double sip = (double)scale * centered;
+    const double est_ip = (double)hdr[1] * sip;
+    const float est = hdr[0] + q->query_norm_sqr - (float)est_ip;
+    const double db_err = (double)hdr[3] * (double)q->query_norm;
+    r.lower_bound = est - (float)((double)eps0 * db_err);
+    r.short_ip = (float)sip;
+    r.error_bound = est - r.lower_bound;
+    r.valid = isfinite(r.lower_bound) ? 1 : 0;
+    return r;
+}
+
+/* centeredNibbleDotTimesFour (:2405-2538): sum (2a-15)(2b-15), exact. */
+int64_t ro_symmetric_ip4(const ro_space *s, const uint8_t *l, const uint8_t *r) {
+    int64_t t = 0;
+    for (size_t i = 0; i < s->code_dim; ++i)
+        t += (int64_t)(2 * (int)get_nibble(l, i) - 15) * (int64_t)(2 * (int)get_nibble(r, i) - 15);
+    return t;
+}
+
+/* distanceBetweenEncoded / symmetricBuildDistanceFromPrepared, K = 1
+ * (:2634-2683, 2706-2747); lhs is the prepared side. */
+float ro_symmetric_distance(const ro_space *s, const uint8_t *lhs, const uint8_t *rhs) {
+    float lh[2], rh[2];
+    memcpy(lh, lhs, 8);
+    memcpy(rh, rhs, 8);
+    const double ln = sqrt((double)lh[0] > 0.0 ? (double)lh[0] : 0.0);
+    const double rn = sqrt((double)rh[0] > 0.0 ? (double)rh[0] : 0.0);
+    const int lvalid = ln > 0.0 && lh[1] > 0.0f && isfinite(lh[1]);
+    const double lipn = lvalid ? (double)lh[1] / (2.0 * ln) : 0.0;
+    if (!lvalid || !(rn > 0.0) || !(rh[1] > 0.0f) || !isfinite(rh[1]))
+        return (float)((double)lh[0] + (double)rh[0] + 0.0);
+    const double ripn = (double)rh[1] / (2.0 * rn);
+    const double code_ip = 0.25 * (double)ro_symmetric_ip4(s, lhs + 16, rhs + 16);
+    double u = code_ip * lipn * ripn;
+    if (!isfinite(u)) u = 0.0;
+    if (u < -1.0) u = -1.0;
+    if (u > 1.0) u = 1.0;
+    const double rip = ln * rn * u;
+    return (float)((double)lh[0] + (double)rh[0] - 2.0 * rip);
+}
+
+/* compute_residual_distance_intervals_batch_by_id (:5432-5548): per block of
+ * 16 dims, 8 lane products (lo*even + hi*odd), sequential lane sum, then
+ * ips += scale * block_ip; skipped blocks have scale 0 or non-finite. */
+ro_interval ro_residual_interval(const ro_space *s, const ro_query *q, const uint8_t *residual, float long_distance) {
+    const uint8_t *scales = residual + 8;
+    const uint8_t *code = residual + 8 + s->scale_bytes;
+    float ips = 0.0f;
+    double errsq = 0.0;
+    for (size_t blk = 0; blk < s->block_count; ++blk) {
+        const float sc = load_scale(s, scales, blk);
+        if (sc == 0.0f || !isfinite(sc)) continue;
+        const size_t pair = blk * 8;
+        float bip = 0.0f;
+        for (int l = 0; l < 8; ++l) {
+            const uint8_t b = code[pair + l];
+            int lo = b & 0x0F, hi = b >> 4;
+            if (lo > 7) lo -= 16;
+            if (hi > 7) hi -= 16;
+            const float ev = q->q[2 * (pair + l)], od = q->q[2 * (pair + l) + 1];
+            float prod;
+            if (SITE(RO_FMA_RES_HI)) prod = fmaf((float)hi, od, (float)lo * ev);
+            else if (SITE(RO_FMA_RES_LO)) prod = fmaf((float)lo, ev, (float)hi * od);
+            else { const float pl = (float)lo * ev, ph = (float)hi * od; prod = pl + ph; }
+            bip += prod;
+        }
+        ips = madd_f(RO_FMA_RES_ACC, sc, bip, ips);
+        const double pde = 0.5 * fabs((double)sc);
+        errsq += 16.0 * pde * pde;
+    }
+    const float dist = long_distance - 2.0f * ips;
+    const float err = (float)(2.0 * (double)q->query_norm * sqrt(errsq));
+    ro_interval r = {dist, dist - err, dist + err};
+    return r;
+}
+
+/* FP64 form of the residual correction: long - 2 * sum_b s_b <r_b, q_b>. */
+double ro_residual_estimate_f64(const ro_space *s, const ro_query *q, const uint8_t *residual, double long_distance) {
+    const uint8_t *scales = residual + 8;
+    const uint8_t *code = residual + 8 + s->scale_bytes;
+    double ips = 0.0;
+    for (size_t blk = 0; blk < s->block_count; ++blk) {
+        const float sc = load_scale(s, scales, blk);
+        if (sc == 0.0f || !isfinite(sc)) continue;
+        double bip = 0.0;
+        for (size_t i = blk * 16; i < blk * 16 + 16; ++i) {
+            int v = get_nibble(code, i);
+            if (v > 7) v -= 16;
+            bip += (double)v * (double)q->q[i];
+        }
+        ips += (double)sc * bip;
+    }
+    return long_distance - 2.0 * ips;
+}
+
+/* ----------------------------------------------------------------- search */
+typedef struct { float d; uint32_t id; uint8_t expanded; } ro_cand;
+static int cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
+    return ad < bd || (ad == bd && aid < bid);
+}
+/* insert_candidate_sorted, ours_diskann.rs:1100-1126. */
+static void pool_insert(ro_cand *pool, size_t *len, size_t L, float d, uint32_t id) {
+    if (L == 0) return;
+    if (*len >= L && !cand_less(d, id, pool[*len - 1].d, pool[*len - 1].id)) return;
+    size_t pos = 0;
+    while (pos < *len && cand_less(pool[pos].d, pool[pos].id, d, id)) ++pos;
+    size_t n = *len < L ? *len : L - 1;
+    memmove(pool + pos + 1, pool + pos, (n - pos) * sizeof(ro_cand));
+    pool[pos].d = d;
+    pool[pos].id = id;
+    pool[pos].expanded = 0;
+    if (*len < L) ++*len;
+}
+
+/* search_ours_paper_active_inner + flush_distance_batch
+ * (ours_diskann.rs:1539-1986) with beam_width 1, start node 0, kth/early
+ * stop off; codec 3 = Int8 (gate C5, mode-2 flush) or 0 = Full (gate C4,
+ * mode-1 flush). Rerank: top min(|pool|, max(rerank, k)) ids, FP32 long
+ * distance recomputed for Int8, residual interval, sort by (estimate, id). */
+size_t ro_search(const ro_space *s, const ro_query *q, int codec,
+                 const uint8_t *cbase, size_t cstride,
+                 const uint8_t *rbase, size_t rstride,
+                 const uint8_t *mbase, size_t mstride,
+                 const ro_factors *factors,
+                 const uint32_t *degrees, const uint32_t *edges, size_t estride,
+                 size_t n, size_t k, size_t L, float eps0, size_t rerank,
+                 uint32_t *out_ids, float *out_d,
+                 uint32_t *pool_ids_out, float *pool_d_out, size_t *pool_len_out,
+                 ro_search_stats *st) {
+    ro_search_stats local;
+    if (!st) st = &local;
+    memset(st, 0, sizeof(*st));
+    if (k == 0 || n == 0) return 0;
+    if (L < k) L = k;
+    ro_cand *pool = (ro_cand *)calloc(L + 1, sizeof(ro_cand));
+    uint8_t *visited = (uint8_t *)calloc(n, 1);
+    size_t len = 0;
+    visited[0] = 1;
+    pool_insert(pool, &len, L, ro_asym_distance(s, q, cbase), 0);
+    st->full_dists++;
+    uint32_t fresh[4096];
+    ro_estimate est[4096];
+    for (;;) {
+        size_t f = len;
+        for (size_t i = 0; i < len; ++i) if (!pool[i].expanded) { f = i; break; }
+        if (f == len) break;
+        pool[f].expanded = 1;
+        const uint32_t node = pool[f].id;
+        st->hops++;
+        size_t nf = 0;
+        const uint32_t deg = degrees[node];
+        for (uint32_t j = 0; j < deg; ++j) {
+            const uint32_t nb = edges[(size_t)node * estride + j];
+            if (!visited[nb]) { visited[nb] = 1; fresh[nf++] = nb; }
+        }
+        if (nf == 0) continue;
+        for (size_t j = 0; j < nf; ++j) {
+            const uint32_t id = fresh[j];
+            est[j] = codec == 3
+                ? ro_gate_int8(s, q, mbase + (size_t)id * mstride, &factors[id], eps0)
+                : ro_gate_fp32(s, q, mbase + (size_t)id * mstride, &factors[id], eps0);
+            st->gate_evals++;
+        }
+        /* gate survivors are batched (at most 64, DIST_BATCH_SIZE) and
+         * re-checked against the current pool when the batch is flushed */
+        uint32_t bid[64];
+        size_t bslot[64], nb = 0;
+        for (size_t j = 0; j <= nf; ++j) {
+            if (j < nf) {
+                const int full = len >= L;
+                const float lb = full ? pool[len - 1].d : 3.402823466e38f;
+                if (full && est[j].valid && est[j].lower_bound > lb) { st->pruned++; continue; }
+                bid[nb] = fresh[j];
+                bslot[nb] = j;
+                nb++;
+                if (nb < 64) continue;
+            }
+            for (size_t b = 0; b < nb; ++b) {
+                const uint32_t id = bid[b];
+                const ro_estimate *e = &est[bslot[b]];
+                const uint8_t *rec = cbase + (size_t)id * cstride;
+                float d;
+                if (!e->valid) d = ro_asym_distance(s, q, rec);
+                else if (codec == 3) d = ro_remaining_distance_int8(s, q, rec, e->short_ip);
+                else d = ro_remaining_distance_fp32(s, q, rec, e->short_ip);
+                st->full_dists++;
+                const int full = len >= L;
+                const float lb = full ? pool[len - 1].d : 3.402823466e38f;
+                if (full && e->valid && e->lower_bound > lb) { st->pruned++; continue; }
+                pool_insert(pool, &len, L, d, id);
+            }
+            nb = 0;
+        }
+    }
+    if (pool_ids_out) for (size_t i = 0; i < len; ++i) { pool_ids_out[i] = pool[i].id; pool_d_out[i] = pool[i].d; }
+    if (pool_len_out) *pool_len_out = len;
+    size_t rc = len < (rerank > k ? rerank : k) ? len : (rerank > k ? rerank : k);
+    float *rd = (float *)malloc((rc + 1) * sizeof(float));
+    uint32_t *ri = (uint32_t *)malloc((rc + 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < rc; ++i) {
+        const uint32_t id = pool[i].id;
+        float ld = pool[i].d;
+        if (codec == 3) ld = ro_asym_distance(s, q, cbase + (size_t)id * cstride);
+        ro_interval iv = ro_residual_interval(s, q, rbase + (size_t)id * rstride, ld);
+        rd[i] = iv.estimate;
+        ri[i] = id;
+    }
+    /* sort by (estimate, id) */
+    for (size_t i = 1; i < rc; ++i) {
+        float d = rd[i]; uint32_t id = ri[i];
+        size_t j = i;
+        while (j > 0 && cand_less(d, id, rd[j - 1], ri[j - 1])) { rd[j] = rd[j - 1]; ri[j] = ri[j - 1]; --j; }
+        rd[j] = d; ri[j] = id;
+    }
+    size_t out = rc < k ? rc : k;
+    for (size_t i = 0; i < out; ++i) { out_ids[i] = ri[i]; out_d[i] = rd[i]; }
+    free(rd); free(ri); free(pool); free(visited);
+    return out;
+}
+
+/* robust_prune_4bit (vamana_prune.h:41-156) with the symmetric distance as
+ * distance_between(candidate, selected) (vamana_index.h:844-849). */
+static int pc_less(const void *a, const void *b) {
+    const ro_cand *x = (const ro_cand *)a, *y = (const ro_cand *)b;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+size_t ro_robust_prune(const ro_space *s, const uint8_t *cbase, size_t cstride,
+                       uint32_t source, const uint32_t *ids, const float *ds,
+                       size_t n, size_t R, float alpha, uint32_t *out) {
+    if (R == 0) return 0;
+    ro_cand *c = (ro_cand *)malloc((n + 1) * sizeof(ro_cand));
+    for (size_t i = 0; i < n; ++i) { c[i].d = ds[i]; c[i].id = ids[i]; c[i].expanded = 0; }
+    qsort(c, n, sizeof(ro_cand), pc_less);
+    size_t u = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (c[i].id == source) continue;
+        int dup = 0;
+        for (size_t j = 0; j < u; ++j) if (c[j].id == c[i].id) { dup = 1; break; }
+        if (!dup) c[u++] = c[i];
+    }
+    uint8_t *dis = (uint8_t *)calloc(u + 1, 1);
+    float *memo = (float *)malloc((u * R + 1) * sizeof(float));
+    for (size_t i = 0; i < u * R; ++i) memo[i] = -1.0f;
+    size_t sel = 0;
+    float cur = 1.0f;
+    const float inc = alpha < 1.2f ? alpha : 1.2f;
+    while (sel < R) {
+        int promoted = 0;
+        for (size_t i = 0; i < u && sel < R; ++i) {
+            if (dis[i]) continue;
+            int occ = 0;
+            for (size_t kk = 0; kk < sel; ++kk) {
+                float *m = &memo[i * R + kk];
+                if (*m < 0.0f)
+                    *m = ro_symmetric_distance(s, cbase + (size_t)c[i].id * cstride,
+                                               cbase + (size_t)out[kk] * cstride);
+                if (cur * *m <= c[i].d) { occ = 1; break; }
+            }
+            if (occ) continue;
+            out[sel++] = c[i].id;
+            dis[i] = 1;
+            promoted = 1;
+        }
+        if (cur == alpha) break;
+        float next = cur * inc;
+        if (next > alpha) next = alpha;
+        if (next == cur) break;
+        cur = next;
+        if (!promoted && sel >= R) break;
+    }
+    free(c); free(dis); free(memo);
+    return sel;
+}

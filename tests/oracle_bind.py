@@ -1,0 +1,368 @@
+"""ctypes bindings for the CPU checkers under oracle/ (test infrastructure).
+
+* ``Oracle`` wraps oracle/lib/liboracle.so, the C restatement of the
+  reference algorithm (oracle/rabitq_oracle.c).
+* ``Ref`` wraps oracle/_ref/librabitq_ref.so, the reference's own C-ABI bridge
+  (experiments/02_diskann_fair/native/rabitq_bridge.cpp) compiled from its
+  sources by oracle/Makefile. It exists only where it was built.
+
+Nothing in the product package imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "librabitq_ref.so")
+
+# Contraction sites of the reference compiler, pinned against oracle/_ref by
+# tests/test_oracle_vs_ref.py (bit names follow oracle/rabitq_oracle.h).
+FMA = dict(ENC_NORM=1 << 0, FQA_T=1 << 1, ENC_OOBAR=1 << 2, ENC_DECNORM=1 << 3,
+           ENC_X4IP=1 << 4, C1_LANE=1 << 5, C1_SUB7=1 << 6, C1_EST=1 << 7,
+           GATE_UPPER=1 << 8, RES_HI=1 << 9, RES_LO=1 << 10, RES_ACC=1 << 11)
+
+
+class ROSpace(C.Structure):
+    _fields_ = [("dim", C.c_size_t), ("code_dim", C.c_size_t), ("block_size", C.c_size_t),
+                ("block_count", C.c_size_t), ("scale_bytes", C.c_size_t),
+                ("compact_code_bytes", C.c_size_t), ("residual_code_bytes", C.c_size_t),
+                ("compact_bytes", C.c_size_t), ("residual_bytes", C.c_size_t),
+                ("full_bytes", C.c_size_t), ("seed", C.c_uint32), ("mse", C.c_int),
+                ("fp16", C.c_int), ("inv_sqrt_code_dim", C.c_float),
+                ("signs", C.POINTER(C.c_float)), ("center", C.POINTER(C.c_float)),
+                ("rotated_center", C.POINTER(C.c_float)), ("center_norm_sqr", C.c_float)]
+
+
+class ROQuery(C.Structure):
+    _fields_ = [("q", C.POINTER(C.c_float)), ("x8", C.POINTER(C.c_int8)),
+                ("query_norm", C.c_float), ("query_norm_sqr", C.c_float),
+                ("half_sum", C.c_float), ("sum", C.c_float), ("positive_sum", C.c_float),
+                ("int8_scale", C.c_float), ("int8_error_norm", C.c_float),
+                ("int8_code_sum", C.c_int64)]
+
+
+class Estimate(C.Structure):
+    """RabitqPaperEstimate (rabitq_bridge.cpp:21-28), 24 bytes."""
+    _fields_ = [("lower_bound", C.c_float), ("short_ip", C.c_float), ("alpha", C.c_float),
+                ("ip_hat", C.c_float), ("error_bound", C.c_float), ("valid", C.c_uint8)]
+
+
+class Interval(C.Structure):
+    """RabitqDistanceInterval (rabitq_bridge.cpp:15-19), 12 bytes."""
+    _fields_ = [("estimate", C.c_float), ("lower_bound", C.c_float), ("upper_bound", C.c_float)]
+
+
+class SearchStats(C.Structure):
+    _fields_ = [("hops", C.c_uint64), ("gate_evals", C.c_uint64),
+                ("full_dists", C.c_uint64), ("pruned", C.c_uint64)]
+
+
+ESTIMATE_DTYPE = np.dtype([("lower_bound", "<f4"), ("short_ip", "<f4"), ("alpha", "<f4"),
+                           ("ip_hat", "<f4"), ("error_bound", "<f4"), ("valid", "u1"),
+                           ("pad", "u1", 3)])
+INTERVAL_DTYPE = np.dtype([("estimate", "<f4"), ("lower_bound", "<f4"), ("upper_bound", "<f4")])
+FACTOR_DTYPE = np.dtype([("norm_sqr", "<f4"), ("data_norm", "<f4"), ("cross_scale", "<f4"),
+                         ("error_cross_scale", "<f4"), ("valid", "u1"), ("pad", "u1", 3)])
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """The C restatement, one space per instance (K = 1, residual 4-bit)."""
+
+    def __init__(self, dim, seed, block=16, mse=True, fp16=True, fma_mask=None):
+        self.lib = C.CDLL(ORACLE_SO)
+        L = self.lib
+        L.ro_space_init.argtypes = [C.POINTER(ROSpace), C.c_size_t, C.c_uint32, C.c_size_t, C.c_int, C.c_int]
+        L.ro_set_fma_mask.argtypes = [C.c_uint]
+        L.ro_asym_distance.restype = C.c_float
+        L.ro_asym_distance_f64.restype = C.c_double
+        L.ro_remaining_distance_fp32.restype = C.c_float
+        L.ro_remaining_distance_fp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_remaining_distance_int8.restype = C.c_float
+        L.ro_remaining_distance_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_gate_fp32.restype = Estimate
+        L.ro_gate_fp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_gate_int8.restype = Estimate
+        L.ro_gate_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_int_codec_estimate.restype = Estimate
+        L.ro_int_codec_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_masked_sum.restype = C.c_int64
+        L.ro_remaining_dot_int8.restype = C.c_int64
+        L.ro_full_dot_int8.restype = C.c_int64
+        L.ro_symmetric_ip4.restype = C.c_int64
+        L.ro_symmetric_distance.restype = C.c_float
+        L.ro_residual_interval.restype = Interval
+        L.ro_residual_interval.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_residual_estimate_f64.restype = C.c_double
+        L.ro_residual_estimate_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
+        L.ro_search.restype = C.c_size_t
+        L.ro_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
+                                C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                C.c_void_p, C.c_size_t, C.c_void_p,
+                                C.c_void_p, C.c_void_p, C.c_size_t,
+                                C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_size_t,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p]
+        L.ro_robust_prune.restype = C.c_size_t
+        L.ro_robust_prune.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32,
+                                      C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                      C.c_float, C.c_void_p]
+        L.ro_fast_quantize_abs.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
+        L.ro_float_to_half.restype = C.c_uint16
+        L.ro_float_to_half.argtypes = [C.c_float]
+        L.ro_half_to_float.restype = C.c_float
+        L.ro_half_to_float.argtypes = [C.c_uint16]
+        self.s = ROSpace()
+        if L.ro_space_init(C.byref(self.s), dim, seed, block, int(mse), int(fp16)) != 0:
+            raise ValueError("bad space")
+        if fma_mask is not None:
+            self.set_fma(fma_mask)
+        self.dim, self.cd = dim, self.s.code_dim
+        self.full_bytes = self.s.full_bytes
+        self.compact_bytes = self.s.compact_bytes
+        self.residual_bytes = self.s.residual_bytes
+        self.msb_bytes = (self.cd + 7) // 8
+        self.sp = C.byref(self.s)
+
+    def __del__(self):
+        try:
+            self.lib.ro_space_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    def set_fma(self, mask):
+        self.lib.ro_set_fma_mask(mask)
+
+    def signs(self):
+        return np.ctypeslib.as_array(self.s.signs, (self.cd,)).copy()
+
+    def set_center(self, c):
+        c = np.ascontiguousarray(c, np.float32)
+        self.lib.ro_space_set_center(self.sp, _p(c))
+
+    def rotate(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(self.cd, np.float32)
+        self.lib.ro_rotate(self.sp, _p(x), _p(out))
+        return out
+
+    def encode_full(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.dim)
+        out = np.zeros((X.shape[0], self.full_bytes), np.uint8)
+        for i in range(X.shape[0]):
+            self.lib.ro_encode_full(self.sp, _p(X[i]), _p(out[i]))
+        return out
+
+    def split(self, full):
+        n = full.shape[0]
+        comp = np.zeros((n, self.compact_bytes), np.uint8)
+        res = np.zeros((n, self.residual_bytes), np.uint8)
+        for i in range(n):
+            self.lib.ro_copy_compact(self.sp, _p(full[i]), _p(comp[i]))
+            self.lib.ro_copy_residual(self.sp, _p(full[i]), _p(res[i]))
+        return comp, res
+
+    def sidecar(self, comp):
+        n = comp.shape[0]
+        msb = np.zeros((n, self.msb_bytes), np.uint8)
+        fac = np.zeros(n, FACTOR_DTYPE)
+        for i in range(n):
+            self.lib.ro_extract_sidecar(self.sp, _p(comp[i]), _p(msb[i]),
+                                        C.c_void_p(fac.ctypes.data + 20 * i))
+        return msb, fac
+
+    def fast_quantize_abs(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        code = np.zeros(a.size, np.uint8)
+        ipn = C.c_float()
+        self.lib.ro_fast_quantize_abs(a.size, _p(a), _p(code), C.byref(ipn))
+        return code, ipn.value
+
+    def prepare(self, raw, codec=3):
+        q = ROQuery()
+        self.lib.ro_query_alloc(self.sp, C.byref(q))
+        raw = np.ascontiguousarray(raw, np.float32)
+        self.lib.ro_prepare_query(self.sp, _p(raw), codec, C.byref(q))
+        q._raw = raw
+        return q
+
+    def free_query(self, q):
+        self.lib.ro_query_free(C.byref(q))
+
+    def qvec(self, q):
+        return np.ctypeslib.as_array(q.q, (self.cd,)).copy()
+
+    def qx8(self, q):
+        return np.ctypeslib.as_array(q.x8, (self.cd,)).copy()
+
+    def asym(self, q, comp):
+        return np.array([self.lib.ro_asym_distance(self.sp, C.byref(q), _p(comp[i]))
+                         for i in range(comp.shape[0])], np.float32)
+
+    def asym_f64(self, q, comp):
+        return np.array([self.lib.ro_asym_distance_f64(self.sp, C.byref(q), _p(comp[i]))
+                         for i in range(comp.shape[0])], np.float64)
+
+
+class Ref:
+    """The reference bridge compiled from source (oracle/_ref)."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_SO)
+
+    def __init__(self, dim, seed, centroid_count=1, residual_bits=4, block=16, mse=True, fp16=True):
+        self.lib = C.CDLL(REF_SO)
+        L = self.lib
+        L.rabitq_space_new.restype = C.c_void_p
+        L.rabitq_space_new.argtypes = [C.c_size_t, C.c_uint32, C.c_size_t, C.c_size_t,
+                                       C.c_size_t, C.c_bool, C.c_bool]
+        L.rabitq_space_delete.argtypes = [C.c_void_p]
+        L.rabitq_space_set_center.restype = C.c_bool
+        L.rabitq_space_set_center.argtypes = [C.c_void_p, C.c_void_p]
+        for f in ("full", "compact", "residual"):
+            getattr(L, "rabitq_%s_record_bytes" % f).restype = C.c_size_t
+            getattr(L, "rabitq_%s_record_bytes" % f).argtypes = [C.c_void_p]
+        L.rabitq_paper_msb_code_bytes.restype = C.c_size_t
+        L.rabitq_paper_msb_code_bytes.argtypes = [C.c_void_p]
+        L.rabitq_encode_full.restype = C.c_bool
+        L.rabitq_encode_full.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_copy_compact.restype = C.c_bool
+        L.rabitq_copy_compact.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_copy_residual_record.restype = C.c_bool
+        L.rabitq_copy_residual_record.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_export_paper_sidecar.restype = C.c_bool
+        L.rabitq_export_paper_sidecar.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                                  C.c_void_p, C.c_void_p]
+        L.rabitq_prepare_query.restype = C.c_void_p
+        L.rabitq_prepare_query.argtypes = [C.c_void_p, C.c_void_p]
+        L.rabitq_set_query_coarse_codec.argtypes = [C.c_void_p, C.c_int]
+        L.rabitq_release_query.argtypes = [C.c_void_p, C.c_void_p]
+        L.rabitq_query_distance.restype = C.c_float
+        L.rabitq_query_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_symmetric_distance.restype = C.c_float
+        L.rabitq_symmetric_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_query_distance_batch.restype = C.c_bool
+        L.rabitq_query_distance_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_size_t, C.c_void_p, C.c_size_t,
+                                                  C.c_void_p, C.c_void_p]
+        L.rabitq_paper_estimate_batch_sidecar.restype = C.c_bool
+        L.rabitq_paper_estimate_batch_sidecar.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                          C.c_size_t, C.c_void_p, C.c_size_t,
+                                                          C.c_void_p, C.c_size_t, C.c_float,
+                                                          C.c_void_p]
+        L.rabitq_paper_estimate_codec.restype = Estimate
+        L.rabitq_paper_estimate_codec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_float]
+        L.rabitq_residual_distance_batch.restype = C.c_bool
+        L.rabitq_residual_distance_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                                     C.c_void_p, C.c_size_t, C.c_void_p,
+                                                     C.c_size_t, C.c_void_p, C.c_void_p]
+        L.rabitq_build_vamana_graph.restype = C.c_bool
+        L.rabitq_build_vamana_graph.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                                C.c_size_t, C.c_size_t, C.c_float, C.c_size_t,
+                                                C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                                C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        self.h = L.rabitq_space_new(dim, seed, centroid_count, residual_bits, block, mse, fp16)
+        if not self.h:
+            raise ValueError("rabitq_space_new failed")
+        self.dim = dim
+        self.full_bytes = L.rabitq_full_record_bytes(self.h)
+        self.compact_bytes = L.rabitq_compact_record_bytes(self.h)
+        self.residual_bytes = L.rabitq_residual_record_bytes(self.h)
+        self.msb_bytes = L.rabitq_paper_msb_code_bytes(self.h)
+
+    def __del__(self):
+        try:
+            self.lib.rabitq_space_delete(self.h)
+        except Exception:
+            pass
+
+    def set_center(self, c):
+        c = np.ascontiguousarray(c, np.float32)
+        assert self.lib.rabitq_space_set_center(self.h, _p(c))
+
+    def set_codec(self, codec):
+        self.lib.rabitq_set_query_coarse_codec(self.h, codec)
+
+    def encode_full(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.dim)
+        out = np.zeros((X.shape[0], self.full_bytes), np.uint8)
+        for i in range(X.shape[0]):
+            assert self.lib.rabitq_encode_full(self.h, _p(X[i]), _p(out[i]))
+        return out
+
+    def split(self, full):
+        n = full.shape[0]
+        comp = np.zeros((n, self.compact_bytes), np.uint8)
+        res = np.zeros((n, self.residual_bytes), np.uint8)
+        for i in range(n):
+            assert self.lib.rabitq_copy_compact(self.h, _p(full[i]), _p(comp[i]))
+            assert self.lib.rabitq_copy_residual_record(self.h, _p(full[i]), _p(res[i]))
+        return comp, res
+
+    def sidecar(self, comp):
+        n = comp.shape[0]
+        msb = np.zeros((n, self.msb_bytes), np.uint8)
+        fac = np.zeros(n, FACTOR_DTYPE)
+        assert self.lib.rabitq_export_paper_sidecar(self.h, _p(comp), n, comp.shape[1],
+                                                    _p(msb), _p(fac))
+        return msb, fac
+
+    def prepare(self, raw):
+        """The returned handle is thread-local storage: valid until the next
+        prepare on this thread (space_rabitq.h:3700-3706)."""
+        self._raw = np.ascontiguousarray(raw, np.float32)
+        return self.lib.rabitq_prepare_query(self.h, _p(self._raw))
+
+    def distance_batch(self, pq, comp, ids, modes, short_ips=None):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        modes = np.ascontiguousarray(modes, np.uint8)
+        if short_ips is None:
+            short_ips = np.zeros(ids.size, np.float32)
+        short_ips = np.ascontiguousarray(short_ips, np.float32)
+        out = np.zeros(ids.size, np.float32)
+        assert self.lib.rabitq_query_distance_batch(self.h, pq, _p(ids), _p(modes), ids.size,
+                                                    _p(comp), comp.shape[1], _p(short_ips), _p(out))
+        return out
+
+    def gate_batch(self, pq, msb, fac, ids, eps0):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros(ids.size, ESTIMATE_DTYPE)
+        assert self.lib.rabitq_paper_estimate_batch_sidecar(self.h, pq, _p(ids), ids.size,
+                                                            _p(msb), msb.shape[1], _p(fac), 20,
+                                                            C.c_float(eps0), _p(out))
+        return out
+
+    def residual_batch(self, pq, comp, res, ids, long_d):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        long_d = np.ascontiguousarray(long_d, np.float32)
+        out = np.zeros(ids.size, INTERVAL_DTYPE)
+        assert self.lib.rabitq_residual_distance_batch(self.h, pq, _p(ids), ids.size, _p(comp),
+                                                       comp.shape[1], _p(res), res.shape[1],
+                                                       _p(long_d), _p(out))
+        return out
+
+    def symmetric(self, a, b):
+        return self.lib.rabitq_symmetric_distance(self.h, _p(a), _p(b))
+
+    def build_graph(self, comp, r, l_build, alpha, beam=1, batch=4096, refine=0, cap=0,
+                    early_stop=0):
+        n = comp.shape[0]
+        deg = np.zeros(n, np.uint32)
+        edges = np.zeros((n, r), np.uint32)
+        msb = np.zeros((n, self.msb_bytes), np.uint8)
+        fac = np.zeros(n, FACTOR_DTYPE)
+        ms, fb, de = C.c_size_t(), C.c_size_t(), C.c_uint64()
+        ok = self.lib.rabitq_build_vamana_graph(self.h, _p(comp), n, comp.shape[1], r, l_build,
+                                                C.c_float(alpha), beam, batch, refine, cap,
+                                                early_stop, _p(deg), _p(edges), r, _p(msb),
+                                                _p(fac), C.byref(ms), C.byref(fb), C.byref(de))
+        assert ok
+        return deg, edges, msb, fac, de.value
