@@ -1,0 +1,204 @@
+/*
+ * qgraph_rabitq.h -- C-ABI of libqgraph_hip.so, the MI355X (gfx950)
+ * implementation of the ExRaBitQ4 distance path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. Device batch API (qg_*): many queries per call, device pointers,
+ *     an explicit hipStream_t (passed as void*), structure-of-arrays index
+ *     views. Status is an int (0 = success); qg_last_error() describes the
+ *     last failure on the calling thread.
+ *
+ *  2. Bridge API (rabitq_*): the exact symbols, argument meanings, struct
+ *     layouts and error sentinels of the reference C-ABI
+ *     experiments/02_diskann_fair/native/rabitq_bridge.cpp, operating on
+ *     host memory. Each call stages its inputs to the GPU, runs the same
+ *     kernels as the batch API, and copies the results back. A caller
+ *     linked against the reference bridge links against this library
+ *     unchanged.
+ *
+ * Numerics: distances are bit-identical to the reference CPU path (same
+ * accumulation order and fused multiply-adds); integer dots are exact.
+ */
+#ifndef QGRAPH_RABITQ_H
+#define QGRAPH_RABITQ_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ POD */
+
+/* rabitq_bridge.cpp:15-19 */
+typedef struct RabitqDistanceInterval {
+    float estimate;
+    float lower_bound;
+    float upper_bound;
+} RabitqDistanceInterval;
+
+/* rabitq_bridge.cpp:21-28 */
+typedef struct RabitqPaperEstimate {
+    float lower_bound;
+    float short_ip;
+    float alpha;
+    float ip_hat;
+    float error_bound;
+    uint8_t valid;
+} RabitqPaperEstimate;
+
+/* rabitq_bridge.cpp:30-36 (PaperPruneFactors<float>, hnswlib.h:298-305) */
+typedef struct RabitqPaperFactors {
+    float norm_sqr;
+    float data_norm;
+    float cross_scale;
+    float error_cross_scale;
+    uint8_t valid;
+} RabitqPaperFactors;
+
+/* Structure-of-arrays device index (row i = base vector i). */
+typedef struct qg_index_desc {
+    uint8_t *codes;     /* [n][cd/2] primary 4-bit codes (reference nibble layout)  */
+    float *hdr;         /* [n][2]    norm_sqr, long_scale                           */
+    float *sfac;        /* [n][2]    short_scale, error_scale                       */
+    uint8_t *msb;       /* [n][cd/8] MSB plane                                      */
+    float *factors;     /* [n][4]    norm_sqr, data_norm, cross_scale, err_cross    */
+    uint8_t *fvalid;    /* [n]       sidecar validity                               */
+    uint16_t *rscale;   /* [n][cd/16] residual block scales, fp16 bits              */
+    uint8_t *rcodes;    /* [n][cd/2] residual 4-bit codes                           */
+    float *rfac;        /* [n][2]    residual_norm_sqr, long_residual_ip            */
+    uint64_t n;
+} qg_index_desc;
+
+/* Prepared queries on the device (QueryContext, space_rabitq.h:101-132). */
+typedef struct qg_query_desc {
+    float *q;           /* [nq][cd] rotated residual                              */
+    int8_t *x8;         /* [nq][cd] INT8 code                                     */
+    int8_t *x8p;        /* [nq][cd] INT8 code regrouped: per 8 dims, 4 even, 4 odd */
+    void *qs;           /* [nq] 48-byte scalar records (qg_query_scalars)          */
+    uint64_t nq;
+} qg_query_desc;
+
+typedef struct qg_query_scalars {
+    float query_norm;
+    float query_norm_sqr;
+    float half_sum;
+    float sum;
+    float positive_sum;
+    float int8_scale;
+    float int8_error_norm;
+    int32_t max_abs_bits;
+    int64_t int8_code_sum;
+    int64_t pad;
+} qg_query_scalars;
+
+/* Query coarse codecs (QueryCoarseCodec, space_rabitq.h:36-44). */
+enum { QG_CODEC_FULL = 0, QG_CODEC_INT8 = 3 };
+
+/* Estimator kernels for qg_scan. */
+enum {
+    QG_SCAN_INT8_FULL = 0, /* C7: INT8 query x full code; out_f = estimate, out_f2 = lower   */
+    QG_SCAN_ASYM = 1,      /* C1/C2: FP32 query x 4-bit code (mode 0)                         */
+    QG_SCAN_REM_FP32 = 2,  /* C3: FP32 remaining bits given short_ips (mode 1)                */
+    QG_SCAN_REM_INT8 = 3,  /* C6: INT8 remaining bits given short_ips (mode 2)                */
+    QG_SCAN_GATE_FP32 = 4, /* C4: FP32 1-bit gate -> RabitqPaperEstimate                      */
+    QG_SCAN_GATE_INT8 = 5, /* C5: INT8 1-bit gate -> RabitqPaperEstimate                      */
+    QG_SCAN_RESIDUAL = 6,  /* E1: residual-4 interval from long_in -> RabitqDistanceInterval  */
+    QG_SCAN_SYMMETRIC = 7  /* C8: code<->code distance, pairs (qidx = lhs row, ids = rhs row)  */
+};
+
+/* -------------------------------------------------------- device batch API */
+
+const char *qg_last_error(void);
+
+/* Space lifecycle. Replaces rabitq_space_new / _delete / _set_center
+ * (rabitq_bridge.cpp:42-87) with explicit device constants. */
+void *qg_space_new(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits,
+                   size_t residual_block_size, bool residual_mse, bool residual_fp16);
+void qg_space_delete(void *space);
+int qg_space_set_center(void *space, const float *center_host);
+size_t qg_space_dim(const void *space);
+size_t qg_space_code_dim(const void *space);
+/* Device pointers to the rotation signs [cd] and the rotated center [cd]. */
+const float *qg_space_signs(const void *space);
+const float *qg_space_rotated_center(const void *space);
+
+/* Encode n base vectors (device, row-major [n][dim]) into the SoA index.
+ * Replaces rabitq_encode_full + copy_compact + copy_residual_record +
+ * export_paper_sidecar (rabitq_bridge.cpp:195-222, 317-343). */
+size_t qg_encode_workspace_bytes(const void *space, uint64_t n);
+int qg_encode(const void *space, const float *x, uint64_t n, qg_index_desc *out,
+              void *workspace, size_t workspace_bytes, void *stream);
+
+/* SoA <-> reference AoS records (compact 16+cd/2+1 bytes; residual
+ * 8+scale_bytes+cd/2 bytes; full record), all device pointers. */
+int qg_pack_records(const void *space, const qg_index_desc *ix, uint8_t *compact, size_t compact_stride,
+                    uint8_t *residual, size_t residual_stride, void *stream);
+int qg_unpack_records(const void *space, const uint8_t *compact, size_t compact_stride,
+                      const uint8_t *residual, size_t residual_stride, uint64_t n,
+                      qg_index_desc *out, void *stream);
+
+/* Prepare nq queries (device [nq][dim]). Replaces rabitq_prepare_query +
+ * rabitq_set_query_coarse_codec (rabitq_bridge.cpp:355-370). Workspace:
+ * qg_prepare_workspace_bytes(nq) bytes of device memory. */
+size_t qg_prepare_workspace_bytes(uint64_t nq);
+int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int codec,
+                       qg_query_desc *out, void *workspace, void *stream);
+
+/* Estimators over candidate lists. fixed != 0: ids is [nq][per_query];
+ * fixed == 0: (qidx[i], ids[i]) pairs, i < n_pairs (for QG_SCAN_SYMMETRIC
+ * both are rows of ix and qv is unused). Replaces
+ * rabitq_query_distance_batch (:885), rabitq_paper_estimate_batch_sidecar
+ * (:702), rabitq_residual_distance_batch (:972) for many queries at once. */
+int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, const qg_query_desc *qv,
+            const uint32_t *ids, const uint32_t *qidx, uint64_t per_query, uint64_t n_pairs, float eps0,
+            const float *short_ips, const float *long_in, float *out_f, float *out_f2,
+            RabitqPaperEstimate *out_est, RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream);
+
+/* Per-query top-k by (value, id) over [nq][per_query] lists (per_query <= 4096). */
+int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_query, uint32_t k,
+            uint32_t *out_ids, float *out_values, uint32_t *out_pos, void *stream);
+
+/* ------------------------------------------------------------ bridge API */
+/* Host-memory drop-ins for rabitq_bridge.cpp; see that file for semantics. */
+
+void *rabitq_space_new(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits,
+                       size_t residual_block_size, bool residual_mse, bool residual_fp16); /* :42  */
+void rabitq_space_delete(void *space);                                                 /* :76  */
+bool rabitq_space_set_center(void *space, const float *center);                       /* :80  */
+size_t rabitq_full_record_bytes(const void *space);                                    /* :174 */
+size_t rabitq_compact_record_bytes(const void *space);                                 /* :178 */
+size_t rabitq_residual_record_bytes(const void *space);                                /* :183 */
+size_t rabitq_residual_block_size(const void *space);                                  /* :187 */
+size_t rabitq_residual_bits(const void *space);                                        /* :191 */
+bool rabitq_encode_full(const void *space, const float *raw, void *encoded_out);       /* :195 */
+bool rabitq_copy_compact(const void *space, const void *full, void *compact_out);      /* :204 */
+bool rabitq_copy_residual_record(const void *space, const void *full, void *out);      /* :214 */
+size_t rabitq_paper_msb_code_bytes(const void *space);                                 /* :305 */
+size_t rabitq_paper_factor_bytes(void);                                                /* :313 */
+bool rabitq_export_paper_sidecar(const void *space, const void *compact_records, size_t record_count,
+                                 size_t compact_stride, uint8_t *msb_out, void *factors_out); /* :317 */
+float rabitq_symmetric_distance(const void *space, const void *lhs, const void *rhs); /* :345 */
+const void *rabitq_prepare_query(const void *space, const float *query);              /* :355 */
+void rabitq_set_query_coarse_codec(void *space, int codec);                           /* :364 */
+void rabitq_release_query(const void *space, const void *prepared);                   /* :372 */
+float rabitq_query_distance(const void *space, const void *prepared, const void *encoded); /* :380 */
+bool rabitq_query_distance_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                 const uint8_t *modes, size_t count, const void *encoded_base,
+                                 size_t encoded_stride, const float *short_ips, float *out); /* :885 */
+bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared, const uint32_t *ids,
+                                         size_t count, const uint8_t *msb_base, size_t msb_stride,
+                                         const void *factors_base, size_t factors_stride,
+                                         float epsilon0, RabitqPaperEstimate *out);   /* :702 */
+bool rabitq_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                    size_t count, const void *encoded_base, size_t encoded_stride,
+                                    const void *residual_base, size_t residual_stride,
+                                    const float *long_distances, RabitqDistanceInterval *out); /* :972 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -104,6 +104,24 @@ static double u01_from(uint64_t x) {
     return r;
 }
 
+/* n-th output (1-based) of each engine from a seed: the standard's check
+ * values are mt19937(5489)[10000] = 4123659995 and mt19937_64(5489)[10000]
+ * = 9981545732273789042 ([rand.predef]). */
+uint32_t ro_mt32_nth(uint32_t seed, size_t n) {
+    mt32 g;
+    uint32_t v = 0;
+    mt32_seed(&g, seed);
+    for (size_t i = 0; i < n; ++i) v = mt32_next(&g);
+    return v;
+}
+uint64_t ro_mt64_nth(uint64_t seed, size_t n) {
+    mt64 g;
+    uint64_t v = 0;
+    mt64_seed(&g, seed);
+    for (size_t i = 0; i < n; ++i) v = mt64_next(&g);
+    return v;
+}
+
 /* ------------------------------------------------------------------ space */
 size_t ro_code_dim(size_t dim) { /* roundUp64, space_rabitq.h:217-224 */
     size_t rounded = ((dim + 63u) / 64u) * 64u;
@@ -642,8 +660,8 @@ static ro_estimate finish_gate(float short_ip, const ro_factors *f, const ro_que
     const float max_cross = 2.0f * f->data_norm * q->query_norm;
     const float ee = eps0 * f->error_cross_scale;
     float up = madd_f(RO_FMA_GATE_UPPER, ee, q->query_norm, cross);
-    if (up > max_cross) up = max_cross;
-    if (up < -max_cross) up = -max_cross;
+    up = (up < max_cross) ? up : max_cross;     /* std::min(max_cross, up) */
+    up = (-max_cross < up) ? up : -max_cross;   /* std::max(-max_cross, up) */
     r.lower_bound = f->norm_sqr + q->query_norm_sqr - up;
     r.short_ip = short_ip;
     r.ip_hat = max_cross > 0.0f ? cross / max_cross : 0.0f;
@@ -730,6 +748,28 @@ ro_estimate ro_int_codec_estimate(const ro_space *s, const ro_query *q, const ui
     r.error_bound = est - r.lower_bound;
     r.valid = isfinite(r.lower_bound) ? 1 : 0;
     return r;
+}
+
+/* The estimate and lower bound of finish_int_paper_prune_estimate
+ * (:5238-5252) for the INT8 query x full 4-bit code. */
+float ro_int8_full_estimate(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0,
+                            float *lower_out) {
+    float hdr[4];
+    memcpy(hdr, compact, 16);
+    const float scale = q->int8_scale;
+    if (!long_scale_ok(hdr[1]) || !(scale > 0.0f) || !isfinite(scale) || q->query_norm <= 0.0f) {
+        const float v = hdr[0] + q->query_norm_sqr;
+        if (lower_out) *lower_out = v;
+        return v;
+    }
+    const int64_t dot = ro_full_dot_int8(s, q, compact);
+    const double centered = (double)dot - 7.5 * (double)q->int8_code_sum;
+    const double sip = (double)scale * centered;
+    const double est_ip = (double)hdr[1] * sip;
+    const float est = hdr[0] + q->query_norm_sqr - (float)est_ip;
+    const double db_err = (double)hdr[3] * (double)q->query_norm;
+    if (lower_out) *lower_out = est - (float)((double)eps0 * db_err);
+    return est;
 }
 
 /* centeredNibbleDotTimesFour (:2405-2538): sum (2a-15)(2b-15), exact. */

@@ -74,6 +74,8 @@ typedef struct ro_space {
     float center_norm_sqr;
 } ro_space;
 
+uint32_t ro_mt32_nth(uint32_t seed, size_t n);
+uint64_t ro_mt64_nth(uint64_t seed, size_t n);
 int ro_space_init(ro_space *s, size_t dim, uint32_t seed, size_t block, int mse, int fp16);
 void ro_space_free(ro_space *s);
 void ro_space_set_center(ro_space *s, const float *center);
@@ -141,6 +143,8 @@ int64_t ro_masked_sum(const ro_space *s, const ro_query *q, const uint8_t *msb);
 int64_t ro_remaining_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact);
 int64_t ro_full_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact);
 float ro_remaining_distance_int8(const ro_space *s, const ro_query *q, const uint8_t *compact, float short_ip);
+float ro_int8_full_estimate(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0,
+                            float *lower_out);
 ro_estimate ro_int_codec_estimate(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0);
 int64_t ro_symmetric_ip4(const ro_space *s, const uint8_t *lhs, const uint8_t *rhs);
 float ro_symmetric_distance(const ro_space *s, const uint8_t *lhs, const uint8_t *rhs);

@@ -1,0 +1,368 @@
+// qg_api.cpp -- C-ABI of libqgraph_hip.so: space lifecycle and the device
+// batch entry points declared in include/qgraph_rabitq.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "qg_args.h"
+#include "qg_space.h"
+
+// kernel launchers (qg_query.hip, qg_scan.hip, qg_select.hip, qg_encode.hip)
+extern "C" {
+int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float *center, uint64_t *seed,
+                       double *norm, double *dot, hipStream_t stream);
+int qg_launch_rotate_rows(const float *rows, uint64_t n, uint32_t dim, uint32_t cd, int identity,
+                          const float *signs, float *out, hipStream_t stream);
+int qg_launch_query_prep(const qg::PrepParams *p, hipStream_t stream);
+int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t stream);
+int qg_launch_topk(const qg::SelectArgs *a, hipStream_t stream);
+int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream);
+int qg_launch_pack(const qg::PackArgs *a, uint64_t n, int unpack, hipStream_t stream);
+}
+
+namespace qg {
+
+thread_local std::string g_last_error;
+
+int fail(const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return -1;
+}
+
+int check_hip(int rc, const char *what) {
+    if (rc != 0) return fail("%s: %s", what, hipGetErrorString((hipError_t)rc));
+    return 0;
+}
+
+size_t round_up64_pow2(size_t value) {  // roundUp64, space_rabitq.h:217-224
+    const size_t rounded = ((value + 63u) / 64u) * 64u;
+    size_t p = 1;
+    while (p < rounded) p <<= 1u;
+    return p;
+}
+
+Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits, size_t block,
+                    bool mse, bool fp16) {
+    if (dim == 0) { fail("dimension must be positive"); return nullptr; }
+    if (centroid_count == 0 || centroid_count > 256) { fail("centroid_count must be in [1, 256]"); return nullptr; }
+    if (centroid_count != 1) { fail("only centroid_count = 1 (the formal configuration) is implemented"); return nullptr; }
+    if (residual_bits != 4) { fail("only 4-bit residual codes are implemented"); return nullptr; }
+    if (block != 16) { fail("only residual block size 16 is implemented"); return nullptr; }
+    if (!fp16) { fail("only fp16 residual scale storage is implemented"); return nullptr; }
+    Space *s = new Space();
+    s->dim = dim;
+    s->cd = round_up64_pow2(dim);
+    if (s->cd > 2048) { delete s; fail("code dimension %zu exceeds 2048", round_up64_pow2(dim)); return nullptr; }
+    s->seed = seed;
+    s->block = block;
+    s->block_count = (s->cd + block - 1) / block;
+    s->mse = mse;
+    s->fp16 = fp16;
+    s->codec = 0;
+    s->compact_bytes = 16 + s->cd / 2 + 1;
+    s->residual_bytes = 8 + 2 * s->block_count + s->cd / 2;
+    s->full_bytes = 24 + 2 * s->block_count + s->cd / 2 + 1 + s->cd / 2;
+    // setRandomRotation (space_rabitq.h:2900-2906): the same libstdc++
+    // engine and distribution, so the sign sequence is identical.
+    const float inv_sqrt = 1.0f / std::sqrt(static_cast<float>(s->cd));
+    s->h_signs.assign(s->cd, inv_sqrt);
+    if (seed != 0) {
+        std::mt19937 rng(seed);
+        std::uniform_int_distribution<int> bernoulli(0, 1);
+        for (size_t i = 0; i < s->cd; ++i) s->h_signs[i] = (bernoulli(rng) ? 1.0f : -1.0f) * inv_sqrt;
+    }
+    s->h_center.assign(dim, 0.0f);
+    if (check_hip(hipMalloc(&s->d_signs, s->cd * 4), "hipMalloc signs") ||
+        check_hip(hipMalloc(&s->d_center, dim * 4), "hipMalloc center") ||
+        check_hip(hipMalloc(&s->d_rot_center, s->cd * 4), "hipMalloc rotated center")) {
+        space_destroy(s);
+        return nullptr;
+    }
+    hipMemcpy(s->d_signs, s->h_signs.data(), s->cd * 4, hipMemcpyHostToDevice);
+    hipMemset(s->d_center, 0, dim * 4);
+    hipMemset(s->d_rot_center, 0, s->cd * 4);
+    s->center_norm_sqr = 0.0f;
+    return s;
+}
+
+void space_destroy(Space *s) {
+    if (!s) return;
+    if (s->d_signs) hipFree(s->d_signs);
+    if (s->d_center) hipFree(s->d_center);
+    if (s->d_rot_center) hipFree(s->d_rot_center);
+    if (s->d_bridge) hipFree(s->d_bridge);
+    delete s;
+}
+
+// setCentroids for K = 1 (space_rabitq.h:2918-2936).
+int space_set_center(Space *s, const float *center_host) {
+    if (!s || !center_host) return fail("null space or center");
+    s->h_center.assign(center_host, center_host + s->dim);
+    if (check_hip(hipMemcpy(s->d_center, center_host, s->dim * 4, hipMemcpyHostToDevice), "upload center")) return -1;
+    double *d_norm = nullptr;
+    if (check_hip(hipMalloc(&d_norm, sizeof(double)), "hipMalloc")) return -1;
+    int rc = qg_launch_row_seed(s->d_center, 1, (uint32_t)s->dim, nullptr, nullptr, d_norm, nullptr, 0);
+    if (!rc) rc = qg_launch_rotate_rows(s->d_center, 1, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0,
+                                        s->d_signs, s->d_rot_center, 0);
+    double norm = 0.0;
+    if (!rc) rc = hipMemcpy(&norm, d_norm, sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(d_norm);
+    if (rc) return check_hip(rc, "set center");
+    s->center_norm_sqr = static_cast<float>(norm);
+    return 0;
+}
+
+IndexView view_of(const qg_index_desc *d) {
+    IndexView v;
+    v.codes = d->codes;
+    v.hdr = reinterpret_cast<const float2 *>(d->hdr);
+    v.sfac = reinterpret_cast<const float2 *>(d->sfac);
+    v.msb = d->msb;
+    v.factors = reinterpret_cast<const float4 *>(d->factors);
+    v.fvalid = d->fvalid;
+    v.rscale = d->rscale;
+    v.rcodes = d->rcodes;
+    v.n = d->n;
+    return v;
+}
+
+QueryView view_of(const qg_query_desc *d) {
+    QueryView v;
+    v.q = d->q;
+    v.x8 = d->x8;
+    v.x8p = d->x8p;
+    v.qs = reinterpret_cast<const QScalars *>(d->qs);
+    v.nq = d->nq;
+    return v;
+}
+
+size_t encode_row_ws_bytes(const Space *s) {
+    return s->cd * 4 /*rot*/ + s->cd * 4 /*sa*/ + s->cd * 2 /*si*/ + sizeof(EncMeta);
+}
+
+int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void *ws, size_t ws_bytes,
+           hipStream_t stream) {
+    if (!s || (!x && n) || !out) return fail("null argument to encode");
+    if (n == 0) return 0;
+    if (!out->codes || !out->hdr || !out->sfac || !out->msb || !out->factors || !out->fvalid || !out->rscale ||
+        !out->rcodes || !out->rfac)
+        return fail("encode output view has a null array");
+    const size_t per = encode_row_ws_bytes(s);
+    const uint64_t batch = ws_bytes / per;
+    if (batch == 0 || !ws) return fail("encode workspace too small: need at least %zu bytes", per);
+    const uint32_t cd = (uint32_t)s->cd;
+    for (uint64_t b0 = 0; b0 < n; b0 += batch) {
+        const uint64_t m = (n - b0 < batch) ? n - b0 : batch;
+        char *w = static_cast<char *>(ws);
+        EncArgs a;
+        a.x = x + b0 * s->dim;
+        a.n = m;
+        a.dim = (uint32_t)s->dim;
+        a.cd = cd;
+        a.block = (uint32_t)s->block;
+        a.block_count = (uint32_t)s->block_count;
+        a.identity = s->seed == 0;
+        a.mse = s->mse;
+        a.signs = s->d_signs;
+        a.center = s->d_center;
+        a.rot = reinterpret_cast<float *>(w);
+        a.sa = reinterpret_cast<float *>(w + m * cd * 4);
+        a.si = reinterpret_cast<uint16_t *>(w + m * cd * 8);
+        a.meta = reinterpret_cast<EncMeta *>(w + ((m * cd * 10 + 15) / 16) * 16);
+        a.codes = out->codes + b0 * (cd / 2);
+        a.hdr = reinterpret_cast<float2 *>(out->hdr) + b0;
+        a.sfac = reinterpret_cast<float2 *>(out->sfac) + b0;
+        a.msb = out->msb + b0 * (cd / 8);
+        a.factors = reinterpret_cast<float4 *>(out->factors) + b0;
+        a.fvalid = out->fvalid + b0;
+        a.rscale = out->rscale + b0 * s->block_count;
+        a.rcodes = out->rcodes + b0 * (cd / 2);
+        a.rfac = reinterpret_cast<float2 *>(out->rfac) + b0;
+        if (check_hip(qg_launch_encode(&a, stream), "encode launch")) return -1;
+    }
+    return 0;
+}
+
+}  // namespace qg
+
+using namespace qg;
+
+extern "C" {
+
+const char *qg_last_error(void) { return g_last_error.c_str(); }
+
+void *qg_space_new(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits, size_t block,
+                   bool mse, bool fp16) {
+    return space_create(dim, seed, centroid_count, residual_bits, block, mse, fp16);
+}
+
+void qg_space_delete(void *space) { space_destroy(static_cast<Space *>(space)); }
+
+int qg_space_set_center(void *space, const float *center_host) {
+    return space_set_center(static_cast<Space *>(space), center_host);
+}
+
+size_t qg_space_dim(const void *space) { return space ? static_cast<const Space *>(space)->dim : 0; }
+size_t qg_space_code_dim(const void *space) { return space ? static_cast<const Space *>(space)->cd : 0; }
+const float *qg_space_signs(const void *space) { return space ? static_cast<const Space *>(space)->d_signs : nullptr; }
+const float *qg_space_rotated_center(const void *space) {
+    return space ? static_cast<const Space *>(space)->d_rot_center : nullptr;
+}
+
+size_t qg_encode_workspace_bytes(const void *space, uint64_t n) {
+    if (!space) return 0;
+    const Space *s = static_cast<const Space *>(space);
+    return n * encode_row_ws_bytes(s) + 64;
+}
+
+int qg_encode(const void *space, const float *x, uint64_t n, qg_index_desc *out, void *ws, size_t ws_bytes,
+              void *stream) {
+    return encode(static_cast<const Space *>(space), x, n, out, ws, ws_bytes, (hipStream_t)stream);
+}
+
+int qg_pack_records(const void *space, const qg_index_desc *ix, uint8_t *compact, size_t compact_stride,
+                    uint8_t *residual, size_t residual_stride, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !ix || !compact || !residual) return fail("null argument to qg_pack_records");
+    if (compact_stride < s->compact_bytes || residual_stride < s->residual_bytes)
+        return fail("record stride smaller than the record size");
+    PackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ix = view_of(ix);
+    a.rfac = reinterpret_cast<const float2 *>(ix->rfac);
+    a.cd = (uint32_t)s->cd;
+    a.block_count = (uint32_t)s->block_count;
+    a.compact = compact;
+    a.compact_stride = compact_stride;
+    a.residual = residual;
+    a.residual_stride = residual_stride;
+    return check_hip(qg_launch_pack(&a, ix->n, 0, (hipStream_t)stream), "pack launch");
+}
+
+int qg_unpack_records(const void *space, const uint8_t *compact, size_t compact_stride, const uint8_t *residual,
+                      size_t residual_stride, uint64_t n, qg_index_desc *out, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !compact || !out) return fail("null argument to qg_unpack_records");
+    if (compact_stride < s->compact_bytes || (residual && residual_stride < s->residual_bytes))
+        return fail("record stride smaller than the record size");
+    PackArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.cd = (uint32_t)s->cd;
+    a.block_count = (uint32_t)s->block_count;
+    a.compact_stride = compact_stride;
+    a.residual_stride = residual_stride;
+    a.in_compact = compact;
+    a.in_residual = residual;
+    a.codes = out->codes;
+    a.hdr = reinterpret_cast<float2 *>(out->hdr);
+    a.sfac = reinterpret_cast<float2 *>(out->sfac);
+    a.rscale = out->rscale;
+    a.rcodes = out->rcodes;
+    a.rfac_out = reinterpret_cast<float2 *>(out->rfac);
+    a.msb = out->msb;
+    a.factors = reinterpret_cast<float4 *>(out->factors);
+    a.fvalid = out->fvalid;
+    out->n = n;
+    return check_hip(qg_launch_pack(&a, n, 1, (hipStream_t)stream), "unpack launch");
+}
+
+size_t qg_prepare_workspace_bytes(uint64_t nq) { return nq * 24 + 64; }
+
+int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int codec, qg_query_desc *out,
+                       void *ws, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || (!queries && nq) || !out || (!ws && nq)) return fail("null argument to qg_prepare_queries");
+    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
+    if (!out->q || !out->x8 || !out->x8p || !out->qs) return fail("query output view has a null array");
+    out->nq = nq;
+    if (nq == 0) return 0;
+    char *w = static_cast<char *>(ws);
+    uint64_t *seed = reinterpret_cast<uint64_t *>(w);
+    double *nrm = reinterpret_cast<double *>(w + nq * 8);
+    double *dot = reinterpret_cast<double *>(w + nq * 16);
+    hipStream_t st = (hipStream_t)stream;
+    if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center, seed, nrm, dot, st), "row seed"))
+        return -1;
+    PrepParams p;
+    p.raw = queries;
+    p.nq = nq;
+    p.dim = (uint32_t)s->dim;
+    p.cd = (uint32_t)s->cd;
+    p.codec = codec;
+    p.identity = s->seed == 0;
+    p.signs = s->d_signs;
+    p.rot_center = s->d_rot_center;
+    p.center_norm_sqr = s->center_norm_sqr;
+    p.seed = seed;
+    p.raw_norm = nrm;
+    p.raw_dot = dot;
+    p.q_out = out->q;
+    p.x8_out = out->x8;
+    p.x8p_out = out->x8p;
+    p.qs_out = reinterpret_cast<QScalars *>(out->qs);
+    return check_hip(qg_launch_query_prep(&p, st), "query prep");
+}
+
+int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, const qg_query_desc *qv,
+            const uint32_t *ids, const uint32_t *qidx, uint64_t per_query, uint64_t n_pairs, float eps0,
+            const float *short_ips, const float *long_in, float *out_f, float *out_f2, RabitqPaperEstimate *out_est,
+            RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !ix || !ids || (!qv && kernel != QG_SCAN_SYMMETRIC)) return fail("null argument to qg_scan");
+    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_SYMMETRIC) return fail("unknown scan kernel %d", kernel);
+    if (!fixed && !qidx) return fail("pair scans need qidx");
+    if (fixed && kernel != QG_SCAN_INT8_FULL && kernel != QG_SCAN_ASYM && kernel != QG_SCAN_RESIDUAL)
+        return fail("kernel %d has no fixed-list form", kernel);
+    const bool gate = kernel == QG_SCAN_GATE_FP32 || kernel == QG_SCAN_GATE_INT8;
+    if (gate && !out_est) return fail("gate kernels need out_est");
+    if (kernel == QG_SCAN_RESIDUAL && (!out_iv || !long_in)) return fail("residual kernel needs long_in and out_iv");
+    if (!gate && kernel != QG_SCAN_RESIDUAL && !out_f) return fail("kernel needs out_f");
+    if ((kernel == QG_SCAN_REM_FP32 || kernel == QG_SCAN_REM_INT8) && !short_ips) return fail("remaining-bit kernels need short_ips");
+    ScanArgs a;
+    a.ix = view_of(ix);
+    if (qv) a.qv = view_of(qv);
+    else std::memset(&a.qv, 0, sizeof(a.qv));
+    a.cd = (uint32_t)s->cd;
+    a.ids = ids;
+    a.qidx = qidx;
+    a.per_query = per_query;
+    a.n_pairs = n_pairs;
+    a.eps0 = eps0;
+    a.short_ips = short_ips;
+    a.long_in = long_in;
+    a.out_f = out_f;
+    a.out_f2 = out_f2;
+    a.out_est = reinterpret_cast<PaperEstimate *>(out_est);
+    a.out_iv = reinterpret_cast<Interval *>(out_iv);
+    a.out_dot = out_dot;
+    return check_hip(qg_launch_scan(kernel, fixed, &a, (hipStream_t)stream), "scan launch");
+}
+
+int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_query, uint32_t k,
+            uint32_t *out_ids, float *out_values, uint32_t *out_pos, void *stream) {
+    if ((!values || !ids || !out_ids) && nq) return fail("null argument to qg_topk");
+    if (per_query > 4096) return fail("qg_topk supports at most 4096 entries per query");
+    SelectArgs a;
+    a.values = values;
+    a.ids = ids;
+    a.nq = nq;
+    a.per_query = per_query;
+    a.k = k;
+    a.out_ids = out_ids;
+    a.out_values = out_values;
+    a.out_pos = out_pos;
+    return check_hip(qg_launch_topk(&a, (hipStream_t)stream), "topk launch");
+}
+
+}  // extern "C"

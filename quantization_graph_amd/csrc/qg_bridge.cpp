@@ -1,0 +1,443 @@
+// qg_bridge.cpp -- host-memory drop-ins for the reference C-ABI
+// (experiments/02_diskann_fair/native/rabitq_bridge.cpp).
+//
+// Every entry point keeps the reference's signature, argument meaning and
+// failure sentinels (false / nullptr / +inf / zeroed estimates / intervals
+// equal to the long distance). Work is staged through a per-space device
+// arena, computed by the same kernels as the batch API, and copied back
+// synchronously. The prepared-query handle lives in thread-local storage,
+// valid on the calling thread until its next rabitq_prepare_query, as in
+// the reference (space_rabitq.h:3700-3706).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "qg_args.h"
+#include "qg_space.h"
+
+namespace qg {
+namespace {
+
+struct Arena {
+    char *base;
+    size_t used, cap;
+    void *take(size_t bytes) {
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off + bytes > cap) return nullptr;
+        used = off + bytes;
+        return base + off;
+    }
+};
+
+bool arena_for(Space *s, size_t bytes, Arena *a) {
+    bytes += 64 * 256;
+    if (s->bridge_bytes < bytes) {
+        if (s->d_bridge) (void)hipFree(s->d_bridge);
+        s->d_bridge = nullptr;
+        s->bridge_bytes = 0;
+        size_t cap = 1 << 20;
+        while (cap < bytes) cap <<= 1;
+        if (hipMalloc(&s->d_bridge, cap) != hipSuccess) return fail("bridge staging allocation failed") == 0;
+        s->bridge_bytes = cap;
+    }
+    a->base = static_cast<char *>(s->d_bridge);
+    a->used = 0;
+    a->cap = s->bridge_bytes;
+    return true;
+}
+
+struct StagedIndex {
+    qg_index_desc d;
+};
+
+// Device SoA rows for `n` records carved from the arena.
+bool stage_index(const Space *s, Arena *a, uint64_t n, qg_index_desc *d) {
+    const size_t cd = s->cd;
+    std::memset(d, 0, sizeof(*d));
+    d->codes = static_cast<uint8_t *>(a->take(n * cd / 2));
+    d->hdr = static_cast<float *>(a->take(n * 8));
+    d->sfac = static_cast<float *>(a->take(n * 8));
+    d->msb = static_cast<uint8_t *>(a->take(n * cd / 8));
+    d->factors = static_cast<float *>(a->take(n * 16));
+    d->fvalid = static_cast<uint8_t *>(a->take(n));
+    d->rscale = static_cast<uint16_t *>(a->take(n * 2 * s->block_count));
+    d->rcodes = static_cast<uint8_t *>(a->take(n * cd / 2));
+    d->rfac = static_cast<float *>(a->take(n * 8));
+    d->n = n;
+    return d->codes && d->hdr && d->sfac && d->msb && d->factors && d->fvalid && d->rscale && d->rcodes && d->rfac;
+}
+
+// Thread-local prepared query: device buffers for one query.
+struct BridgeQuery {
+    const Space *owner = nullptr;
+    int codec = 0;
+    float *q = nullptr;
+    int8_t *x8 = nullptr;
+    int8_t *x8p = nullptr;
+    void *qs = nullptr;
+    void *ws = nullptr;
+    float *raw = nullptr;
+    size_t cd = 0, dim = 0;
+    ~BridgeQuery() { release(); }
+    void release() {
+        if (q) (void)hipFree(q);
+        if (x8) (void)hipFree(x8);
+        if (x8p) (void)hipFree(x8p);
+        if (qs) (void)hipFree(qs);
+        if (ws) (void)hipFree(ws);
+        if (raw) (void)hipFree(raw);
+        q = nullptr; x8 = nullptr; x8p = nullptr; qs = nullptr; ws = nullptr; raw = nullptr;
+    }
+    bool ensure(const Space *s) {
+        if (owner == s && cd == s->cd && dim == s->dim) return true;
+        release();
+        cd = s->cd;
+        dim = s->dim;
+        owner = s;
+        return hipMalloc(&q, cd * 4) == hipSuccess && hipMalloc(&x8, cd) == hipSuccess &&
+               hipMalloc(&x8p, cd) == hipSuccess && hipMalloc(&qs, 48) == hipSuccess &&
+               hipMalloc(&ws, 256) == hipSuccess && hipMalloc(&raw, dim * 4) == hipSuccess;
+    }
+    qg_query_desc desc() const {
+        qg_query_desc d;
+        d.q = q; d.x8 = x8; d.x8p = x8p; d.qs = qs; d.nq = 1;
+        return d;
+    }
+};
+
+thread_local BridgeQuery t_query;
+
+// Gather rows ids[] of a strided host array into a dense host buffer.
+void gather(const void *base, size_t stride, size_t row_bytes, const uint32_t *ids, size_t count,
+            std::vector<uint8_t> *out) {
+    out->resize(count * row_bytes);
+    const uint8_t *b = static_cast<const uint8_t *>(base);
+    for (size_t i = 0; i < count; ++i)
+        std::memcpy(out->data() + i * row_bytes, b + static_cast<size_t>(ids[i]) * stride, row_bytes);
+}
+
+// Stage `count` compact records (+ optional residual records) and unpack
+// them to SoA rows 0..count-1.
+bool upload_records(Space *s, Arena *a, const std::vector<uint8_t> &comp, const std::vector<uint8_t> *res,
+                    size_t count, qg_index_desc *d) {
+    if (!stage_index(s, a, count, d)) return false;
+    void *dc = a->take(comp.size());
+    void *dr = res ? a->take(res->size()) : nullptr;
+    if (!dc || (res && !dr)) return false;
+    if (hipMemcpy(dc, comp.data(), comp.size(), hipMemcpyHostToDevice) != hipSuccess) return false;
+    if (res && hipMemcpy(dr, res->data(), res->size(), hipMemcpyHostToDevice) != hipSuccess) return false;
+    if (!res) {
+        (void)hipMemset(d->rscale, 0, count * 2 * s->block_count);
+        (void)hipMemset(d->rcodes, 0, count * s->cd / 2);
+    }
+    return qg_unpack_records(s, static_cast<const uint8_t *>(dc), s->compact_bytes, static_cast<const uint8_t *>(dr),
+                             s->residual_bytes, count, d, nullptr) == 0;
+}
+
+uint32_t *upload_iota(Arena *a, size_t count) {
+    std::vector<uint32_t> h(count);
+    for (size_t i = 0; i < count; ++i) h[i] = static_cast<uint32_t>(i);
+    uint32_t *d = static_cast<uint32_t *>(a->take(count * 4 + 4));
+    if (d && count) (void)hipMemcpy(d, h.data(), count * 4, hipMemcpyHostToDevice);
+    return d;
+}
+
+uint32_t *upload_zeros_u32(Arena *a, size_t count) {
+    uint32_t *d = static_cast<uint32_t *>(a->take(count * 4 + 4));
+    if (d && count) (void)hipMemset(d, 0, count * 4);
+    return d;
+}
+
+}  // namespace
+}  // namespace qg
+
+using namespace qg;
+
+extern "C" {
+
+void *rabitq_space_new(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits,
+                       size_t residual_block_size, bool residual_mse, bool residual_fp16) {
+    return space_create(dim, seed, centroid_count, residual_bits, residual_block_size, residual_mse, residual_fp16);
+}
+
+void rabitq_space_delete(void *space) { space_destroy(static_cast<Space *>(space)); }
+
+bool rabitq_space_set_center(void *space, const float *center) {
+    return space_set_center(static_cast<Space *>(space), center) == 0;
+}
+
+size_t rabitq_full_record_bytes(const void *space) { return static_cast<const Space *>(space)->full_bytes; }
+size_t rabitq_compact_record_bytes(const void *space) { return static_cast<const Space *>(space)->compact_bytes; }
+size_t rabitq_residual_record_bytes(const void *space) { return static_cast<const Space *>(space)->residual_bytes; }
+size_t rabitq_residual_block_size(const void *space) { return static_cast<const Space *>(space)->block; }
+size_t rabitq_residual_bits(const void *space) { (void)space; return 4; }
+size_t rabitq_paper_msb_code_bytes(const void *space) { return (static_cast<const Space *>(space)->cd + 7) / 8; }
+size_t rabitq_paper_factor_bytes(void) { return 20; }
+
+bool rabitq_encode_full(const void *space, const float *raw, void *encoded_out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    if (!s || !raw || !encoded_out) return false;
+    Arena a;
+    const size_t ws_bytes = qg_encode_workspace_bytes(s, 1);
+    if (!arena_for(s, s->dim * 4 + ws_bytes + s->compact_bytes + s->residual_bytes + 4096 + s->cd * 8, &a)) return false;
+    float *dx = static_cast<float *>(a.take(s->dim * 4));
+    qg_index_desc d;
+    if (!dx || !stage_index(s, &a, 1, &d)) return false;
+    void *ws = a.take(ws_bytes);
+    uint8_t *dc = static_cast<uint8_t *>(a.take(s->compact_bytes));
+    uint8_t *dr = static_cast<uint8_t *>(a.take(s->residual_bytes));
+    if (!ws || !dc || !dr) return false;
+    if (hipMemcpy(dx, raw, s->dim * 4, hipMemcpyHostToDevice) != hipSuccess) return false;
+    if (qg_encode(s, dx, 1, &d, ws, ws_bytes, nullptr) != 0) return false;
+    if (qg_pack_records(s, &d, dc, s->compact_bytes, dr, s->residual_bytes, nullptr) != 0) return false;
+    std::vector<uint8_t> c(s->compact_bytes), r(s->residual_bytes);
+    if (hipMemcpy(c.data(), dc, c.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpy(r.data(), dr, r.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    // full record (space_rabitq.h:261-266): header | short factors |
+    // residual factors | scales | codes | centroid id | residual codes
+    uint8_t *o = static_cast<uint8_t *>(encoded_out);
+    const size_t sb = 2 * s->block_count, cb = s->cd / 2;
+    std::memcpy(o, c.data(), 16);
+    std::memcpy(o + 16, r.data(), 8);
+    std::memcpy(o + 24, r.data() + 8, sb);
+    std::memcpy(o + 24 + sb, c.data() + 16, cb);
+    o[24 + sb + cb] = 0;
+    std::memcpy(o + 25 + sb + cb, r.data() + 8 + sb, cb);
+    return true;
+}
+
+bool rabitq_copy_compact(const void *space, const void *full_encoded, void *compact_out) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !full_encoded || !compact_out) return false;
+    const uint8_t *f = static_cast<const uint8_t *>(full_encoded);
+    uint8_t *o = static_cast<uint8_t *>(compact_out);
+    const size_t sb = 2 * s->block_count, cb = s->cd / 2;
+    std::memcpy(o, f, 16);
+    std::memcpy(o + 16, f + 24 + sb, cb);
+    o[16 + cb] = f[24 + sb + cb];
+    return true;
+}
+
+bool rabitq_copy_residual_record(const void *space, const void *full_encoded, void *residual_out) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !full_encoded || !residual_out) return false;
+    const uint8_t *f = static_cast<const uint8_t *>(full_encoded);
+    uint8_t *o = static_cast<uint8_t *>(residual_out);
+    const size_t sb = 2 * s->block_count, cb = s->cd / 2;
+    std::memcpy(o, f + 16, 8);
+    std::memcpy(o + 8, f + 24, sb);
+    std::memcpy(o + 8 + sb, f + 25 + sb + cb, cb);
+    return true;
+}
+
+bool rabitq_export_paper_sidecar(const void *space, const void *compact_records_in, size_t record_count,
+                                 size_t compact_stride, uint8_t *msb_out, void *factors_out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    if (!s || (!compact_records_in && record_count) || !msb_out || !factors_out) return false;
+    if (record_count == 0) return true;
+    if (compact_stride < s->compact_bytes) return false;
+    std::vector<uint32_t> ids(record_count);
+    for (size_t i = 0; i < record_count; ++i) ids[i] = static_cast<uint32_t>(i);
+    std::vector<uint8_t> comp;
+    gather(compact_records_in, compact_stride, s->compact_bytes, ids.data(), record_count, &comp);
+    Arena a;
+    if (!arena_for(s, record_count * (s->compact_bytes + 2 * s->cd + 64), &a)) return false;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, nullptr, record_count, &d)) return false;
+    const size_t mb = s->cd / 8;
+    std::vector<float> f(record_count * 4);
+    std::vector<uint8_t> v(record_count);
+    if (hipMemcpy(msb_out, d.msb, record_count * mb, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpy(f.data(), d.factors, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpy(v.data(), d.fvalid, v.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    RabitqPaperFactors *o = static_cast<RabitqPaperFactors *>(factors_out);
+    for (size_t i = 0; i < record_count; ++i) {
+        o[i].norm_sqr = f[4 * i];
+        o[i].data_norm = f[4 * i + 1];
+        o[i].cross_scale = f[4 * i + 2];
+        o[i].error_cross_scale = f[4 * i + 3];
+        o[i].valid = v[i];
+    }
+    return true;
+}
+
+float rabitq_symmetric_distance(const void *space, const void *lhs, const void *rhs) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const float inf = std::numeric_limits<float>::infinity();
+    if (!s || !lhs || !rhs) return inf;
+    std::vector<uint8_t> comp(2 * s->compact_bytes);
+    std::memcpy(comp.data(), lhs, s->compact_bytes);
+    std::memcpy(comp.data() + s->compact_bytes, rhs, s->compact_bytes);
+    Arena a;
+    if (!arena_for(s, 2 * (s->compact_bytes + 2 * s->cd + 64) + 64, &a)) return inf;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, nullptr, 2, &d)) return inf;
+    uint32_t *l = upload_zeros_u32(&a, 1);
+    uint32_t *r = upload_iota(&a, 2) + 1;
+    float *out = static_cast<float *>(a.take(4));
+    if (!l || !out) return inf;
+    if (qg_scan(s, QG_SCAN_SYMMETRIC, 0, &d, nullptr, r, l, 0, 1, 0.0f, nullptr, nullptr, out, nullptr, nullptr,
+                nullptr, nullptr, nullptr) != 0)
+        return inf;
+    float v = inf;
+    if (hipMemcpy(&v, out, 4, hipMemcpyDeviceToHost) != hipSuccess) return inf;
+    return v;
+}
+
+const void *rabitq_prepare_query(const void *space, const float *query) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !query) return nullptr;
+    BridgeQuery &bq = t_query;
+    if (!bq.ensure(s)) return nullptr;
+    bq.codec = s->codec;
+    if (hipMemcpy(bq.raw, query, s->dim * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    qg_query_desc d = bq.desc();
+    const int codec = (s->codec == QG_CODEC_INT8) ? QG_CODEC_INT8 : QG_CODEC_FULL;
+    if (qg_prepare_queries(s, bq.raw, 1, codec, &d, bq.ws, nullptr) != 0) return nullptr;
+    if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+    return &bq;
+}
+
+void rabitq_set_query_coarse_codec(void *space, int codec) {
+    Space *s = static_cast<Space *>(space);
+    if (s) s->codec = codec;
+}
+
+void rabitq_release_query(const void *space, const void *prepared) {
+    (void)space;
+    (void)prepared;
+}
+
+bool rabitq_query_distance_batch(const void *space, const void *prepared, const uint32_t *ids, const uint8_t *modes,
+                                 size_t count, const void *encoded_base, size_t encoded_stride,
+                                 const float *short_ips, float *out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
+    if (!s || !bq || bq->owner != s || (count && (!ids || !modes || !encoded_base || !out))) return false;
+    if (count == 0) return true;
+    std::vector<uint8_t> comp;
+    gather(encoded_base, encoded_stride, s->compact_bytes, ids, count, &comp);
+    Arena a;
+    if (!arena_for(s, count * (s->compact_bytes + 2 * s->cd + 96) + 4096, &a)) return false;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, nullptr, count, &d)) return false;
+    uint32_t *rows = upload_iota(&a, count);
+    uint32_t *qidx = upload_zeros_u32(&a, count);
+    float *dsi = static_cast<float *>(a.take(count * 4 + 4));
+    float *dout = static_cast<float *>(a.take(count * 4 + 4));
+    if (!rows || !qidx || !dsi || !dout) return false;
+    std::vector<float> si(count, 0.0f);
+    if (short_ips) std::memcpy(si.data(), short_ips, count * 4);
+    (void)hipMemcpy(dsi, si.data(), count * 4, hipMemcpyHostToDevice);
+    const qg_query_desc qd = bq->desc();
+    // mode 0: full recompute; 1 / 3: FP32 reuse of the gate's short_ip;
+    // 2: INT8 reuse (rabitq_bridge.cpp:885-970)
+    std::vector<float> res(count), tmp(count);
+    const int kernels[4] = {QG_SCAN_ASYM, QG_SCAN_REM_FP32, QG_SCAN_REM_INT8, QG_SCAN_REM_FP32};
+    bool used[4] = {false, false, false, false};
+    for (size_t i = 0; i < count; ++i) used[modes[i] < 4 ? modes[i] : 0] = true;
+    for (int m = 0; m < 4; ++m) {
+        if (!used[m]) continue;
+        if (qg_scan(s, kernels[m], 0, &d, &qd, rows, qidx, 0, count, 0.0f, dsi, nullptr, dout, nullptr, nullptr,
+                    nullptr, nullptr, nullptr) != 0)
+            return false;
+        if (hipMemcpy(tmp.data(), dout, count * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        for (size_t i = 0; i < count; ++i)
+            if ((modes[i] < 4 ? modes[i] : 0) == m) res[i] = tmp[i];
+    }
+    std::memcpy(out, res.data(), count * 4);
+    return true;
+}
+
+float rabitq_query_distance(const void *space, const void *prepared, const void *encoded) {
+    const uint32_t id = 0;
+    const uint8_t mode = 0;
+    float out = std::numeric_limits<float>::infinity();
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !encoded) return out;
+    if (!rabitq_query_distance_batch(space, prepared, &id, &mode, 1, encoded, s->compact_bytes, nullptr, &out))
+        return std::numeric_limits<float>::infinity();
+    return out;
+}
+
+bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                         const uint8_t *msb_base, size_t msb_stride, const void *factors_base,
+                                         size_t factors_stride, float epsilon0, RabitqPaperEstimate *out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
+    if (!s || !bq || bq->owner != s || (count && (!ids || !msb_base || !factors_base || !out))) return false;
+    if (count == 0) return true;
+    if (bq->codec != QG_CODEC_INT8 && bq->codec != QG_CODEC_FULL) return fail("query codec %d has no gate", bq->codec) == 0;
+    const size_t mb = s->cd / 8;
+    std::vector<uint8_t> msb;
+    gather(msb_base, msb_stride, mb, ids, count, &msb);
+    std::vector<float> fac(count * 4);
+    std::vector<uint8_t> valid(count);
+    const uint8_t *fb = static_cast<const uint8_t *>(factors_base);
+    for (size_t i = 0; i < count; ++i) {
+        RabitqPaperFactors f;
+        std::memcpy(&f, fb + static_cast<size_t>(ids[i]) * factors_stride, 20);
+        fac[4 * i] = f.norm_sqr;
+        fac[4 * i + 1] = f.data_norm;
+        fac[4 * i + 2] = f.cross_scale;
+        fac[4 * i + 3] = f.error_cross_scale;
+        valid[i] = f.valid;
+    }
+    Arena a;
+    if (!arena_for(s, count * (mb + 16 + 1 + 64) + 4096, &a)) return false;
+    qg_index_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.msb = static_cast<uint8_t *>(a.take(count * mb));
+    d.factors = static_cast<float *>(a.take(count * 16));
+    d.fvalid = static_cast<uint8_t *>(a.take(count));
+    d.n = count;
+    uint32_t *rows = upload_iota(&a, count);
+    uint32_t *qidx = upload_zeros_u32(&a, count);
+    RabitqPaperEstimate *dout = static_cast<RabitqPaperEstimate *>(a.take(count * 24));
+    if (!d.msb || !d.factors || !d.fvalid || !rows || !qidx || !dout) return false;
+    (void)hipMemcpy(d.msb, msb.data(), msb.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(d.factors, fac.data(), fac.size() * 4, hipMemcpyHostToDevice);
+    (void)hipMemcpy(d.fvalid, valid.data(), count, hipMemcpyHostToDevice);
+    const qg_query_desc qd = bq->desc();
+    const int k = bq->codec == QG_CODEC_INT8 ? QG_SCAN_GATE_INT8 : QG_SCAN_GATE_FP32;
+    if (qg_scan(s, k, 0, &d, &qd, rows, qidx, 0, count, epsilon0, nullptr, nullptr, nullptr, nullptr, dout, nullptr,
+                nullptr, nullptr) != 0)
+        return false;
+    return hipMemcpy(out, dout, count * 24, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+bool rabitq_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                    const void *encoded_base, size_t encoded_stride, const void *residual_base,
+                                    size_t residual_stride, const float *long_distances, RabitqDistanceInterval *out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
+    if (!s || !bq || bq->owner != s ||
+        (count && (!ids || !encoded_base || !residual_base || !long_distances || !out)))
+        return false;
+    if (count == 0) return true;
+    std::vector<uint8_t> comp, res;
+    gather(encoded_base, encoded_stride, s->compact_bytes, ids, count, &comp);
+    gather(residual_base, residual_stride, s->residual_bytes, ids, count, &res);
+    Arena a;
+    if (!arena_for(s, count * (s->compact_bytes + s->residual_bytes + 2 * s->cd + 96) + 4096, &a)) return false;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, &res, count, &d)) return false;
+    uint32_t *rows = upload_iota(&a, count);
+    uint32_t *qidx = upload_zeros_u32(&a, count);
+    float *dl = static_cast<float *>(a.take(count * 4));
+    RabitqDistanceInterval *dout = static_cast<RabitqDistanceInterval *>(a.take(count * 12));
+    if (!rows || !qidx || !dl || !dout) return false;
+    (void)hipMemcpy(dl, long_distances, count * 4, hipMemcpyHostToDevice);
+    const qg_query_desc qd = bq->desc();
+    if (qg_scan(s, QG_SCAN_RESIDUAL, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, dl, nullptr, nullptr, nullptr,
+                dout, nullptr, nullptr) != 0) {
+        for (size_t i = 0; i < count; ++i) out[i] = RabitqDistanceInterval{long_distances[i], long_distances[i], long_distances[i]};
+        return true;
+    }
+    return hipMemcpy(out, dout, count * 12, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+// qg_query.hip -- batched query preparation (FP32 rotated residual + INT8
+// dithered code) for N queries in one launch.
+//
+// Restates initializePreparedQuery + buildCentroidQuery
+// (Ours/core/hnswlib/space_rabitq.h:3429-3698) for K = 1:
+//   rotated   = FHT(diag(signs) * pad(query))            (:990-1000, 944-988)
+//   q         = rotated - rotated_center
+//   sum       = sequential float sum of (even + odd) per pair; half = 0.5*sum
+//   pos       = sequential float sum of max(0,even) + max(0,odd)
+//   ||q-c||^2 = float(||raw||^2 + ||c||^2 - 2<raw,c>)   (doubles, :3447-3454)
+//   INT8: scale = max|q|/128, x = clamp(floor(q/scale + u), -128, 127) with u
+//   from mt19937_64 seeded by FNV-1a over the raw bytes (:3540-3659), two
+//   draws per dimension, the first used for INT8.
+// Serial recurrences (the float sums, FNV-1a, the mt19937_64 seeding) run on
+// single lanes in the reference order; everything else is data-parallel.
+#include "qg_args.h"
+
+namespace qg {
+
+constexpr int kPrepThreads = 256;
+
+// One thread per row: FNV-1a over the row's bytes, ||row||^2 and <row, c> in
+// double, accumulated in index order exactly as the reference loops do.
+__global__ void k_row_seed(const float *__restrict__ rows, uint64_t n, uint32_t dim,
+                           const float *__restrict__ center, uint64_t *__restrict__ seed_out,
+                           double *__restrict__ norm_out, double *__restrict__ dot_out) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float *x = rows + r * dim;
+    uint64_t h = 1469598103934665603ull;
+    double nrm = 0.0, dot = 0.0;
+    for (uint32_t i = 0; i < dim; ++i) {
+        const float v = x[i];
+        const uint32_t b = __float_as_uint(v);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            h ^= (uint64_t)((b >> (8 * k)) & 0xFFu);
+            h *= 1099511628211ull;
+        }
+        const double dv = (double)v;
+        nrm += dv * dv;  // exact product: ordering of the add is what matters
+        if (center) dot += dv * (double)center[i];
+    }
+    if (seed_out) seed_out[r] = h;
+    if (norm_out) norm_out[r] = nrm;
+    if (dot_out) dot_out[r] = dot;
+}
+
+// In-LDS Walsh-Hadamard butterflies, stage step 1, 2, 4, ... (hadamard,
+// space_rabitq.h:944-988). Only add/sub: bit-exact in any schedule.
+__device__ void fht_lds(float *v, uint32_t cd) {
+    for (uint32_t step = 1; step < cd; step <<= 1) {
+        for (uint32_t b = threadIdx.x; b < cd / 2; b += blockDim.x) {
+            const uint32_t blk = (b / step) * (step << 1);
+            const uint32_t i = blk + (b % step);
+            const float a = v[i], c = v[i + step];
+            v[i] = a + c;
+            v[i + step] = a - c;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint64_t mt64_temper(uint64_t x) {
+    x ^= (x >> 29) & 0x5555555555555555ull;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ull;
+    x ^= (x << 37) & 0xFFF7EEE000000000ull;
+    x ^= x >> 43;
+    return x;
+}
+
+// One std::mt19937_64 state refill (n = 312, m = 156), parallel form: the
+// first 156 words depend only on old words, the rest on old words and the
+// first half's new words.
+__device__ void mt64_twist(uint64_t *mt) {
+    const uint64_t UM = 0xFFFFFFFF80000000ull, LM = 0x7FFFFFFFull, A = 0xB5026F5AA96619E9ull;
+    uint64_t v[2] = {0, 0};
+    int c = 0;
+    for (uint32_t i = threadIdx.x; i < 156; i += blockDim.x, ++c) {
+        const uint64_t y = (mt[i] & UM) | (mt[i + 1] & LM);
+        v[c] = mt[i + 156] ^ (y >> 1) ^ ((y & 1ull) ? A : 0ull);
+    }
+    __syncthreads();
+    c = 0;
+    for (uint32_t i = threadIdx.x; i < 156; i += blockDim.x, ++c) mt[i] = v[c];
+    __syncthreads();
+    c = 0;
+    for (uint32_t i = 156 + threadIdx.x; i < 312; i += blockDim.x, ++c) {
+        const uint64_t y = (mt[i] & UM) | (mt[(i + 1) % 312] & LM);
+        v[c] = mt[i - 156] ^ (y >> 1) ^ ((y & 1ull) ? A : 0ull);
+    }
+    __syncthreads();
+    c = 0;
+    for (uint32_t i = 156 + threadIdx.x; i < 312; i += blockDim.x, ++c) mt[i] = v[c];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
+    extern __shared__ unsigned char smem[];
+    const uint32_t cd = p.cd;
+    float *v = (float *)smem;                          // cd floats
+    double *err = (double *)(smem + 4 * cd);           // cd doubles
+    uint64_t *mt = (uint64_t *)(smem + 12 * cd);       // 312 words
+    __shared__ float s_sum, s_pos, s_max;
+    __shared__ long long s_csum;
+    __shared__ double s_err;
+    const uint64_t qi = blockIdx.x;
+    const float *raw = p.raw + qi * p.dim;
+    const uint32_t t = threadIdx.x;
+
+    for (uint32_t i = t; i < cd; i += blockDim.x)
+        v[i] = (i < p.dim) ? (p.identity ? raw[i] : raw[i] * p.signs[i]) : 0.0f;
+    if (t == 0) { s_max = 0.0f; s_csum = 0; }
+    __syncthreads();
+    if (!p.identity) fht_lds(v, cd);
+    for (uint32_t i = t; i < cd; i += blockDim.x) {
+        const float d = v[i] - p.rot_center[i];
+        v[i] = d;
+        p.q_out[qi * cd + i] = d;
+    }
+    __syncthreads();
+    // Serial float recurrences of buildCentroidQuery (:3471-3486) on two
+    // lanes of different waves; max|q| by reduction (order-free).
+    if (t == 0) {
+        float s = 0.0f;
+        for (uint32_t pr = 0; pr < cd / 2; ++pr) { const float e = v[2 * pr], o = v[2 * pr + 1]; s += e + o; }
+        s_sum = s;
+    } else if (t == 64) {
+        float s = 0.0f;
+        for (uint32_t pr = 0; pr < cd / 2; ++pr) {
+            const float e = v[2 * pr], o = v[2 * pr + 1];
+            const float pe = (0.0f < e) ? e : 0.0f, po = (0.0f < o) ? o : 0.0f;  // std::max(0.0f, x)
+            s += pe + po;
+        }
+        s_pos = s;
+    }
+    float m = 0.0f;
+    for (uint32_t i = t; i < cd; i += blockDim.x) { const float a = fabsf(v[i]); m = (m < a) ? a : m; }
+    for (int off = 32; off > 0; off >>= 1) { const float o = __shfl_xor(m, off); m = (m < o) ? o : m; }
+    if ((t & 63) == 0) atomicMax((int *)&s_max, __float_as_int(m));  // non-negative floats order as ints
+    __syncthreads();
+
+    double qn2d = p.raw_norm[qi] + (double)p.center_norm_sqr - 2.0 * p.raw_dot[qi];
+    float qn2 = (float)qn2d;
+    qn2 = (0.0f < qn2) ? qn2 : 0.0f;  // std::max(0.0f, x)
+    const float qn = sqrtf(qn2);
+    const float max_abs = s_max;
+    float scale = 1.0f;
+    const bool int8 = (p.codec == 3) && (max_abs > 0.0f);
+    if (int8) {
+        scale = max_abs / 128.0f;
+        if (t == 0) {
+            uint64_t sd = p.seed[qi];
+            sd ^= 0ull + 0x9E3779B97F4A7C15ull + (sd << 6) + (sd >> 2);  // centroid id 0
+            mt[0] = sd;
+            for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ull * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+        }
+        __syncthreads();
+        const uint32_t draws = 2 * cd;
+        long long csum = 0;
+        for (uint32_t base = 0; base < draws; base += 312) {
+            mt64_twist(mt);
+            for (uint32_t k = t; k < 312; k += blockDim.x) {
+                const uint32_t idx = base + k;
+                if (idx < draws && (idx & 1u) == 0) {
+                    const uint32_t i = idx >> 1;
+                    double u = (double)mt64_temper(mt[k]) * 0x1p-64;
+                    if (u >= 1.0) u = 0x1.fffffffffffffp-1;  // nextafter(1.0, 0.0)
+                    int x = (int)floor((double)v[i] / (double)scale + u);
+                    x = x > 127 ? 127 : (x < -128 ? -128 : x);
+                    const double e = (double)v[i] - (double)scale * (double)x;
+                    err[i] = e;
+                    p.x8_out[qi * cd + i] = (int8_t)x;
+                    // regroup: per 8 dims, 4 even dims then 4 odd dims
+                    const uint32_t g = i >> 3, w = i & 7u;
+                    p.x8p_out[qi * cd + 8 * g + ((w & 1u) ? 4 : 0) + (w >> 1)] = (int8_t)x;
+                    csum += x;
+                }
+            }
+            __syncthreads();
+        }
+        for (int off = 32; off > 0; off >>= 1) csum += __shfl_xor(csum, off);
+        if ((t & 63) == 0) atomicAdd((unsigned long long *)&s_csum, (unsigned long long)csum);
+        __syncthreads();
+        if (t == 0) {
+            double a = 0.0;
+            for (uint32_t i = 0; i < cd; ++i) a = fma(err[i], err[i], a);
+            s_err = a;
+        }
+    } else {
+        for (uint32_t i = t; i < cd; i += blockDim.x) { p.x8_out[qi * cd + i] = 0; p.x8p_out[qi * cd + i] = 0; }
+    }
+    __syncthreads();
+    if (t == 0) {
+        QScalars s;
+        s.query_norm = qn;
+        s.query_norm_sqr = qn2;
+        s.half_sum = s_sum * 0.5f;
+        s.sum = s_sum;
+        s.positive_sum = s_pos;
+        s.int8_scale = scale;
+        s.int8_error_norm = int8 ? (float)sqrt(s_err) : 0.0f;
+        s.max_abs_bits = __float_as_int(max_abs);
+        s.int8_code_sum = int8 ? (int64_t)s_csum : 0;
+        s.pad = 0;
+        p.qs_out[qi] = s;
+    }
+}
+
+// Rotation only (setCentroids' rotate, :2918-2936): rows -> FHT(signs * x).
+__global__ void __launch_bounds__(kPrepThreads) k_rotate_rows(const float *rows, uint64_t n, uint32_t dim,
+                                                              uint32_t cd, int identity,
+                                                              const float *signs, float *out) {
+    extern __shared__ unsigned char smem[];
+    float *v = (float *)smem;
+    const float *x = rows + (uint64_t)blockIdx.x * dim;
+    for (uint32_t i = threadIdx.x; i < cd; i += blockDim.x)
+        v[i] = (i < dim) ? (identity ? x[i] : x[i] * signs[i]) : 0.0f;
+    __syncthreads();
+    if (!identity) fht_lds(v, cd);
+    for (uint32_t i = threadIdx.x; i < cd; i += blockDim.x) out[(uint64_t)blockIdx.x * cd + i] = v[i];
+}
+
+}  // namespace qg
+
+extern "C" {
+
+int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float *center,
+                       uint64_t *seed, double *norm, double *dot, hipStream_t stream) {
+    if (n == 0) return 0;
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    hipLaunchKernelGGL(qg::k_row_seed, dim3(blocks), dim3(128), 0, stream, rows, n, dim, center, seed,
+                       norm, dot);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_query_prep(const qg::PrepParams *p, hipStream_t stream) {
+    if (p->nq == 0) return 0;
+    const size_t smem = (size_t)p->cd * 12 + 312 * 8;
+    hipLaunchKernelGGL(qg::k_query_prep, dim3((unsigned)p->nq), dim3(qg::kPrepThreads), smem, stream, *p);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_rotate_rows(const float *rows, uint64_t n, uint32_t dim, uint32_t cd, int identity,
+                          const float *signs, float *out, hipStream_t stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(qg::k_rotate_rows, dim3((unsigned)n), dim3(qg::kPrepThreads), (size_t)cd * 4, stream,
+                       rows, n, dim, cd, identity, signs, out);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

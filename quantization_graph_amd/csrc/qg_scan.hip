@@ -1,0 +1,427 @@
+// qg_scan.hip -- the estimator kernels over candidate lists.
+//
+// Two list shapes:
+//  * fixed:  every query q owns `per_query` candidate ids at ids[q*per_query..];
+//            grid = (ceil(per_query / 256), nq), so a workgroup serves one
+//            query and stages that query's vectors in LDS once.
+//  * pairs:  a flat list of (query index, id) pairs; query vectors are read
+//            through the cache (used for ragged batches).
+// One lane evaluates one (query, candidate) pair and reads the candidate's
+// code row with 16-byte loads. Integer dots are exact (v_dot4 on unpacked
+// nibbles); float estimators keep the reference's 16-lane accumulation order.
+#include "qg_args.h"
+
+namespace qg {
+
+constexpr int kScanThreads = 256;
+
+
+
+__device__ __forceinline__ bool fixed_item(const ScanArgs &a, uint64_t &qi, uint64_t &pos) {
+    qi = blockIdx.y;
+    pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    return pos < a.per_query;
+}
+
+// ---------------------------------------------------------------- C7 -----
+// INT8 query x full 4-bit code: dot = sum c_i x_i (exact int32), then
+// finish_int_paper_prune_estimate (space_rabitq.h:5230-5260).
+template <bool kFixed>
+__global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
+    __shared__ uint32_t xs[2048 / 4];
+    uint64_t qi, pos, item;
+    bool live;
+    if (kFixed) {
+        live = fixed_item(a, qi, pos);
+        item = qi * a.per_query + pos;
+        const uint32_t *src = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) xs[i] = src[i];
+        __syncthreads();
+    } else {
+        item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        live = item < a.n_pairs;
+        qi = live ? a.qidx[item] : 0;
+    }
+    if (!live) return;
+    const uint32_t id = a.ids[item];
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    const uint32_t *xg = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+    int acc = 0;
+    const uint32_t chunks = a.cd / 32;  // 16 code bytes = 32 dims per chunk
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t d = 4 * c + k;  // code dword: dims 8d..8d+7
+            const uint32_t xl = kFixed ? xs[2 * d] : xg[2 * d];
+            const uint32_t xh = kFixed ? xs[2 * d + 1] : xg[2 * d + 1];
+            acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x0F0F0F0Fu), (int)xl, acc, false);
+            acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x0F0F0F0Fu), (int)xh, acc, false);
+        }
+    }
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    const float scale = s.int8_scale;
+    float est, lower;
+    if (!long_scale_ok(h.y) || !(scale > 0.0f) || !__builtin_isfinite(scale) || s.query_norm <= 0.0f) {
+        est = h.x + s.query_norm_sqr;
+        lower = est;
+    } else {
+        const double centered = (double)acc - 7.5 * (double)s.int8_code_sum;
+        const double sip = (double)scale * centered;
+        const double est_ip = (double)h.y * sip;
+        est = h.x + s.query_norm_sqr - (float)est_ip;
+        const double db_err = (double)a.ix.sfac[id].y * (double)s.query_norm;
+        lower = est - (float)((double)a.eps0 * db_err);
+    }
+    a.out_f[item] = est;
+    if (a.out_f2) a.out_f2[item] = lower;
+    if (a.out_dot) a.out_dot[item] = acc;
+}
+
+// ---------------------------------------------------------------- C1 -----
+// FP32 query x 4-bit code (queryDistanceLong + longCodeIpsAvx512,
+// space_rabitq.h:1551-1607): 16 lane accumulators per sum, lane j takes
+// pairs j, j+16, ...; remaining bits fused (FMA), MSB half by masked add.
+template <bool kFixed>
+__global__ void __launch_bounds__(kScanThreads) k_asym(ScanArgs a) {
+    __shared__ float qsh[2048];
+    uint64_t qi, pos, item;
+    bool live;
+    if (kFixed) {
+        live = fixed_item(a, qi, pos);
+        item = qi * a.per_query + pos;
+        const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
+        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
+        __syncthreads();
+    } else {
+        item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        live = item < a.n_pairs;
+        qi = live ? a.qidx[item] : 0;
+    }
+    if (!live) return;
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    if (!long_scale_ok(h.y)) { a.out_f[item] = h.x + s.query_norm_sqr; return; }
+    const float *qg = kFixed ? qsh : a.qv.q + qi * a.cd;
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    float sh[16], rm[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { sh[j] = 0.0f; rm[j] = 0.0f; }
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const float4 *qp = (const float4 *)(qg + 32 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q0 = qp[2 * k], q1 = qp[2 * k + 1];  // dims of code bytes 4k..4k+3
+            const float qe[4] = {q0.x, q0.z, q1.x, q1.z};
+            const float qo[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t byte = (ws[k] >> (8 * b)) & 0xFFu;
+                const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
+                rm[j] = fmaf((float)(lo & 7u), qe[b], rm[j]);
+                rm[j] = fmaf((float)(hi & 7u), qo[b], rm[j]);
+                if (lo > 7u) sh[j] = sh[j] + qe[b];
+                if (hi > 7u) sh[j] = sh[j] + qo[b];
+            }
+        }
+    }
+    const float short_ip = reduce16(sh) - s.half_sum;
+    const float rem_ip = reduce16(rm);
+    a.out_f[item] = finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, rem_ip);
+}
+
+// ---------------------------------------------------------------- C3 -----
+// FP32 remaining three bits after the gate (query_distance_with_paper_msb_
+// batch_by_id, space_rabitq.h:3972-4073).
+__global__ void __launch_bounds__(kScanThreads) k_rem_fp32(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    if (!long_scale_ok(h.y)) { a.out_f[item] = h.x + s.query_norm_sqr; return; }
+    const float *qg = a.qv.q + qi * a.cd;
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    float rm[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) rm[j] = 0.0f;
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const float4 *qp = (const float4 *)(qg + 32 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q0 = qp[2 * k], q1 = qp[2 * k + 1];
+            const float qe[4] = {q0.x, q0.z, q1.x, q1.z};
+            const float qo[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t byte = (ws[k] >> (8 * b)) & 0xFFu;
+                rm[j] = fmaf((float)(byte & 7u), qe[b], rm[j]);
+                rm[j] = fmaf((float)((byte >> 4) & 7u), qo[b], rm[j]);
+            }
+        }
+    }
+    a.out_f[item] = finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, a.short_ips[item], reduce16(rm));
+}
+
+// ---------------------------------------------------------------- C6 -----
+// INT8 remaining three bits (query_distance_with_quantized_paper_msb_batch_
+// by_id + finishQuantizedTraversalDistance, space_rabitq.h:4234-4358).
+__global__ void __launch_bounds__(kScanThreads) k_rem_int8(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    if (!long_scale_ok(h.y)) { a.out_f[item] = h.x + s.query_norm_sqr; return; }
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    const uint32_t *xg = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+    int acc = 0;
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t d = 4 * c + k;
+            acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x07070707u), (int)xg[2 * d], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x07070707u), (int)xg[2 * d + 1], acc, false);
+        }
+    }
+    const double centered = (double)s.int8_scale * ((double)acc - 3.5 * (double)s.int8_code_sum);
+    const double signed_ip = 8.0 * (double)a.short_ips[item] + centered;
+    a.out_f[item] = h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
+    if (a.out_dot) a.out_dot[item] = acc;
+}
+
+// ---------------------------------------------------------------- C4 -----
+// FP32 1-bit gate (compute_full_paper_prune_estimate_batch_by_id,
+// space_rabitq.h:4427-4508): 16 lane accumulators over the MSB plane.
+__global__ void __launch_bounds__(kScanThreads) k_gate_fp32(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float *qg = a.qv.q + qi * a.cd;
+    const uint4 *m = (const uint4 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    float acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+    for (uint32_t c = 0; c < a.cd / 128; ++c) {  // 16 bytes of MSB = 128 dims
+        const uint4 w = m[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {  // one 16-dim group per 16 bits
+                const uint32_t bits = (ws[k] >> (16 * half)) & 0xFFFFu;
+                const float4 *qp = (const float4 *)(qg + 128 * c + 32 * k + 16 * half);
+                const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+                const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                                      q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + (((bits >> j) & 1u) ? qv[j] : 0.0f);
+            }
+        }
+    }
+    const float short_ip = reduce16(acc) - s.half_sum;
+    a.out_est[item] = finish_gate(short_ip, a.ix.factors[id], a.ix.fvalid[id], s.query_norm, s.query_norm_sqr, a.eps0);
+}
+
+// ---------------------------------------------------------------- C5 -----
+// INT8 1-bit gate (compute_int_paper_prune_estimate_batch_by_id,
+// space_rabitq.h:4991-5093): selected = sum of x_i over set MSB bits (exact).
+__global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float4 f = a.ix.factors[id];
+    const uint8_t fv = a.ix.fvalid[id];
+    PaperEstimate r;
+    r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
+    r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) { a.out_est[item] = r; return; }
+    const uint32_t *xg = (const uint32_t *)(a.qv.x8 + qi * a.cd);
+    const uint4 *m = (const uint4 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    int sel = 0;
+    for (uint32_t c = 0; c < a.cd / 128; ++c) {
+        const uint4 w = m[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int nib = 0; nib < 8; ++nib) {  // 4 dims per MSB nibble -> one dot4
+                const uint32_t bits = (ws[k] >> (4 * nib)) & 0xFu;
+                // expand 4 bits to a byte mask 0x01 per selected dim
+                const uint32_t mask = (bits & 1u) | ((bits & 2u) << 7) | ((bits & 4u) << 14) | ((bits & 8u) << 21);
+                const uint32_t d4 = 128 * c + 32 * k + 4 * nib;  // first dim of this group
+                sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[d4 / 4], sel, false);
+            }
+        }
+    }
+    const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);
+    a.out_est[item] = finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, a.eps0);
+}
+
+// ---------------------------------------------------------------- E1 -----
+// Residual-4 rerank (compute_residual_distance_intervals_batch_by_id,
+// space_rabitq.h:5432-5548): per 16-dim block, 8 lane products
+// fma(hi, odd, lo*even), lane sum in order, ips = fma(scale, block, ips);
+// blocks whose fp16 scale decodes to 0 or non-finite are skipped.
+template <bool kFixed>
+__global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
+    __shared__ float qsh[2048];
+    uint64_t qi, pos, item;
+    bool live;
+    if (kFixed) {
+        live = fixed_item(a, qi, pos);
+        item = qi * a.per_query + pos;
+        const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
+        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
+        __syncthreads();
+    } else {
+        item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        live = item < a.n_pairs;
+        qi = live ? a.qidx[item] : 0;
+    }
+    if (!live) return;
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float *qg = kFixed ? qsh : a.qv.q + qi * a.cd;
+    const uint32_t blocks = a.cd / 16;
+    const uint16_t *sc = a.ix.rscale + (uint64_t)id * blocks;
+    const uint2 *rc = (const uint2 *)(a.ix.rcodes + (uint64_t)id * (a.cd / 2));
+    float ips = 0.0f;
+    double errsq = 0.0;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const float scale = half_to_float_ref(sc[b]);
+        if (scale == 0.0f || !__builtin_isfinite(scale)) continue;
+        const uint2 w = rc[b];
+        const float4 *qp = (const float4 *)(qg + 16 * b);
+        const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+        const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                              q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+        float bip = 0.0f;
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            const uint32_t byte = ((l < 4 ? w.x : w.y) >> (8 * (l & 3))) & 0xFFu;
+            int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
+            lo -= (lo > 7) ? 16 : 0;
+            hi -= (hi > 7) ? 16 : 0;
+            const float pl = (float)lo * qv[2 * l];
+            bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+        }
+        ips = fmaf(scale, bip, ips);
+        const double pde = 0.5 * fabs((double)scale);
+        errsq += 16.0 * pde * pde;
+    }
+    const float dist = a.long_in[item] - 2.0f * ips;
+    const float err = (float)(2.0 * (double)s.query_norm * sqrt(errsq));
+    Interval iv;
+    iv.estimate = dist;
+    iv.lower_bound = dist - err;
+    iv.upper_bound = dist + err;
+    a.out_iv[item] = iv;
+    if (a.out_f) a.out_f[item] = dist;
+}
+
+// ---------------------------------------------------------------- C8 -----
+// Symmetric code<->code distance (distanceBetweenEncoded / symmetric
+// BuildDistanceFromPrepared, space_rabitq.h:2634-2747, K = 1): ip4 =
+// sum (2a-15)(2b-15) exactly, then the double-precision finish with the lhs
+// (prepared) side first. Pairs (qidx[i] = lhs row, ids[i] = rhs row).
+__device__ __forceinline__ uint32_t centered_bytes(uint32_t nib4) {
+    // four nibbles c (one per byte) -> four signed bytes 2c - 15, carry-free
+    const uint32_t x = nib4 ^ 0x08080808u;                           // c - 8, 4-bit two's complement
+    const uint32_t sx = x | (((x & 0x08080808u) >> 3) * 0xF0u);      // sign-extend each byte
+    return ((sx << 1) & 0xFEFEFEFEu) | 0x01010101u;                   // 2(c - 8) + 1
+}
+
+__device__ __forceinline__ float symmetric_finish(float2 lh, float2 rh, int64_t ip4) {
+    const double ln = sqrt((0.0 < (double)lh.x) ? (double)lh.x : 0.0);
+    const double rn = sqrt((0.0 < (double)rh.x) ? (double)rh.x : 0.0);
+    const bool lvalid = ln > 0.0 && lh.y > 0.0f && __builtin_isfinite(lh.y);
+    const double lipn = lvalid ? (double)lh.y / (2.0 * ln) : 0.0;
+    if (!lvalid || !(rn > 0.0) || !(rh.y > 0.0f) || !__builtin_isfinite(rh.y))
+        return (float)((double)lh.x + (double)rh.x + 0.0);
+    const double ripn = (double)rh.y / (2.0 * rn);
+    const double code_ip = 0.25 * (double)ip4;
+    double u = code_ip * lipn * ripn;
+    if (!__builtin_isfinite(u)) u = 0.0;
+    u = (u < 1.0) ? u : 1.0;
+    u = (-1.0 < u) ? u : -1.0;
+    const double rip = ln * rn * u;
+    return (float)((double)lh.x + (double)rh.x - 2.0 * rip);
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_symmetric(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint32_t l = a.qidx[item], r = a.ids[item];
+    const uint4 *lr = (const uint4 *)(a.ix.codes + (uint64_t)l * (a.cd / 2));
+    const uint4 *rr = (const uint4 *)(a.ix.codes + (uint64_t)r * (a.cd / 2));
+    int acc = 0;
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 wl = lr[c], wr = rr[c];
+        const uint32_t ls[4] = {wl.x, wl.y, wl.z, wl.w}, rs[4] = {wr.x, wr.y, wr.z, wr.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc = __builtin_amdgcn_sdot4((int)centered_bytes(ls[k] & 0x0F0F0F0Fu),
+                                         (int)centered_bytes(rs[k] & 0x0F0F0F0Fu), acc, false);
+            acc = __builtin_amdgcn_sdot4((int)centered_bytes((ls[k] >> 4) & 0x0F0F0F0Fu),
+                                         (int)centered_bytes((rs[k] >> 4) & 0x0F0F0F0Fu), acc, false);
+        }
+    }
+    a.out_f[item] = symmetric_finish(a.ix.hdr[l], a.ix.hdr[r], acc);
+    if (a.out_dot) a.out_dot[item] = acc;
+}
+
+}  // namespace qg
+
+extern "C" {
+
+enum { QG_K_INT8_FULL = 0, QG_K_ASYM = 1, QG_K_REM_FP32 = 2, QG_K_REM_INT8 = 3, QG_K_GATE_FP32 = 4,
+       QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7 };
+
+int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t stream) {
+    const dim3 block(qg::kScanThreads);
+    if (fixed) {
+        if (a->per_query == 0 || a->qv.nq == 0) return 0;
+        const dim3 grid((unsigned)((a->per_query + qg::kScanThreads - 1) / qg::kScanThreads), (unsigned)a->qv.nq);
+        switch (kernel) {
+            case QG_K_INT8_FULL: hipLaunchKernelGGL(qg::k_int8_full<true>, grid, block, 0, stream, *a); break;
+            case QG_K_ASYM: hipLaunchKernelGGL(qg::k_asym<true>, grid, block, 0, stream, *a); break;
+            case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<true>, grid, block, 0, stream, *a); break;
+            default: return (int)hipErrorInvalidValue;
+        }
+    } else {
+        if (a->n_pairs == 0) return 0;
+        const dim3 grid((unsigned)((a->n_pairs + qg::kScanThreads - 1) / qg::kScanThreads));
+        switch (kernel) {
+            case QG_K_INT8_FULL: hipLaunchKernelGGL(qg::k_int8_full<false>, grid, block, 0, stream, *a); break;
+            case QG_K_ASYM: hipLaunchKernelGGL(qg::k_asym<false>, grid, block, 0, stream, *a); break;
+            case QG_K_REM_FP32: hipLaunchKernelGGL(qg::k_rem_fp32, grid, block, 0, stream, *a); break;
+            case QG_K_REM_INT8: hipLaunchKernelGGL(qg::k_rem_int8, grid, block, 0, stream, *a); break;
+            case QG_K_GATE_FP32: hipLaunchKernelGGL(qg::k_gate_fp32, grid, block, 0, stream, *a); break;
+            case QG_K_GATE_INT8: hipLaunchKernelGGL(qg::k_gate_int8, grid, block, 0, stream, *a); break;
+            case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<false>, grid, block, 0, stream, *a); break;
+            case QG_K_SYMMETRIC: hipLaunchKernelGGL(qg::k_symmetric, grid, block, 0, stream, *a); break;
+            default: return (int)hipErrorInvalidValue;
+        }
+    }
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

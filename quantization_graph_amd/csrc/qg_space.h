@@ -1,0 +1,36 @@
+// qg_space.h -- host-side space object: configuration of one ExRaBitQ4
+// space (RaBitQSpace ctor, space_rabitq.h:2836-2889) plus its device
+// constants (rotation signs, center, rotated center).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/qgraph_rabitq.h"
+
+namespace qg {
+
+struct Space {
+    size_t dim = 0, cd = 0, block = 16, block_count = 0;
+    uint32_t seed = 100;
+    bool mse = true, fp16 = true;
+    int codec = 0;  // query coarse codec selected through the bridge API
+    size_t compact_bytes = 0, residual_bytes = 0, full_bytes = 0;
+    std::vector<float> h_signs, h_center;
+    float *d_signs = nullptr;
+    float *d_center = nullptr;
+    float *d_rot_center = nullptr;
+    float center_norm_sqr = 0.0f;
+    void *d_bridge = nullptr;  // staging buffer of the host-memory bridge
+    size_t bridge_bytes = 0;
+};
+
+Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits, size_t block,
+                    bool mse, bool fp16);
+void space_destroy(Space *s);
+int space_set_center(Space *s, const float *center_host);
+int fail(const char *fmt, ...);
+int check_hip(int rc, const char *what);
+
+}  // namespace qg

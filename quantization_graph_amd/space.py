@@ -1,0 +1,318 @@
+"""Host-side mirror of the reference's ExRaBitQ4 space on MI355X.
+
+``RabitqSpace`` plays the role of the Rust wrapper ``RabitqSpace``
+(experiments/02_diskann_fair/src/ours_diskann.rs:244-330) and of
+``hnswlib::RaBitQSpace`` behind the bridge: same constructor arguments
+(``rabitq_space_new(dim, seed, K, residual_bits, block, mse, fp16)``,
+rabitq_bridge.cpp:42-74), same record sizes, same codecs. Instead of one
+query per call it works on device-resident batches: ``encode`` turns N base
+vectors into a structure-of-arrays ``DeviceIndex``; ``prepare`` turns N
+queries into ``DeviceQueries``; the estimator entry points score many
+(query, candidate) pairs per launch.
+
+PyTorch supplies device memory and the current HIP stream; all arithmetic
+runs in libqgraph_hip.so.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import IndexDesc, QueryDesc, check
+
+CODEC_FULL = 0
+CODEC_INT8 = 3
+_CODECS = {"full": CODEC_FULL, "int8": CODEC_INT8, CODEC_FULL: CODEC_FULL, CODEC_INT8: CODEC_INT8}
+
+SCAN_INT8_FULL = 0
+SCAN_ASYM = 1
+SCAN_REM_FP32 = 2
+SCAN_REM_INT8 = 3
+SCAN_GATE_FP32 = 4
+SCAN_GATE_INT8 = 5
+SCAN_RESIDUAL = 6
+
+# RabitqPaperEstimate (rabitq_bridge.cpp:21-28) and RabitqDistanceInterval
+# (:15-19) as numpy record types.
+ESTIMATE_DTYPE = np.dtype([("lower_bound", "<f4"), ("short_ip", "<f4"), ("alpha", "<f4"),
+                           ("ip_hat", "<f4"), ("error_bound", "<f4"), ("valid", "u1"),
+                           ("pad", "u1", 3)])
+INTERVAL_DTYPE = np.dtype([("estimate", "<f4"), ("lower_bound", "<f4"), ("upper_bound", "<f4")])
+QSCALAR_DTYPE = np.dtype([("query_norm", "<f4"), ("query_norm_sqr", "<f4"), ("half_sum", "<f4"),
+                          ("sum", "<f4"), ("positive_sum", "<f4"), ("int8_scale", "<f4"),
+                          ("int8_error_norm", "<f4"), ("max_abs_bits", "<i4"),
+                          ("int8_code_sum", "<i8"), ("pad", "<i8")])
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t, dtype):
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(np.asarray(t))
+    return t.to(device="cuda", dtype=dtype).contiguous()
+
+
+def code_dim_for(dim):
+    """roundUp64 (space_rabitq.h:217-224): next power of two of dim rounded up to 64."""
+    rounded = ((dim + 63) // 64) * 64
+    p = 1
+    while p < rounded:
+        p <<= 1
+    return p
+
+
+class DeviceIndex:
+    """Structure-of-arrays encoded base set resident in HBM.
+
+    codes   uint8 [n, cd/2]   primary 4-bit codes (reference nibble layout)
+    hdr     f32   [n, 2]      norm_sqr, long_scale
+    sfac    f32   [n, 2]      short_scale, error_scale
+    msb     uint8 [n, cd/8]   MSB plane of the codes (paper-prune sidecar)
+    factors f32   [n, 4]      norm_sqr, data_norm, cross_scale, error_cross_scale
+    fvalid  uint8 [n]         sidecar validity
+    rscale  int16 [n, cd/16]  residual block scales (fp16 bit patterns)
+    rcodes  uint8 [n, cd/2]   residual 4-bit two's-complement codes
+    rfac    f32   [n, 2]      residual_norm_sqr, long_residual_ip
+    """
+
+    def __init__(self, space, n):
+        cd = space.code_dim
+        dev = "cuda"
+        self.space = space
+        self.n = n
+        self.codes = torch.empty((n, cd // 2), dtype=torch.uint8, device=dev)
+        self.hdr = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        self.sfac = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        self.msb = torch.empty((n, cd // 8), dtype=torch.uint8, device=dev)
+        self.factors = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        self.fvalid = torch.empty((n,), dtype=torch.uint8, device=dev)
+        self.rscale = torch.empty((n, cd // 16), dtype=torch.int16, device=dev)
+        self.rcodes = torch.empty((n, cd // 2), dtype=torch.uint8, device=dev)
+        self.rfac = torch.empty((n, 2), dtype=torch.float32, device=dev)
+
+    def desc(self):
+        return IndexDesc(self.codes.data_ptr(), self.hdr.data_ptr(), self.sfac.data_ptr(),
+                         self.msb.data_ptr(), self.factors.data_ptr(), self.fvalid.data_ptr(),
+                         self.rscale.data_ptr(), self.rcodes.data_ptr(), self.rfac.data_ptr(),
+                         self.n)
+
+    def nbytes(self):
+        return sum(t.numel() * t.element_size() for t in (
+            self.codes, self.hdr, self.sfac, self.msb, self.factors, self.fvalid, self.rscale,
+            self.rcodes, self.rfac))
+
+    def to_records(self):
+        """Reference AoS records: compact [n, 16+cd/2+1] and residual
+        [n, 8+2*cd/16+cd/2] (copyCompactPayloadFromFull /
+        copyResidualRecordFromFull, space_rabitq.h:6005-6029)."""
+        sp = self.space
+        comp = torch.empty((self.n, sp.compact_record_bytes), dtype=torch.uint8, device="cuda")
+        res = torch.empty((self.n, sp.residual_record_bytes), dtype=torch.uint8, device="cuda")
+        d = self.desc()
+        check(_lib.lib().qg_pack_records(sp.handle, C.byref(d), _ptr(comp), comp.shape[1], _ptr(res),
+                                         res.shape[1], _stream()))
+        return comp, res
+
+    @classmethod
+    def from_records(cls, space, compact, residual=None):
+        """Build the SoA index from reference AoS records (device or host)."""
+        compact = _dev(compact, torch.uint8)
+        n = compact.shape[0]
+        ix = cls(space, n)
+        if residual is not None:
+            residual = _dev(residual, torch.uint8)
+        else:
+            ix.rscale.zero_()
+            ix.rcodes.zero_()
+            ix.rfac.zero_()
+        d = ix.desc()
+        check(_lib.lib().qg_unpack_records(space.handle, _ptr(compact), compact.shape[1],
+                                           _ptr(residual), residual.shape[1] if residual is not None else 0,
+                                           n, C.byref(d), _stream()))
+        return ix
+
+
+class DeviceQueries:
+    """Prepared queries resident in HBM (QueryContext, space_rabitq.h:101-132)."""
+
+    def __init__(self, space, nq, codec):
+        cd = space.code_dim
+        self.space = space
+        self.nq = nq
+        self.codec = codec
+        self.q = torch.empty((nq, cd), dtype=torch.float32, device="cuda")
+        self.x8 = torch.empty((nq, cd), dtype=torch.int8, device="cuda")
+        self.x8p = torch.empty((nq, cd), dtype=torch.int8, device="cuda")
+        self.qs = torch.empty((nq, 48), dtype=torch.uint8, device="cuda")
+
+    def desc(self):
+        return QueryDesc(self.q.data_ptr(), self.x8.data_ptr(), self.x8p.data_ptr(),
+                         self.qs.data_ptr(), self.nq)
+
+    def scalars(self):
+        """Per-query scalars as a numpy record array (QSCALAR_DTYPE)."""
+        return self.qs.cpu().numpy().view(QSCALAR_DTYPE).reshape(self.nq)
+
+
+class RabitqSpace:
+    """One ExRaBitQ4 space: rabitq_space_new(dim, seed, K, bits, block, mse,
+    fp16) (rabitq_bridge.cpp:42-74). The formal configuration of the
+    reference runner is (dim, 100, 1, 4, 16, True, True)
+    (ours_diskann.rs:277)."""
+
+    def __init__(self, dim, seed=100, centroid_count=1, residual_bits=4, residual_block_size=16,
+                 residual_mse=True, residual_fp16=True):
+        L = _lib.lib()
+        h = L.qg_space_new(dim, seed, centroid_count, residual_bits, residual_block_size,
+                           bool(residual_mse), bool(residual_fp16))
+        if not h:
+            raise _lib.QGError(L.qg_last_error().decode())
+        self.handle = C.c_void_p(h)
+        self.dim = dim
+        self.seed = seed
+        self.code_dim = L.qg_space_code_dim(self.handle)
+        self.block_count = self.code_dim // residual_block_size
+        self.compact_record_bytes = 16 + self.code_dim // 2 + 1
+        self.residual_record_bytes = 8 + 2 * self.block_count + self.code_dim // 2
+        self.full_record_bytes = 24 + 2 * self.block_count + self.code_dim + 1
+        self.paper_msb_code_bytes = (self.code_dim + 7) // 8
+        self.paper_factor_bytes = 20
+        self.codec = CODEC_INT8
+
+    @classmethod
+    def new_with_centroids(cls, dim, seed, centroid_count, centroids):
+        """RabitqSpace::new_with_centroids (ours_diskann.rs:264-297)."""
+        centroids = np.asarray(centroids, np.float32).reshape(-1)
+        if centroids.size != dim * centroid_count:
+            raise ValueError("ExRaBitQ centroids dimension mismatch: centroids=%d dim*K=%d"
+                             % (centroids.size, dim * centroid_count))
+        sp = cls(dim, seed, centroid_count, 4, 16, True, True)
+        sp.set_center(centroids)
+        return sp
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib.lib().qg_space_delete(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def set_center(self, center):
+        """rabitq_space_set_center (rabitq_bridge.cpp:80-87)."""
+        c = np.ascontiguousarray(np.asarray(center, np.float32).reshape(-1))
+        if c.size != self.dim:
+            raise ValueError("center has %d values, expected %d" % (c.size, self.dim))
+        check(_lib.lib().qg_space_set_center(self.handle, c.ctypes.data_as(C.c_void_p)))
+
+    def set_query_coarse_codec(self, codec):
+        """RabitqSpace::set_query_coarse_codec (ours_diskann.rs:299-312):
+        'full' (0) or 'int8' (3)."""
+        if codec not in _CODECS:
+            raise ValueError("unsupported query coarse codec %r" % (codec,))
+        self.codec = _CODECS[codec]
+
+    # ----------------------------------------------------------- encode
+    def encode(self, x, batch=None):
+        """Encode base vectors x [n, dim] (float32) into a DeviceIndex."""
+        x = _dev(x, torch.float32)
+        n = x.shape[0]
+        if x.shape[1] != self.dim:
+            raise ValueError("x has dimension %d, expected %d" % (x.shape[1], self.dim))
+        ix = DeviceIndex(self, n)
+        if n == 0:
+            return ix
+        L = _lib.lib()
+        rows = n if batch is None else max(1, min(n, batch))
+        ws_bytes = L.qg_encode_workspace_bytes(self.handle, rows)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        d = ix.desc()
+        check(L.qg_encode(self.handle, _ptr(x), n, C.byref(d), _ptr(ws), ws_bytes, _stream()))
+        return ix
+
+    # ------------------------------------------------------------ query
+    def prepare(self, queries, codec=None):
+        """Prepare queries [nq, dim] (rabitq_prepare_query for a batch)."""
+        codec = self.codec if codec is None else _CODECS[codec]
+        qx = _dev(queries, torch.float32)
+        if qx.dim() == 1:
+            qx = qx.reshape(1, -1)
+        nq = qx.shape[0]
+        out = DeviceQueries(self, nq, codec)
+        L = _lib.lib()
+        ws = torch.empty(L.qg_prepare_workspace_bytes(nq), dtype=torch.uint8, device="cuda")
+        d = out.desc()
+        check(L.qg_prepare_queries(self.handle, _ptr(qx), nq, codec, C.byref(d), _ptr(ws), _stream()))
+        return out
+
+    # -------------------------------------------------------- estimators
+    def _scan(self, kernel, fixed, index, queries, ids, qidx=None, per_query=0, n_pairs=0, eps0=1.9,
+              short_ips=None, long_in=None, want=("f",)):
+        n = ids.numel()
+        out_f = torch.empty(ids.shape, dtype=torch.float32, device="cuda") if "f" in want else None
+        out_f2 = torch.empty(ids.shape, dtype=torch.float32, device="cuda") if "f2" in want else None
+        out_est = torch.empty((n, 24), dtype=torch.uint8, device="cuda") if "est" in want else None
+        out_iv = torch.empty((n, 12), dtype=torch.uint8, device="cuda") if "iv" in want else None
+        out_dot = torch.empty(ids.shape, dtype=torch.int64, device="cuda") if "dot" in want else None
+        di, dq = index.desc(), queries.desc()
+        check(_lib.lib().qg_scan(self.handle, kernel, 1 if fixed else 0, C.byref(di), C.byref(dq),
+                                 _ptr(ids), _ptr(qidx), per_query, n_pairs, C.c_float(eps0),
+                                 _ptr(short_ips), _ptr(long_in), _ptr(out_f), _ptr(out_f2),
+                                 _ptr(out_est), _ptr(out_iv), _ptr(out_dot), _stream()))
+        return out_f, out_f2, out_est, out_iv, out_dot
+
+    def scan_fixed(self, kernel, index, queries, ids, eps0=1.9, long_in=None, want=("f",)):
+        """Score fixed candidate lists ids [nq, P] for every prepared query."""
+        ids = _dev(ids, torch.int32)
+        if ids.dim() != 2 or ids.shape[0] != queries.nq:
+            raise ValueError("ids must be [nq, per_query]")
+        if long_in is not None:
+            long_in = _dev(long_in, torch.float32)
+        return self._scan(kernel, True, index, queries, ids, per_query=ids.shape[1], eps0=eps0,
+                          long_in=long_in, want=want)
+
+    def scan_pairs(self, kernel, index, queries, qidx, ids, eps0=1.9, short_ips=None, long_in=None,
+                   want=("f",)):
+        """Score (qidx[i], ids[i]) pairs."""
+        ids = _dev(ids, torch.int32).reshape(-1)
+        qidx = _dev(qidx, torch.int32).reshape(-1)
+        if qidx.numel() != ids.numel():
+            raise ValueError("qidx and ids must have the same length")
+        if short_ips is not None:
+            short_ips = _dev(short_ips, torch.float32).reshape(-1)
+        if long_in is not None:
+            long_in = _dev(long_in, torch.float32).reshape(-1)
+        return self._scan(kernel, False, index, queries, ids, qidx=qidx, n_pairs=ids.numel(),
+                          eps0=eps0, short_ips=short_ips, long_in=long_in, want=want)
+
+
+def topk(values, ids, k):
+    """Per-row top-k of values [nq, P] by (value, id); returns (ids, values,
+    positions), each [nq, k]. Rows shorter than k are padded with id
+    0xFFFFFFFF and +inf."""
+    values = values.contiguous()
+    ids = _dev(ids, torch.int32)
+    nq, p = values.shape
+    out_ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    out_v = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+    out_pos = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    check(_lib.lib().qg_topk(_ptr(values), _ptr(ids), nq, p, k, _ptr(out_ids), _ptr(out_v),
+                             _ptr(out_pos), _stream()))
+    return out_ids, out_v, out_pos
+
+
+def estimates_np(t):
+    """Device [n, 24] estimate bytes -> numpy ESTIMATE_DTYPE array."""
+    return t.cpu().numpy().reshape(-1).view(ESTIMATE_DTYPE)
+
+
+def intervals_np(t):
+    """Device [n, 12] interval bytes -> numpy INTERVAL_DTYPE array."""
+    return t.cpu().numpy().reshape(-1).view(INTERVAL_DTYPE)

@@ -89,9 +89,23 @@ class Oracle:
         L.ro_gate_fp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.ro_gate_int8.restype = Estimate
         L.ro_gate_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_int8_full_estimate.restype = C.c_float
+        L.ro_int8_full_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                            C.POINTER(C.c_float)]
+        L.ro_mt32_nth.restype = C.c_uint32
+        L.ro_mt32_nth.argtypes = [C.c_uint32, C.c_size_t]
+        L.ro_mt64_nth.restype = C.c_uint64
+        L.ro_mt64_nth.argtypes = [C.c_uint64, C.c_size_t]
         L.ro_int_codec_estimate.restype = Estimate
         L.ro_int_codec_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.ro_masked_sum.restype = C.c_int64
+        L.ro_masked_sum.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_remaining_dot_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_full_dot_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_symmetric_ip4.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_symmetric_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_asym_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_asym_distance_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.ro_remaining_dot_int8.restype = C.c_int64
         L.ro_full_dot_int8.restype = C.c_int64
         L.ro_symmetric_ip4.restype = C.c_int64
@@ -207,6 +221,83 @@ class Oracle:
     def asym_f64(self, q, comp):
         return np.array([self.lib.ro_asym_distance_f64(self.sp, C.byref(q), _p(comp[i]))
                          for i in range(comp.shape[0])], np.float64)
+
+    # ---- batched helpers over compact records / sidecars (rows of arrays)
+    def _each(self, fn, rows):
+        return [fn(rows[i]) for i in range(rows.shape[0])]
+
+    def int8_full(self, q, comp, eps0=1.9):
+        est = np.zeros(comp.shape[0], np.float32)
+        low = np.zeros(comp.shape[0], np.float32)
+        dot = np.zeros(comp.shape[0], np.int64)
+        lo = C.c_float()
+        for i in range(comp.shape[0]):
+            est[i] = self.lib.ro_int8_full_estimate(self.sp, C.byref(q), _p(comp[i]), C.c_float(eps0), C.byref(lo))
+            low[i] = lo.value
+            dot[i] = self.lib.ro_full_dot_int8(self.sp, C.byref(q), _p(comp[i]))
+        return est, low, dot
+
+    def rem_int8(self, q, comp, short_ips):
+        d = np.array([self.lib.ro_remaining_distance_int8(self.sp, C.byref(q), _p(comp[i]), C.c_float(short_ips[i]))
+                      for i in range(comp.shape[0])], np.float32)
+        dots = np.array([self.lib.ro_remaining_dot_int8(self.sp, C.byref(q), _p(comp[i]))
+                         for i in range(comp.shape[0])], np.int64)
+        return d, dots
+
+    def rem_fp32(self, q, comp, short_ips):
+        return np.array([self.lib.ro_remaining_distance_fp32(self.sp, C.byref(q), _p(comp[i]), C.c_float(short_ips[i]))
+                         for i in range(comp.shape[0])], np.float32)
+
+    def gate(self, q, msb, fac, codec, eps0=1.9):
+        fn = self.lib.ro_gate_int8 if codec == 3 else self.lib.ro_gate_fp32
+        out = np.zeros(msb.shape[0], ESTIMATE_DTYPE)
+        for i in range(msb.shape[0]):
+            e = fn(self.sp, C.byref(q), _p(msb[i]), C.c_void_p(fac.ctypes.data + 20 * i), C.c_float(eps0))
+            out[i] = (e.lower_bound, e.short_ip, e.alpha, e.ip_hat, e.error_bound, e.valid, (0, 0, 0))
+        return out
+
+    def masked_sum(self, q, msb):
+        return np.array([self.lib.ro_masked_sum(self.sp, C.byref(q), _p(msb[i])) for i in range(msb.shape[0])],
+                        np.int64)
+
+    def residual(self, q, res, long_d):
+        out = np.zeros(res.shape[0], INTERVAL_DTYPE)
+        for i in range(res.shape[0]):
+            iv = self.lib.ro_residual_interval(self.sp, C.byref(q), _p(res[i]), C.c_float(long_d[i]))
+            out[i] = (iv.estimate, iv.lower_bound, iv.upper_bound)
+        return out
+
+    def residual_f64(self, q, res, long_d):
+        return np.array([self.lib.ro_residual_estimate_f64(self.sp, C.byref(q), _p(res[i]), float(long_d[i]))
+                         for i in range(res.shape[0])], np.float64)
+
+    def symmetric(self, a, b):
+        return self.lib.ro_symmetric_distance(self.sp, _p(a), _p(b))
+
+    def symmetric_ip4(self, a, b):
+        return self.lib.ro_symmetric_ip4(self.sp, _p(a[16:]), _p(b[16:]))
+
+    def search(self, q, codec, comp, res, msb, fac, degrees, edges, k=10, L=100, eps0=1.9, rerank=100):
+        n = comp.shape[0]
+        ids = np.zeros(k, np.uint32)
+        ds = np.zeros(k, np.float32)
+        pool_ids = np.zeros(max(L, k) + 1, np.uint32)
+        pool_d = np.zeros(max(L, k) + 1, np.float32)
+        plen = C.c_size_t()
+        st = SearchStats()
+        got = self.lib.ro_search(self.sp, C.byref(q), codec, _p(comp), comp.shape[1], _p(res), res.shape[1],
+                                 _p(msb), msb.shape[1], _p(fac), _p(degrees), _p(edges), edges.shape[1],
+                                 n, k, L, C.c_float(eps0), rerank, _p(ids), _p(ds), _p(pool_ids), _p(pool_d),
+                                 C.byref(plen), C.byref(st))
+        return ids[:got], ds[:got], pool_ids[:plen.value], pool_d[:plen.value], st
+
+    def robust_prune(self, comp, source, pool_ids, pool_d, r, alpha):
+        pool_ids = np.ascontiguousarray(pool_ids, np.uint32)
+        pool_d = np.ascontiguousarray(pool_d, np.float32)
+        out = np.zeros(r, np.uint32)
+        got = self.lib.ro_robust_prune(self.sp, _p(comp), comp.shape[1], source, _p(pool_ids), _p(pool_d),
+                                       pool_ids.size, r, C.c_float(alpha), _p(out))
+        return out[:got]
 
 
 class Ref:
