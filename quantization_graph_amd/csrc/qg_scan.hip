@@ -72,8 +72,11 @@ __global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
         const double sip = (double)scale * centered;
         const double est_ip = (double)h.y * sip;
         est = h.x + s.query_norm_sqr - (float)est_ip;
-        const double db_err = (double)a.ix.sfac[id].y * (double)s.query_norm;
-        lower = est - (float)((double)a.eps0 * db_err);
+        lower = est;
+        if (a.out_f2) {
+            const double db_err = (double)a.ix.sfac[id].y * (double)s.query_norm;
+            lower = est - (float)((double)a.eps0 * db_err);
+        }
     }
     a.out_f[item] = est;
     if (a.out_f2) a.out_f2[item] = lower;
