@@ -347,19 +347,23 @@ class Ref:
                                                           C.c_size_t, C.c_void_p, C.c_size_t,
                                                           C.c_void_p, C.c_size_t, C.c_float,
                                                           C.c_void_p]
-        L.rabitq_paper_estimate_codec.restype = Estimate
-        L.rabitq_paper_estimate_codec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                                  C.c_void_p, C.c_float]
         L.rabitq_residual_distance_batch.restype = C.c_bool
         L.rabitq_residual_distance_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                                      C.c_void_p, C.c_size_t, C.c_void_p,
                                                      C.c_size_t, C.c_void_p, C.c_void_p]
+        if not hasattr(L, "rabitq_build_vamana_graph"):
+            self._init_space(dim, seed, centroid_count, residual_bits, block, mse, fp16)
+            return
         L.rabitq_build_vamana_graph.restype = C.c_bool
         L.rabitq_build_vamana_graph.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                                 C.c_size_t, C.c_size_t, C.c_float, C.c_size_t,
                                                 C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
                                                 C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        self._init_space(dim, seed, centroid_count, residual_bits, block, mse, fp16)
+
+    def _init_space(self, dim, seed, centroid_count, residual_bits, block, mse, fp16):
+        L = self.lib
         self.h = L.rabitq_space_new(dim, seed, centroid_count, residual_bits, block, mse, fp16)
         if not self.h:
             raise ValueError("rabitq_space_new failed")
