@@ -83,6 +83,75 @@ __global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
     if (a.out_dot) a.out_dot[item] = acc;
 }
 
+// C7 for fixed lists, coalesced form: 8 lanes per candidate. Lane `sub` of
+// a group owns the 16-byte pieces sub, sub+8, sub+16, ... of the code row, so
+// every load instruction reads 128 contiguous bytes per candidate, and keeps
+// the matching slice of the INT8 query in registers. The integer dot is
+// exact in any order; the 8 partial sums meet through lane shuffles.
+template <int CD>
+__global__ void __launch_bounds__(kScanThreads) k_int8_full_grouped(ScanArgs a) {
+    constexpr int kLanes = 8;
+    constexpr int kPieces = CD / 2 / 16 / kLanes;  // 16-byte pieces per lane
+    const uint64_t qi = blockIdx.y;
+    const int sub = threadIdx.x & (kLanes - 1);
+    const int slot = threadIdx.x / kLanes;
+    constexpr int kSlots = kScanThreads / kLanes;
+    // query slice: piece p of this lane covers code bytes 16*(sub + 8p) ..,
+    // i.e. dims 32*(sub + 8p) .. +32 -> 8 dwords of x8p
+    uint32_t xv[kPieces][8];
+    const uint32_t *xq = (const uint32_t *)(a.qv.x8p + qi * CD);
+#pragma unroll
+    for (int p = 0; p < kPieces; ++p)
+#pragma unroll
+        for (int w = 0; w < 8; ++w) xv[p][w] = xq[8 * (sub + kLanes * p) + w];
+    const QScalars s = a.qv.qs[qi];
+    const uint64_t first = (uint64_t)blockIdx.x * kSlots * 4;
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t pos = first + (uint64_t)r * kSlots + slot;
+        const bool live = pos < a.per_query;
+        const uint64_t item = qi * a.per_query + (live ? pos : 0);
+        const uint32_t id = a.ids[item];
+        const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (CD / 2));
+        uint4 w[kPieces];
+#pragma unroll
+        for (int p = 0; p < kPieces; ++p) w[p] = row[sub + kLanes * p];
+        int acc = 0;
+#pragma unroll
+        for (int p = 0; p < kPieces; ++p) {
+            const uint32_t ws[4] = {w[p].x, w[p].y, w[p].z, w[p].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x0F0F0F0Fu), (int)xv[p][2 * k], acc, false);
+                acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x0F0F0F0Fu), (int)xv[p][2 * k + 1], acc, false);
+            }
+        }
+        acc += __shfl_xor(acc, 1);
+        acc += __shfl_xor(acc, 2);
+        acc += __shfl_xor(acc, 4);
+        if (sub != 0 || !live) continue;
+        const float2 h = a.ix.hdr[id];
+        const float scale = s.int8_scale;
+        float est, lower;
+        if (!long_scale_ok(h.y) || !(scale > 0.0f) || !__builtin_isfinite(scale) || s.query_norm <= 0.0f) {
+            est = h.x + s.query_norm_sqr;
+            lower = est;
+        } else {
+            const double centered = (double)acc - 7.5 * (double)s.int8_code_sum;
+            const double sip = (double)scale * centered;
+            const double est_ip = (double)h.y * sip;
+            est = h.x + s.query_norm_sqr - (float)est_ip;
+            lower = est;
+            if (a.out_f2) {
+                const double db_err = (double)a.ix.sfac[id].y * (double)s.query_norm;
+                lower = est - (float)((double)a.eps0 * db_err);
+            }
+        }
+        a.out_f[item] = est;
+        if (a.out_f2) a.out_f2[item] = lower;
+        if (a.out_dot) a.out_dot[item] = acc;
+    }
+}
+
 // ---------------------------------------------------------------- C1 -----
 // FP32 query x 4-bit code (queryDistanceLong + longCodeIpsAvx512,
 // space_rabitq.h:1551-1607): 16 lane accumulators per sum, lane j takes
@@ -404,7 +473,14 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
         if (a->per_query == 0 || a->qv.nq == 0) return 0;
         const dim3 grid((unsigned)((a->per_query + qg::kScanThreads - 1) / qg::kScanThreads), (unsigned)a->qv.nq);
         switch (kernel) {
-            case QG_K_INT8_FULL: hipLaunchKernelGGL(qg::k_int8_full<true>, grid, block, 0, stream, *a); break;
+            case QG_K_INT8_FULL: {
+                const unsigned per_block = qg::kScanThreads / 8 * 4;
+                const dim3 g2((unsigned)((a->per_query + per_block - 1) / per_block), (unsigned)a->qv.nq);
+                if (a->cd == 1024) hipLaunchKernelGGL(qg::k_int8_full_grouped<1024>, g2, block, 0, stream, *a);
+                else if (a->cd == 2048) hipLaunchKernelGGL(qg::k_int8_full_grouped<2048>, g2, block, 0, stream, *a);
+                else hipLaunchKernelGGL(qg::k_int8_full<true>, grid, block, 0, stream, *a);
+                break;
+            }
             case QG_K_ASYM: hipLaunchKernelGGL(qg::k_asym<true>, grid, block, 0, stream, *a); break;
             case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<true>, grid, block, 0, stream, *a); break;
             default: return (int)hipErrorInvalidValue;
