@@ -460,6 +460,92 @@ __global__ void __launch_bounds__(kScanThreads) k_symmetric(ScanArgs a) {
     if (a.out_dot) a.out_dot[item] = acc;
 }
 
+// E1 for fixed lists, two-phase form. Phase 1: 8 lanes per candidate load
+// the residual row with 128-byte coalesced pieces (lane `sub` owns 16-byte
+// pieces sub, sub+8, ... = blocks 2(sub+8p), 2(sub+8p)+1) and compute each
+// block's 8-lane product sum in the reference order; block sums and decoded
+// scales go to LDS. Phase 2: one lane per candidate runs the order-dependent
+// recurrences ips = fma(scale, block_ip, ips) and the error norm over the
+// blocks in index order.
+template <int CD>
+__global__ void __launch_bounds__(kScanThreads) k_residual_grouped(ScanArgs a) {
+    constexpr int kLanes = 8;
+    constexpr int kSlots = kScanThreads / kLanes;      // candidates per pass
+    constexpr int kBlocks = CD / 16;
+    constexpr int kPieces = CD / 2 / 16 / kLanes;       // 16-byte pieces per lane
+    __shared__ float qsh[CD];
+    __shared__ float bip_s[kSlots][kBlocks + 1];
+    __shared__ float scl_s[kSlots][kBlocks + 1];
+    const uint64_t qi = blockIdx.y;
+    const float4 *src = (const float4 *)(a.qv.q + qi * CD);
+    for (uint32_t i = threadIdx.x; i < CD / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
+    const QScalars s = a.qv.qs[qi];
+    const int sub = threadIdx.x & (kLanes - 1);
+    const int slot = threadIdx.x / kLanes;
+    const uint64_t first = (uint64_t)blockIdx.x * kSlots * 2;
+    __syncthreads();
+    for (int r = 0; r < 2; ++r) {
+        const uint64_t pos = first + (uint64_t)r * kSlots + slot;
+        const bool live = pos < a.per_query;
+        const uint64_t item = qi * a.per_query + (live ? pos : 0);
+        const uint32_t id = a.ids[item];
+        const uint4 *row = (const uint4 *)(a.ix.rcodes + (uint64_t)id * (CD / 2));
+        const uint32_t *srow = (const uint32_t *)(a.ix.rscale + (uint64_t)id * kBlocks);
+        uint4 w[kPieces];
+        uint32_t sc2[kPieces];
+#pragma unroll
+        for (int p = 0; p < kPieces; ++p) {
+            w[p] = row[sub + kLanes * p];
+            sc2[p] = srow[sub + kLanes * p];  // fp16 scales of the piece's two blocks
+        }
+#pragma unroll
+        for (int p = 0; p < kPieces; ++p) {
+            const uint32_t ws[4] = {w[p].x, w[p].y, w[p].z, w[p].w};
+#pragma unroll
+            for (int hb = 0; hb < 2; ++hb) {
+                const int blk = 2 * (sub + kLanes * p) + hb;
+                const float4 *qp = (const float4 *)(qsh + 16 * blk);
+                const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+                const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                                      q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+                float bip = 0.0f;
+#pragma unroll
+                for (int l = 0; l < 8; ++l) {
+                    const uint32_t byte = (ws[2 * hb + (l >> 2)] >> (8 * (l & 3))) & 0xFFu;
+                    int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
+                    lo -= (lo > 7) ? 16 : 0;
+                    hi -= (hi > 7) ? 16 : 0;
+                    const float pl = (float)lo * qv[2 * l];
+                    bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+                }
+                bip_s[slot][blk] = bip;
+                scl_s[slot][blk] = half_to_float_ref((uint16_t)(sc2[p] >> (16 * hb)));
+            }
+        }
+        __syncthreads();
+        if (sub == 0 && live) {
+            float ips = 0.0f;
+            double errsq = 0.0;
+            for (int b = 0; b < kBlocks; ++b) {
+                const float scale = scl_s[slot][b];
+                if (scale == 0.0f || !__builtin_isfinite(scale)) continue;
+                ips = fmaf(scale, bip_s[slot][b], ips);
+                const double pde = 0.5 * fabs((double)scale);
+                errsq += 16.0 * pde * pde;
+            }
+            const float dist = a.long_in[item] - 2.0f * ips;
+            const float err = (float)(2.0 * (double)s.query_norm * sqrt(errsq));
+            Interval iv;
+            iv.estimate = dist;
+            iv.lower_bound = dist - err;
+            iv.upper_bound = dist + err;
+            a.out_iv[item] = iv;
+            if (a.out_f) a.out_f[item] = dist;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace qg
 
 extern "C" {
@@ -482,7 +568,14 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
                 break;
             }
             case QG_K_ASYM: hipLaunchKernelGGL(qg::k_asym<true>, grid, block, 0, stream, *a); break;
-            case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<true>, grid, block, 0, stream, *a); break;
+            case QG_K_RESIDUAL: {
+                const unsigned per_block = qg::kScanThreads / 8 * 2;
+                const dim3 g2((unsigned)((a->per_query + per_block - 1) / per_block), (unsigned)a->qv.nq);
+                if (a->cd == 1024) hipLaunchKernelGGL(qg::k_residual_grouped<1024>, g2, block, 0, stream, *a);
+                else if (a->cd == 2048) hipLaunchKernelGGL(qg::k_residual_grouped<2048>, g2, block, 0, stream, *a);
+                else hipLaunchKernelGGL(qg::k_residual<true>, grid, block, 0, stream, *a);
+                break;
+            }
             default: return (int)hipErrorInvalidValue;
         }
     } else {
