@@ -162,6 +162,24 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
 int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_query, uint32_t k,
             uint32_t *out_ids, float *out_values, uint32_t *out_pos, void *stream);
 
+/* Batched graph search: greedy beam search (beam width 1, start node 0)
+ * with the 1-bit gate, remaining-bit distances and the residual-4 rerank of
+ * the first max(rerank, k) pool entries, one wavefront per query.
+ * Replaces the per-query loop search_ours_paper_active_inner +
+ * flush_distance_batch + rerank (ours_diskann.rs:1539-1986) and its bridge
+ * calls. Graph: degrees [n], edges [n][edge_stride], edge_stride <= 64.
+ * Outputs [nq][k] ids / estimates (missing entries: 0xFFFFFFFF / +inf);
+ * optional stats [nq][4] = hops, gate evaluations, distances, pruned and
+ * the final pool [nq][max(l_search, k)]. Workspace:
+ * qg_search_workspace_bytes(n, slots) of device memory (per-slot visited
+ * bitsets and the work-queue head); slots = concurrent queries. */
+size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots);
+int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+              const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+              float eps0, uint32_t rerank, uint32_t slots, uint32_t *out_ids, float *out_dists,
+              uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
+              void *stream);
+
 /* ------------------------------------------------------------ bridge API */
 /* Host-memory drop-ins for rabitq_bridge.cpp; see that file for semantics. */
 

@@ -63,6 +63,10 @@ def lib():
     L.qg_scan.argtypes = [vp, i32, i32, C.POINTER(IndexDesc), C.POINTER(QueryDesc), vp, vp, u64, u64,
                           f32, vp, vp, vp, vp, vp, vp, vp, vp]
     L.qg_topk.argtypes = [vp, vp, u64, u32, u32, vp, vp, vp, vp]
+    L.qg_search_workspace_bytes.restype = sz
+    L.qg_search_workspace_bytes.argtypes = [u64, u32]
+    L.qg_search.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, f32,
+                            u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
     _lib = L
     return L
 

@@ -24,6 +24,7 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
 int qg_launch_topk(const qg::SelectArgs *a, hipStream_t stream);
 int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream);
 int qg_launch_pack(const qg::PackArgs *a, uint64_t n, int unpack, hipStream_t stream);
+int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream);
 }
 
 namespace qg {
@@ -363,6 +364,55 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
     a.out_values = out_values;
     a.out_pos = out_pos;
     return check_hip(qg_launch_topk(&a, (hipStream_t)stream), "topk launch");
+}
+
+size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots) {
+    return (size_t)slots * ((n + 32) / 32) * 4 + 256;
+}
+
+int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+              const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+              float eps0, uint32_t rerank, uint32_t slots, uint32_t *out_ids, float *out_dists,
+              uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
+              void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !ix || !qv || !degrees || !edges || !out_ids || !out_dists || !workspace)
+        return fail("null argument to qg_search");
+    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
+    if (k == 0) return fail("k must be positive");
+    if (edge_stride == 0 || edge_stride > 64) return fail("graph degree bound must be in [1, 64]");
+    if (ix->n == 0 || ix->n > 0xFFFFFFFEull) return fail("index must hold 1 .. 2^32-2 points");
+    const uint32_t L = l_search > k ? l_search : k;
+    if (L > 1024) return fail("search list size %u exceeds 1024", L);
+    if ((pool_ids == nullptr) != (pool_dists == nullptr)) return fail("pool outputs come in pairs");
+    if (slots == 0) slots = 1;
+    if (workspace_bytes < qg_search_workspace_bytes(ix->n, slots))
+        return fail("search workspace too small: need %zu bytes", qg_search_workspace_bytes(ix->n, slots));
+    if (!qv->nq) return 0;
+    SearchArgs a;
+    a.ix = view_of(ix);
+    a.qv = view_of(qv);
+    a.cd = (uint32_t)s->cd;
+    a.codec = codec;
+    a.L = L;
+    a.k = k;
+    a.rerank = rerank;
+    a.eps0 = eps0;
+    a.degrees = degrees;
+    a.edges = edges;
+    a.edge_stride = edge_stride;
+    a.n = ix->n;
+    char *w = static_cast<char *>(workspace);
+    a.counter = reinterpret_cast<uint32_t *>(w);
+    a.err = reinterpret_cast<uint32_t *>(w + 4);
+    a.visited = reinterpret_cast<uint32_t *>(w + 256);
+    a.visited_words = (ix->n + 32) / 32;
+    a.out_ids = out_ids;
+    a.out_d = out_dists;
+    a.stats = out_stats;
+    a.pool_ids = pool_ids;
+    a.pool_d = pool_dists;
+    return check_hip(qg_launch_search(&a, slots, (hipStream_t)stream), "search launch");
 }
 
 }  // extern "C"

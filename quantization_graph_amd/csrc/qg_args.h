@@ -99,4 +99,26 @@ struct PackArgs {
     uint8_t *msb; float4 *factors; uint8_t *fvalid;
 };
 
+struct SearchArgs {
+    IndexView ix;
+    QueryView qv;
+    uint32_t cd;
+    int codec;                 // 0 Full, 3 Int8
+    uint32_t L, k, rerank;
+    float eps0;
+    const uint32_t *degrees;   // [n]
+    const uint32_t *edges;     // [n][edge_stride]
+    uint64_t edge_stride;
+    uint64_t n;
+    uint32_t *visited;         // [slots][visited_words] scratch bitsets
+    uint64_t visited_words;
+    uint32_t *counter;         // work-queue head
+    uint32_t *err;             // set to non-zero on an out-of-range neighbour
+    uint32_t *out_ids;         // [nq][k]
+    float *out_d;              // [nq][k]
+    uint64_t *stats;           // [nq][4] hops, gate evaluations, distances, pruned (optional)
+    uint32_t *pool_ids;        // [nq][L] final pool (optional)
+    float *pool_d;             // [nq][L]
+};
+
 }  // namespace qg

@@ -1,0 +1,189 @@
+// qg_device.h -- one-candidate forms of the estimators, for kernels in which
+// a single lane owns a (query, candidate) pair (graph search, rerank).
+// The arithmetic is the one of the batch kernels in qg_scan.hip, so results
+// are bit-identical between the two.
+#pragma once
+
+#include "qg_common.h"
+
+namespace qg {
+
+// C1: FP32 query x 4-bit code (queryDistanceLong + longCodeIpsAvx512,
+// space_rabitq.h:1551-1607).
+__device__ __forceinline__ float asym_one(const float *q, const uint8_t *code_row, uint32_t cd, float2 h,
+                                          const QScalars &s) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    const uint4 *row = (const uint4 *)code_row;
+    float sh[16], rm[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { sh[j] = 0.0f; rm[j] = 0.0f; }
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const float4 *qp = (const float4 *)(q + 32 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q0 = qp[2 * k], q1 = qp[2 * k + 1];
+            const float qe[4] = {q0.x, q0.z, q1.x, q1.z};
+            const float qo[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t byte = (ws[k] >> (8 * b)) & 0xFFu;
+                const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
+                rm[j] = fmaf((float)(lo & 7u), qe[b], rm[j]);
+                rm[j] = fmaf((float)(hi & 7u), qo[b], rm[j]);
+                if (lo > 7u) sh[j] = sh[j] + qe[b];
+                if (hi > 7u) sh[j] = sh[j] + qo[b];
+            }
+        }
+    }
+    const float short_ip = reduce16(sh) - s.half_sum;
+    return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, reduce16(rm));
+}
+
+// C3: FP32 remaining three bits (query_distance_with_paper_msb_batch_by_id).
+__device__ __forceinline__ float rem_fp32_one(const float *q, const uint8_t *code_row, uint32_t cd, float2 h,
+                                              const QScalars &s, float short_ip) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    const uint4 *row = (const uint4 *)code_row;
+    float rm[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) rm[j] = 0.0f;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const float4 *qp = (const float4 *)(q + 32 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q0 = qp[2 * k], q1 = qp[2 * k + 1];
+            const float qe[4] = {q0.x, q0.z, q1.x, q1.z};
+            const float qo[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t byte = (ws[k] >> (8 * b)) & 0xFFu;
+                rm[j] = fmaf((float)(byte & 7u), qe[b], rm[j]);
+                rm[j] = fmaf((float)((byte >> 4) & 7u), qo[b], rm[j]);
+            }
+        }
+    }
+    return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, reduce16(rm));
+}
+
+// C6: INT8 remaining three bits (finishQuantizedTraversalDistance).
+__device__ __forceinline__ float rem_int8_one(const int8_t *x8p, const uint8_t *code_row, uint32_t cd, float2 h,
+                                              const QScalars &s, float short_ip) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    const uint4 *row = (const uint4 *)code_row;
+    const uint32_t *xg = (const uint32_t *)x8p;
+    int acc = 0;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t d = 4 * c + k;
+            acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x07070707u), (int)xg[2 * d], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x07070707u), (int)xg[2 * d + 1], acc, false);
+        }
+    }
+    const double centered = (double)s.int8_scale * ((double)acc - 3.5 * (double)s.int8_code_sum);
+    const double signed_ip = 8.0 * (double)short_ip + centered;
+    return h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
+}
+
+// C4: FP32 1-bit gate.
+__device__ __forceinline__ PaperEstimate gate_fp32_one(const float *q, const uint8_t *msb_row, uint32_t cd, float4 f,
+                                                       uint8_t fv, const QScalars &s, float eps0) {
+    const uint4 *m = (const uint4 *)msb_row;
+    float acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+    for (uint32_t c = 0; c < cd / 128; ++c) {
+        const uint4 w = m[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const uint32_t bits = (ws[k] >> (16 * half)) & 0xFFFFu;
+                const float4 *qp = (const float4 *)(q + 128 * c + 32 * k + 16 * half);
+                const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+                const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                                      q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + (((bits >> j) & 1u) ? qv[j] : 0.0f);
+            }
+        }
+    }
+    return finish_gate(reduce16(acc) - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
+}
+
+// C5: INT8 1-bit gate (exact masked sum).
+__device__ __forceinline__ PaperEstimate gate_int8_one(const int8_t *x8, const uint8_t *msb_row, uint32_t cd, float4 f,
+                                                       uint8_t fv, const QScalars &s, float eps0) {
+    if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) {
+        PaperEstimate r;
+        r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
+        r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
+        return r;
+    }
+    const uint32_t *xg = (const uint32_t *)x8;
+    const uint4 *m = (const uint4 *)msb_row;
+    int sel = 0;
+    for (uint32_t c = 0; c < cd / 128; ++c) {
+        const uint4 w = m[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int nib = 0; nib < 8; ++nib) {
+                const uint32_t bits = (ws[k] >> (4 * nib)) & 0xFu;
+                const uint32_t mask = (bits & 1u) | ((bits & 2u) << 7) | ((bits & 4u) << 14) | ((bits & 8u) << 21);
+                sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[32 * c + 8 * k + nib], sel, false);
+            }
+        }
+    }
+    const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);
+    return finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, eps0);
+}
+
+// E1: residual-4 interval for one candidate.
+__device__ __forceinline__ Interval residual_one(const float *q, const uint16_t *scale_row, const uint8_t *rcode_row,
+                                                 uint32_t cd, const QScalars &s, float long_d) {
+    const uint2 *rc = (const uint2 *)rcode_row;
+    float ips = 0.0f;
+    double errsq = 0.0;
+    for (uint32_t b = 0; b < cd / 16; ++b) {
+        const float scale = half_to_float_ref(scale_row[b]);
+        if (scale == 0.0f || !__builtin_isfinite(scale)) continue;
+        const uint2 w = rc[b];
+        const float4 *qp = (const float4 *)(q + 16 * b);
+        const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+        const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                              q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+        float bip = 0.0f;
+#pragma unroll
+        for (int l = 0; l < 8; ++l) {
+            const uint32_t byte = ((l < 4 ? w.x : w.y) >> (8 * (l & 3))) & 0xFFu;
+            int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
+            lo -= (lo > 7) ? 16 : 0;
+            hi -= (hi > 7) ? 16 : 0;
+            const float pl = (float)lo * qv[2 * l];
+            bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+        }
+        ips = fmaf(scale, bip, ips);
+        const double pde = 0.5 * fabs((double)scale);
+        errsq += 16.0 * pde * pde;
+    }
+    const float dist = long_d - 2.0f * ips;
+    const float err = (float)(2.0 * (double)s.query_norm * sqrt(errsq));
+    Interval iv;
+    iv.estimate = dist;
+    iv.lower_bound = dist - err;
+    iv.upper_bound = dist + err;
+    return iv;
+}
+
+}  // namespace qg

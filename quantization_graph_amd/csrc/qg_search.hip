@@ -1,0 +1,260 @@
+// qg_search.hip -- batched greedy beam search over a Vamana graph with the
+// 1-bit gate, INT8 / FP32 remaining-bit distances and the residual-4 rerank,
+// one wavefront per query in a persistent work-queue launch.
+//
+// Semantics follow search_ours_paper_active_inner + flush_distance_batch
+// (experiments/02_diskann_fair/src/ours_diskann.rs:1539-1986) with beam
+// width 1, start node 0 and the k-th / early stops off:
+//   hop: expand the first unexpanded pool entry; its unvisited neighbours
+//        (adjacency order, first occurrence) are gated against the pool as
+//        it stood at the start of the hop (pruned if the pool is full, the
+//        gate is valid and lower_bound > pool.back); survivors get their
+//        distance (mode 2 INT8 / mode 1 FP32 reuse of the gate's short_ip,
+//        mode 0 when the gate is invalid) and are inserted in order, each
+//        re-checked against the current pool back (insert_candidate_sorted,
+//        :1100-1126, (distance, id) order, truncated to L).
+//   end:  the first min(|pool|, max(rerank, k)) entries get the FP32 4-bit
+//        distance (recomputed for the INT8 codec, :1850-1866) and the
+//        residual-4 interval; the answer is the top-k by (estimate, id).
+// Degrees are at most 64, so a hop's fresh neighbours form one distance
+// batch (DIST_BATCH_SIZE = 64), as in the reference.
+#include "qg_args.h"
+#include "qg_device.h"
+
+namespace qg {
+
+constexpr int kSearchLanes = 64;
+constexpr uint32_t kMaxL = 1024;
+
+__device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
+    return ad < bd || (ad == bd && aid < bid);
+}
+
+__global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
+    extern __shared__ unsigned char smem[];
+    const uint32_t cd = a.cd;
+    const uint32_t L = a.L;
+    float *q = (float *)smem;                                   // cd
+    int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd
+    int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd
+    float *pd = (float *)(smem + 6 * cd);                       // L
+    uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * L);        // L
+    uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * L);         // L (+pad)
+    __shared__ uint32_t fresh[64], surv[64];
+    __shared__ float f_lb[64], f_sip[64], s_d[64];
+    __shared__ uint8_t f_valid[64];
+    __shared__ float r_est[kMaxL];
+    __shared__ uint32_t r_id[kMaxL];
+    __shared__ uint32_t s_qi;
+    const int lane = threadIdx.x;
+    uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
+
+    for (;;) {
+        if (lane == 0) s_qi = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t qi = s_qi;
+        __syncthreads();
+        if (qi >= a.qv.nq) break;
+        // stage the prepared query, clear this slot's visited bits
+        for (uint32_t i = lane; i < cd / 4; i += kSearchLanes) {
+            ((float4 *)q)[i] = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
+        }
+        for (uint32_t i = lane; i < cd / 4; i += kSearchLanes) {
+            ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
+            ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
+        }
+        for (uint32_t i = lane; i < a.visited_words; i += kSearchLanes) visited[i] = 0u;
+        const QScalars s = a.qv.qs[qi];
+        __threadfence();
+        __syncthreads();
+
+        uint32_t len = 0;
+        uint64_t hops = 0, gates = 0, dists = 1, pruned = 0;
+        // start node 0 with its full FP32 distance
+        if (lane == 0) {
+            atomicOr(&visited[0], 1u);
+            pd[0] = asym_one(q, a.ix.codes, cd, a.ix.hdr[0], s);
+            pid[0] = 0;
+            pexp[0] = 0;
+        }
+        len = (a.n > 0 && L > 0) ? 1 : 0;
+        __syncthreads();
+
+        for (;;) {
+            // first unexpanded entry
+            uint32_t f = 0xFFFFFFFFu;
+            for (uint32_t i = lane; i < len; i += kSearchLanes)
+                if (!pexp[i]) { f = i; break; }
+            for (int off = 32; off > 0; off >>= 1) f = min(f, (uint32_t)__shfl_xor((int)f, off));
+            if (f == 0xFFFFFFFFu) break;
+            const uint32_t node = pid[f];
+            __syncthreads();
+            if (lane == 0) pexp[f] = 1;
+            ++hops;
+            const uint32_t deg = a.degrees[node];
+            uint32_t nb = 0xFFFFFFFFu;
+            if ((uint32_t)lane < deg) nb = a.edges[(uint64_t)node * a.edge_stride + lane];
+            // first occurrence within the list, then test-and-set visited
+            bool first_occ = (uint32_t)lane < deg;
+            for (int j = 0; j < 63; ++j) {
+                const uint32_t other = (uint32_t)__shfl((int)nb, j);
+                if (j < lane && other == nb) first_occ = false;
+            }
+            bool is_fresh = false;
+            if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
+                atomicOr(a.err, 1u);
+                first_occ = false;
+            }
+            if (first_occ) {
+                const uint32_t bit = 1u << (nb & 31u);
+                const uint32_t old = atomicOr(&visited[nb >> 5], bit);
+                is_fresh = (old & bit) == 0u;
+            }
+            const uint64_t fm = __ballot(is_fresh);
+            const uint32_t nf = (uint32_t)__popcll(fm);
+            if (nf == 0) { __syncthreads(); continue; }
+            const uint32_t rank = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
+            if (is_fresh) fresh[rank] = nb;
+            __syncthreads();
+            // gate every fresh neighbour
+            PaperEstimate e;
+            e.valid = 0;
+            e.lower_bound = 0.0f;
+            e.short_ip = 0.0f;
+            if ((uint32_t)lane < nf) {
+                const uint32_t id = fresh[lane];
+                const float4 fac = a.ix.factors[id];
+                const uint8_t fv = a.ix.fvalid[id];
+                const uint8_t *mrow = a.ix.msb + (uint64_t)id * (cd / 8);
+                e = a.codec == 3 ? gate_int8_one(x8, mrow, cd, fac, fv, s, a.eps0)
+                                 : gate_fp32_one(q, mrow, cd, fac, fv, s, a.eps0);
+                f_lb[lane] = e.lower_bound;
+                f_sip[lane] = e.short_ip;
+                f_valid[lane] = e.valid;
+            }
+            gates += nf;
+            // prune against the pool as it stood at the start of the hop
+            const bool full0 = len >= L;
+            const float back0 = full0 ? pd[len - 1] : 3.402823466e38f;
+            const bool survive = (uint32_t)lane < nf && !(full0 && e.valid && e.lower_bound > back0);
+            const uint64_t sm = __ballot(survive);
+            const uint32_t ns = (uint32_t)__popcll(sm);
+            pruned += nf - ns;
+            const uint32_t srank = (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
+            if (survive) surv[srank] = (uint32_t)lane;
+            __syncthreads();
+            // distances of the survivors
+            if ((uint32_t)lane < ns) {
+                const uint32_t slot = surv[lane];
+                const uint32_t id = fresh[slot];
+                const uint8_t *row = a.ix.codes + (uint64_t)id * (cd / 2);
+                const float2 h = a.ix.hdr[id];
+                float d;
+                if (!f_valid[slot]) d = asym_one(q, row, cd, h, s);
+                else if (a.codec == 3) d = rem_int8_one(x8p, row, cd, h, s, f_sip[slot]);
+                else d = rem_fp32_one(q, row, cd, h, s, f_sip[slot]);
+                s_d[lane] = d;
+            }
+            dists += ns;
+            __syncthreads();
+            // ordered insertion with the re-check against the current back
+            for (uint32_t b = 0; b < ns; ++b) {
+                const uint32_t slot = surv[b];
+                const uint32_t id = fresh[slot];
+                const float d = s_d[b];
+                const bool full = len >= L;
+                if (full && f_valid[slot] && f_lb[slot] > pd[len - 1]) { ++pruned; continue; }
+                if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
+                // position = number of entries ordered before (d, id)
+                uint32_t cnt = 0;
+                for (uint32_t i = lane; i < len; i += kSearchLanes) cnt += cand_less(pd[i], pid[i], d, id) ? 1u : 0u;
+                for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
+                const uint32_t pos = cnt;
+                const uint32_t last = full ? L - 1 : len;  // entries [pos, last) move one slot right
+                // move in descending chunks of 64: each chunk is read before it is
+                // written, and lower chunks only overwrite entries already moved
+                for (uint32_t c = 0; c + pos < last; c += kSearchLanes) {
+                    const uint32_t idx = last - 1 - c - (uint32_t)lane;
+                    const bool mv = c + (uint32_t)lane < last - pos;
+                    float md = 0.0f;
+                    uint32_t mi = 0;
+                    uint8_t me = 0;
+                    if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
+                    __syncthreads();
+                    if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
+                    __syncthreads();
+                }
+                if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
+                if (!full) ++len;
+                __syncthreads();
+            }
+        }
+
+        // rerank: first rc pool entries, FP32 long distance, residual-4
+        const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
+        const uint32_t rc = len < want ? len : want;
+        for (uint32_t i = lane; i < rc; i += kSearchLanes) {
+            const uint32_t id = pid[i];
+            float ld = pd[i];
+            if (a.codec == 3) ld = asym_one(q, a.ix.codes + (uint64_t)id * (cd / 2), cd, a.ix.hdr[id], s);
+            const Interval iv = residual_one(q, a.ix.rscale + (uint64_t)id * (cd / 16),
+                                             a.ix.rcodes + (uint64_t)id * (cd / 2), cd, s, ld);
+            r_est[i] = iv.estimate;
+            r_id[i] = id;
+        }
+        __syncthreads();
+        // top-k by (estimate, id): k rounds of a wave-wide argmin
+        const uint32_t kout = rc < a.k ? rc : a.k;
+        for (uint32_t r = 0; r < a.k; ++r) {
+            if (r >= kout) {
+                if (lane == 0) { a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu; a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf(); }
+                continue;
+            }
+            float bd = __builtin_huge_valf();
+            uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
+            for (uint32_t i = lane; i < rc; i += kSearchLanes) {
+                if (r_id[i] == 0xFFFFFFFFu) continue;
+                if (bpos == 0xFFFFFFFFu || cand_less(r_est[i], r_id[i], bd, bid)) { bd = r_est[i]; bid = r_id[i]; bpos = i; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const float od = __shfl_xor(bd, off);
+                const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+                const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
+                const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || cand_less(od, oid, bd, bid));
+                if (take) { bd = od; bid = oid; bpos = opos; }
+            }
+            if (lane == 0) {
+                a.out_ids[(uint64_t)qi * a.k + r] = bid;
+                a.out_d[(uint64_t)qi * a.k + r] = bd;
+                r_id[bpos] = 0xFFFFFFFFu;
+            }
+            __syncthreads();
+        }
+        if (lane == 0 && a.stats) {
+            uint64_t *st = a.stats + (uint64_t)qi * 4;
+            st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
+        }
+        if (a.pool_ids && lane == 0) {
+            for (uint32_t i = 0; i < L; ++i) {
+                a.pool_ids[(uint64_t)qi * L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
+                a.pool_d[(uint64_t)qi * L + i] = i < len ? pd[i] : __builtin_huge_valf();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}  // namespace qg
+
+extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream) {
+    if (a->qv.nq == 0) return 0;
+    hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
+    const size_t smem = (size_t)a->cd * 6 + (size_t)a->L * 9 + 16;
+    hipLaunchKernelGGL(qg::k_search, dim3(slots), dim3(qg::kSearchLanes), smem, stream, *a);
+    return (int)hipGetLastError();
+}

@@ -292,6 +292,43 @@ class RabitqSpace:
         return self._scan(kernel, False, index, queries, ids, qidx=qidx, n_pairs=ids.numel(),
                           eps0=eps0, short_ips=short_ips, long_in=long_in, want=want)
 
+    # ------------------------------------------------------------ search
+    def search(self, index, queries, degrees, edges, k=10, l_search=100, eps0=1.9, rerank=100,
+               slots=None, want_pool=False):
+        """Batched graph search (search_ours_paper_active_inner, ours_diskann.rs:1539-1911)
+        over a graph given as degrees [n] and edges [n, R] (R <= 64), start
+        node 0, with the query codec the queries were prepared with.
+
+        Returns (ids [nq, k] int32, estimates [nq, k], stats [nq, 4] int64 =
+        hops, gate evaluations, distances, pruned[, pool ids, pool
+        distances [nq, max(l_search, k)]])."""
+        degrees = _dev(degrees, torch.int32).reshape(-1)
+        edges = _dev(edges, torch.int32)
+        if degrees.numel() != index.n or edges.shape[0] != index.n:
+            raise ValueError("graph does not match the index size")
+        nq = queries.nq
+        L = max(l_search, k)
+        if slots is None:
+            slots = max(1, min(nq, 4096))
+        ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        stats = torch.zeros((nq, 4), dtype=torch.int64, device="cuda")
+        pool_ids = torch.empty((nq, L), dtype=torch.int32, device="cuda") if want_pool else None
+        pool_d = torch.empty((nq, L), dtype=torch.float32, device="cuda") if want_pool else None
+        L_ = _lib.lib()
+        ws_bytes = L_.qg_search_workspace_bytes(index.n, slots)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        di, dq = index.desc(), queries.desc()
+        check(L_.qg_search(self.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
+                           edges.shape[1], k, l_search, C.c_float(eps0), rerank, slots, _ptr(ids), _ptr(dists),
+                           _ptr(stats), _ptr(pool_ids), _ptr(pool_d), _ptr(ws), ws_bytes, _stream()))
+        err = ws[4:8].view(torch.int32).item()
+        if err:
+            raise _lib.QGError("graph contains an out-of-range neighbour")
+        if want_pool:
+            return ids, dists, stats, pool_ids, pool_d
+        return ids, dists, stats
+
 
 def topk(values, ids, k):
     """Per-row top-k of values [nq, P] by (value, id); returns (ids, values,

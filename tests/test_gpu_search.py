@@ -1,0 +1,75 @@
+"""Batched GPU graph search against the oracle's restatement of the
+reference search loop (ours_diskann.rs:1539-1986): identical final ids and
+estimates, identical final pools and identical hop / gate / distance /
+prune counts, for both query codecs and several list sizes, on a Vamana
+graph built by the reference bridge compiled from source."""
+import numpy as np
+import pytest
+
+from conftest import gaussian_mixture
+from oracle_bind import Oracle, Ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def graph_case():
+    n, dim = 3000, 960
+    x, q = gaussian_mixture(n, dim, 16, 0.3, 21, nq=12)
+    c = x.mean(0).astype(np.float32)
+    orc = Oracle(dim, 100)
+    orc.set_center(c)
+    comp, res = orc.split(orc.encode_full(x))
+    msb, fac = orc.sidecar(comp)
+    if Ref.available():
+        ref = Ref(dim, 100)
+        ref.set_center(c)
+        deg, edges, _, _, _ = ref.build_graph(comp, 32, 64, 1.2, beam=1, batch=512, refine=1, cap=128)
+    else:  # a connected random graph when the reference build is absent
+        rng = np.random.default_rng(0)
+        deg = np.full(n, 32, np.uint32)
+        edges = rng.integers(0, n, size=(n, 32)).astype(np.uint32)
+        edges[:, 0] = (np.arange(n) + 1) % n
+    return dict(x=x, q=q, c=c, orc=orc, comp=comp, res=res, msb=msb, fac=fac, deg=deg, edges=edges)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+@pytest.mark.parametrize("L", [16, 64, 200])
+def test_search_matches_oracle(gpu, graph_case, codec, L):
+    import quantization_graph_amd as qg
+    g = graph_case
+    sp = qg.RabitqSpace(960, 100)
+    sp.set_center(g["c"])
+    ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
+    qv = sp.prepare(gpu.from_numpy(g["q"]).cuda(), codec)
+    ids, dists, stats, pool, pool_d = sp.search(ix, qv, g["deg"].astype(np.int32), g["edges"].astype(np.int32),
+                                                k=10, l_search=L, eps0=1.9, rerank=100, want_pool=True)
+    ids, dists, stats = ids.cpu().numpy(), dists.cpu().numpy(), stats.cpu().numpy()
+    pool, pool_d = pool.cpu().numpy(), pool_d.cpu().numpy()
+    orc = g["orc"]
+    for i in range(g["q"].shape[0]):
+        oq = orc.prepare(g["q"][i], codec)
+        o_ids, o_d, o_pool, o_pd, st = orc.search(oq, codec, g["comp"], g["res"], g["msb"], g["fac"], g["deg"],
+                                                  g["edges"], k=10, L=L, eps0=1.9, rerank=100)
+        m = len(o_pool)
+        assert np.array_equal(pool[i, :m].astype(np.uint32), o_pool), "pool ids (query %d)" % i
+        assert np.array_equal(pool_d[i, :m].view(np.uint32), o_pd.view(np.uint32)), "pool distances"
+        assert np.array_equal(ids[i, :len(o_ids)].astype(np.uint32), o_ids)
+        assert np.array_equal(dists[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
+        assert tuple(stats[i]) == (st.hops, st.gate_evals, st.full_dists, st.pruned)
+        orc.free_query(oq)
+
+
+def test_search_rejects_bad_graph(gpu, graph_case):
+    import quantization_graph_amd as qg
+    g = graph_case
+    sp = qg.RabitqSpace(960, 100)
+    sp.set_center(g["c"])
+    ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
+    qv = sp.prepare(gpu.from_numpy(g["q"][:2]).cuda(), 3)
+    edges = g["edges"].astype(np.int64).copy()
+    edges[0, 0] = g["x"].shape[0] + 5  # out of range from the start node
+    with pytest.raises(qg._lib.QGError):
+        sp.search(ix, qv, g["deg"].astype(np.int32), edges.astype(np.int32), k=10, l_search=32)
+    with pytest.raises(Exception):
+        sp.search(ix, qv, g["deg"].astype(np.int32), np.zeros((g["x"].shape[0], 65), np.int32), k=10)
