@@ -180,6 +180,21 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
               uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
               void *stream);
 
+/* Vamana graph build with the symmetric 4-bit distance over the index's
+ * primary codes: VamanaIndex::buildEncodedSymmetricBulk + refineGraphSymmetric
+ * as driven by rabitq_build_vamana_graph (rabitq_bridge.cpp:224-303,
+ * vamana_index.h:308-528). Start node 0, batches of inserts from id 1,
+ * grouped back-edges, refine_passes full passes. degrees [n] and edges
+ * [n][r] are device arrays (unused edge slots 0xFFFFFFFF);
+ * *distance_evaluations receives the reference's counter. r <= 64,
+ * l_build <= 1024, beam_width <= 64. Synchronises the stream.
+ * Workspace: qg_build_workspace_bytes(n, r, l_build, batch, refine_passes). */
+size_t qg_build_workspace_bytes(uint64_t n, uint32_t r, uint32_t l_build, uint32_t batch, uint32_t refine_passes);
+int qg_build_graph(const void *space, const qg_index_desc *ix, uint32_t r, uint32_t l_build, float alpha,
+                   uint32_t beam_width, uint32_t batch, uint32_t refine_passes, uint32_t prune_cap,
+                   uint32_t early_stop_hops, uint32_t *degrees, uint32_t *edges, uint64_t *distance_evaluations,
+                   void *workspace, size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------------------ bridge API */
 /* Host-memory drop-ins for rabitq_bridge.cpp; see that file for semantics. */
 

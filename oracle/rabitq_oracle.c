@@ -988,9 +988,9 @@ static int pc_less(const void *a, const void *b) {
     if (x->d != y->d) return x->d < y->d ? -1 : 1;
     return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
 }
-size_t ro_robust_prune(const ro_space *s, const uint8_t *cbase, size_t cstride,
-                       uint32_t source, const uint32_t *ids, const float *ds,
-                       size_t n, size_t R, float alpha, uint32_t *out) {
+static size_t robust_prune_counted(const ro_space *s, const uint8_t *cbase, size_t cstride,
+                                   uint32_t source, const uint32_t *ids, const float *ds,
+                                   size_t n, size_t R, float alpha, uint32_t *out, uint64_t *evals) {
     if (R == 0) return 0;
     ro_cand *c = (ro_cand *)malloc((n + 1) * sizeof(ro_cand));
     for (size_t i = 0; i < n; ++i) { c[i].d = ds[i]; c[i].id = ids[i]; c[i].expanded = 0; }
@@ -1015,9 +1015,11 @@ size_t ro_robust_prune(const ro_space *s, const uint8_t *cbase, size_t cstride,
             int occ = 0;
             for (size_t kk = 0; kk < sel; ++kk) {
                 float *m = &memo[i * R + kk];
-                if (*m < 0.0f)
+                if (*m < 0.0f) {
                     *m = ro_symmetric_distance(s, cbase + (size_t)c[i].id * cstride,
                                                cbase + (size_t)out[kk] * cstride);
+                    if (evals) ++*evals;
+                }
                 if (cur * *m <= c[i].d) { occ = 1; break; }
             }
             if (occ) continue;
@@ -1034,4 +1036,188 @@ size_t ro_robust_prune(const ro_space *s, const uint8_t *cbase, size_t cstride,
     }
     free(c); free(dis); free(memo);
     return sel;
+}
+
+size_t ro_robust_prune(const ro_space *s, const uint8_t *cbase, size_t cstride,
+                       uint32_t source, const uint32_t *ids, const float *ds,
+                       size_t n, size_t R, float alpha, uint32_t *out, uint64_t *evals) {
+    if (evals) *evals = 0;
+    return robust_prune_counted(s, cbase, cstride, source, ids, ds, n, R, alpha, out, evals);
+}
+
+/* ------------------------------------------------------------------ build */
+/* search_vamana (vamana_search.h:97-198) with BuildEvaluator
+ * (vamana_index.h:41-69): lhs = the inserted record, rhs = graph nodes.
+ * Visited marks are epoch tags; returns the pool length or (size_t)-1 on an
+ * out-of-range neighbour. */
+static size_t build_search(const ro_space *s, const uint8_t *cbase, size_t cstride, uint32_t qid,
+                           const uint32_t *deg, const uint32_t *edges, size_t R, size_t node_count,
+                           size_t L, size_t beam, size_t early_stop, uint32_t *tags, uint32_t tag,
+                           ro_cand *pool, uint64_t *evals) {
+    if (node_count == 0 || L == 0) return 0;
+    const uint8_t *qrec = cbase + (size_t)qid * cstride;
+    size_t len = 0;
+    tags[0] = tag;
+    pool[len].d = ro_symmetric_distance(s, qrec, cbase);
+    pool[len].id = 0;
+    pool[len].expanded = 0;
+    ++len;
+    ++*evals;
+    uint32_t frontier[64];
+    size_t stall = 0;
+    for (;;) {
+        const float back_before = pool[len - 1].d;
+        size_t nfr = 0;
+        for (size_t i = 0; i < len && nfr < beam; ++i)
+            if (!pool[i].expanded) { pool[i].expanded = 1; frontier[nfr++] = pool[i].id; }
+        if (nfr == 0) break;
+        for (size_t f = 0; f < nfr; ++f) {
+            const uint32_t node = frontier[f];
+            uint32_t fresh[256];
+            size_t nf = 0;
+            for (uint32_t j = 0; j < deg[node]; ++j) {
+                const uint32_t nb = edges[(size_t)node * R + j];
+                if (nb >= node_count) return (size_t)-1;
+                if (tags[nb] != tag) { tags[nb] = tag; fresh[nf++] = nb; }
+            }
+            for (size_t j = 0; j < nf; ++j) {
+                const float d = ro_symmetric_distance(s, qrec, cbase + (size_t)fresh[j] * cstride);
+                ++*evals;
+                pool_insert(pool, &len, L, d, fresh[j]);
+            }
+        }
+        if (early_stop != 0 && len >= L) {
+            stall = pool[len - 1].d >= back_before ? stall + 1 : 0;
+            if (stall >= early_stop) break;
+        }
+    }
+    return len;
+}
+
+typedef struct { uint32_t src, dst; } be_pair;
+static int be_less(const void *a, const void *b) {
+    const be_pair *x = (const be_pair *)a, *y = (const be_pair *)b;
+    if (x->src != y->src) return x->src < y->src ? -1 : 1;
+    return x->dst < y->dst ? -1 : (x->dst > y->dst ? 1 : 0);
+}
+
+/* addBackedgesGroupedAndPrune + addBackedgeGroupAndPrune
+ * (vamana_index.h:995-1117). */
+static void add_backedges(const ro_space *s, const uint8_t *cbase, size_t cstride, be_pair *pairs, size_t np,
+                          uint32_t *deg, uint32_t *edges, size_t R, float alpha, uint64_t *evals) {
+    if (np == 0) return;
+    qsort(pairs, np, sizeof(be_pair), be_less);
+    uint32_t *nb = (uint32_t *)malloc((R + np + 1) * sizeof(uint32_t));
+    float *nd = (float *)malloc((R + np + 1) * sizeof(float));
+    for (size_t b = 0; b < np;) {
+        size_t e = b;
+        const uint32_t src = pairs[b].src;
+        while (e < np && pairs[e].src == src) ++e;
+        size_t len = deg[src];
+        memcpy(nb, edges + (size_t)src * R, len * sizeof(uint32_t));
+        size_t added = 0;
+        for (size_t i = b; i < e; ++i) {
+            const uint32_t t = pairs[i].dst;
+            if (t == src || (i > b && pairs[i - 1].dst == t)) continue;
+            int present = 0;
+            for (size_t j = 0; j < len; ++j) if (nb[j] == t) { present = 1; break; }
+            if (!present) { nb[len++] = t; ++added; }
+        }
+        if (added != 0) {
+            if (len <= R) {
+                memcpy(edges + (size_t)src * R, nb, len * sizeof(uint32_t));
+                deg[src] = (uint32_t)len;
+            } else {
+                size_t m = 0;
+                for (size_t j = 0; j < len; ++j) if (nb[j] != src) nb[m++] = nb[j];
+                for (size_t j = 0; j < m; ++j)
+                    nd[j] = ro_symmetric_distance(s, cbase + (size_t)src * cstride, cbase + (size_t)nb[j] * cstride);
+                *evals += m;
+                uint32_t out[256];
+                const size_t got = robust_prune_counted(s, cbase, cstride, src, nb, nd, m, R, alpha, out, evals);
+                memcpy(edges + (size_t)src * R, out, got * sizeof(uint32_t));
+                deg[src] = (uint32_t)got;
+            }
+        }
+        b = e;
+    }
+    free(nb); free(nd);
+}
+
+static int cap_less(const void *a, const void *b) { return pc_less(a, b); }
+
+/* VamanaIndex::buildEncodedSymmetricBulk + refineGraphSymmetric
+ * (vamana_index.h:308-422, 452-528), driven as rabitq_build_vamana_graph
+ * does (rabitq_bridge.cpp:224-303). Start node 0; batches of inserts from
+ * id 1 search the graph as it stood when the batch began. */
+int ro_build_vamana(const ro_space *s, const uint8_t *cbase, size_t cstride, size_t n, size_t R, size_t L,
+                    float alpha, size_t beam, size_t batch, size_t refine, size_t cap, size_t early_stop,
+                    uint32_t *deg, uint32_t *edges, uint64_t *evals_out) {
+    if (R == 0 || R > 256 || beam == 0 || beam > 64 || !(alpha >= 1.0f) || !isfinite(alpha)) return -1;
+    if (batch == 0) batch = 1;
+    uint64_t evals = 0;
+    memset(deg, 0, n * sizeof(uint32_t));
+    uint32_t *tags = (uint32_t *)calloc(n + 1, sizeof(uint32_t));
+    uint32_t tag = 0;
+    ro_cand *pool = (ro_cand *)malloc((L + 1) * sizeof(ro_cand));
+    uint32_t *pid = (uint32_t *)malloc((L + R + 1) * sizeof(uint32_t));
+    float *pd = (float *)malloc((L + R + 1) * sizeof(float));
+    uint32_t *nbr = (uint32_t *)malloc((batch * R + 1) * sizeof(uint32_t));
+    uint32_t *cnt = (uint32_t *)malloc((batch + 1) * sizeof(uint32_t));
+    be_pair *pairs = (be_pair *)malloc((n * R + 1) * sizeof(be_pair));
+    int rc = 0;
+    for (size_t begin = 1; begin < n && rc == 0; begin += batch) {
+        const size_t end = begin + batch < n ? begin + batch : n;
+        for (size_t id = begin; id < end; ++id) {
+            size_t len = build_search(s, cbase, cstride, (uint32_t)id, deg, edges, R, begin, L, beam, early_stop,
+                                      tags, ++tag, pool, &evals);
+            if (len == (size_t)-1) { rc = -2; break; }
+            if (cap != 0 && len > cap) { qsort(pool, len, sizeof(ro_cand), cap_less); len = cap; }
+            for (size_t i = 0; i < len; ++i) { pid[i] = pool[i].id; pd[i] = pool[i].d; }
+            cnt[id - begin] = (uint32_t)robust_prune_counted(s, cbase, cstride, (uint32_t)id, pid, pd, len, R, alpha,
+                                                             nbr + (id - begin) * R, &evals);
+        }
+        size_t np = 0;
+        for (size_t id = begin; id < end && rc == 0; ++id) {
+            const uint32_t *row = nbr + (id - begin) * R;
+            memcpy(edges + id * R, row, cnt[id - begin] * sizeof(uint32_t));
+            deg[id] = cnt[id - begin];
+            for (uint32_t j = 0; j < cnt[id - begin]; ++j) { pairs[np].src = row[j]; pairs[np].dst = (uint32_t)id; ++np; }
+        }
+        if (rc == 0) add_backedges(s, cbase, cstride, pairs, np, deg, edges, R, alpha, &evals);
+    }
+    if (refine != 0 && n > 1 && rc == 0) {
+        uint32_t *rdeg = (uint32_t *)malloc(n * sizeof(uint32_t));
+        uint32_t *redges = (uint32_t *)malloc((n * R + 1) * sizeof(uint32_t));
+        for (size_t pass = 0; pass < refine && rc == 0; ++pass) {
+            for (size_t id = 0; id < n; ++id) {
+                size_t len = build_search(s, cbase, cstride, (uint32_t)id, deg, edges, R, n, L, beam, early_stop,
+                                          tags, ++tag, pool, &evals);
+                if (len == (size_t)-1) { rc = -2; break; }
+                if (cap != 0 && len > cap) { qsort(pool, len, sizeof(ro_cand), cap_less); len = cap; }
+                for (size_t i = 0; i < len; ++i) { pid[i] = pool[i].id; pd[i] = pool[i].d; }
+                const uint8_t *rec = cbase + id * cstride;
+                for (uint32_t j = 0; j < deg[id]; ++j) {
+                    const uint32_t e = edges[id * R + j];
+                    pid[len] = e;
+                    pd[len] = ro_symmetric_distance(s, rec, cbase + (size_t)e * cstride);
+                    ++evals;
+                    ++len;
+                }
+                rdeg[id] = (uint32_t)robust_prune_counted(s, cbase, cstride, (uint32_t)id, pid, pd, len, R, alpha,
+                                                          redges + id * R, &evals);
+            }
+            if (rc != 0) break;
+            memcpy(deg, rdeg, n * sizeof(uint32_t));
+            memcpy(edges, redges, n * R * sizeof(uint32_t));
+            size_t np = 0;
+            for (size_t id = 0; id < n; ++id)
+                for (uint32_t j = 0; j < deg[id]; ++j) { pairs[np].src = edges[id * R + j]; pairs[np].dst = (uint32_t)id; ++np; }
+            add_backedges(s, cbase, cstride, pairs, np, deg, edges, R, alpha, &evals);
+        }
+        free(rdeg); free(redges);
+    }
+    free(tags); free(pool); free(pid); free(pd); free(nbr); free(cnt); free(pairs);
+    if (evals_out) *evals_out = evals;
+    return rc;
 }

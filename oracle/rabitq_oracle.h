@@ -170,10 +170,19 @@ size_t ro_search(const ro_space *s, const ro_query *q, int codec,
                  uint32_t *pool_ids_out, float *pool_dists_out, size_t *pool_len_out,
                  ro_search_stats *stats);
 
-/* RobustPrune over a candidate pool (vamana_prune.h:41-156). */
+/* RobustPrune over a candidate pool (vamana_prune.h:41-156); *evals (may
+ * be NULL) receives the distance_between calls it makes. */
 size_t ro_robust_prune(const ro_space *s, const uint8_t *compact_base, size_t compact_stride,
                        uint32_t source, const uint32_t *pool_ids, const float *pool_dists,
-                       size_t pool_len, size_t r, float alpha, uint32_t *out);
+                       size_t pool_len, size_t r, float alpha, uint32_t *out, uint64_t *evals);
+
+/* Vamana build with the symmetric distance (rabitq_build_vamana_graph,
+ * rabitq_bridge.cpp:224-303): batched inserts, grouped back-edges, refine
+ * passes. edges has stride r; distance_evaluations as the reference counts
+ * them. Returns 0, -1 on bad parameters, -2 on an out-of-range neighbour. */
+int ro_build_vamana(const ro_space *s, const uint8_t *compact_base, size_t compact_stride, size_t n, size_t r,
+                    size_t l_build, float alpha, size_t beam, size_t batch, size_t refine, size_t cap,
+                    size_t early_stop, uint32_t *degrees, uint32_t *edges, uint64_t *distance_evaluations);
 
 #ifdef __cplusplus
 }

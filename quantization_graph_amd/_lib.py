@@ -67,6 +67,10 @@ def lib():
     L.qg_search_workspace_bytes.argtypes = [u64, u32]
     L.qg_search.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, f32,
                             u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.qg_build_workspace_bytes.restype = sz
+    L.qg_build_workspace_bytes.argtypes = [u64, u32, u32, u32, u32]
+    L.qg_build_graph.argtypes = [vp, C.POINTER(IndexDesc), u32, u32, f32, u32, u32, u32, u32, u32, vp, vp,
+                                 C.POINTER(C.c_uint64), vp, sz, vp]
     _lib = L
     return L
 

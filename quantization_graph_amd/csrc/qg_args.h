@@ -121,4 +121,79 @@ struct SearchArgs {
     float *pool_d;             // [nq][L]
 };
 
+// Vamana build (qg_build.hip)
+struct BuildSearchArgs {
+    const uint8_t *codes;      // [n][cd/2]
+    const float2 *hdr;         // [n]
+    uint32_t cd;
+    const uint32_t *degrees;   // [n]
+    const uint32_t *edges;     // [n][R]
+    uint32_t R;
+    uint64_t node_count;       // neighbours must be < node_count
+    uint32_t L, beam, early_stop;
+    uint32_t jobs;
+    uint64_t qid_base;         // job j searches for record qid_base + j
+    uint32_t *tags;            // [slots][tag_stride] visited epochs
+    uint64_t tag_stride;
+    uint32_t tag_base;         // job j marks with tag_base + j + 1
+    uint32_t *counter;         // work-queue head
+    uint32_t *err;             // bit 0: out-of-range neighbour
+    unsigned long long *evals; // distance evaluations
+    float *pool_d;             // [jobs][L]
+    uint32_t *pool_id;         // [jobs][L]
+    uint32_t *pool_len;        // [jobs]
+};
+
+struct PruneArgs {
+    const uint8_t *codes;
+    const float2 *hdr;
+    uint32_t cd;
+    uint32_t jobs;
+    uint64_t src_base;         // source of job j: sources[j] or src_base + j
+    const uint32_t *sources;
+    // part A: a sorted search pool, first min(len, cap) entries
+    const float *pool_d;
+    const uint32_t *pool_id;
+    const uint32_t *pool_len;
+    uint64_t pool_stride;
+    uint32_t cap;
+    // part B: ids whose distance from the source is computed here, either a
+    // graph row of the source (graph_deg != null) or a list (extra)
+    const uint32_t *graph_deg;
+    const uint32_t *graph_edges;
+    const uint32_t *extra;
+    const uint64_t *extra_off;
+    const uint32_t *extra_len;
+    uint32_t R;
+    float alpha;
+    // per-candidate state for pools above the LDS capacity (part B lists only),
+    // indexed like extra
+    unsigned long long *big_keys;
+    float *big_min;
+    float *big_pre;
+    uint32_t *big_state;
+    // output rows: out_edges[src][R], out_deg[src]
+    uint32_t *out_edges;
+    uint32_t *out_deg;
+    uint32_t *err;             // bit 1: pool above LDS capacity without state storage
+    unsigned long long *evals;
+};
+
+struct MergeArgs {
+    const uint32_t *group_src;   // [ngroups] source of each back-edge group
+    const uint32_t *group_begin; // [ngroups] offset in the sorted targets
+    const uint32_t *group_len;   // [ngroups]
+    const uint32_t *targets;     // sorted (source, target) pairs, target half
+    uint32_t ngroups;
+    uint32_t R;
+    uint32_t *degrees;           // graph, updated in place for groups that fit
+    uint32_t *edges;
+    const uint64_t *merged_off;  // [ngroups]
+    uint32_t *merged;            // merged neighbour lists of overflowing groups
+    uint32_t *job_src;           // prune jobs (overflowing groups)
+    uint64_t *job_off;
+    uint32_t *job_len;
+    uint32_t *job_count;
+};
+
 }  // namespace qg

@@ -1,0 +1,847 @@
+// qg_build.hip -- Vamana graph construction on the GPU with the symmetric
+// 4-bit distance (rabitq_build_vamana_graph, rabitq_bridge.cpp:224-303 ->
+// VamanaIndex::buildEncodedSymmetricBulk / refineGraphSymmetric,
+// Ours/core/hnswlib/vamana_index.h:308-528, 995-1117).
+//
+// The reference build is deterministic by construction: each batch of
+// inserts searches the graph as it stood when the batch began, prunes
+// privately, commits, and then adds back-edges grouped by source so that no
+// two groups touch the same adjacency row. The GPU build keeps that
+// structure, one kernel per stage:
+//   k_build_search   one wavefront per inserted id: search_vamana
+//                    (vamana_search.h:97-198) with the BuildEvaluator
+//                    distance (lhs = inserted record), node_count limit,
+//                    beam width, early stop. Visited marks are per-slot
+//                    epoch tags, so nothing is cleared between jobs.
+//   k_prune          one workgroup per job: robust_prune_4bit
+//                    (vamana_prune.h:41-156). The pool is sorted by
+//                    (distance, id) in LDS, de-duplicated and swept level by
+//                    level. Every round evaluates, for all open candidates
+//                    at or after the scan position in parallel, the lazy
+//                    memoised occlusion test of the reference (per candidate:
+//                    count of memoised pair distances and their minimum;
+//                    float rounding of alpha * x is monotone in x, so
+//                    alpha * min <= d is the same "some selected node
+//                    occludes" predicate); the first open candidate that is
+//                    not occluded is selected. Candidates are committed as
+//                    visited exactly when the reference's sequential sweep
+//                    reaches them, which reproduces its distance_evaluations.
+//   back-edges       (neighbour << 32 | id) keys radix-sorted, run-length
+//                    encoded into groups per source, merged one wavefront per
+//                    group (addBackedgeGroupAndPrune, vamana_index.h:1047-1117);
+//                    groups whose list exceeds R become prune jobs.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+#include "qg_args.h"
+#include "qg_device.h"
+#include "qg_space.h"
+
+namespace qg {
+
+int fail(const char *fmt, ...);
+int check_hip(int rc, const char *what);
+
+constexpr int kSearchLanes = 64;
+constexpr int kPruneThreads = 256;
+constexpr uint32_t kPruneLds = 2048;      // candidates whose state fits in LDS
+constexpr uint32_t kMaxCodeDwords = 512;  // centred code of cd <= 2048
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// prune state word: byte 0 = memoised pair distances, byte 1 = the count at
+// the last visit, byte 2 = index of the first negative memoised distance
+// (0xFF: none), byte 3 = flags
+constexpr uint32_t kSelected = 1u << 24;
+constexpr uint32_t kKeep = 1u << 25;
+
+__device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
+    // candidate_less / prune_candidate_less: distance, then id
+    return ad < bd || (ad == bd && aid < bid);
+}
+
+__device__ __forceinline__ unsigned long long prune_key(float d, uint32_t id) {
+    return ((unsigned long long)float_order_key(d) << 32) | id;
+}
+
+__device__ __forceinline__ void stage_centered(uint32_t *dst, const uint8_t *row, uint32_t cd, int tid, int nthreads) {
+    const uint32_t *w = (const uint32_t *)row;
+    for (uint32_t d = tid; d < cd / 8; d += nthreads) {
+        const uint32_t v = w[d];
+        dst[2 * d] = centered_bytes(v & 0x0F0F0F0Fu);
+        dst[2 * d + 1] = centered_bytes((v >> 4) & 0x0F0F0F0Fu);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------- search --
+__global__ void __launch_bounds__(kSearchLanes) k_build_search(BuildSearchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t cd = a.cd, L = a.L;
+    uint32_t *lc = (uint32_t *)smem;                                // cd/4 dwords
+    float *pd = (float *)(smem + cd);                               // [L]
+    uint32_t *pid = (uint32_t *)(smem + cd + 4 * (size_t)L);       // [L]
+    uint8_t *pexp = smem + cd + 8 * (size_t)L;                      // [L]
+    __shared__ uint32_t fresh[kSearchLanes];
+    __shared__ float fdist[kSearchLanes];
+    __shared__ uint32_t frontier[kSearchLanes];
+    const int lane = threadIdx.x;
+    uint32_t *tags = a.tags + (uint64_t)blockIdx.x * a.tag_stride;
+    for (;;) {
+        uint32_t job = 0;
+        if (lane == 0) job = atomicAdd(a.counter, 1u);
+        job = (uint32_t)__shfl((int)job, 0);
+        if (job >= a.jobs) break;
+        const uint32_t qid = (uint32_t)(a.qid_base + job);
+        const uint32_t tag = a.tag_base + job + 1u;
+        stage_centered(lc, a.codes + (uint64_t)qid * (cd / 2), cd, lane, kSearchLanes);
+        const float2 qh = a.hdr[qid];
+        __syncthreads();
+        unsigned long long evals = 1;
+        if (lane == 0) {
+            tags[0] = tag;
+            pd[0] = symmetric_finish(qh, a.hdr[0], symmetric_ip4_prepared(lc, a.codes, cd));
+            pid[0] = 0;
+            pexp[0] = 0;
+        }
+        uint32_t len = 1, stall = 0;
+        bool bad = false;
+        __syncthreads();
+        for (;;) {
+            const float back_before = pd[len - 1];
+            // frontier: the first `beam` unexpanded entries in pool order
+            uint32_t nfr = 0;
+            for (uint32_t base = 0; base < len && nfr < a.beam; base += kSearchLanes) {
+                const uint32_t i = base + lane;
+                const bool un = i < len && pexp[i] == 0;
+                const uint64_t m = __ballot(un);
+                const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                const uint32_t room = a.beam - nfr;
+                if (un && r < room) {
+                    frontier[nfr + r] = pid[i];
+                    pexp[i] = 1;
+                }
+                const uint32_t cnt = (uint32_t)__popcll(m);
+                nfr += cnt < room ? cnt : room;
+            }
+            __syncthreads();
+            if (nfr == 0) break;
+            for (uint32_t f = 0; f < nfr; ++f) {
+                const uint32_t node = frontier[f];
+                const uint32_t deg = a.degrees[node];
+                const bool in = (uint32_t)lane < deg;
+                const uint32_t nb = in ? a.edges[(uint64_t)node * a.R + lane] : kNone;
+                if (__ballot(in && nb >= a.node_count)) {  // vamana_search.h:157-159
+                    bad = true;
+                    break;
+                }
+                // try_mark in adjacency order: the first occurrence of an id
+                bool first_occ = in;
+                for (int j = 0; j < kSearchLanes - 1; ++j) {
+                    const uint32_t o = (uint32_t)__shfl((int)nb, j);
+                    if (j < lane && o == nb) first_occ = false;
+                }
+                bool is_fresh = false;
+                if (first_occ) {
+                    is_fresh = tags[nb] != tag;
+                    if (is_fresh) tags[nb] = tag;
+                }
+                const uint64_t fm = __ballot(is_fresh);
+                const uint32_t nf = (uint32_t)__popcll(fm);
+                if (is_fresh) {
+                    const uint32_t rank = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
+                    fresh[rank] = nb;
+                    fdist[rank] = symmetric_finish(qh, a.hdr[nb],
+                                                   symmetric_ip4_prepared(lc, a.codes + (uint64_t)nb * (cd / 2), cd));
+                }
+                evals += nf;
+                __syncthreads();
+                for (uint32_t b = 0; b < nf; ++b) {  // insert_candidate_sorted, in fresh order
+                    const uint32_t id = fresh[b];
+                    const float d = fdist[b];
+                    const bool full = len >= L;
+                    if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
+                    uint32_t cnt = 0;
+                    for (uint32_t i = lane; i < len; i += kSearchLanes) cnt += cand_less(pd[i], pid[i], d, id) ? 1u : 0u;
+                    for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
+                    const uint32_t pos = cnt;
+                    const uint32_t last = full ? L - 1 : len;  // [pos, last) moves up by one
+                    for (uint32_t c = 0; pos + c < last; c += kSearchLanes) {
+                        const bool mv = c + (uint32_t)lane < last - pos;
+                        const uint32_t idx = last - 1 - c - (uint32_t)lane;
+                        float md = 0.0f;
+                        uint32_t mi = 0;
+                        uint8_t me = 0;
+                        if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
+                        __syncthreads();
+                        if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
+                        __syncthreads();
+                    }
+                    if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
+                    if (!full) ++len;
+                    __syncthreads();
+                }
+            }
+            if (bad) break;
+            if (a.early_stop != 0 && len >= L) {
+                const float back_after = pd[len - 1];
+                stall = (back_after >= back_before) ? stall + 1 : 0;
+                if (stall >= a.early_stop) break;
+            }
+        }
+        for (uint32_t i = lane; i < len; i += kSearchLanes) {
+            a.pool_d[(uint64_t)job * L + i] = pd[i];
+            a.pool_id[(uint64_t)job * L + i] = pid[i];
+        }
+        if (lane == 0) {
+            a.pool_len[job] = bad ? 0u : len;
+            if (bad) atomicOr(a.err, 1u);
+            atomicAdd(a.evals, evals);
+        }
+        __syncthreads();
+    }
+}
+
+// -------------------------------------------------------------- prune --
+__device__ __forceinline__ uint32_t block_min_u32(uint32_t v, uint32_t *red) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = red[0];
+    for (int w = 1; w < kPruneThreads / 64; ++w) r = min(r, red[w]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long r = 0;
+    for (int w = 0; w < kPruneThreads / 64; ++w) r += red[w];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kPruneThreads) k_prune(PruneArgs a) {
+    __shared__ unsigned long long s_keys[kPruneLds];
+    __shared__ float s_min[kPruneLds];
+    __shared__ float s_pre[kPruneLds];
+    __shared__ uint32_t s_state[kPruneLds];
+    __shared__ uint32_t s_src[kMaxCodeDwords];
+    __shared__ uint32_t s_sel[64];
+    __shared__ uint32_t s_red[kPruneThreads / 64];
+    __shared__ unsigned long long s_red64[kPruneThreads / 64];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wv = t >> 6;
+    const uint32_t job = blockIdx.x;
+    const uint32_t cd = a.cd, R = a.R;
+    const uint32_t src = a.sources ? a.sources[job] : (uint32_t)(a.src_base + job);
+    uint32_t na = 0;
+    if (a.pool_len) {
+        na = a.pool_len[job];
+        if (a.cap != 0 && na > a.cap) na = a.cap;  // capBuildPrunePool on a sorted pool
+    }
+    const uint32_t *ex = nullptr;
+    uint32_t nb = 0;
+    uint64_t off = 0;
+    if (a.graph_deg) {
+        nb = a.graph_deg[src];
+        ex = a.graph_edges + (uint64_t)src * R;
+    } else if (a.extra) {
+        off = a.extra_off[job];
+        nb = a.extra_len[job];
+        ex = a.extra + off;
+    }
+    const uint32_t n = na + nb;
+    const bool big = n > kPruneLds;
+    if (big && (na != 0 || a.extra == nullptr || a.big_keys == nullptr)) {
+        if (t == 0) {
+            atomicOr(a.err, 2u);
+            a.out_deg[src] = 0;
+        }
+        return;
+    }
+    unsigned long long *keys = big ? a.big_keys + off : s_keys;
+    float *mind = big ? a.big_min + off : s_min;
+    float *pre = big ? a.big_pre + off : s_pre;
+    uint32_t *state = big ? a.big_state + off : s_state;
+
+    // pool: search entries as given, listed ids with their distance from
+    // the source (BuildEvaluator::distance_batch / distance_batch_from: lhs
+    // = source)
+    if (nb) stage_centered(s_src, a.codes + (uint64_t)src * (cd / 2), cd, t, kPruneThreads);
+    __syncthreads();
+    const float2 sh = a.hdr[src];
+    unsigned long long ev = 0;
+    for (uint32_t i = t; i < n; i += kPruneThreads) {
+        if (i < na) {
+            keys[i] = prune_key(a.pool_d[(uint64_t)job * a.pool_stride + i], a.pool_id[(uint64_t)job * a.pool_stride + i]);
+        } else {
+            const uint32_t id = ex[i - na];
+            const float d = symmetric_finish(sh, a.hdr[id], symmetric_ip4_prepared(s_src, a.codes + (uint64_t)id * (cd / 2), cd));
+            keys[i] = prune_key(d, id);
+            ++ev;
+        }
+    }
+    __syncthreads();
+    // bitonic sort, ascending, over the next power of two; entries past n are
+    // +inf and never move, so pairs with the upper index >= n are skipped
+    uint32_t np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        const uint32_t half = size >> 1;
+        for (uint32_t b = t; b < np2 / 2; b += kPruneThreads) {
+            const uint32_t blk = b / half, r = b % half;
+            const uint32_t lo = blk * size + r, hi = blk * size + size - 1 - r;
+            if (hi < n) {
+                const unsigned long long k0 = keys[lo], k1 = keys[hi];
+                if (k0 > k1) { keys[lo] = k1; keys[hi] = k0; }
+            }
+        }
+        __syncthreads();
+        for (uint32_t stride = half >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t b = t; b < np2 / 2; b += kPruneThreads) {
+                const uint32_t lo = 2 * stride * (b / stride) + (b % stride), hi = lo + stride;
+                if (hi < n) {
+                    const unsigned long long k0 = keys[lo], k1 = keys[hi];
+                    if (k0 > k1) { keys[lo] = k1; keys[hi] = k0; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // drop the source and repeated ids. A repeated id carries the same
+    // distance (one deterministic function of the pair), so its copies are
+    // adjacent after the sort and the first is the one kept.
+    for (uint32_t i = t; i < n; i += kPruneThreads) {
+        const unsigned long long k = keys[i];
+        const bool keep = (uint32_t)k != src && !(i > 0 && keys[i - 1] == k);
+        state[i] = keep ? kKeep : 0u;
+    }
+    __syncthreads();
+    uint32_t u = 0;
+    for (uint32_t base = 0; base < n; base += kPruneThreads) {
+        const uint32_t i = base + t;
+        const bool keep = i < n && (state[i] & kKeep);
+        const unsigned long long k = keep ? keys[i] : 0ull;
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) s_red[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < kPruneThreads / 64; ++w) {
+            before += (w < wv) ? s_red[w] : 0u;
+            total += s_red[w];
+        }
+        if (keep) keys[u + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = k;
+        u += total;
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < u; i += kPruneThreads) {
+        mind[i] = __builtin_nanf("");
+        pre[i] = __builtin_nanf("");
+        state[i] = 0x00FF0000u;
+    }
+    __syncthreads();
+
+    const float alpha = a.alpha;
+    const float inc = (alpha < 1.2f) ? alpha : 1.2f;  // std::min(alpha, 1.2f)
+    float cur = 1.0f;
+    uint32_t nsel = 0;
+    while (nsel < R) {
+        bool promoted = false;
+        uint32_t start = 0;
+        while (nsel < R) {
+            // lazy occlusion test of every open candidate at or after start
+            uint32_t first = kNone;
+            for (uint32_t j = start + t; j < u; j += kPruneThreads) {
+                const uint32_t st = state[j];
+                if (st & kSelected) continue;
+                const unsigned long long k = keys[j];
+                const float D = float_from_order_key((uint32_t)(k >> 32));
+                float m = mind[j];
+                uint32_t c = st & 0xFFu;
+                bool occ = c > 0 && cur * m <= D;
+                if (!occ && c < nsel) {
+                    const uint32_t cj = (uint32_t)k;
+                    const uint8_t *crow = a.codes + (uint64_t)cj * (cd / 2);
+                    const float2 ch = a.hdr[cj];
+                    uint32_t g = (st >> 16) & 0xFFu;
+                    float p = pre[j];
+                    do {
+                        // distance_between(candidate, selected): lhs = candidate
+                        const uint32_t sid = s_sel[c];
+                        const float d = symmetric_finish(
+                            ch, a.hdr[sid], symmetric_ip4_rows(crow, a.codes + (uint64_t)sid * (cd / 2), cd));
+                        if (d < 0.0f && g == 0xFFu) { g = c; p = m; }
+                        m = (m != m || d < m) ? d : m;
+                        ++c;
+                        occ = cur * d <= D;
+                    } while (!occ && c < nsel);
+                    mind[j] = m;
+                    pre[j] = p;
+                    state[j] = (st & 0xFF00FF00u) | (g << 16) | c;
+                }
+                if (!occ) { first = j; break; }
+            }
+            first = block_min_u32(first, s_red);
+            // the sequential sweep visits every open candidate in
+            // [start, first] (all of [start, u) when none is selectable)
+            const uint32_t vend = (first == kNone) ? u : first + 1;
+            for (uint32_t j = start + t; j < vend; j += kPruneThreads) {
+                uint32_t st = state[j];
+                if (st & kSelected) continue;
+                const uint32_t c = st & 0xFFu, com = (st >> 8) & 0xFFu, g = (st >> 16) & 0xFFu;
+                ev += c - com;
+                if (g != 0xFFu && g < com) {
+                    // a memoised negative distance (cached < 0) is computed
+                    // again when the sweep reaches it
+                    const float D = float_from_order_key((uint32_t)(keys[j] >> 32));
+                    if (!(cur * pre[j] <= D)) ++ev;
+                }
+                st = (st & ~0x0000FF00u) | (c << 8);
+                if (j == first) st |= kSelected;
+                state[j] = st;
+            }
+            if (first == kNone) break;
+            if (t == 0) s_sel[nsel] = (uint32_t)keys[first];
+            __syncthreads();
+            ++nsel;
+            start = first + 1;
+            promoted = true;
+        }
+        if (cur == alpha) break;
+        const float nx = cur * inc;
+        const float next = (nx < alpha) ? nx : alpha;  // std::min(alpha, current_alpha * increment)
+        if (next == cur) break;
+        cur = next;
+        if (!promoted && nsel >= R) break;
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < R; i += kPruneThreads) a.out_edges[(uint64_t)src * R + i] = i < nsel ? s_sel[i] : kNone;
+    if (t == 0) a.out_deg[src] = nsel;
+    ev = block_sum_u64(ev, s_red64);
+    if (t == 0 && ev) atomicAdd(a.evals, ev);
+}
+
+// ---------------------------------------------------------- back-edges --
+// (neighbour << 32 | row) for rows [row_base, row_base + rows); empty slots
+// are ~0 and sort last
+__global__ void k_edge_keys(const uint32_t *deg, const uint32_t *edges, uint32_t R, uint64_t row_base, uint64_t rows,
+                            unsigned long long *keys) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows * R) return;
+    const uint64_t r = row_base + g / R;
+    const uint32_t i = (uint32_t)(g % R);
+    keys[g] = i < deg[r] ? (((unsigned long long)edges[r * R + i] << 32) | r) : ~0ull;
+}
+
+__global__ void k_split_keys(const unsigned long long *keys, uint64_t n, uint32_t *src, uint32_t *tgt) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = keys[i];
+    src[i] = (uint32_t)(k >> 32);
+    tgt[i] = (uint32_t)k;
+}
+
+__global__ void k_merged_sizes(const uint32_t *group_src, const uint32_t *group_len, const uint32_t *ngroups,
+                               const uint32_t *deg, uint64_t *sizes, uint64_t cap) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= cap) return;
+    uint64_t s = 0;
+    if (g < *ngroups && group_src[g] != kNone) s = (uint64_t)deg[group_src[g]] + group_len[g];
+    sizes[g] = s;
+}
+
+// addBackedgeGroupAndPrune, merge half: the current list, then each sorted
+// unique target that is neither the source nor already present
+__global__ void __launch_bounds__(64) k_backedge_merge(MergeArgs a) {
+    const uint32_t g = blockIdx.x;
+    if (g >= a.ngroups) return;
+    const uint32_t src = a.group_src[g];
+    if (src == kNone) return;
+    const int lane = threadIdx.x;
+    const uint32_t begin = a.group_begin[g], glen = a.group_len[g];
+    const uint32_t deg = a.degrees[src];
+    const uint32_t *cur = a.edges + (uint64_t)src * a.R;
+    uint32_t *out = a.merged + a.merged_off[g];
+    const uint32_t mine = (uint32_t)lane < deg ? cur[lane] : kNone;
+    if ((uint32_t)lane < deg) out[lane] = mine;
+    uint32_t total = deg;
+    for (uint32_t base = begin; base < begin + glen; base += 64) {
+        const uint32_t i = base + lane;
+        bool take = false;
+        uint32_t tg = kNone;
+        if (i < begin + glen) {
+            tg = a.targets[i];
+            take = tg != src && (i == begin || a.targets[i - 1] != tg);
+        }
+        for (int j = 0; j < 64; ++j) {
+            const uint32_t o = (uint32_t)__shfl((int)mine, j);
+            if ((uint32_t)j < deg && o == tg) take = false;
+        }
+        const uint64_t m = __ballot(take);
+        if (take) out[total + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = tg;
+        total += (uint32_t)__popcll(m);
+    }
+    if (total == deg) return;  // nothing added
+    __syncthreads();
+    if (total <= a.R) {
+        for (uint32_t i = lane; i < total; i += 64) a.edges[(uint64_t)src * a.R + i] = out[i];
+        if (lane == 0) a.degrees[src] = total;
+    } else if (lane == 0) {
+        const uint32_t j = atomicAdd(a.job_count, 1u);
+        a.job_src[j] = src;
+        a.job_off[j] = a.merged_off[g];
+        a.job_len[j] = total;
+    }
+}
+
+// ------------------------------------------------------------ host side --
+struct BuildLayout {
+    uint64_t n, R, L, J, P, M, slots;
+    size_t cub_bytes;
+    size_t off_ctl, off_tags, off_pool_d, off_pool_id, off_pool_len, off_rdeg, off_redges, off_keys_a, off_keys_b,
+        off_src, off_tgt, off_gsrc, off_glen, off_gbeg, off_msize, off_moff, off_merged, off_bkeys, off_bmin,
+        off_bpre, off_bstate, off_jsrc, off_joff, off_jlen, off_cub, total;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int cub_temp_bytes(uint64_t P, size_t *out) {
+    size_t a = 0, b = 0, c = 0, d = 0;
+    int rc = (int)hipcub::DeviceRadixSort::SortKeys(nullptr, a, (const unsigned long long *)nullptr,
+                                                    (unsigned long long *)nullptr, (int)P, 0, 64);
+    if (rc) return rc;
+    rc = (int)hipcub::DeviceRunLengthEncode::Encode(nullptr, b, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                    (uint32_t *)nullptr, (uint32_t *)nullptr, (int)P);
+    if (rc) return rc;
+    rc = (int)hipcub::DeviceScan::ExclusiveSum(nullptr, c, (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)P);
+    if (rc) return rc;
+    rc = (int)hipcub::DeviceScan::ExclusiveSum(nullptr, d, (const uint64_t *)nullptr, (uint64_t *)nullptr, (int)P);
+    if (rc) return rc;
+    size_t m = a;
+    if (b > m) m = b;
+    if (c > m) m = c;
+    if (d > m) m = d;
+    *out = m;
+    return 0;
+}
+
+static int build_layout(uint64_t n, uint32_t R, uint32_t L, uint32_t batch, uint32_t refine, BuildLayout *l) {
+    l->n = n;
+    l->R = R;
+    l->L = L;
+    const uint64_t jb = batch < n ? batch : n;
+    l->J = refine ? (n < 65536 ? n : (jb > 65536 ? jb : 65536)) : jb;
+    if (l->J == 0) l->J = 1;
+    const uint64_t rows = refine ? n : jb;  // rows whose edges feed one back-edge round
+    l->P = rows * R;
+    if (l->P == 0) l->P = 1;
+    l->M = (l->P < n ? l->P : n) * R + l->P;  // groups <= min(P, n), each grows by at most R
+    l->slots = l->J < 2048 ? l->J : 2048;
+    int rc = cub_temp_bytes(l->P, &l->cub_bytes);
+    if (rc) return rc;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { const size_t at = o; o = align256(o + bytes); return at; };
+    l->off_ctl = take(256);
+    l->off_tags = take(l->slots * n * 4);
+    l->off_pool_d = take(l->J * L * 4);
+    l->off_pool_id = take(l->J * L * 4);
+    l->off_pool_len = take(l->J * 4);
+    l->off_rdeg = take(refine ? n * 4 : 0);
+    l->off_redges = take(refine ? n * R * 4 : 0);
+    l->off_keys_a = take(l->P * 8);
+    l->off_keys_b = take(l->P * 8);
+    l->off_src = take(l->P * 4);
+    l->off_tgt = take(l->P * 4);
+    l->off_gsrc = take(l->P * 4);
+    l->off_glen = take(l->P * 4);
+    l->off_gbeg = take(l->P * 4);
+    l->off_msize = take(l->P * 8);
+    l->off_moff = take(l->P * 8);
+    l->off_merged = take(l->M * 4);
+    l->off_bkeys = take(l->M * 8);
+    l->off_bmin = take(l->M * 4);
+    l->off_bpre = take(l->M * 4);
+    l->off_bstate = take(l->M * 4);
+    l->off_jsrc = take(l->P * 4);
+    l->off_joff = take(l->P * 8);
+    l->off_jlen = take(l->P * 4);
+    l->off_cub = take(l->cub_bytes);
+    l->total = o;
+    return 0;
+}
+
+struct Ctl {
+    uint32_t counter, err, ngroups, njobs;
+    unsigned long long evals;
+};
+
+static int launch_fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return (int)hipGetLastError();
+}
+
+struct Builder {
+    const Space *s;
+    const uint8_t *codes;
+    const float2 *hdr;
+    BuildLayout l;
+    char *w;
+    hipStream_t st;
+    uint32_t R, L, beam, early_stop, cap;
+    float alpha;
+    uint32_t tag_next = 0;
+
+    template <class T> T *at(size_t off) { return reinterpret_cast<T *>(w + off); }
+    Ctl *ctl() { return at<Ctl>(l.off_ctl); }
+
+    int read_ctl(Ctl *h) {
+        int rc = (int)hipMemcpyAsync(h, ctl(), sizeof(Ctl), hipMemcpyDeviceToHost, st);
+        if (rc) return check_hip(rc, "build control read");
+        return check_hip((int)hipStreamSynchronize(st), "build synchronize");
+    }
+
+    int reset_counter() { return check_hip(launch_fill(&ctl()->counter, 1, 0u, st), "build counter reset"); }
+
+    // searches for records qid_base .. qid_base + jobs - 1 into the pools
+    int search(uint64_t qid_base, uint32_t jobs, uint64_t node_count, const uint32_t *deg, const uint32_t *edges) {
+        if (reset_counter()) return -1;
+        BuildSearchArgs a;
+        a.codes = codes;
+        a.hdr = hdr;
+        a.cd = (uint32_t)s->cd;
+        a.degrees = deg;
+        a.edges = edges;
+        a.R = R;
+        a.node_count = node_count;
+        a.L = L;
+        a.beam = beam;
+        a.early_stop = early_stop;
+        a.jobs = jobs;
+        a.qid_base = qid_base;
+        a.tags = at<uint32_t>(l.off_tags);
+        a.tag_stride = l.n;
+        a.tag_base = tag_next;
+        tag_next += jobs;
+        a.counter = &ctl()->counter;
+        a.err = &ctl()->err;
+        a.evals = &ctl()->evals;
+        a.pool_d = at<float>(l.off_pool_d);
+        a.pool_id = at<uint32_t>(l.off_pool_id);
+        a.pool_len = at<uint32_t>(l.off_pool_len);
+        const uint32_t slots = (uint32_t)(jobs < l.slots ? jobs : l.slots);
+        const size_t smem = (size_t)s->cd + (size_t)L * 9;
+        hipLaunchKernelGGL(k_build_search, dim3(slots), dim3(kSearchLanes), smem, st, a);
+        return check_hip((int)hipGetLastError(), "build search launch");
+    }
+
+    PruneArgs prune_args() {
+        PruneArgs p;
+        memset(&p, 0, sizeof(p));
+        p.codes = codes;
+        p.hdr = hdr;
+        p.cd = (uint32_t)s->cd;
+        p.R = R;
+        p.alpha = alpha;
+        p.err = &ctl()->err;
+        p.evals = &ctl()->evals;
+        return p;
+    }
+
+    int prune(const PruneArgs &p) {
+        if (p.jobs == 0) return 0;
+        hipLaunchKernelGGL(k_prune, dim3(p.jobs), dim3(kPruneThreads), 0, st, p);
+        return check_hip((int)hipGetLastError(), "prune launch");
+    }
+
+    // prune the pools of records id_base .. id_base + jobs - 1 into out rows;
+    // with existing lists (refine), each record's current row joins its pool
+    int prune_pools(uint64_t id_base, uint32_t jobs, const uint32_t *exist_deg, const uint32_t *exist_edges,
+                    uint32_t *out_deg, uint32_t *out_edges) {
+        PruneArgs p = prune_args();
+        p.jobs = jobs;
+        p.src_base = id_base;
+        p.pool_d = at<float>(l.off_pool_d);
+        p.pool_id = at<uint32_t>(l.off_pool_id);
+        p.pool_len = at<uint32_t>(l.off_pool_len);
+        p.pool_stride = L;
+        p.cap = cap;
+        p.graph_deg = exist_deg;
+        p.graph_edges = exist_edges;
+        p.out_deg = out_deg;
+        p.out_edges = out_edges;
+        return prune(p);
+    }
+
+    // addBackedgesGroupedAndPrune over the edges of rows [row_base, row_base + rows)
+    int backedges(uint64_t row_base, uint64_t rows, uint32_t *deg, uint32_t *edges) {
+        const uint64_t P = rows * R;
+        if (P == 0) return 0;
+        unsigned long long *ka = at<unsigned long long>(l.off_keys_a), *kb = at<unsigned long long>(l.off_keys_b);
+        hipLaunchKernelGGL(k_edge_keys, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, deg, edges, R, row_base,
+                           rows, ka);
+        if (check_hip((int)hipGetLastError(), "edge keys")) return -1;
+        int end_bit = 32;
+        while (end_bit < 64 && (1ull << (end_bit - 32)) <= l.n) ++end_bit;
+        size_t tb = l.cub_bytes;
+        void *temp = w + l.off_cub;
+        if (check_hip((int)hipcub::DeviceRadixSort::SortKeys(temp, tb, ka, kb, (int)P, 0, end_bit, st), "edge sort"))
+            return -1;
+        uint32_t *src = at<uint32_t>(l.off_src), *tgt = at<uint32_t>(l.off_tgt);
+        hipLaunchKernelGGL(k_split_keys, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, kb, P, src, tgt);
+        if (check_hip((int)hipGetLastError(), "edge split")) return -1;
+        uint32_t *gsrc = at<uint32_t>(l.off_gsrc), *glen = at<uint32_t>(l.off_glen), *gbeg = at<uint32_t>(l.off_gbeg);
+        tb = l.cub_bytes;
+        if (check_hip((int)hipcub::DeviceRunLengthEncode::Encode(temp, tb, src, gsrc, glen, &ctl()->ngroups, (int)P, st),
+                      "edge groups"))
+            return -1;
+        tb = l.cub_bytes;
+        if (check_hip((int)hipcub::DeviceScan::ExclusiveSum(temp, tb, glen, gbeg, (int)P, st), "group offsets"))
+            return -1;
+        uint64_t *msize = at<uint64_t>(l.off_msize), *moff = at<uint64_t>(l.off_moff);
+        hipLaunchKernelGGL(k_merged_sizes, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, gsrc, glen,
+                           &ctl()->ngroups, deg, msize, P);
+        if (check_hip((int)hipGetLastError(), "merged sizes")) return -1;
+        tb = l.cub_bytes;
+        if (check_hip((int)hipcub::DeviceScan::ExclusiveSum(temp, tb, msize, moff, (int)P, st), "merged offsets"))
+            return -1;
+        if (check_hip(launch_fill(&ctl()->njobs, 1, 0u, st), "job count reset")) return -1;
+        Ctl h;
+        if (read_ctl(&h)) return -1;
+        MergeArgs m;
+        m.group_src = gsrc;
+        m.group_begin = gbeg;
+        m.group_len = glen;
+        m.targets = tgt;
+        m.ngroups = h.ngroups;
+        m.R = R;
+        m.degrees = deg;
+        m.edges = edges;
+        m.merged_off = moff;
+        m.merged = at<uint32_t>(l.off_merged);
+        m.job_src = at<uint32_t>(l.off_jsrc);
+        m.job_off = at<uint64_t>(l.off_joff);
+        m.job_len = at<uint32_t>(l.off_jlen);
+        m.job_count = &ctl()->njobs;
+        if (h.ngroups) {
+            hipLaunchKernelGGL(k_backedge_merge, dim3(h.ngroups), dim3(64), 0, st, m);
+            if (check_hip((int)hipGetLastError(), "back-edge merge")) return -1;
+        }
+        if (read_ctl(&h)) return -1;
+        PruneArgs p = prune_args();
+        p.jobs = h.njobs;
+        p.sources = m.job_src;
+        p.extra = m.merged;
+        p.extra_off = m.job_off;
+        p.extra_len = m.job_len;
+        p.big_keys = at<unsigned long long>(l.off_bkeys);
+        p.big_min = at<float>(l.off_bmin);
+        p.big_pre = at<float>(l.off_bpre);
+        p.big_state = at<uint32_t>(l.off_bstate);
+        p.out_deg = deg;
+        p.out_edges = edges;
+        return prune(p);
+    }
+};
+
+}  // namespace qg
+
+extern "C" {
+
+size_t qg_build_workspace_bytes(uint64_t n, uint32_t r, uint32_t l_build, uint32_t batch, uint32_t refine_passes) {
+    qg::BuildLayout l;
+    if (batch == 0) batch = 1;
+    if (qg::build_layout(n, r, l_build, batch, refine_passes, &l)) return 0;
+    return l.total;
+}
+
+int qg_build_graph(const void *space, const qg_index_desc *ix, uint32_t r, uint32_t l_build, float alpha,
+                   uint32_t beam_width, uint32_t batch, uint32_t refine_passes, uint32_t prune_cap,
+                   uint32_t early_stop_hops, uint32_t *degrees, uint32_t *edges, uint64_t *distance_evaluations,
+                   void *workspace, size_t workspace_bytes, void *stream) {
+    using namespace qg;
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !ix || !degrees || !edges || !workspace) return fail("null argument to qg_build_graph");
+    if (r == 0 || r > 64) return fail("graph degree R must be in [1, 64]");
+    if (l_build == 0 || l_build > 1024) return fail("build list size must be in [1, 1024]");
+    if (!(alpha >= 1.0f) || !std::isfinite(alpha)) return fail("Vamana alpha must be finite and >= 1.0");
+    if (beam_width == 0 || beam_width > 64) return fail("beam width must be in [1, 64]");
+    if (batch == 0) batch = 1;
+    const uint64_t n = ix->n;
+    if (n > 0xFFFFFFF0ull) return fail("index too large for 32-bit node ids");
+    if (n * (uint64_t)(1 + refine_passes) >= 0xFFFFFFF0ull) return fail("too many searches for 32-bit visit tags");
+    Builder b;
+    b.s = s;
+    b.codes = ix->codes;
+    b.hdr = reinterpret_cast<const float2 *>(ix->hdr);
+    if (build_layout(n, r, l_build, batch, refine_passes, &b.l)) return fail("build workspace sizing failed");
+    if (workspace_bytes < b.l.total) return fail("build workspace too small: need %zu bytes", b.l.total);
+    b.w = static_cast<char *>(workspace);
+    b.st = (hipStream_t)stream;
+    b.R = r;
+    b.L = l_build;
+    b.beam = beam_width;
+    b.early_stop = early_stop_hops;
+    b.cap = prune_cap;
+    b.alpha = alpha;
+    hipStream_t st = b.st;
+    if (check_hip(launch_fill(reinterpret_cast<uint32_t *>(b.w + b.l.off_ctl), 64, 0u, st), "build init") ||
+        check_hip(launch_fill(reinterpret_cast<uint32_t *>(b.w + b.l.off_tags), b.l.slots * n, 0u, st), "tag init") ||
+        check_hip(launch_fill(degrees, n, 0u, st), "degree init") ||
+        check_hip(launch_fill(edges, n * r, 0xFFFFFFFFu, st), "edge init"))
+        return -1;
+    // buildGraphFromLoadedPayloads (vamana_index.h:452-528)
+    for (uint64_t begin = 1; begin < n; begin += batch) {
+        const uint64_t end = begin + batch < n ? begin + batch : n;
+        const uint32_t jobs = (uint32_t)(end - begin);
+        if (b.search(begin, jobs, begin, degrees, edges)) return -1;
+        if (b.prune_pools(begin, jobs, nullptr, nullptr, degrees, edges)) return -1;
+        if (b.backedges(begin, jobs, degrees, edges)) return -1;
+        Ctl h;
+        if (b.read_ctl(&h)) return -1;
+        if (h.err & 1u) return fail("Vamana graph contains out-of-range neighbor");
+        if (h.err) return fail("prune pool exceeds the workspace layout");
+    }
+    // refineGraphSymmetric (vamana_index.h:328-422): every record searches
+    // the frozen graph; results go to the second graph and are swapped in
+    if (refine_passes != 0 && n > 1) {
+        uint32_t *cur_deg = degrees, *cur_edges = edges;
+        uint32_t *alt_deg = reinterpret_cast<uint32_t *>(b.w + b.l.off_rdeg);
+        uint32_t *alt_edges = reinterpret_cast<uint32_t *>(b.w + b.l.off_redges);
+        for (uint32_t pass = 0; pass < refine_passes; ++pass) {
+            for (uint64_t c0 = 0; c0 < n; c0 += b.l.J) {
+                const uint64_t c1 = c0 + b.l.J < n ? c0 + b.l.J : n;
+                const uint32_t jobs = (uint32_t)(c1 - c0);
+                if (b.search(c0, jobs, n, cur_deg, cur_edges)) return -1;
+                if (b.prune_pools(c0, jobs, cur_deg, cur_edges, alt_deg, alt_edges)) return -1;
+            }
+            std::swap(cur_deg, alt_deg);
+            std::swap(cur_edges, alt_edges);
+            if (b.backedges(0, n, cur_deg, cur_edges)) return -1;
+            Ctl h;
+            if (b.read_ctl(&h)) return -1;
+            if (h.err & 1u) return fail("Vamana graph contains out-of-range neighbor");
+            if (h.err) return fail("prune pool exceeds the workspace layout");
+        }
+        if (cur_deg != degrees) {
+            if (check_hip((int)hipMemcpyAsync(degrees, cur_deg, n * 4, hipMemcpyDeviceToDevice, st), "graph copy") ||
+                check_hip((int)hipMemcpyAsync(edges, cur_edges, n * r * 4, hipMemcpyDeviceToDevice, st), "graph copy"))
+                return -1;
+        }
+    }
+    Ctl h;
+    if (b.read_ctl(&h)) return -1;
+    if (distance_evaluations) *distance_evaluations = h.evals;
+    return 0;
+}
+
+}  // extern "C"

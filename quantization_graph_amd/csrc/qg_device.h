@@ -186,4 +186,70 @@ __device__ __forceinline__ Interval residual_one(const float *q, const uint16_t 
     return iv;
 }
 
+// C8: symmetric code<->code distance (distanceBetweenEncoded /
+// symmetricBuildDistanceFromPrepared, space_rabitq.h:2634-2747, K = 1).
+// ip4 = sum (2a-15)(2b-15) is exact in int32 (|term| <= 225, cd <= 2048);
+// the finish runs in double with the lhs (prepared) side first.
+__device__ __forceinline__ uint32_t centered_bytes(uint32_t nib4) {
+    // four nibbles c (one per byte) -> four signed bytes 2c - 15, carry-free
+    const uint32_t x = nib4 ^ 0x08080808u;                           // c - 8, 4-bit two's complement
+    const uint32_t sx = x | (((x & 0x08080808u) >> 3) * 0xF0u);      // sign-extend each byte
+    return ((sx << 1) & 0xFEFEFEFEu) | 0x01010101u;                   // 2(c - 8) + 1
+}
+
+__device__ __forceinline__ float symmetric_finish(float2 lh, float2 rh, int64_t ip4) {
+    const double ln = sqrt((0.0 < (double)lh.x) ? (double)lh.x : 0.0);
+    const double rn = sqrt((0.0 < (double)rh.x) ? (double)rh.x : 0.0);
+    const bool lvalid = ln > 0.0 && lh.y > 0.0f && __builtin_isfinite(lh.y);
+    const double lipn = lvalid ? (double)lh.y / (2.0 * ln) : 0.0;
+    if (!lvalid || !(rn > 0.0) || !(rh.y > 0.0f) || !__builtin_isfinite(rh.y))
+        return (float)((double)lh.x + (double)rh.x + 0.0);
+    const double ripn = (double)rh.y / (2.0 * rn);
+    const double code_ip = 0.25 * (double)ip4;
+    double u = code_ip * lipn * ripn;
+    if (!__builtin_isfinite(u)) u = 0.0;
+    u = (u < 1.0) ? u : 1.0;
+    u = (-1.0 < u) ? u : -1.0;
+    const double rip = ln * rn * u;
+    return (float)((double)lh.x + (double)rh.x - 2.0 * rip);
+}
+
+// ip4 of two raw code rows (cd/2 bytes each, 16-byte aligned)
+__device__ __forceinline__ int symmetric_ip4_rows(const uint8_t *lhs, const uint8_t *rhs, uint32_t cd) {
+    const uint4 *lr = (const uint4 *)lhs;
+    const uint4 *rr = (const uint4 *)rhs;
+    int acc = 0;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 wl = lr[c], wr = rr[c];
+        const uint32_t ls[4] = {wl.x, wl.y, wl.z, wl.w}, rs[4] = {wr.x, wr.y, wr.z, wr.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            acc = __builtin_amdgcn_sdot4((int)centered_bytes(ls[k] & 0x0F0F0F0Fu),
+                                         (int)centered_bytes(rs[k] & 0x0F0F0F0Fu), acc, false);
+            acc = __builtin_amdgcn_sdot4((int)centered_bytes((ls[k] >> 4) & 0x0F0F0F0Fu),
+                                         (int)centered_bytes((rs[k] >> 4) & 0x0F0F0F0Fu), acc, false);
+        }
+    }
+    return acc;
+}
+
+// ip4 with the lhs already centred: lc[2w] / lc[2w+1] are the low / high
+// nibbles of code dword w as signed bytes 2c - 15 (cd/4 dwords)
+__device__ __forceinline__ int symmetric_ip4_prepared(const uint32_t *lc, const uint8_t *rhs, uint32_t cd) {
+    const uint4 *rr = (const uint4 *)rhs;
+    int acc = 0;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 w = rr[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t d = 4 * c + k;
+            acc = __builtin_amdgcn_sdot4((int)lc[2 * d], (int)centered_bytes(ws[k] & 0x0F0F0F0Fu), acc, false);
+            acc = __builtin_amdgcn_sdot4((int)lc[2 * d + 1], (int)centered_bytes((ws[k] >> 4) & 0x0F0F0F0Fu), acc,
+                                         false);
+        }
+    }
+    return acc;
+}
+
 }  // namespace qg

@@ -10,6 +10,7 @@
 // code row with 16-byte loads. Integer dots are exact (v_dot4 on unpacked
 // nibbles); float estimators keep the reference's 16-lane accumulation order.
 #include "qg_args.h"
+#include "qg_device.h"
 
 namespace qg {
 
@@ -414,48 +415,12 @@ __global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
 // BuildDistanceFromPrepared, space_rabitq.h:2634-2747, K = 1): ip4 =
 // sum (2a-15)(2b-15) exactly, then the double-precision finish with the lhs
 // (prepared) side first. Pairs (qidx[i] = lhs row, ids[i] = rhs row).
-__device__ __forceinline__ uint32_t centered_bytes(uint32_t nib4) {
-    // four nibbles c (one per byte) -> four signed bytes 2c - 15, carry-free
-    const uint32_t x = nib4 ^ 0x08080808u;                           // c - 8, 4-bit two's complement
-    const uint32_t sx = x | (((x & 0x08080808u) >> 3) * 0xF0u);      // sign-extend each byte
-    return ((sx << 1) & 0xFEFEFEFEu) | 0x01010101u;                   // 2(c - 8) + 1
-}
-
-__device__ __forceinline__ float symmetric_finish(float2 lh, float2 rh, int64_t ip4) {
-    const double ln = sqrt((0.0 < (double)lh.x) ? (double)lh.x : 0.0);
-    const double rn = sqrt((0.0 < (double)rh.x) ? (double)rh.x : 0.0);
-    const bool lvalid = ln > 0.0 && lh.y > 0.0f && __builtin_isfinite(lh.y);
-    const double lipn = lvalid ? (double)lh.y / (2.0 * ln) : 0.0;
-    if (!lvalid || !(rn > 0.0) || !(rh.y > 0.0f) || !__builtin_isfinite(rh.y))
-        return (float)((double)lh.x + (double)rh.x + 0.0);
-    const double ripn = (double)rh.y / (2.0 * rn);
-    const double code_ip = 0.25 * (double)ip4;
-    double u = code_ip * lipn * ripn;
-    if (!__builtin_isfinite(u)) u = 0.0;
-    u = (u < 1.0) ? u : 1.0;
-    u = (-1.0 < u) ? u : -1.0;
-    const double rip = ln * rn * u;
-    return (float)((double)lh.x + (double)rh.x - 2.0 * rip);
-}
-
 __global__ void __launch_bounds__(kScanThreads) k_symmetric(ScanArgs a) {
     const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= a.n_pairs) return;
     const uint32_t l = a.qidx[item], r = a.ids[item];
-    const uint4 *lr = (const uint4 *)(a.ix.codes + (uint64_t)l * (a.cd / 2));
-    const uint4 *rr = (const uint4 *)(a.ix.codes + (uint64_t)r * (a.cd / 2));
-    int acc = 0;
-    for (uint32_t c = 0; c < a.cd / 32; ++c) {
-        const uint4 wl = lr[c], wr = rr[c];
-        const uint32_t ls[4] = {wl.x, wl.y, wl.z, wl.w}, rs[4] = {wr.x, wr.y, wr.z, wr.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            acc = __builtin_amdgcn_sdot4((int)centered_bytes(ls[k] & 0x0F0F0F0Fu),
-                                         (int)centered_bytes(rs[k] & 0x0F0F0F0Fu), acc, false);
-            acc = __builtin_amdgcn_sdot4((int)centered_bytes((ls[k] >> 4) & 0x0F0F0F0Fu),
-                                         (int)centered_bytes((rs[k] >> 4) & 0x0F0F0F0Fu), acc, false);
-        }
-    }
+    const int acc = symmetric_ip4_rows(a.ix.codes + (uint64_t)l * (a.cd / 2), a.ix.codes + (uint64_t)r * (a.cd / 2),
+                                       a.cd);
     a.out_f[item] = symmetric_finish(a.ix.hdr[l], a.ix.hdr[r], acc);
     if (a.out_dot) a.out_dot[item] = acc;
 }

@@ -329,6 +329,30 @@ class RabitqSpace:
             return ids, dists, stats, pool_ids, pool_d
         return ids, dists, stats
 
+    def build_graph(self, index, r=32, l_build=400, alpha=1.2, beam_width=1, batch=4096, refine_passes=0,
+                    prune_cap=0, early_stop_hops=0):
+        """Vamana graph over the index's codes with the symmetric distance
+        (rabitq_build_vamana_graph, rabitq_bridge.cpp:224-303; defaults are
+        the reference's: R 32, L_build 400, alpha 1.2, beam 1, batch 4096).
+
+        Returns (degrees [n] int32, edges [n, r] int32 with -1 past each
+        degree, distance_evaluations)."""
+        n = index.n
+        degrees = torch.empty(n, dtype=torch.int32, device="cuda")
+        edges = torch.empty((n, r), dtype=torch.int32, device="cuda")
+        L_ = _lib.lib()
+        ws_bytes = L_.qg_build_workspace_bytes(n, r, l_build, batch, refine_passes)
+        if ws_bytes == 0:
+            raise _lib.QGError("build workspace sizing failed")
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        evals = C.c_uint64()
+        di = index.desc()
+        check(L_.qg_build_graph(self.handle, C.byref(di), r, l_build, C.c_float(alpha), beam_width, batch,
+                                refine_passes, prune_cap, early_stop_hops, _ptr(degrees), _ptr(edges),
+                                C.byref(evals), _ptr(ws), ws_bytes, _stream()))
+        del ws
+        return degrees, edges, evals.value
+
 
 def topk(values, ids, k):
     """Per-row top-k of values [nq, P] by (value, id); returns (ids, values,

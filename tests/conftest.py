@@ -32,3 +32,9 @@ def gpu():
         pytest.skip("no GPU")
     torch.cuda.set_device(0)
     return torch
+
+
+@pytest.fixture(scope="session")
+def golden():
+    """Vectors written by the reference bridge (tests/golden/make_golden.py)."""
+    return np.load(os.path.join(ROOT, "tests", "golden", "ref_vectors.npz"))

@@ -125,7 +125,11 @@ class Oracle:
         L.ro_robust_prune.restype = C.c_size_t
         L.ro_robust_prune.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32,
                                       C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
-                                      C.c_float, C.c_void_p]
+                                      C.c_float, C.c_void_p, C.c_void_p]
+        L.ro_build_vamana.restype = C.c_int
+        L.ro_build_vamana.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                      C.c_float, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
         L.ro_fast_quantize_abs.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]
         L.ro_float_to_half.restype = C.c_uint16
         L.ro_float_to_half.argtypes = [C.c_float]
@@ -291,13 +295,25 @@ class Oracle:
                                  C.byref(plen), C.byref(st))
         return ids[:got], ds[:got], pool_ids[:plen.value], pool_d[:plen.value], st
 
-    def robust_prune(self, comp, source, pool_ids, pool_d, r, alpha):
+    def robust_prune(self, comp, source, pool_ids, pool_d, r, alpha, with_evals=False):
         pool_ids = np.ascontiguousarray(pool_ids, np.uint32)
         pool_d = np.ascontiguousarray(pool_d, np.float32)
         out = np.zeros(r, np.uint32)
+        ev = C.c_uint64()
         got = self.lib.ro_robust_prune(self.sp, _p(comp), comp.shape[1], source, _p(pool_ids), _p(pool_d),
-                                       pool_ids.size, r, C.c_float(alpha), _p(out))
-        return out[:got]
+                                       pool_ids.size, r, C.c_float(alpha), _p(out), C.byref(ev))
+        return (out[:got], ev.value) if with_evals else out[:got]
+
+    def build_graph(self, comp, r, l_build, alpha, beam=1, batch=4096, refine=0, cap=0, early_stop=0):
+        n = comp.shape[0]
+        deg = np.zeros(n, np.uint32)
+        edges = np.zeros((n, r), np.uint32)
+        ev = C.c_uint64()
+        rc = self.lib.ro_build_vamana(self.sp, _p(comp), comp.shape[1], n, r, l_build, C.c_float(alpha), beam,
+                                      batch, refine, cap, early_stop, _p(deg), _p(edges), C.byref(ev))
+        if rc != 0:
+            raise ValueError("ro_build_vamana failed: %d" % rc)
+        return deg, edges, ev.value
 
 
 class Ref:

@@ -6,7 +6,9 @@
   orthogonality of the rotation, exact integer dots by brute force, optimality
   of fastQuantizeAbs over all codes on tiny inputs, the FP64 form of every
   float estimator, IEEE half values, and brute-force RobustPrune / search on
-  tiny inputs.
+  tiny inputs;
+* the Vamana build against the reference bridge's graphs (fixture and, where
+  oracle/_ref is built, live).
 """
 import ctypes as C
 import itertools
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 from conftest import gaussian_mixture
-from oracle_bind import ESTIMATE_DTYPE, FACTOR_DTYPE, INTERVAL_DTYPE, Oracle
+from oracle_bind import ESTIMATE_DTYPE, FACTOR_DTYPE, INTERVAL_DTYPE, Oracle, Ref
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
 
@@ -187,8 +189,17 @@ def test_robust_prune_against_brute_force():
         pool = np.concatenate([pool, pool[:3]])  # duplicates are dropped
         d = np.array([o.symmetric(comp[src], comp[j]) for j in pool], np.float32)
         R, alpha = 6, 1.2
-        got = o.robust_prune(comp, src, pool, d, R, alpha)
-        # brute force: vamana_prune.h:41-156 re-derived
+        got, evals = o.robust_prune(comp, src, pool, d, R, alpha, with_evals=True)
+        # brute force: vamana_prune.h:41-156 re-derived, with the lazily
+        # filled (candidate, selected-slot) memo whose fills are the count
+        memo = {}
+
+        def pair(j, k):
+            if (j, k) not in memo or memo[(j, k)] < 0:
+                memo[(j, k)] = np.float32(o.symmetric(comp[j], comp[sel[k]]))
+                count[0] += 1
+            return memo[(j, k)]
+        count = [0]
         cand = sorted({int(j): float(dd) for j, dd in zip(pool, d)}.items(), key=lambda t: (np.float32(t[1]), t[0]))
         cand = [(j, dd) for j, dd in cand if j != src]
         sel, dis, cur = [], set(), np.float32(1.0)
@@ -200,7 +211,7 @@ def test_robust_prune_against_brute_force():
                     break
                 if j in dis:
                     continue
-                occ = any(np.float32(cur * np.float32(o.symmetric(comp[j], comp[s]))) <= np.float32(dd) for s in sel)
+                occ = any(np.float32(cur * pair(j, k)) <= np.float32(dd) for k in range(len(sel)))
                 if not occ:
                     sel.append(j)
                     dis.add(j)
@@ -214,7 +225,43 @@ def test_robust_prune_against_brute_force():
             if not promoted and len(sel) >= R:
                 break
         assert list(got) == sel
+        assert evals == count[0]
         assert src not in got and len(set(got.tolist())) == len(got) <= R
+
+
+def test_build_matches_reference_fixture(golden):
+    """Oracle build vs the graph the reference bridge built (300 x 128, R 16,
+    L 64, batch 64, one refine pass, prune cap 128): same adjacency lists in
+    order and the same distance_evaluations."""
+    o = Oracle(128, 100)
+    o.set_center(golden["g_center"])
+    deg, edges, evals = o.build_graph(golden["g_compact"], 16, 64, 1.2, beam=1, batch=64, refine=1, cap=128)
+    assert np.array_equal(deg, golden["g_degrees"])
+    for i in range(deg.size):
+        assert np.array_equal(edges[i, :deg[i]], golden["g_edges"][i, :deg[i]])
+    assert evals == int(golden["g_evals"][0])
+
+
+@pytest.mark.skipif(not Ref.available(), reason="reference bridge not built (oracle/_ref)")
+@pytest.mark.parametrize("case", [(900, 64, 32, 100, 1.2, 1, 256, 0, 0, 0), (900, 64, 16, 50, 1.5, 2, 100, 2, 40, 0),
+                                  (700, 96, 24, 80, 2.0, 3, 300, 1, 0, 3)])
+def test_build_matches_reference_bridge(case):
+    """Oracle build vs the reference bridge compiled from source, across
+    beam widths, early stop, prune caps, refine passes and alpha."""
+    n, dim, R, L, alpha, beam, batch, refine, cap, es = case
+    x, _ = gaussian_mixture(n, dim, 8, 0.3, 5)
+    c = x.mean(0).astype(np.float32)
+    o = Oracle(dim, 100)
+    o.set_center(c)
+    comp, _ = o.split(o.encode_full(x))
+    r = Ref(dim, 100)
+    r.set_center(c)
+    d1, e1, v1 = o.build_graph(comp, R, L, alpha, beam, batch, refine, cap, es)
+    d2, e2, _, _, v2 = r.build_graph(comp, R, L, alpha, beam, batch, refine, cap, es)
+    assert np.array_equal(d1, d2)
+    for i in range(n):
+        assert np.array_equal(e1[i, :d1[i]], e2[i, :d2[i]])
+    assert v1 == v2
 
 
 def test_search_visits_everything_when_pool_never_fills():
