@@ -409,6 +409,8 @@ __global__ void __launch_bounds__(kPruneThreads) k_prune(PruneArgs a) {
                 if (j == first) st |= kSelected;
                 state[j] = st;
             }
+            // the next sweep maps candidates to other threads
+            __syncthreads();
             if (first == kNone) break;
             if (t == 0) s_sel[nsel] = (uint32_t)keys[first];
             __syncthreads();
