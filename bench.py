@@ -1,20 +1,26 @@
 #!/usr/bin/env python
 """Headline benchmark on one MI355X.
 
-Workload (the 1M x 960 precision-form configuration): 1,000,000 x 960 fp32
-vectors from a 256-component Gaussian mixture (centers N(0, 1), per-cluster
-std 0.3, seed 2), ExRaBitQ4 block16 encode (seed-100 FHT rotation, MSE fp16
-residual scales, mean center), 10,000 queries (base points + N(0, 0.05)
-noise) quantized to INT8, 1,000 fixed candidates per query (the exact-L2
-top-10 plus 990 drawn from exact ranks 11..2000, shuffled). One step =
-prepare the 10k queries + INT8 x 4-bit scoring of all 10M (query,
-candidate) pairs + per-query top-`depth` + FP32 restore + residual-4 rerank
-+ top-10. Recall@10 is measured against brute-force fp32 ground truth; the
-rerank depth is swept and the headline is the fastest depth that reaches
-Recall@10 >= 0.95.
+Graph workload (the full build-and-search configuration, the one the
+metric is quoted on): 1,000,000 x 1536 fp32 vectors with a power-law
+spectrum (z ~ N(0, diag(i^-0.8)), rotated by a seeded orthogonal basis,
+L2-normalised, seed 1); ExRaBitQ4 block16 encode (seed-100 FHT rotation,
+MSE fp16 residual scales, mean center); Vamana graph built on the GPU with
+symmetric 4-bit distances (M = R = 64, L_build = 400, alpha = 1.2, batch
+4096, one refine pass, prune cap 256); 1,000 queries = base points plus
+N(0, (0.1 / sqrt(1536))^2) noise, re-normalised. One step = prepare the
+1,000 queries (FP32 codec) + one batched persistent graph search (1-bit
+gate, remaining-bit distances, residual-4 top-100 rerank, top-10). The
+search list L is swept over {100, 200, 400}; the headline is the fastest
+L whose Recall@10 (vs brute-force fp32 ground truth) is >= 0.95.
 
-Prints one JSON line (see the keys below). ``--gpus N`` with N > 1 exits:
-this build runs on one GPU.
+A second section repeats the fixed-candidate form on 1,000,000 x 960
+(256-cluster Gaussian mixture, seed 2): 10,000 INT8 queries x 1,000
+candidates, INT8 x 4-bit scoring, residual-4 rerank; it carries the
+code-scan kernel's roofline.
+
+Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on
+one GPU.
 """
 import argparse
 import json
@@ -126,46 +132,208 @@ def cpu_reference_baseline(comp, res, cand, gt, q_host, center, dim, depth, samp
             "seconds": float(np.sum(times)), "recall_at_10": float(np.mean(hits)) / 10.0}, found
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--dim", type=int, default=960)
-    ap.add_argument("--nq", type=int, default=10_000)
-    ap.add_argument("--per", type=int, default=1000)
-    ap.add_argument("--depths", default="100,150,200,300,400,600",
-                    help="residual-rerank depths swept; the headline is the smallest with Recall@10 >= 0.95")
-    ap.add_argument("--target-recall", type=float, default=0.95)
-    ap.add_argument("--cpu-queries", type=int, default=2000)
-    ap.add_argument("--cpu-repeats", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
-                    help="only run 5 device-to-device copies of BYTES (a plain copy kernel of known size, "
-                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only; --gpus %d is not supported by this build" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
 
+
+def make_powerlaw(torch, n, dim, nq, seed, exponent=0.8, qnoise=0.1, chunk=131072):
+    """Power-law spectrum vectors: z ~ N(0, diag(i^-exponent)), rotated by a
+    seeded orthogonal basis (QR of a Gaussian matrix), L2-normalised; queries
+    are base points plus N(0, (qnoise/sqrt(dim))^2) noise, re-normalised."""
     import numpy as np
-    import torch
-    import quantization_graph_amd as qg
-    from quantization_graph_amd import fixed
+    basis, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(dim, dim)))
+    basis = torch.from_numpy(basis.astype(np.float32)).cuda()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    scale = torch.arange(1, dim + 1, device="cuda", dtype=torch.float32).pow(-exponent / 2.0)
+    x = torch.empty((n, dim), device="cuda")
+    for s in range(0, n, chunk):
+        z = torch.randn((min(chunk, n - s), dim), generator=g, device="cuda") * scale
+        x[s:s + z.shape[0]] = torch.nn.functional.normalize(z @ basis.T, dim=1)
+    pick = torch.randint(0, n, (nq,), generator=g, device="cuda")
+    noise = (qnoise / dim ** 0.5) * torch.randn((nq, dim), generator=g, device="cuda")
+    q = torch.nn.functional.normalize(x[pick] + noise, dim=1)
+    return x, q.contiguous()
 
-    torch.cuda.set_device(0)
-    if args.calibrate_copy:
-        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
-        return
+
+def search_bytes_per_query(cd, gates, dists, reranked, int8):
+    """Algorithmic HBM bytes of one query's search: gate reads (MSB plane +
+    20-byte factors + id + estimate), full 4-bit distance reads (codes +
+    8-byte header + id + distance), rerank reads (fp16 scales + residual
+    codes + long distance + id + estimate; plus an FP32 long-distance
+    recompute of the codes for INT8 queries)."""
+    gate_b = cd // 8 + 20 + 4 + 8
+    dist_b = cd // 2 + 8 + 4 + 4
+    rr_b = 2 * (cd // 16) + cd // 2 + 12 + (cd // 2 + 8 if int8 else 0)
+    return gates * gate_b + dists * dist_b + reranked * rr_b
+
+
+def cpu_graph_baseline(sp_dim, center, comp, res, deg, edges, q_host, gt_host, L, repeats, threads):
+    """The reference's CPU search path (its compiled bridge kernels driven by
+    the C restatement of its Rust search loop, oracle/ref_search.c), FP32
+    codec: one warm-up pass, `repeats` timed single-thread passes (median),
+    and one pass over `threads` host threads."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Ref, RefSearch
+    if not RefSearch.available():
+        return None, None
+    ref = Ref(sp_dim, 100)
+    ref.set_center(center)
+    ref.set_codec(0)
+    msb, fac = ref.sidecar(comp)
+    rs = RefSearch(ref)
+    nq = q_host.shape[0]
+    out = (np.zeros((nq, 10), np.uint32), np.zeros((nq, 10), np.float32))
+    rs.search(q_host, 0, comp, res, msb, fac, deg, edges, k=10, L=L, q1=min(nq, 32), out=out)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        rs.search(q_host, 0, comp, res, msb, fac, deg, edges, k=10, L=L, out=out)
+        times.append(time.perf_counter() - t0)
+    single = nq / float(np.median(times))
+    multi_out = (np.zeros((nq, 10), np.uint32), np.zeros((nq, 10), np.float32))
+    bounds = [(nq * t // threads, nq * (t + 1) // threads) for t in range(threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda b: rs.search(q_host, 0, comp, res, msb, fac, deg, edges, k=10, L=L, q0=b[0], q1=b[1],
+                                        out=multi_out), bounds))
+    multi = nq / (time.perf_counter() - t0)
+    hits = [len(set(out[0][i].tolist()) & set(gt_host[i].tolist())) for i in range(nq)]
+    return {"qps": single, "qps_all_cores": multi, "threads": threads, "queries": nq,
+            "seconds": float(np.sum(times)), "recall_at_10": float(np.mean(hits)) / 10.0}, out[0]
+
+
+def cpu_build_baseline(dim, center, comp, sub, R, L, alpha, refine, cap):
+    """The reference build (rabitq_build_vamana_graph compiled from source,
+    OpenMP over the host cores) on the first `sub` records."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Ref
+    if not Ref.available():
+        return None
+    ref = Ref(dim, 100)
+    ref.set_center(center)
+    t0 = time.perf_counter()
+    _, _, _, _, evals = ref.build_graph(comp[:sub], R, L, alpha, beam=1, batch=4096, refine=refine, cap=cap)
+    secs = time.perf_counter() - t0
+    return {"records": sub, "seconds": secs, "distance_evaluations": evals, "evals_per_s": evals / secs,
+            "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+
+
+def run_graph(args, torch, np, qg):
     t_setup = time.perf_counter()
-    x, q, g = make_data(torch, args.n, args.dim, args.nq, 256, 0.3, 2, 0.05)
+    n, dim, nq = args.n, args.dim, args.nq
+    x, q = make_powerlaw(torch, n, dim, nq, 1)
+    gt = exact_topk(torch, x, q, 10)
+    center = x.double().mean(0).float().cpu().numpy()
+    log("graph data ready", tuple(x.shape), tuple(q.shape))
+    sp = qg.RabitqSpace(dim, 100)
+    sp.set_center(center)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = sp.encode(x, batch=131072)
+    torch.cuda.synchronize()
+    encode_s = time.perf_counter() - t0
+    del x
+    torch.cuda.empty_cache()
+    log("encoded %d x %d in %.2f s" % (n, dim, encode_s))
+    t0 = time.perf_counter()
+    deg, edges, evals = sp.build_graph(index, r=args.R, l_build=args.l_build, alpha=1.2, batch=4096,
+                                       refine_passes=args.refine, prune_cap=args.prune_cap)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log("built graph in %.1f s: %.4g distance evaluations (%.3g /s), mean degree %.2f"
+        % (build_s, evals, evals / build_s, deg.float().mean().item()))
+    stream = torch.cuda.current_stream()
+    cd = sp.code_dim
+
+    def step(L, codec, ev=None):
+        qv = sp.prepare(q, codec)
+        return sp.search(index, qv, deg, edges, k=10, l_search=L, eps0=1.9, rerank=100, events=ev)
+
+    def timed(L, codec, steps, warmup):
+        for _ in range(warmup):
+            step(L, codec)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for i in range(steps):
+            ids, _, st = step(L, codec, evs[i])
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / steps
+        kms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
+        mean = st.double().mean(0).tolist()
+        return {"L": L, "codec": codec, "ms_per_step": ms, "kernel_ms": kms, "qps": nq / (ms / 1000.0),
+                "recall_at_10": rec, "hops": mean[0], "gate_evals": mean[1], "distances": mean[2],
+                "pruned": mean[3]}, ids
+
+    lists = [int(v) for v in args.lists.split(",") if v]
+    sweep = []
+    for codec in ("full", "int8"):
+        for L in lists:
+            r, _ = timed(L, codec, max(3, args.steps // 4), 1)
+            sweep.append(r)
+            log("codec %-4s L=%4d: %.3f ms/step  %.0f q/s  recall@10 %.4f  (gates %.0f, distances %.0f)"
+                % (codec, L, r["ms_per_step"], r["qps"], r["recall_at_10"], r["gate_evals"], r["distances"]))
+    full = [r for r in sweep if r["codec"] == "full"]
+    ok = [r for r in full if r["recall_at_10"] >= args.target_recall]
+    pick = min(ok, key=lambda r: r["ms_per_step"]) if ok else max(full, key=lambda r: r["recall_at_10"])
+    head, ids = timed(pick["L"], "full", args.steps, args.warmup)
+    reranked = min(100, head["L"])
+    bpq = search_bytes_per_query(cd, head["gate_evals"], head["distances"], reranked, False)
+    achieved = nq * bpq / (head["kernel_ms"] / 1000.0) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "search_1Mx1536_pmc.json")
+    if os.path.exists(pmc) and n == 1_000_000 and dim == 1536:
+        with open(pmc) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
+
+    cpu, parity, cpu_build = None, None, None
+    if not args.no_cpu:
+        comp, res = index.to_records()
+        comp_h, res_h = comp.cpu().numpy(), res.cpu().numpy()
+        del comp, res
+        deg_h = deg.cpu().numpy().astype(np.uint32)
+        edges_h = edges.cpu().numpy().astype(np.uint32)
+        threads = min(16, os.cpu_count() or 1)
+        base, ref_ids = cpu_graph_baseline(dim, center, comp_h, res_h, deg_h, edges_h, q.cpu().numpy(),
+                                           gt.cpu().numpy(), head["L"], args.cpu_repeats, threads)
+        if base is not None:
+            parity = float((ids.cpu().numpy().astype(np.uint32) == ref_ids).all(1).mean())
+            cpu = {"value": base["qps"], "unit": "queries/s", "cores": 1, "kind": "reference",
+                   "sample": "all %d queries of this workload at L=%d on the same GPU-built graph and records, "
+                             "warm-up + %d repeats (median), single thread; the reference bridge's compiled "
+                             "AVX-512 kernels (gate, remaining-bit distances, residual rerank) driven by the C "
+                             "restatement of its Rust search loop" % (nq, head["L"], args.cpu_repeats),
+                   "value_all_cores": base["qps_all_cores"], "all_cores_threads": base["threads"],
+                   "recall_at_10": base["recall_at_10"]}
+            log("cpu reference search: %.1f q/s (1 thread), %.1f q/s (%d threads); GPU vs reference top-10 "
+                "id parity %.4f" % (base["qps"], base["qps_all_cores"], base["threads"], parity))
+        if args.cpu_build_records:
+            cpu_build = cpu_build_baseline(dim, center, comp_h, min(args.cpu_build_records, n), args.R,
+                                           args.l_build, 1.2, args.refine, args.prune_cap)
+            if cpu_build:
+                log("cpu reference build of %d records: %.1f s, %.3g evals/s"
+                    % (cpu_build["records"], cpu_build["seconds"], cpu_build["evals_per_s"]))
+    del index, deg, edges
+    torch.cuda.empty_cache()
+    return {
+        "head": head, "sweep": sweep, "encode_seconds": encode_s, "build_seconds": build_s,
+        "build_distance_evaluations": evals, "build_evals_per_s": evals / build_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "qg::k_search (persistent batched graph search, one wavefront per query)",
+                     "kernel_ms": head["kernel_ms"], "bytes_per_unit": bpq, "unit_of_work": "query",
+                     "units_per_launch": nq},
+        "cpu_baseline": cpu, "cpu_build_baseline": cpu_build, "gpu_vs_reference_id_parity": parity,
+        "setup_seconds": time.perf_counter() - t_setup,
+    }
+
+
+def run_fixed(args, torch, np, qg):
+    from quantization_graph_amd import fixed
+    t_setup = time.perf_counter()
+    x, q, g = make_data(torch, args.fixed_n, args.fixed_dim, args.fixed_nq, 256, 0.3, 2, 0.05)
     center = x.double().mean(0).float().cpu().numpy()
     log("data ready", tuple(x.shape), tuple(q.shape))
     top2000 = exact_topk(torch, x, q, 2000)
@@ -174,16 +342,16 @@ def main():
     del top2000
     log("ground truth + candidates ready")
 
-    sp = qg.RabitqSpace(args.dim, 100)
+    sp = qg.RabitqSpace(args.fixed_dim, 100)
     sp.set_center(center)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     index = sp.encode(x, batch=262144)
     torch.cuda.synchronize()
     encode_s = time.perf_counter() - t0
-    log("encoded %d vectors in %.2f s" % (args.n, encode_s))
+    log("encoded %d vectors in %.2f s" % (args.fixed_n, encode_s))
 
-    nq, per = args.nq, args.per
+    nq, per = args.fixed_nq, args.per
     stream = torch.cuda.current_stream()
     depths = [int(d) for d in args.depths.split(",") if d]
 
@@ -231,7 +399,7 @@ def main():
     achieved = units * unit_bytes / (kernel_ms / 1000.0) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "int8_full_fixed_1Mx960_pmc.json")
-    if os.path.exists(pmc) and args.n == 1_000_000 and args.dim == 960:
+    if os.path.exists(pmc) and args.fixed_n == 1_000_000 and args.fixed_dim == 960:
         with open(pmc) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
@@ -246,7 +414,7 @@ def main():
         q_h = q[:ns].cpu().numpy()
         gt_h = gt[:ns].cpu().numpy()
         threads = min(16, os.cpu_count() or 1)
-        base, ref_found = cpu_reference_baseline(comp_h, res_h, cand_h, gt_h, q_h, center, args.dim, depth,
+        base, ref_found = cpu_reference_baseline(comp_h, res_h, cand_h, gt_h, q_h, center, args.fixed_dim, depth,
                                                  ns, args.cpu_repeats, threads)
         if base is not None:
             # the same FP32-first-stage pipeline on the GPU, compared id-for-id
@@ -262,38 +430,112 @@ def main():
             log("cpu reference: %.1f q/s (1 thread), %.1f q/s (%d threads); GPU vs reference top-10 id "
                 "parity %.4f" % (base["qps"], base["qps_all_cores"], base["threads"], parity))
 
-    out = {
-        "metric": "QPS at Recall@10>=0.95 (fixed candidates, residual4 rerank); Gdist/s vs HBM roofline",
-        "value": qps,
-        "unit": "queries/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int8",
-        "data": "synthetic",
+    res = {
+        "value": qps, "ms_per_step": ms_per_step, "recall_at_10": recall,
+        "recall_target_met": recall >= args.target_recall, "recall_plateau": best, "depth_sweep": sweep,
         "config": {"workload": "fixed-candidate INT8 x 4-bit scoring + residual4 rerank, %d x %d GMM(256, std 0.3, "
-                               "seed 2), %d queries x %d candidates" % (args.n, args.dim, nq, per),
-                   "base": args.n, "dim": args.dim, "queries": nq, "candidates_per_query": per,
+                               "seed 2), %d queries x %d candidates" % (args.fixed_n, args.fixed_dim, nq, per),
                    "rerank_depth": depth, "codec": "int8"},
-        "recall_at_10": recall,
-        "recall_target_met": recall >= args.target_recall,
-        "recall_plateau": best,
-        "depth_sweep": sweep,
         "gdist_per_s": units / (kernel_ms / 1000.0) / 1e9,
-        "encode_seconds": encode_s,
-        "encode_mvec_per_s": args.n / encode_s / 1e6,
+        "encode_seconds": encode_s, "encode_mvec_per_s": args.fixed_n / encode_s / 1e6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_int8_full<true> (INT8 query x 4-bit code, fixed lists)",
+                     "kernel": "qg::k_int8_full_grouped<1024> (INT8 query x 4-bit code, fixed lists)",
                      "kernel_ms": kernel_ms, "bytes_per_unit": unit_bytes, "units_per_launch": units},
-        "cpu_baseline": cpu,
-        "gpu_vs_reference_id_parity": parity,
+        "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity,
         "setup_seconds": time.perf_counter() - t_setup,
     }
+    del index, x, q, cand, gt
+    torch.cuda.empty_cache()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("all", "graph", "fixed"), default="all")
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=1536)
+    ap.add_argument("--nq", type=int, default=1000)
+    ap.add_argument("--R", type=int, default=64)
+    ap.add_argument("--l-build", type=int, default=400)
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--prune-cap", type=int, default=256)
+    ap.add_argument("--lists", default="100,200,400", help="search list sizes swept")
+    ap.add_argument("--fixed-n", type=int, default=1_000_000)
+    ap.add_argument("--fixed-dim", type=int, default=960)
+    ap.add_argument("--fixed-nq", type=int, default=10_000)
+    ap.add_argument("--per", type=int, default=1000)
+    ap.add_argument("--depths", default="100,150,200,300,400,600",
+                    help="fixed-candidate rerank depths swept")
+    ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--cpu-queries", type=int, default=2000)
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--cpu-build-records", type=int, default=20000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (a plain copy kernel of known size, "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only; --gpus %d is not supported by this build" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import quantization_graph_amd as qg
+
+    torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
+        return
+
+    graph = run_graph(args, torch, np, qg) if args.workload in ("all", "graph") else None
+    fixed_res = run_fixed(args, torch, np, qg) if args.workload in ("all", "fixed") else None
+
+    metric = "QPS at Recall@10>=0.95, learned vs plain rerank; Gdist/s vs HBM roofline"
+    if graph is not None:
+        h = graph["head"]
+        out = {
+            "metric": metric, "value": h["qps"], "unit": "queries/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": h["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "Vamana build (M=%d, L_build=%d, alpha 1.2, refine %d, prune cap %d) + batched "
+                                   "search of %d queries (FP32 codec, L=%d, residual4 top-100) over %d x %d "
+                                   "power-law normalised vectors (seed 1)"
+                                   % (args.R, args.l_build, args.refine, args.prune_cap, args.nq, h["L"], args.n,
+                                      args.dim),
+                       "base": args.n, "dim": args.dim, "queries": args.nq, "search_list": h["L"],
+                       "rerank": 100, "codec": "full", "graph_degree": args.R, "l_build": args.l_build},
+            "recall_at_10": h["recall_at_10"], "recall_target_met": h["recall_at_10"] >= args.target_recall,
+            "rerank": "plain top-100 residual4",
+            "search_sweep": graph["sweep"],
+            "encode_seconds": graph["encode_seconds"], "build_seconds": graph["build_seconds"],
+            "build_distance_evaluations": graph["build_distance_evaluations"],
+            "build_evals_per_s": graph["build_evals_per_s"],
+            "roofline": graph["roofline"], "cpu_baseline": graph["cpu_baseline"],
+            "cpu_build_baseline": graph["cpu_build_baseline"],
+            "gpu_vs_reference_id_parity": graph["gpu_vs_reference_id_parity"],
+            "fixed_candidates": fixed_res,
+        }
+    else:
+        f = fixed_res
+        out = {
+            "metric": metric, "value": f["value"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int8", "data": "synthetic", "config": f["config"],
+            "recall_at_10": f["recall_at_10"], "roofline": f["roofline"], "cpu_baseline": f["cpu_baseline"],
+            "fixed_candidates": f,
+        }
     print(json.dumps(out), flush=True)
 
 

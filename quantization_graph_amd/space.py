@@ -294,14 +294,15 @@ class RabitqSpace:
 
     # ------------------------------------------------------------ search
     def search(self, index, queries, degrees, edges, k=10, l_search=100, eps0=1.9, rerank=100,
-               slots=None, want_pool=False):
+               slots=None, want_pool=False, events=None):
         """Batched graph search (search_ours_paper_active_inner, ours_diskann.rs:1539-1911)
         over a graph given as degrees [n] and edges [n, R] (R <= 64), start
         node 0, with the query codec the queries were prepared with.
 
         Returns (ids [nq, k] int32, estimates [nq, k], stats [nq, 4] int64 =
         hops, gate evaluations, distances, pruned[, pool ids, pool
-        distances [nq, max(l_search, k)]])."""
+        distances [nq, max(l_search, k)]]). `events` = (start, end) CUDA
+        events recorded on the launch stream around the search kernel."""
         degrees = _dev(degrees, torch.int32).reshape(-1)
         edges = _dev(edges, torch.int32)
         if degrees.numel() != index.n or edges.shape[0] != index.n:
@@ -319,9 +320,13 @@ class RabitqSpace:
         ws_bytes = L_.qg_search_workspace_bytes(index.n, slots)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
         di, dq = index.desc(), queries.desc()
+        if events is not None:
+            events[0].record(torch.cuda.current_stream())
         check(L_.qg_search(self.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
                            edges.shape[1], k, l_search, C.c_float(eps0), rerank, slots, _ptr(ids), _ptr(dists),
                            _ptr(stats), _ptr(pool_ids), _ptr(pool_d), _ptr(ws), ws_bytes, _stream()))
+        if events is not None:
+            events[1].record(torch.cuda.current_stream())
         err = ws[4:8].view(torch.int32).item()
         if err:
             raise _lib.QGError("graph contains an out-of-range neighbour")

@@ -477,3 +477,49 @@ class Ref:
                                                 _p(fac), C.byref(ms), C.byref(fb), C.byref(de))
         assert ok
         return deg, edges, msb, fac, de.value
+
+
+REF_SEARCH_SO = os.path.join(ROOT, "oracle", "lib", "libref_search.so")
+
+
+class RefSearch:
+    """The Rust search loop restated in C (oracle/ref_search.c) driving the
+    reference bridge's compiled kernels; one instance per Ref space."""
+
+    _lib = None
+
+    @staticmethod
+    def available():
+        return Ref.available() and os.path.exists(REF_SEARCH_SO)
+
+    def __init__(self, ref):
+        if RefSearch._lib is None:
+            L = C.CDLL(REF_SEARCH_SO)
+            L.rs_open.restype = C.c_int
+            L.rs_open.argtypes = [C.c_char_p]
+            if L.rs_open(REF_SO.encode()) != 0:
+                raise OSError("cannot bind the reference bridge")
+            L.rs_search_range.restype = C.c_int
+            L.rs_search_range.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int,
+                                          C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                          C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_size_t, C.c_float, C.c_size_t, C.c_void_p, C.c_void_p]
+            RefSearch._lib = L
+        self.ref = ref
+
+    def search(self, queries, codec, comp, res, msb, fac, degrees, edges, k=10, L=100, eps0=1.9, rerank=100,
+               q0=0, q1=None, out=None):
+        """Searches queries[q0:q1] into out = (ids [nq, k], dists [nq, k]);
+        the space's codec must already be set to `codec`."""
+        queries = np.ascontiguousarray(queries, np.float32)
+        nq, dim = queries.shape
+        q1 = nq if q1 is None else q1
+        if out is None:
+            out = (np.zeros((nq, k), np.uint32), np.zeros((nq, k), np.float32))
+        rc = RefSearch._lib.rs_search_range(self.ref.h, _p(queries), dim, q0, q1, codec, _p(comp), comp.shape[1],
+                                            _p(res), res.shape[1], _p(msb), msb.shape[1], _p(fac), fac.itemsize,
+                                            _p(degrees), _p(edges), edges.shape[1], comp.shape[0], k, L,
+                                            C.c_float(eps0), rerank, _p(out[0]), _p(out[1]))
+        if rc != 0:
+            raise RuntimeError("reference search failed")
+        return out

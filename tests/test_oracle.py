@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from conftest import gaussian_mixture
-from oracle_bind import ESTIMATE_DTYPE, FACTOR_DTYPE, INTERVAL_DTYPE, Oracle, Ref
+from oracle_bind import ESTIMATE_DTYPE, FACTOR_DTYPE, INTERVAL_DTYPE, Oracle, Ref, RefSearch
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.npz")
 
@@ -296,4 +296,32 @@ def test_search_visits_everything_when_pool_never_fills():
             est = o.residual(oq, res[top], long_d)["estimate"]
             final = sorted(range(100), key=lambda j: (est[j], top[j]))[:10]
             assert list(ids) == [top[j] for j in final]
+            o.free_query(oq)
+
+
+@pytest.mark.skipif(not RefSearch.available(), reason="reference bridge not built (oracle/_ref)")
+def test_search_matches_reference_kernels():
+    """The oracle's search (restated estimators) vs the same loop driving the
+    reference bridge's compiled kernels: identical ids and estimates, for the
+    FP32 and INT8 query codecs, on a graph the reference built."""
+    n, dim = 1500, 128
+    x, q = gaussian_mixture(n, dim, 8, 0.3, 21, nq=8)
+    c = x.mean(0).astype(np.float32)
+    o = Oracle(dim, 100)
+    o.set_center(c)
+    comp, res = o.split(o.encode_full(x))
+    msb, fac = o.sidecar(comp)
+    r = Ref(dim, 100)
+    r.set_center(c)
+    deg, edges, _, _, _ = r.build_graph(comp, 16, 64, 1.2, beam=1, batch=256, refine=1, cap=128)
+    rs = RefSearch(r)
+    for codec in (0, 3):
+        r.set_codec(codec)
+        ids, ds = rs.search(q, codec, comp, res, msb, fac, deg, edges, k=10, L=48, eps0=1.9, rerank=100)
+        for i in range(q.shape[0]):
+            oq = o.prepare(q[i], codec)
+            o_ids, o_d, _, _, _ = o.search(oq, codec, comp, res, msb, fac, deg, edges, k=10, L=48, eps0=1.9,
+                                           rerank=100)
+            assert np.array_equal(ids[i, :len(o_ids)], o_ids), (codec, i)
+            assert np.array_equal(ds[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
             o.free_query(oq)
