@@ -190,10 +190,11 @@ struct MergeArgs {
     uint32_t *edges;
     const uint64_t *merged_off;  // [ngroups]
     uint32_t *merged;            // merged neighbour lists of overflowing groups
-    uint32_t *job_src;           // prune jobs (overflowing groups)
-    uint64_t *job_off;
+    uint32_t *job_src;           // prune jobs (overflowing groups), one region of
+    uint64_t *job_off;           // job_cap entries per size class
     uint32_t *job_len;
-    uint32_t *job_count;
+    uint32_t *job_count;         // [4] jobs per size class
+    uint64_t job_cap;
 };
 
 }  // namespace qg

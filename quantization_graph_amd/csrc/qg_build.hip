@@ -432,6 +432,203 @@ __global__ void __launch_bounds__(kPruneThreads) k_prune(PruneArgs a) {
     if (t == 0 && ev) atomicAdd(a.evals, ev);
 }
 
+// --------------------------------------------------------- gram prune --
+// robust_prune_4bit for a listed pool of up to MP-1 candidates whose
+// distances from the source are computed here (the back-edge re-prunes of
+// addBackedgeGroupAndPrune, vamana_index.h:1091-1116). One wavefront per
+// job. The exact integer parts ip4(a, b) = sum (2a_i - 15)(2b_i - 15) of every
+// pair among {source} + pool are one int8 GEMM G = C C^T on the matrix cores
+// (v_mfma_i32_16x16x32_i8, exact int32 accumulation; the dimension order
+// inside a K block is the same permutation for both operands, which leaves
+// every dot product unchanged); G is kept as a triangle in LDS. The sweep
+// then follows the reference's sequential loop: the k-th selected node sits
+// in lane k, so "some selected node occludes the visited candidate" is a
+// ballot and the first occluder in selection order is its lowest set bit,
+// which gives the memoised-pair counts of the reference exactly.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t tri_index(uint32_t i, uint32_t j) {  // i < j
+    return j * (j - 1) / 2 + i;
+}
+
+__device__ __forceinline__ long pack_centered(uint32_t w) {
+    return (long)(((unsigned long long)centered_bytes((w >> 4) & 0x0F0F0F0Fu) << 32) |
+                  centered_bytes(w & 0x0F0F0F0Fu));
+}
+
+template <int CD, int MP>
+__global__ void __launch_bounds__(64) k_prune_gram(PruneArgs a) {
+    constexpr int KQ = CD / 32;  // code dwords per lane group
+    constexpr int NP2 = MP <= 128 ? 128 : 256;
+    __shared__ int sG[MP * (MP - 1) / 2];
+    __shared__ unsigned long long sKey[NP2];
+    __shared__ uint16_t sRow[NP2];
+    __shared__ float2 sH[MP];
+    __shared__ uint32_t sId[MP];
+    __shared__ float sD[NP2];
+    __shared__ uint8_t sCnt[NP2];
+    __shared__ uint8_t sSel[NP2];
+    const int lane = threadIdx.x;
+    const uint32_t job = blockIdx.x;
+    const uint32_t R = a.R;
+    const uint32_t src = a.sources[job];
+    const uint32_t m = a.extra_len[job];
+    const uint32_t *ex = a.extra + a.extra_off[job];
+    if (m + 1 > MP) {  // the host sorts jobs into size classes
+        if (lane == 0) { atomicOr(a.err, 2u); a.out_deg[src] = 0; }
+        return;
+    }
+    for (uint32_t r = lane; r < MP; r += 64) {
+        const uint32_t id = r == 0 ? src : (r <= m ? ex[r - 1] : src);
+        sId[r] = id;
+        sH[r] = a.hdr[id];
+    }
+    __syncthreads();
+    // G over rows 0..m (row 0 = source), 16-row blocks
+    const uint32_t nb = (m + 16) / 16;
+    const int g = lane >> 4, rl = lane & 15;
+    for (uint32_t bi = 0; bi < nb; ++bi) {
+        const uint32_t ra = 16 * bi + rl;
+        uint32_t A[KQ];
+        {
+            const uint32_t *rp = (const uint32_t *)(a.codes + (uint64_t)sId[ra < MP ? ra : 0] * (CD / 2)) + g * KQ;
+            const bool va = ra <= m;
+#pragma unroll
+            for (int t = 0; t < KQ; ++t) A[t] = va ? rp[t] : 0u;
+        }
+        for (uint32_t bj = bi; bj < nb; ++bj) {
+            const uint32_t rb = 16 * bj + rl;
+            const uint32_t *rp = (const uint32_t *)(a.codes + (uint64_t)sId[rb < MP ? rb : 0] * (CD / 2)) + g * KQ;
+            const bool vb = rb <= m;
+            v4i_t acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < KQ; ++t) {
+                const uint32_t b = vb ? rp[t] : 0u;
+                acc = __builtin_amdgcn_mfma_i32_16x16x32_i8(pack_centered(A[t]), pack_centered(b), acc, 0, 0, 0);
+            }
+            // lane holds rows 16bi + 4g + r, column 16bj + rl
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = 16 * bi + 4 * g + r, j = rb;
+                if (i < j && j <= m) sG[tri_index(i, j)] = acc[r];
+            }
+        }
+    }
+    __syncthreads();
+    // pool distances from the source (lhs = source), sorted by (distance, id)
+    const float2 sh = sH[0];
+    for (uint32_t r = 1 + lane; r <= m; r += 64) {
+        const float d = symmetric_finish(sh, sH[r], sG[tri_index(0, r)]);
+        sKey[r - 1] = prune_key(d, sId[r]);
+        sRow[r - 1] = (uint16_t)r;
+    }
+    unsigned long long ev = m;
+    __syncthreads();
+    uint32_t np2 = 1;
+    while (np2 < m) np2 <<= 1;
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t b = lane; b < np2 / 2; b += 64) {
+                uint32_t lo, hi;
+                if (stride == size >> 1) {
+                    const uint32_t blk = b / stride, r = b % stride;
+                    lo = blk * size + r;
+                    hi = blk * size + size - 1 - r;
+                } else {
+                    lo = 2 * stride * (b / stride) + (b % stride);
+                    hi = lo + stride;
+                }
+                if (hi < m) {
+                    const unsigned long long k0 = sKey[lo], k1 = sKey[hi];
+                    if (k0 > k1) {
+                        sKey[lo] = k1;
+                        sKey[hi] = k0;
+                        const uint16_t t0 = sRow[lo];
+                        sRow[lo] = sRow[hi];
+                        sRow[hi] = t0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // drop the source and repeated ids (adjacent after the sort), compact
+    uint32_t u = 0;
+    unsigned long long prev = ~0ull;
+    for (uint32_t base = 0; base < m; base += 64) {
+        const uint32_t p = base + lane;
+        const unsigned long long k = p < m ? sKey[p] : ~0ull;
+        const uint16_t row = p < m ? sRow[p] : 0;
+        const unsigned long long before = (unsigned long long)__shfl_up((long long)k, 1);
+        const unsigned long long pk = lane == 0 ? prev : before;
+        const bool keep = p < m && (uint32_t)k != src && k != pk;
+        const uint64_t msk = __ballot(keep);
+        prev = (unsigned long long)__shfl((long long)k, 63);
+        __syncthreads();
+        if (keep) {
+            const uint32_t dst = u + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+            sKey[dst] = k;
+            sRow[dst] = row;
+        }
+        u += (uint32_t)__popcll(msk);
+        __syncthreads();
+    }
+    for (uint32_t p = lane; p < u; p += 64) {
+        sD[p] = float_from_order_key((uint32_t)(sKey[p] >> 32));
+        sCnt[p] = 0;
+        sSel[p] = 0;
+    }
+    __syncthreads();
+
+    const float alpha = a.alpha;
+    const float inc = (alpha < 1.2f) ? alpha : 1.2f;
+    float cur = 1.0f;
+    uint32_t nsel = 0;
+    uint32_t my_row = 0;  // row of the selected node held by this lane
+    float2 my_h = make_float2(0.0f, 0.0f);
+    while (nsel < R) {
+        bool promoted = false;
+        for (uint32_t p = 0; p < u && nsel < R; ++p) {
+            if (sSel[p]) continue;
+            const uint32_t ri = sRow[p];
+            const float Di = sD[p];
+            const float2 hi = sH[ri];
+            bool occ = false, neg = false;
+            if ((uint32_t)lane < nsel) {
+                const uint32_t lo = ri < my_row ? ri : my_row, hi_r = ri < my_row ? my_row : ri;
+                const float d = symmetric_finish(hi, my_h, sG[tri_index(lo, hi_r)]);  // lhs = candidate
+                occ = cur * d <= Di;
+                neg = d < 0.0f;
+            }
+            const uint64_t om = __ballot(occ), nm = __ballot(neg);
+            const uint32_t e = om ? (uint32_t)__ffsll((unsigned long long)om) : nsel;
+            const uint32_t cp = sCnt[p];
+            const uint32_t lim = e < cp ? e : cp;
+            // new memo fills, plus memoised negative distances computed again
+            ev += (e > cp ? e - cp : 0u) + (uint32_t)__popcll(nm & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull)));
+            sCnt[p] = (uint8_t)(e > cp ? e : cp);
+            if (!om) {
+                sSel[p] = 1;
+                if ((uint32_t)lane == nsel) { my_row = ri; my_h = hi; }
+                ++nsel;
+                promoted = true;
+            }
+        }
+        if (cur == alpha) break;
+        const float nx = cur * inc;
+        const float next = (nx < alpha) ? nx : alpha;
+        if (next == cur) break;
+        cur = next;
+        if (!promoted && nsel >= R) break;
+    }
+    for (uint32_t i = lane; i < R; i += 64)
+        a.out_edges[(uint64_t)src * R + i] = i < nsel ? sId[my_row] : kNone;
+    if (lane == 0) {
+        a.out_deg[src] = nsel;
+        atomicAdd(a.evals, ev);
+    }
+}
+
 // ---------------------------------------------------------- back-edges --
 // (neighbour << 32 | row) for rows [row_base, row_base + rows); empty slots
 // are ~0 and sort last
@@ -459,6 +656,13 @@ __global__ void k_merged_sizes(const uint32_t *group_src, const uint32_t *group_
     uint64_t s = 0;
     if (g < *ngroups && group_src[g] != kNone) s = (uint64_t)deg[group_src[g]] + group_len[g];
     sizes[g] = s;
+}
+
+// prune size classes of listed pools: k_prune_gram<., 80 / 128 / 168> and,
+// above 167 candidates, k_prune
+constexpr int kGramMP[3] = {80, 128, 168};
+__device__ __forceinline__ uint32_t gram_class(uint32_t m) {
+    return m < 80 ? 0u : (m < 128 ? 1u : (m < 168 ? 2u : 3u));
 }
 
 // addBackedgeGroupAndPrune, merge half: the current list, then each sorted
@@ -498,7 +702,8 @@ __global__ void __launch_bounds__(64) k_backedge_merge(MergeArgs a) {
         for (uint32_t i = lane; i < total; i += 64) a.edges[(uint64_t)src * a.R + i] = out[i];
         if (lane == 0) a.degrees[src] = total;
     } else if (lane == 0) {
-        const uint32_t j = atomicAdd(a.job_count, 1u);
+        const uint32_t c = gram_class(total);
+        const uint64_t j = (uint64_t)c * a.job_cap + atomicAdd(&a.job_count[c], 1u);
         a.job_src[j] = src;
         a.job_off[j] = a.merged_off[g];
         a.job_len[j] = total;
@@ -507,7 +712,7 @@ __global__ void __launch_bounds__(64) k_backedge_merge(MergeArgs a) {
 
 // ------------------------------------------------------------ host side --
 struct BuildLayout {
-    uint64_t n, R, L, J, P, M, slots;
+    uint64_t n, R, L, J, P, M, Jc, slots;
     size_t cub_bytes;
     size_t off_ctl, off_tags, off_pool_d, off_pool_id, off_pool_len, off_rdeg, off_redges, off_keys_a, off_keys_b,
         off_src, off_tgt, off_gsrc, off_glen, off_gbeg, off_msize, off_moff, off_merged, off_bkeys, off_bmin,
@@ -573,16 +778,19 @@ static int build_layout(uint64_t n, uint32_t R, uint32_t L, uint32_t batch, uint
     l->off_bmin = take(l->M * 4);
     l->off_bpre = take(l->M * 4);
     l->off_bstate = take(l->M * 4);
-    l->off_jsrc = take(l->P * 4);
-    l->off_joff = take(l->P * 8);
-    l->off_jlen = take(l->P * 4);
+    l->Jc = l->P < n ? l->P : n;  // prune jobs per back-edge round <= distinct sources
+    if (l->Jc == 0) l->Jc = 1;
+    l->off_jsrc = take(4 * l->Jc * 4);
+    l->off_joff = take(4 * l->Jc * 8);
+    l->off_jlen = take(4 * l->Jc * 4);
     l->off_cub = take(l->cub_bytes);
     l->total = o;
     return 0;
 }
 
 struct Ctl {
-    uint32_t counter, err, ngroups, njobs;
+    uint32_t counter, err, ngroups, pad;
+    uint32_t njobs[4];
     unsigned long long evals;
 };
 
@@ -716,7 +924,7 @@ struct Builder {
         tb = l.cub_bytes;
         if (check_hip((int)hipcub::DeviceScan::ExclusiveSum(temp, tb, msize, moff, (int)P, st), "merged offsets"))
             return -1;
-        if (check_hip(launch_fill(&ctl()->njobs, 1, 0u, st), "job count reset")) return -1;
+        if (check_hip(launch_fill(ctl()->njobs, 4, 0u, st), "job count reset")) return -1;
         Ctl h;
         if (read_ctl(&h)) return -1;
         MergeArgs m;
@@ -733,25 +941,48 @@ struct Builder {
         m.job_src = at<uint32_t>(l.off_jsrc);
         m.job_off = at<uint64_t>(l.off_joff);
         m.job_len = at<uint32_t>(l.off_jlen);
-        m.job_count = &ctl()->njobs;
+        m.job_count = ctl()->njobs;
+        m.job_cap = l.Jc;
         if (h.ngroups) {
             hipLaunchKernelGGL(k_backedge_merge, dim3(h.ngroups), dim3(64), 0, st, m);
             if (check_hip((int)hipGetLastError(), "back-edge merge")) return -1;
         }
         if (read_ctl(&h)) return -1;
-        PruneArgs p = prune_args();
-        p.jobs = h.njobs;
-        p.sources = m.job_src;
-        p.extra = m.merged;
-        p.extra_off = m.job_off;
-        p.extra_len = m.job_len;
-        p.big_keys = at<unsigned long long>(l.off_bkeys);
-        p.big_min = at<float>(l.off_bmin);
-        p.big_pre = at<float>(l.off_bpre);
-        p.big_state = at<uint32_t>(l.off_bstate);
-        p.out_deg = deg;
-        p.out_edges = edges;
-        return prune(p);
+        for (int c = 0; c < 4; ++c) {
+            PruneArgs p = prune_args();
+            p.jobs = h.njobs[c];
+            p.sources = m.job_src + c * l.Jc;
+            p.extra = m.merged;
+            p.extra_off = m.job_off + c * l.Jc;
+            p.extra_len = m.job_len + c * l.Jc;
+            p.big_keys = at<unsigned long long>(l.off_bkeys);
+            p.big_min = at<float>(l.off_bmin);
+            p.big_pre = at<float>(l.off_bpre);
+            p.big_state = at<uint32_t>(l.off_bstate);
+            p.out_deg = deg;
+            p.out_edges = edges;
+            if ((c < 3 ? prune_gram(p, c) : prune(p)) != 0) return -1;
+        }
+        return 0;
+    }
+
+    int prune_gram(const PruneArgs &p, int c) {
+        if (p.jobs == 0) return 0;
+        switch (s->cd * 4 + c) {
+#define QG_GRAM_CASE(CD)                                                                                        \
+    case CD * 4 + 0: hipLaunchKernelGGL((k_prune_gram<CD, kGramMP[0]>), dim3(p.jobs), dim3(64), 0, st, p); break; \
+    case CD * 4 + 1: hipLaunchKernelGGL((k_prune_gram<CD, kGramMP[1]>), dim3(p.jobs), dim3(64), 0, st, p); break; \
+    case CD * 4 + 2: hipLaunchKernelGGL((k_prune_gram<CD, kGramMP[2]>), dim3(p.jobs), dim3(64), 0, st, p); break;
+            QG_GRAM_CASE(64)
+            QG_GRAM_CASE(128)
+            QG_GRAM_CASE(256)
+            QG_GRAM_CASE(512)
+            QG_GRAM_CASE(1024)
+            QG_GRAM_CASE(2048)
+#undef QG_GRAM_CASE
+            default: return fail("no Gram prune kernel for code dimension %zu", s->cd);
+        }
+        return check_hip((int)hipGetLastError(), "Gram prune launch");
     }
 };
 
