@@ -210,6 +210,13 @@ size_t rabitq_residual_bits(const void *space);                                 
 bool rabitq_encode_full(const void *space, const float *raw, void *encoded_out);       /* :195 */
 bool rabitq_copy_compact(const void *space, const void *full, void *compact_out);      /* :204 */
 bool rabitq_copy_residual_record(const void *space, const void *full, void *out);      /* :214 */
+bool rabitq_build_vamana_graph(void *space, const void *compact_records, size_t record_count,
+                               size_t compact_stride, size_t r, size_t l_build, float alpha, size_t beam_width,
+                               size_t batch_size, size_t refine_passes, size_t prune_candidate_cap,
+                               size_t build_early_stop_hops, uint32_t *degrees_out, uint32_t *edges_out,
+                               size_t edge_stride, uint8_t *paper_msb_out, void *paper_factors_out,
+                               size_t *paper_msb_stride_out, size_t *paper_factor_bytes_out,
+                               uint64_t *distance_evaluations_out);                   /* :224 */
 size_t rabitq_paper_msb_code_bytes(const void *space);                                 /* :305 */
 size_t rabitq_paper_factor_bytes(void);                                                /* :313 */
 bool rabitq_export_paper_sidecar(const void *space, const void *compact_records, size_t record_count,

@@ -264,6 +264,82 @@ bool rabitq_export_paper_sidecar(const void *space, const void *compact_records_
     return true;
 }
 
+// rabitq_build_vamana_graph (rabitq_bridge.cpp:224-303): the records are
+// staged to the device once, the graph is built by qg_build_graph, and
+// degrees, edges (degree entries per row; the rest of each row untouched),
+// the paper sidecar and the evaluation counter are copied back. R <= 64 and
+// L_build <= 1024 are limits of the device build (false beyond them).
+bool rabitq_build_vamana_graph(void *space, const void *compact_records_in, size_t record_count,
+                               size_t compact_stride, size_t r, size_t l_build, float alpha, size_t beam_width,
+                               size_t batch_size, size_t refine_passes, size_t prune_candidate_cap,
+                               size_t build_early_stop_hops, uint32_t *degrees_out, uint32_t *edges_out,
+                               size_t edge_stride, uint8_t *paper_msb_out, void *paper_factors_out,
+                               size_t *paper_msb_stride_out, size_t *paper_factor_bytes_out,
+                               uint64_t *distance_evaluations_out) {
+    Space *s = static_cast<Space *>(space);
+    if (!s || (!compact_records_in && record_count) || !degrees_out || !edges_out || edge_stride < r ||
+        !paper_msb_out || !paper_factors_out || !paper_msb_stride_out || !paper_factor_bytes_out)
+        return false;
+    if (compact_stride < s->compact_bytes) return false;
+    if (r == 0 || l_build == 0 || beam_width == 0) return false;  // VamanaIndex constructor
+    if (record_count > 1 && (!(alpha >= 1.0f) || !std::isfinite(alpha))) return false;
+    if (r > 64 || l_build > 1024 || beam_width > 64 || batch_size > 0xFFFFFFFFu || refine_passes > 0xFFFFu ||
+        prune_candidate_cap > 0xFFFFFFFFu || build_early_stop_hops > 0xFFFFFFFFu || record_count > 0xFFFFFFF0u)
+        return false;
+    *paper_msb_stride_out = (s->cd + 7) / 8;
+    *paper_factor_bytes_out = sizeof(RabitqPaperFactors);
+    if (record_count == 0) {
+        if (distance_evaluations_out) *distance_evaluations_out = 0;
+        return true;
+    }
+    std::vector<uint32_t> ids(record_count);
+    for (size_t i = 0; i < record_count; ++i) ids[i] = static_cast<uint32_t>(i);
+    std::vector<uint8_t> comp;
+    gather(compact_records_in, compact_stride, s->compact_bytes, ids.data(), record_count, &comp);
+    Arena a;
+    if (!arena_for(s, record_count * (s->compact_bytes + 2 * s->cd + 64) + 2 * record_count * (r + 1) * 4, &a))
+        return false;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, nullptr, record_count, &d)) return false;
+    uint32_t *deg = static_cast<uint32_t *>(a.take(record_count * 4));
+    uint32_t *edges = static_cast<uint32_t *>(a.take(record_count * r * 4));
+    if (!deg || !edges) return false;
+    uint64_t evals = 0;
+    std::vector<uint32_t> hdeg(record_count, 0), hedges(record_count * r);
+    if (record_count > 1) {
+        const size_t ws_bytes = qg_build_workspace_bytes(record_count, (uint32_t)r, (uint32_t)l_build,
+                                                         batch_size ? (uint32_t)batch_size : 1u,
+                                                         (uint32_t)refine_passes);
+        void *ws = nullptr;
+        if (ws_bytes == 0 || hipMalloc(&ws, ws_bytes) != hipSuccess) return false;
+        const int rc = qg_build_graph(s, &d, (uint32_t)r, (uint32_t)l_build, alpha, (uint32_t)beam_width,
+                                      (uint32_t)batch_size, (uint32_t)refine_passes, (uint32_t)prune_candidate_cap,
+                                      (uint32_t)build_early_stop_hops, deg, edges, &evals, ws, ws_bytes, nullptr);
+        (void)hipFree(ws);
+        if (rc != 0) return false;
+        if (hipMemcpy(hdeg.data(), deg, record_count * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        if (hipMemcpy(hedges.data(), edges, record_count * r * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    }
+    const size_t mb = s->cd / 8;
+    std::vector<float> f(record_count * 4);
+    std::vector<uint8_t> v(record_count);
+    if (hipMemcpy(paper_msb_out, d.msb, record_count * mb, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpy(f.data(), d.factors, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (hipMemcpy(v.data(), d.fvalid, v.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    RabitqPaperFactors *o = static_cast<RabitqPaperFactors *>(paper_factors_out);
+    for (size_t i = 0; i < record_count; ++i) {
+        o[i].norm_sqr = f[4 * i];
+        o[i].data_norm = f[4 * i + 1];
+        o[i].cross_scale = f[4 * i + 2];
+        o[i].error_cross_scale = f[4 * i + 3];
+        o[i].valid = v[i];
+        degrees_out[i] = hdeg[i];
+        std::memcpy(edges_out + i * edge_stride, hedges.data() + i * r, hdeg[i] * 4);
+    }
+    if (distance_evaluations_out) *distance_evaluations_out = evals;
+    return true;
+}
+
 float rabitq_symmetric_distance(const void *space, const void *lhs, const void *rhs) {
     Space *s = const_cast<Space *>(static_cast<const Space *>(space));
     const float inf = std::numeric_limits<float>::infinity();

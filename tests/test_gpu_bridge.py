@@ -91,3 +91,38 @@ def test_bridge_error_sentinels(gpu):
     assert not L.rabitq_encode_full(ours.h, None, None)
     L.rabitq_symmetric_distance.restype = C.c_float
     assert np.isinf(L.rabitq_symmetric_distance(ours.h, None, None))
+
+
+@pytest.mark.parametrize("refine", [0, 1])
+def test_bridge_build_matches_reference(gpu, refine):
+    """rabitq_build_vamana_graph through both libraries on the same records:
+    same degrees, adjacency lists, sidecar and distance_evaluations."""
+    if not Ref.available():
+        pytest.skip("oracle/_ref not built")
+    n, dim = 700, 256
+    x, _ = gaussian_mixture(n, dim, 8, 0.3, 13)
+    c = x.mean(0).astype(np.float32)
+    ref, ours = Ref(dim, 100), Ours(dim, 100)
+    for s in (ref, ours):
+        s.set_center(c)
+    comp, _ = ref.split(ref.encode_full(x))
+    d1, e1, m1, f1, v1 = ref.build_graph(comp, 24, 48, 1.2, beam=1, batch=128, refine=refine, cap=0)
+    d2, e2, m2, f2, v2 = ours.build_graph(comp, 24, 48, 1.2, beam=1, batch=128, refine=refine, cap=0)
+    assert np.array_equal(d1, d2)
+    for i in range(n):
+        assert np.array_equal(e1[i, :d1[i]], e2[i, :d2[i]])
+    assert np.array_equal(m1, m2)
+    assert f1.tobytes() == f2.tobytes()
+    assert v1 == v2
+
+
+def test_bridge_build_rejects_what_the_reference_rejects(gpu):
+    ours = Ours(128, 100)
+    x, _ = gaussian_mixture(32, 128, 2, 0.3, 1)
+    comp, _ = ours.split(ours.encode_full(x))
+    with pytest.raises(AssertionError):
+        ours.build_graph(comp, 0, 48, 1.2)        # R = 0
+    with pytest.raises(AssertionError):
+        ours.build_graph(comp, 16, 48, 0.5)       # alpha < 1
+    with pytest.raises(AssertionError):
+        ours.build_graph(comp, 16, 0, 1.2)        # L_build = 0
