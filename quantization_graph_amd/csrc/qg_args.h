@@ -149,6 +149,7 @@ struct PruneArgs {
     const float2 *hdr;
     uint32_t cd;
     uint32_t jobs;
+    const uint32_t *job_ids;   // optional: block b runs job job_ids[b]
     uint64_t src_base;         // source of job j: sources[j] or src_base + j
     const uint32_t *sources;
     // part A: a sorted search pool, first min(len, cap) entries

@@ -81,15 +81,35 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 // ------------------------------------------------------------- search --
+// Without a gate, inserting the fresh neighbours one by one into a pool
+// truncated at L (insert_candidate_sorted) leaves exactly the L smallest of
+// pool + fresh by (distance, id): the pool's back only shrinks, so a
+// candidate rejected at its turn would be rejected later too. Each
+// expansion therefore sorts its fresh neighbours across the wavefront
+// (bitonic network over lane shuffles) and merges them with the pool into
+// the second LDS buffer at their merge ranks.
+__device__ __forceinline__ void wave_sort_keys(unsigned long long &key, uint32_t &payload, int lane) {
+    for (int k = 2; k <= 64; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const unsigned long long ok = (unsigned long long)__shfl_xor((long long)key, j);
+            const uint32_t op = (uint32_t)__shfl_xor((int)payload, j);
+            const bool up = (lane & k) == 0;     // ascending block
+            const bool lower = (lane & j) == 0;  // lower index of the pair
+            const bool take = (lower == up) ? (ok < key) : (ok > key);
+            if (take) { key = ok; payload = op; }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kSearchLanes) k_build_search(BuildSearchArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd, L = a.L;
     uint32_t *lc = (uint32_t *)smem;                                // cd/4 dwords
-    float *pd = (float *)(smem + cd);                               // [L]
-    uint32_t *pid = (uint32_t *)(smem + cd + 4 * (size_t)L);       // [L]
-    uint8_t *pexp = smem + cd + 8 * (size_t)L;                      // [L]
-    __shared__ uint32_t fresh[kSearchLanes];
-    __shared__ float fdist[kSearchLanes];
+    float *bd[2] = {(float *)(smem + cd), (float *)(smem + cd + 4 * (size_t)L)};
+    uint32_t *bid[2] = {(uint32_t *)(smem + cd + 8 * (size_t)L), (uint32_t *)(smem + cd + 12 * (size_t)L)};
+    uint8_t *bexp[2] = {smem + cd + 16 * (size_t)L, smem + cd + 17 * (size_t)L};
+    __shared__ float fd_s[kSearchLanes];
+    __shared__ uint32_t fid_s[kSearchLanes];
     __shared__ uint32_t frontier[kSearchLanes];
     const int lane = threadIdx.x;
     uint32_t *tags = a.tags + (uint64_t)blockIdx.x * a.tag_stride;
@@ -103,17 +123,21 @@ __global__ void __launch_bounds__(kSearchLanes) k_build_search(BuildSearchArgs a
         stage_centered(lc, a.codes + (uint64_t)qid * (cd / 2), cd, lane, kSearchLanes);
         const float2 qh = a.hdr[qid];
         __syncthreads();
+        int cur = 0;
         unsigned long long evals = 1;
         if (lane == 0) {
             tags[0] = tag;
-            pd[0] = symmetric_finish(qh, a.hdr[0], symmetric_ip4_prepared(lc, a.codes, cd));
-            pid[0] = 0;
-            pexp[0] = 0;
+            bd[0][0] = symmetric_finish(qh, a.hdr[0], symmetric_ip4_prepared(lc, a.codes, cd));
+            bid[0][0] = 0;
+            bexp[0][0] = 0;
         }
         uint32_t len = 1, stall = 0;
         bool bad = false;
         __syncthreads();
         for (;;) {
+            float *pd = bd[cur];
+            uint32_t *pid = bid[cur];
+            uint8_t *pexp = bexp[cur];
             const float back_before = pd[len - 1];
             // frontier: the first `beam` unexpanded entries in pool order
             uint32_t nfr = 0;
@@ -152,52 +176,64 @@ __global__ void __launch_bounds__(kSearchLanes) k_build_search(BuildSearchArgs a
                     is_fresh = tags[nb] != tag;
                     if (is_fresh) tags[nb] = tag;
                 }
-                const uint64_t fm = __ballot(is_fresh);
-                const uint32_t nf = (uint32_t)__popcll(fm);
-                if (is_fresh) {
-                    const uint32_t rank = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
-                    fresh[rank] = nb;
-                    fdist[rank] = symmetric_finish(qh, a.hdr[nb],
-                                                   symmetric_ip4_prepared(lc, a.codes + (uint64_t)nb * (cd / 2), cd));
-                }
+                const uint32_t nf = (uint32_t)__popcll(__ballot(is_fresh));
+                if (nf == 0) continue;
                 evals += nf;
-                __syncthreads();
-                for (uint32_t b = 0; b < nf; ++b) {  // insert_candidate_sorted, in fresh order
-                    const uint32_t id = fresh[b];
-                    const float d = fdist[b];
-                    const bool full = len >= L;
-                    if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
-                    uint32_t cnt = 0;
-                    for (uint32_t i = lane; i < len; i += kSearchLanes) cnt += cand_less(pd[i], pid[i], d, id) ? 1u : 0u;
-                    for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
-                    const uint32_t pos = cnt;
-                    const uint32_t last = full ? L - 1 : len;  // [pos, last) moves up by one
-                    for (uint32_t c = 0; pos + c < last; c += kSearchLanes) {
-                        const bool mv = c + (uint32_t)lane < last - pos;
-                        const uint32_t idx = last - 1 - c - (uint32_t)lane;
-                        float md = 0.0f;
-                        uint32_t mi = 0;
-                        uint8_t me = 0;
-                        if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
-                        __syncthreads();
-                        if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
-                        __syncthreads();
-                    }
-                    if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
-                    if (!full) ++len;
-                    __syncthreads();
+                float d = 0.0f;
+                if (is_fresh)
+                    d = symmetric_finish(qh, a.hdr[nb], symmetric_ip4_prepared(lc, a.codes + (uint64_t)nb * (cd / 2), cd));
+                unsigned long long key = is_fresh ? prune_key(d, nb) : ~0ull;
+                uint32_t dbits = __float_as_uint(d);
+                wave_sort_keys(key, dbits, lane);
+                if ((uint32_t)lane < nf) {
+                    fd_s[lane] = __uint_as_float(dbits);
+                    fid_s[lane] = (uint32_t)key;
                 }
+                __syncthreads();
+                float *nd = bd[cur ^ 1];
+                uint32_t *nid = bid[cur ^ 1];
+                uint8_t *nexp = bexp[cur ^ 1];
+                // pool entries move up by the number of fresh entries before them
+                for (uint32_t i = lane; i < len; i += kSearchLanes) {
+                    const float pdv = pd[i];
+                    const uint32_t piv = pid[i];
+                    uint32_t lo = 0, hi = nf;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (cand_less(fd_s[mid], fid_s[mid], pdv, piv)) lo = mid + 1; else hi = mid;
+                    }
+                    const uint32_t at = i + lo;
+                    if (at < L) { nd[at] = pdv; nid[at] = piv; nexp[at] = pexp[i]; }
+                }
+                // fresh entries land after the pool entries before them
+                if ((uint32_t)lane < nf) {
+                    const float fv = fd_s[lane];
+                    const uint32_t fi = fid_s[lane];
+                    uint32_t lo = 0, hi = len;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (cand_less(pd[mid], pid[mid], fv, fi)) lo = mid + 1; else hi = mid;
+                    }
+                    const uint32_t at = lane + lo;
+                    if (at < L) { nd[at] = fv; nid[at] = fi; nexp[at] = 0; }
+                }
+                len = (len + nf < L) ? len + nf : L;
+                cur ^= 1;
+                pd = nd;
+                pid = nid;
+                pexp = nexp;
+                __syncthreads();
             }
             if (bad) break;
             if (a.early_stop != 0 && len >= L) {
-                const float back_after = pd[len - 1];
+                const float back_after = bd[cur][len - 1];
                 stall = (back_after >= back_before) ? stall + 1 : 0;
                 if (stall >= a.early_stop) break;
             }
         }
         for (uint32_t i = lane; i < len; i += kSearchLanes) {
-            a.pool_d[(uint64_t)job * L + i] = pd[i];
-            a.pool_id[(uint64_t)job * L + i] = pid[i];
+            a.pool_d[(uint64_t)job * L + i] = bd[cur][i];
+            a.pool_id[(uint64_t)job * L + i] = bid[cur][i];
         }
         if (lane == 0) {
             a.pool_len[job] = bad ? 0u : len;
@@ -240,7 +276,7 @@ __global__ void __launch_bounds__(kPruneThreads) k_prune(PruneArgs a) {
     __shared__ unsigned long long s_red64[kPruneThreads / 64];
     const int t = threadIdx.x;
     const int lane = t & 63, wv = t >> 6;
-    const uint32_t job = blockIdx.x;
+    const uint32_t job = a.job_ids ? a.job_ids[blockIdx.x] : blockIdx.x;
     const uint32_t cd = a.cd, R = a.R;
     const uint32_t src = a.sources ? a.sources[job] : (uint32_t)(a.src_base + job);
     uint32_t na = 0;
@@ -629,6 +665,243 @@ __global__ void __launch_bounds__(64) k_prune_gram(PruneArgs a) {
     }
 }
 
+// ---------------------------------------------------- pool gram prune --
+// robust_prune_4bit for search pools (the batch inserts and refine passes,
+// vamana_index.h:372-396, 486-499): part A = the first min(len, cap)
+// entries of the job's sorted search pool, part B = the record's current
+// adjacency row (refine only; its distances from the record are counted).
+// Four wavefronts: ids are de-duplicated (a repeated id carries one
+// distance, so dropping repeats by id equals the reference's dedup of
+// adjacent equal (distance, id) pairs) and the source is dropped; the Gram
+// matrix of {source} + pool comes from int8 MFMA with the block rows spread
+// over the waves; every pool distance is then re-derived from G with the
+// source as lhs (the function the search used, so the values are the
+// search's). G lives in dynamic LDS as 24-bit entries (|ip4| <= 225 cd <
+// 2^19). Wave 0 then runs the same sequential sweep as k_prune_gram. Jobs
+// whose de-duplicated pool does not fit mp rows are appended to the
+// overflow list and pruned by k_prune.
+__device__ __forceinline__ int g24(const uint16_t *lo, const int8_t *hi, uint32_t t) {
+    return ((int)hi[t] << 16) | (int)lo[t];
+}
+
+template <int CD>
+__global__ void __launch_bounds__(256) k_prune_pool_gram(PruneArgs a, uint32_t mp, uint32_t *overflow,
+                                                         uint32_t *overflow_count) {
+    constexpr int KQ = CD / 32;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t t_count = mp * (mp - 1) / 2;
+    // layout: [ids, later keys + rows: 8 KB][G lo16][G hi8][sH][sId][sD][sCnt][sSel]
+    uint32_t *sIds = (uint32_t *)smem;                          // [<= 2048] ids, later keys
+    unsigned long long *sKey = (unsigned long long *)smem;      // [<= 512]
+    uint16_t *sRow = (uint16_t *)(smem + 4096);                 // [<= 512]
+    uint16_t *gLo = (uint16_t *)(smem + 8192);
+    int8_t *gHi = (int8_t *)(smem + 8192 + 2 * (size_t)t_count);
+    size_t o = (8192 + 3 * (size_t)t_count + 15) & ~(size_t)15;
+    float2 *sH = (float2 *)(smem + o);
+    o += 8 * (size_t)mp;
+    uint32_t *sId = (uint32_t *)(smem + o);
+    o += 4 * (size_t)mp;
+    float *sD = (float *)(smem + o);
+    o += 4 * (size_t)mp;
+    uint8_t *sCnt = smem + o;
+    o += mp;
+    uint8_t *sSel = smem + o;
+    __shared__ uint32_t s_red[4];
+    __shared__ uint32_t s_mu;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const uint32_t job = blockIdx.x;
+    const uint32_t R = a.R;
+    const uint32_t src = (uint32_t)(a.src_base + job);
+    uint32_t na = a.pool_len[job];
+    if (a.cap != 0 && na > a.cap) na = a.cap;
+    uint32_t nb = 0;
+    const uint32_t *ex = nullptr;
+    if (a.graph_deg) {
+        nb = a.graph_deg[src];
+        ex = a.graph_edges + (uint64_t)src * R;
+    }
+    const uint32_t m = na + nb;  // <= 2048 (L <= 1024, R <= 64)
+    for (uint32_t i = t; i < m; i += 256) sIds[i] = i < na ? a.pool_id[(uint64_t)job * a.pool_stride + i] : ex[i - na];
+    __syncthreads();
+    uint32_t np2 = 1;
+    while (np2 < m) np2 <<= 1;
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t b = t; b < np2 / 2; b += 256) {
+                uint32_t lo, hi;
+                if (stride == size >> 1) {
+                    lo = (b / stride) * size + (b % stride);
+                    hi = (b / stride) * size + size - 1 - (b % stride);
+                } else {
+                    lo = 2 * stride * (b / stride) + (b % stride);
+                    hi = lo + stride;
+                }
+                if (hi < m) {
+                    const uint32_t k0 = sIds[lo], k1 = sIds[hi];
+                    if (k0 > k1) { sIds[lo] = k1; sIds[hi] = k0; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // unique ids other than the source, in id order
+    uint32_t mu = 0;
+    for (uint32_t base = 0; base < m; base += 256) {
+        const uint32_t i = base + t;
+        const uint32_t id = i < m ? sIds[i] : kNone;
+        const bool keep = i < m && id != src && !(i > 0 && sIds[i - 1] == id);
+        const uint64_t msk = __ballot(keep);
+        if (lane == 0) s_red[wv] = (uint32_t)__popcll(msk);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < 4; ++w) { before += w < wv ? s_red[w] : 0u; total += s_red[w]; }
+        if (keep && mu + before + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull)) + 1 < mp)
+            sId[1 + mu + before + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull))] = id;
+        mu += total;
+        __syncthreads();
+    }
+    if (mu + 1 > mp) {
+        if (t == 0) overflow[atomicAdd(overflow_count, 1u)] = job;
+        return;
+    }
+    for (uint32_t r = t; r <= mu; r += 256) {
+        const uint32_t id = r == 0 ? src : sId[r];
+        if (r == 0) sId[0] = src;
+        sH[r] = a.hdr[id];
+    }
+    __syncthreads();
+    // Gram matrix over rows 0..mu, block rows bi = wv, wv + 4, ...
+    const uint32_t nblk = (mu + 16) / 16;
+    const int g = lane >> 4, rl = lane & 15;
+    for (uint32_t bi = wv; bi < nblk; bi += 4) {
+        const uint32_t ra = 16 * bi + rl;
+        uint32_t A[KQ];
+        {
+            const bool va = ra <= mu;
+            const uint32_t *rp = (const uint32_t *)(a.codes + (uint64_t)sId[va ? ra : 0] * (CD / 2)) + g * KQ;
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) A[q] = va ? rp[q] : 0u;
+        }
+        for (uint32_t bj = bi; bj < nblk; ++bj) {
+            const uint32_t rb = 16 * bj + rl;
+            const bool vb = rb <= mu;
+            const uint32_t *rp = (const uint32_t *)(a.codes + (uint64_t)sId[vb ? rb : 0] * (CD / 2)) + g * KQ;
+            v4i_t acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < KQ; ++q) {
+                const uint32_t bw = vb ? rp[q] : 0u;
+                acc = __builtin_amdgcn_mfma_i32_16x16x32_i8(pack_centered(A[q]), pack_centered(bw), acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t i = 16 * bi + 4 * g + r, j = rb;
+                if (i < j && j <= mu) {
+                    const uint32_t ti = tri_index(i, j);
+                    gLo[ti] = (uint16_t)(acc[r] & 0xFFFF);
+                    gHi[ti] = (int8_t)(acc[r] >> 16);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // pool distances with the source as lhs, sorted by (distance, id)
+    const float2 sh = sH[0];
+    for (uint32_t r = 1 + t; r <= mu; r += 256) {
+        const float d = symmetric_finish(sh, sH[r], g24(gLo, gHi, tri_index(0, r)));
+        sKey[r - 1] = prune_key(d, sId[r]);
+        sRow[r - 1] = (uint16_t)r;
+    }
+    __syncthreads();
+    np2 = 1;
+    while (np2 < mu) np2 <<= 1;
+    for (uint32_t size = 2; size <= np2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t b = t; b < np2 / 2; b += 256) {
+                uint32_t lo, hi;
+                if (stride == size >> 1) {
+                    lo = (b / stride) * size + (b % stride);
+                    hi = (b / stride) * size + size - 1 - (b % stride);
+                } else {
+                    lo = 2 * stride * (b / stride) + (b % stride);
+                    hi = lo + stride;
+                }
+                if (hi < mu) {
+                    const unsigned long long k0 = sKey[lo], k1 = sKey[hi];
+                    if (k0 > k1) {
+                        sKey[lo] = k1;
+                        sKey[hi] = k0;
+                        const uint16_t t0 = sRow[lo];
+                        sRow[lo] = sRow[hi];
+                        sRow[hi] = t0;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t p = t; p < mu; p += 256) {
+        sD[p] = float_from_order_key((uint32_t)(sKey[p] >> 32));
+        sCnt[p] = 0;
+        sSel[p] = 0;
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    // sequential sweep on wave 0 (as k_prune_gram)
+    const uint32_t u = mu;
+    unsigned long long ev = nb;
+    const float alpha = a.alpha;
+    const float inc = (alpha < 1.2f) ? alpha : 1.2f;
+    float cur = 1.0f;
+    uint32_t nsel = 0;
+    uint32_t my_row = 0;
+    float2 my_h = make_float2(0.0f, 0.0f);
+    while (nsel < R) {
+        bool promoted = false;
+        for (uint32_t p = 0; p < u && nsel < R; ++p) {
+            if (sSel[p]) continue;
+            const uint32_t ri = sRow[p];
+            const float Di = sD[p];
+            const float2 hi = sH[ri];
+            bool occ = false, neg = false;
+            if ((uint32_t)lane < nsel) {
+                const uint32_t lo = ri < my_row ? ri : my_row, hr = ri < my_row ? my_row : ri;
+                const float d = symmetric_finish(hi, my_h, g24(gLo, gHi, tri_index(lo, hr)));
+                occ = cur * d <= Di;
+                neg = d < 0.0f;
+            }
+            const uint64_t om = __ballot(occ), nm = __ballot(neg);
+            const uint32_t e = om ? (uint32_t)__ffsll((unsigned long long)om) : nsel;
+            const uint32_t cp = sCnt[p];
+            const uint32_t lim = e < cp ? e : cp;
+            ev += (e > cp ? e - cp : 0u) + (uint32_t)__popcll(nm & (lim >= 64 ? ~0ull : ((1ull << lim) - 1ull)));
+            sCnt[p] = (uint8_t)(e > cp ? e : cp);
+            if (!om) {
+                sSel[p] = 1;
+                if ((uint32_t)lane == nsel) { my_row = ri; my_h = hi; }
+                ++nsel;
+                promoted = true;
+            }
+        }
+        if (cur == alpha) break;
+        const float nx = cur * inc;
+        const float next = (nx < alpha) ? nx : alpha;
+        if (next == cur) break;
+        cur = next;
+        if (!promoted && nsel >= R) break;
+    }
+    for (uint32_t i = lane; i < R; i += 64) a.out_edges[(uint64_t)src * R + i] = i < nsel ? sId[my_row] : kNone;
+    if (lane == 0) {
+        a.out_deg[src] = nsel;
+        atomicAdd(a.evals, ev);
+    }
+}
+
+static size_t pool_gram_lds(uint32_t mp) {
+    const size_t t_count = (size_t)mp * (mp - 1) / 2;
+    size_t o = (8192 + 3 * t_count + 15) & ~(size_t)15;
+    return o + 8 * (size_t)mp + 4 * (size_t)mp + 4 * (size_t)mp + 2 * (size_t)mp;
+}
+
 // ---------------------------------------------------------- back-edges --
 // (neighbour << 32 | row) for rows [row_base, row_base + rows); empty slots
 // are ~0 and sort last
@@ -661,6 +934,7 @@ __global__ void k_merged_sizes(const uint32_t *group_src, const uint32_t *group_
 // prune size classes of listed pools: k_prune_gram<., 80 / 128 / 168> and,
 // above 167 candidates, k_prune
 constexpr int kGramMP[3] = {80, 128, 168};
+constexpr uint32_t kPoolGramMaxRows = 312;  // largest pool triangle that fits 160 KB of LDS
 __device__ __forceinline__ uint32_t gram_class(uint32_t m) {
     return m < 80 ? 0u : (m < 128 ? 1u : (m < 168 ? 2u : 3u));
 }
@@ -715,7 +989,7 @@ struct BuildLayout {
     uint64_t n, R, L, J, P, M, Jc, slots;
     size_t cub_bytes;
     size_t off_ctl, off_tags, off_pool_d, off_pool_id, off_pool_len, off_rdeg, off_redges, off_keys_a, off_keys_b,
-        off_src, off_tgt, off_gsrc, off_glen, off_gbeg, off_msize, off_moff, off_merged, off_bkeys, off_bmin,
+        off_src, off_tgt, off_overflow, off_gsrc, off_glen, off_gbeg, off_msize, off_moff, off_merged, off_bkeys, off_bmin,
         off_bpre, off_bstate, off_jsrc, off_joff, off_jlen, off_cub, total;
 };
 
@@ -752,7 +1026,11 @@ static int build_layout(uint64_t n, uint32_t R, uint32_t L, uint32_t batch, uint
     l->P = rows * R;
     if (l->P == 0) l->P = 1;
     l->M = (l->P < n ? l->P : n) * R + l->P;  // groups <= min(P, n), each grows by at most R
-    l->slots = l->J < 2048 ? l->J : 2048;
+    // concurrent searches: enough wavefronts to hide row-fetch latency, with
+    // the visit tags (slots x n u32) held to 16 GiB
+    uint64_t slots = (16ull << 30) / (4 * (n ? n : 1));
+    slots = slots < 256 ? 256 : (slots > 8192 ? 8192 : slots);
+    l->slots = l->J < slots ? l->J : slots;
     int rc = cub_temp_bytes(l->P, &l->cub_bytes);
     if (rc) return rc;
     size_t o = 0;
@@ -762,6 +1040,7 @@ static int build_layout(uint64_t n, uint32_t R, uint32_t L, uint32_t batch, uint
     l->off_pool_d = take(l->J * L * 4);
     l->off_pool_id = take(l->J * L * 4);
     l->off_pool_len = take(l->J * 4);
+    l->off_overflow = take(l->J * 4);
     l->off_rdeg = take(refine ? n * 4 : 0);
     l->off_redges = take(refine ? n * R * 4 : 0);
     l->off_keys_a = take(l->P * 8);
@@ -789,7 +1068,7 @@ static int build_layout(uint64_t n, uint32_t R, uint32_t L, uint32_t batch, uint
 }
 
 struct Ctl {
-    uint32_t counter, err, ngroups, pad;
+    uint32_t counter, err, ngroups, noverflow;
     uint32_t njobs[4];
     unsigned long long evals;
 };
@@ -849,7 +1128,7 @@ struct Builder {
         a.pool_id = at<uint32_t>(l.off_pool_id);
         a.pool_len = at<uint32_t>(l.off_pool_len);
         const uint32_t slots = (uint32_t)(jobs < l.slots ? jobs : l.slots);
-        const size_t smem = (size_t)s->cd + (size_t)L * 9;
+        const size_t smem = (size_t)s->cd + (size_t)L * 18;
         hipLaunchKernelGGL(k_build_search, dim3(slots), dim3(kSearchLanes), smem, st, a);
         return check_hip((int)hipGetLastError(), "build search launch");
     }
@@ -889,6 +1168,42 @@ struct Builder {
         p.graph_edges = exist_edges;
         p.out_deg = out_deg;
         p.out_edges = out_edges;
+        // Gram prune when the capped pool fits the LDS triangle; jobs whose
+        // de-duplicated pool does not fit go to k_prune
+        const uint32_t pool_max = (cap != 0 && cap < L) ? cap : L;
+        if (jobs == 0 || pool_max + 1 > kPoolGramMaxRows) return prune(p);
+        uint32_t need = pool_max + (exist_deg ? R : 0) + 1;
+        need = (need + 15) & ~15u;
+        const uint32_t mp = need < kPoolGramMaxRows ? need : kPoolGramMaxRows;
+        uint32_t *ovf = at<uint32_t>(l.off_overflow);
+        if (check_hip(launch_fill(&ctl()->noverflow, 1, 0u, st), "overflow reset")) return -1;
+        const size_t lds = pool_gram_lds(mp);
+        int rc = 0;
+        switch (s->cd) {
+#define QG_POOL_CASE(CD)                                                                                  \
+    case CD:                                                                                              \
+        rc = (int)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_prune_pool_gram<CD>),            \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);              \
+        if (rc == 0)                                                                                      \
+            hipLaunchKernelGGL(k_prune_pool_gram<CD>, dim3(jobs), dim3(256), lds, st, p, mp, ovf,         \
+                               &ctl()->noverflow);                                                        \
+        break;
+            QG_POOL_CASE(64)
+            QG_POOL_CASE(128)
+            QG_POOL_CASE(256)
+            QG_POOL_CASE(512)
+            QG_POOL_CASE(1024)
+            QG_POOL_CASE(2048)
+#undef QG_POOL_CASE
+            default: return prune(p);
+        }
+        if (check_hip(rc, "pool Gram attribute") || check_hip((int)hipGetLastError(), "pool Gram prune launch"))
+            return -1;
+        Ctl h;
+        if (read_ctl(&h)) return -1;
+        if (h.noverflow == 0) return 0;
+        p.jobs = h.noverflow;
+        p.job_ids = ovf;
         return prune(p);
     }
 

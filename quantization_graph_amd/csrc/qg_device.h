@@ -238,6 +238,7 @@ __device__ __forceinline__ int symmetric_ip4_rows(const uint8_t *lhs, const uint
 __device__ __forceinline__ int symmetric_ip4_prepared(const uint32_t *lc, const uint8_t *rhs, uint32_t cd) {
     const uint4 *rr = (const uint4 *)rhs;
     int acc = 0;
+#pragma unroll 8
     for (uint32_t c = 0; c < cd / 32; ++c) {
         const uint4 w = rr[c];
         const uint32_t ws[4] = {w.x, w.y, w.z, w.w};

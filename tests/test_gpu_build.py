@@ -44,6 +44,8 @@ CASES = [
     (800, 960, 32, 64, 1.2, 1, 128, 1, 0, 0),
     (600, 1536, 64, 96, 1.2, 1, 200, 1, 0, 0),
     (2600, 64, 16, 32, 1.2, 1, 2500, 0, 0, 0),  # a back-edge group above the LDS pool capacity
+    (1500, 64, 32, 320, 1.2, 1, 256, 1, 300, 0),  # refine pools above the LDS Gram triangle (overflow path)
+    (1200, 96, 24, 400, 1.2, 1, 300, 0, 0, 0),    # uncapped pools too large for the Gram triangle
 ]
 
 
