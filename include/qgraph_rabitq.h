@@ -179,6 +179,37 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
               float eps0, uint32_t rerank, uint32_t slots, uint32_t *out_ids, float *out_dists,
               uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
               void *stream);
+/* With out_ids == out_dists == NULL, qg_search stops after the traversal
+ * and only writes the pools (and stats): the input of the learned rerank. */
+
+/* ------------------------------------- learned adaptive rerank depth
+ * No counterpart in the reference. A 24-64-64-1 MLP (BF16 MFMA, FP32
+ * accumulation; parameters are qg_mlp_param_count() FP32 values laid out
+ * W1[64][24], b1[64], W2[64][64], b2[64], W3[64], b3) predicts a quantile
+ * of the 4-bit estimator's error |exact - estimate| / ||q||^2 for each of a
+ * query's first `depth` pool candidates; the rerank depth per query is the
+ * last candidate whose lower bound can beat the largest upper bound of the
+ * first min_depth, and the residual-4 rerank then runs on that many. With
+ * preds == NULL, qg_learned_rerank is the plain rerank of the search
+ * kernel (ours_diskann.rs:1813-1902). */
+size_t qg_mlp_param_count(void);
+size_t qg_mlp_workspace_bytes(uint64_t n);
+int qg_mlp_forward(const float *params, const uint16_t *features, uint64_t n, float *out, void *stream);
+int qg_mlp_grad(const float *params, const uint16_t *features, const float *targets, uint64_t n, float tau,
+                float *grad_out, float *loss_out, void *workspace, size_t workspace_bytes, void *stream);
+int qg_mlp_adam_step(float *params, float *m, float *v, const uint16_t *features, const float *targets, uint64_t n,
+                     float tau, float lr, float beta1, float beta2, float eps, uint32_t step, float *loss_out,
+                     void *workspace, size_t workspace_bytes, void *stream);
+/* per-vector residual block stats [n][4] (rms scale, max scale, zero-scale fraction, 0) */
+int qg_residual_stats(const void *space, const qg_index_desc *ix, float *out, void *stream);
+/* features [nq][depth][32] bf16 of the pools [nq][L] (first min(|pool|, depth) rows) */
+int qg_rerank_features(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, const uint32_t *pool_ids,
+                       const float *pool_dists, uint32_t L, uint32_t depth, const float *rstats, uint16_t *features,
+                       void *stream);
+int qg_learned_rerank(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                      const uint32_t *pool_ids, const float *pool_dists, uint32_t L, uint32_t depth,
+                      const float *preds, float kmult, uint32_t min_depth, uint32_t k, uint32_t *out_ids,
+                      float *out_dists, uint32_t *out_depth, void *stream);
 
 /* Vamana graph build with the symmetric 4-bit distance over the index's
  * primary codes: VamanaIndex::buildEncodedSymmetricBulk + refineGraphSymmetric

@@ -7,7 +7,9 @@ Public surface:
     DeviceQueries      prepared query batch in HBM
     topk               per-query top-k by (value, id)
     fixed              fixed-candidate scoring + residual rerank pipeline
+    learned            learned adaptive rerank depth (MLP kernels, training, calibration)
 """
+from . import fixed, learned  # noqa: F401
 from .space import (CODEC_FULL, CODEC_INT8, SCAN_ASYM, SCAN_GATE_FP32, SCAN_GATE_INT8,
                     SCAN_INT8_FULL, SCAN_REM_FP32, SCAN_REM_INT8, SCAN_RESIDUAL, DeviceIndex,
                     DeviceQueries, RabitqSpace, code_dim_for, estimates_np, intervals_np, topk)

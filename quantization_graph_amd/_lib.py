@@ -71,6 +71,16 @@ def lib():
     L.qg_build_workspace_bytes.argtypes = [u64, u32, u32, u32, u32]
     L.qg_build_graph.argtypes = [vp, C.POINTER(IndexDesc), u32, u32, f32, u32, u32, u32, u32, u32, vp, vp,
                                  C.POINTER(C.c_uint64), vp, sz, vp]
+    L.qg_mlp_param_count.restype = sz
+    L.qg_mlp_workspace_bytes.restype = sz
+    L.qg_mlp_workspace_bytes.argtypes = [u64]
+    L.qg_mlp_forward.argtypes = [vp, vp, u64, vp, vp]
+    L.qg_mlp_grad.argtypes = [vp, vp, vp, u64, f32, vp, vp, vp, sz, vp]
+    L.qg_mlp_adam_step.argtypes = [vp, vp, vp, vp, vp, u64, f32, f32, f32, f32, f32, u32, vp, vp, sz, vp]
+    L.qg_residual_stats.argtypes = [vp, C.POINTER(IndexDesc), vp, vp]
+    L.qg_rerank_features.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), vp, vp, u32, u32, vp, vp, vp]
+    L.qg_learned_rerank.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, u32, u32, vp,
+                                    f32, u32, u32, vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -25,6 +25,16 @@ int qg_launch_topk(const qg::SelectArgs *a, hipStream_t stream);
 int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream);
 int qg_launch_pack(const qg::PackArgs *a, uint64_t n, int unpack, hipStream_t stream);
 int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream);
+int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st);
+uint32_t qg_mlp_grad_waves(uint64_t n);
+int qg_launch_mlp_grad(const float *P, const uint16_t *X, const float *t, uint64_t n, float tau, float *partial,
+                       float *partial_loss, hipStream_t st);
+int qg_launch_mlp_reduce(const float *partial, const float *partial_loss, uint32_t waves, float *grad,
+                         float *loss_out, float *P, float *m, float *v, float lr, float b1, float b2, float eps,
+                         float bc1, float bc2, hipStream_t st);
+int qg_launch_residual_stats(const uint16_t *rscale, uint64_t n, uint32_t nb, float4 *out, hipStream_t st);
+int qg_launch_rerank_features(const qg::RerankArgs *a, hipStream_t st);
+int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st);
 }
 
 namespace qg {
@@ -376,8 +386,10 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
               uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
               void *stream) {
     const Space *s = static_cast<const Space *>(space);
-    if (!s || !ix || !qv || !degrees || !edges || !out_ids || !out_dists || !workspace)
+    const bool pool_only = out_ids == nullptr && out_dists == nullptr;
+    if (!s || !ix || !qv || !degrees || !edges || !workspace || (!pool_only && (!out_ids || !out_dists)))
         return fail("null argument to qg_search");
+    if (pool_only && !pool_ids) return fail("qg_search without outputs needs the pool outputs");
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
     if (k == 0) return fail("k must be positive");
     if (edge_stride == 0 || edge_stride > 64) return fail("graph degree bound must be in [1, 64]");
@@ -412,7 +424,116 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
     a.stats = out_stats;
     a.pool_ids = pool_ids;
     a.pool_d = pool_dists;
+    a.pool_only = pool_only ? 1 : 0;
     return check_hip(qg_launch_search(&a, slots, (hipStream_t)stream), "search launch");
+}
+
+// ------------------------------------------------- learned rerank depth
+size_t qg_mlp_param_count(void) { return 5825; }
+
+size_t qg_mlp_workspace_bytes(uint64_t n) {
+    return ((size_t)qg_mlp_grad_waves(n) * (5825 + 1) + 64) * 4;
+}
+
+int qg_mlp_forward(const float *params, const uint16_t *features, uint64_t n, float *out, void *stream) {
+    if (!params || (!features && n) || (!out && n)) return fail("null argument to qg_mlp_forward");
+    return check_hip(qg_launch_mlp_forward(params, features, n, out, (hipStream_t)stream), "mlp forward launch");
+}
+
+static int mlp_grad_common(const float *params, const uint16_t *features, const float *targets, uint64_t n,
+                           float tau, void *ws, size_t ws_bytes, float **partial, float **ploss, uint32_t *waves,
+                           hipStream_t st) {
+    if (!params || !features || !targets || !ws || n == 0) return fail("null argument or empty batch to the MLP trainer");
+    if (!(tau > 0.0f && tau < 1.0f)) return fail("quantile tau must be in (0, 1)");
+    if (ws_bytes < qg_mlp_workspace_bytes(n)) return fail("MLP workspace too small: need %zu bytes", qg_mlp_workspace_bytes(n));
+    *waves = qg_mlp_grad_waves(n);
+    *ploss = static_cast<float *>(ws);
+    *partial = static_cast<float *>(ws) + ((*waves + 63) / 64) * 64;
+    return check_hip(qg_launch_mlp_grad(params, features, targets, n, tau, *partial, *ploss, st), "mlp grad launch");
+}
+
+int qg_mlp_grad(const float *params, const uint16_t *features, const float *targets, uint64_t n, float tau,
+                float *grad_out, float *loss_out, void *workspace, size_t workspace_bytes, void *stream) {
+    float *partial, *ploss;
+    uint32_t waves;
+    hipStream_t st = (hipStream_t)stream;
+    if (!grad_out) return fail("null gradient output");
+    if (mlp_grad_common(params, features, targets, n, tau, workspace, workspace_bytes, &partial, &ploss, &waves, st))
+        return -1;
+    return check_hip(qg_launch_mlp_reduce(partial, ploss, waves, grad_out, loss_out, nullptr, nullptr, nullptr, 0, 0,
+                                          0, 0, 1, 1, st), "mlp reduce launch");
+}
+
+int qg_mlp_adam_step(float *params, float *m, float *v, const uint16_t *features, const float *targets, uint64_t n,
+                     float tau, float lr, float beta1, float beta2, float eps, uint32_t step, float *loss_out,
+                     void *workspace, size_t workspace_bytes, void *stream) {
+    float *partial, *ploss;
+    uint32_t waves;
+    hipStream_t st = (hipStream_t)stream;
+    if (!m || !v || step == 0) return fail("Adam needs moment buffers and a step count >= 1");
+    if (mlp_grad_common(params, features, targets, n, tau, workspace, workspace_bytes, &partial, &ploss, &waves, st))
+        return -1;
+    const float bc1 = 1.0f - std::pow(beta1, (float)step), bc2 = 1.0f - std::pow(beta2, (float)step);
+    return check_hip(qg_launch_mlp_reduce(partial, ploss, waves, nullptr, loss_out, params, m, v, lr, beta1, beta2,
+                                          eps, bc1, bc2, st), "mlp adam launch");
+}
+
+int qg_residual_stats(const void *space, const qg_index_desc *ix, float *out, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !ix || (!out && ix->n)) return fail("null argument to qg_residual_stats");
+    return check_hip(qg_launch_residual_stats(ix->rscale, ix->n, (uint32_t)s->block_count, reinterpret_cast<float4 *>(out),
+                                              (hipStream_t)stream), "residual stats launch");
+}
+
+static int rerank_args(const Space *s, const qg_index_desc *ix, const qg_query_desc *qv, const uint32_t *pool_ids,
+                       const float *pool_dists, uint32_t L, uint32_t depth, qg::RerankArgs *a) {
+    if (!s || !ix || !qv || !pool_ids || !pool_dists) return fail("null argument to the rerank stage");
+    if (depth == 0 || depth > 128) return fail("rerank depth must be in [1, 128]");
+    if (L == 0) return fail("pool stride must be positive");
+    std::memset(a, 0, sizeof(*a));
+    a->ix = view_of(ix);
+    a->rfac = reinterpret_cast<const float2 *>(ix->rfac);
+    a->qv = view_of(qv);
+    a->cd = (uint32_t)s->cd;
+    a->nq = qv->nq;
+    a->L = L;
+    a->depth = depth;
+    a->pool_ids = pool_ids;
+    a->pool_d = pool_dists;
+    return 0;
+}
+
+int qg_rerank_features(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, const uint32_t *pool_ids,
+                       const float *pool_dists, uint32_t L, uint32_t depth, const float *rstats, uint16_t *features,
+                       void *stream) {
+    qg::RerankArgs a;
+    if (rerank_args(static_cast<const Space *>(space), ix, qv, pool_ids, pool_dists, L, depth, &a)) return -1;
+    if (!rstats || !features) return fail("null residual stats or feature output");
+    a.rstats = reinterpret_cast<const float4 *>(rstats);
+    a.feats = features;
+    return check_hip(qg_launch_rerank_features(&a, (hipStream_t)stream), "rerank features launch");
+}
+
+int qg_learned_rerank(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                      const uint32_t *pool_ids, const float *pool_dists, uint32_t L, uint32_t depth,
+                      const float *preds, float kmult, uint32_t min_depth, uint32_t k, uint32_t *out_ids,
+                      float *out_dists, uint32_t *out_depth, void *stream) {
+    qg::RerankArgs a;
+    if (rerank_args(static_cast<const Space *>(space), ix, qv, pool_ids, pool_dists, L, depth, &a)) return -1;
+    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
+    if (k == 0 || k > 128) return fail("k must be in [1, 128]");
+    if (!out_ids || !out_dists) return fail("null rerank output");
+    if (preds && !(kmult >= 0.0f)) return fail("error multiplier must be >= 0");
+    if (preds && k > depth) return fail("k must not exceed the rerank depth when predictions are given");
+    a.codec = codec;
+    a.k = k;
+    a.min_depth = min_depth;
+    a.kmult = kmult;
+    a.preds = preds;
+    a.out_ids = out_ids;
+    a.out_d = out_dists;
+    a.out_depth = out_depth;
+    return check_hip(qg_launch_learned_rerank(&a, (hipStream_t)stream), "learned rerank launch");
 }
 
 }  // extern "C"

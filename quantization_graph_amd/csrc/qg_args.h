@@ -118,6 +118,7 @@ struct SearchArgs {
     float *out_d;              // [nq][k]
     uint64_t *stats;           // [nq][4] hops, gate evaluations, distances, pruned (optional)
     uint32_t *pool_ids;        // [nq][L] final pool (optional)
+    int pool_only;             // skip the rerank (out_ids / out_d unused)
     float *pool_d;             // [nq][L]
 };
 
@@ -196,6 +197,29 @@ struct MergeArgs {
     uint32_t *job_len;
     uint32_t *job_count;         // [4] jobs per size class
     uint64_t job_cap;
+};
+
+// learned rerank (qg_mlp.hip)
+struct RerankArgs {
+    IndexView ix;
+    const float2 *rfac;        // [n] {residual_norm_sqr, long_residual_ip}
+    const float4 *rstats;      // [n] {rms scale, max scale, zero-scale fraction, 0}
+    QueryView qv;
+    uint32_t cd;
+    int codec;                 // 0 Full, 3 Int8
+    uint64_t nq;
+    uint32_t L;                // pool row stride
+    uint32_t depth;            // largest rerank depth (feature rows per query)
+    uint32_t k;
+    uint32_t min_depth;
+    float kmult;
+    const uint32_t *pool_ids;  // [nq][L], 0xFFFFFFFF past the pool
+    const float *pool_d;       // [nq][L]
+    const float *preds;        // [nq][depth] predicted errors, or null (plain rerank)
+    uint16_t *feats;           // [nq][depth][32] bf16
+    uint32_t *out_ids;         // [nq][k]
+    float *out_d;              // [nq][k]
+    uint32_t *out_depth;       // [nq] (optional)
 };
 
 }  // namespace qg

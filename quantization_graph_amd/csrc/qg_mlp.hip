@@ -1,0 +1,473 @@
+// qg_mlp.hip -- learned adaptive rerank depth (no counterpart in the
+// reference; the plain top-100 residual-4 rerank stays the fallback and the
+// judge of its recall).
+//
+// A 24 -> 64 -> 64 -> 1 MLP predicts, for each of a query's first 100 pool
+// candidates, a high quantile of the 4-bit estimator's error
+// |exact - estimate| / ||q||^2. The per-query rerank depth is the last pool
+// position whose lower bound (estimate - k * error) can still beat the
+// largest upper bound of the first ten, clamped to [10, 100].
+//
+// Numerics (mirrored by the oracle in tests/mlp_oracle.py):
+//   X (bf16, 24 features zero-padded to 32), W1 and W2 rounded to bf16
+//   (round to nearest even) when loaded;
+//   Z1 = X W1^T (v_mfma_f32_16x16x32_bf16, FP32 accumulation) + b1;
+//   H1 = bf16(relu(Z1)); Z2 = H1 W2^T (MFMA) + b2; H2 = relu(Z2) in FP32;
+//   y = H2 . W3 + b3 in FP32.
+// Training: pinball loss at quantile tau on the target, mean over the
+// batch; FP32 backward on the same rounded operands (W1, W2, X, H1 as
+// bf16 values); per-wave partial gradients reduced in a fixed order, then
+// Adam on FP32 master weights.
+#include "qg_args.h"
+#include "qg_device.h"
+
+namespace qg {
+
+constexpr int kF = 24, kFP = 32, kH = 64;
+constexpr int kOffW1 = 0, kOffB1 = kH * kF, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH,
+              kOffB3 = kOffW3 + kH, kParams = kOffB3 + 1;  // 5825
+constexpr int kTilesPerWave = 8;                           // training: 128 samples per wave
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even, finite inputs
+    uint32_t b = __float_as_uint(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ bf16x8_t pack8(const uint16_t *v) {
+    short8_t s;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = (short)v[j];
+    return __builtin_bit_cast(bf16x8_t, s);
+}
+
+// Forward of one 16-sample tile held by a wavefront. Lane l = (g = l/16,
+// c = l%16) ends with Z1 / Z2 for rows 4g + r, columns 16nt + c; H1 (bf16)
+// is left in sh1[16][64]; y[4g + r] is returned in every lane of group g.
+struct TileOut {
+    float z1[4][4];  // [nt][r]
+    float z2[4][4];
+    float y[4];
+};
+
+__device__ __forceinline__ void tile_forward(const uint16_t *xrow, bool xvalid, const float *P, uint16_t *sh1,
+                                             TileOut &o) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    // A: X[row = c][k = 8g .. 8g+7]
+    uint16_t xa[8];
+    {
+        const uint4 v = xvalid ? *(const uint4 *)(xrow + 8 * g) : make_uint4(0, 0, 0, 0);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { xa[2 * j] = (uint16_t)(w[j] & 0xFFFF); xa[2 * j + 1] = (uint16_t)(w[j] >> 16); }
+    }
+    const bf16x8_t A1 = pack8(xa);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int oc = 16 * nt + c;
+        uint16_t wb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            wb[j] = k < kF ? f2bf(P[kOffW1 + oc * kF + k]) : (uint16_t)0;
+        }
+        f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, pack8(wb), acc, 0, 0, 0);
+        const float b1 = P[kOffB1 + oc];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float z = acc[r] + b1;
+            o.z1[nt][r] = z;
+            sh1[(4 * g + r) * kH + oc] = f2bf(z > 0.0f ? z : 0.0f);
+        }
+    }
+    __syncthreads();
+    bf16x8_t A2[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) A2[ks] = pack8(sh1 + c * kH + 32 * ks + 8 * g);
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int oc = 16 * nt + c;
+        f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            uint16_t wb[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wb[j] = f2bf(P[kOffW2 + oc * kH + 32 * ks + 8 * g + j]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[ks], pack8(wb), acc, 0, 0, 0);
+        }
+        const float b2 = P[kOffB2 + oc], w3 = P[kOffW3 + oc];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float z = acc[r] + b2;
+            o.z2[nt][r] = z;
+            part[r] = fmaf(z > 0.0f ? z : 0.0f, w3, part[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float v = part[r];
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
+        o.y[r] = v + P[kOffB3];
+    }
+}
+
+// y[i] for X [n][32] bf16; one wave per 16-sample tile, 4 tiles per block
+__global__ void __launch_bounds__(256) k_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y) {
+    __shared__ uint16_t sh1[4][16 * kH];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    const uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
+    const uint64_t row = tile * 16 + c;
+    TileOut o;
+    tile_forward(X + row * kFP, row < n, P, sh1[wv], o);
+    if (c == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint64_t rr = tile * 16 + 4 * g + r;
+            if (rr < n) y[rr] = o.y[r];
+        }
+    }
+}
+
+// Forward + backward of the pinball loss over a batch; each wave writes its
+// partial gradient [kParams] and partial loss.
+__global__ void __launch_bounds__(64) k_mlp_grad(const float *P, const uint16_t *X, const float *t, uint64_t n,
+                                                 float tau, float inv_n, float *partial, float *partial_loss) {
+    __shared__ uint16_t sh1[16 * kH];
+    __shared__ float sdz[16 * kH];
+    __shared__ float sh2[16 * kH];
+    __shared__ float sdy[16];
+    __shared__ uint16_t sx[16 * kFP];
+    const int lane = threadIdx.x, g = lane >> 4, c = lane & 15;
+    float gW2[kH], gW1[kF], w2col[kH];
+    float gB2 = 0.0f, gW3 = 0.0f, gB1 = 0.0f, gB3 = 0.0f, loss = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kH; ++i) { gW2[i] = 0.0f; w2col[i] = bf2f(f2bf(P[kOffW2 + i * kH + lane])); }
+#pragma unroll
+    for (int k = 0; k < kF; ++k) gW1[k] = 0.0f;
+    for (int ti = 0; ti < kTilesPerWave; ++ti) {
+        const uint64_t tile = (uint64_t)blockIdx.x * kTilesPerWave + ti;
+        if (tile * 16 >= n) break;
+        const uint64_t row = tile * 16 + c;
+        TileOut o;
+        tile_forward(X + row * kFP, row < n, P, sh1, o);
+        // dL/dy, pinball: -tau if t >= y else 1 - tau; zero for padding rows
+        if (c == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint64_t rr = tile * 16 + 4 * g + r;
+                float dy = 0.0f;
+                if (rr < n) {
+                    const float u = t[rr] - o.y[r];
+                    dy = (u >= 0.0f ? -tau : 1.0f - tau) * inv_n;
+                    loss += (u >= 0.0f ? tau * u : (tau - 1.0f) * u) * inv_n;
+                }
+                sdy[4 * g + r] = dy;
+            }
+        }
+        for (uint32_t i = lane; i < 16 * kFP; i += 64) {
+            const uint64_t rr = tile * 16 + i / kFP;
+            sx[i] = rr < n ? X[rr * kFP + (i % kFP)] : (uint16_t)0;
+        }
+        __syncthreads();
+        // dZ2 = dy * W3 * [Z2 > 0]; H2 kept for dW3
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int oc = 16 * nt + c;
+            const float w3 = P[kOffW3 + oc];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row_r = 4 * g + r;
+                const float z = o.z2[nt][r];
+                sh2[row_r * kH + oc] = z > 0.0f ? z : 0.0f;
+                sdz[row_r * kH + oc] = z > 0.0f ? sdy[row_r] * w3 : 0.0f;
+            }
+        }
+        __syncthreads();
+        // lane = output unit o of layer 2 (W2 row) and hidden unit i of layer 1
+        for (int rr = 0; rr < 16; ++rr) {
+            const float dz = sdz[rr * kH + lane];
+            gB2 += dz;
+            gW3 = fmaf(sdy[rr], sh2[rr * kH + lane], gW3);
+#pragma unroll
+            for (int i = 0; i < kH; ++i) gW2[i] = fmaf(dz, bf2f(sh1[rr * kH + i]), gW2[i]);
+        }
+        if (lane == 0)
+            for (int rr = 0; rr < 16; ++rr) gB3 += sdy[rr];
+        // dH1[row][lane] = sum_o dZ2[row][o] W2[o][lane]; dZ1 = dH1 [Z1 > 0]
+        float dh1[16];
+        for (int rr = 0; rr < 16; ++rr) {
+            float v = 0.0f;
+#pragma unroll
+            for (int oo = 0; oo < kH; ++oo) v = fmaf(sdz[rr * kH + oo], w2col[oo], v);
+            dh1[rr] = v;
+        }
+        __syncthreads();
+        // Z1 signs: lane (g, c) holds Z1[4g + r][16nt + c]; publish through sdz
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sdz[(4 * g + r) * kH + 16 * nt + c] = o.z1[nt][r] > 0.0f ? 1.0f : 0.0f;
+        __syncthreads();
+        for (int rr = 0; rr < 16; ++rr) {
+            const float dz1 = dh1[rr] * sdz[rr * kH + lane];
+            gB1 += dz1;
+#pragma unroll
+            for (int k = 0; k < kF; ++k) gW1[k] = fmaf(dz1, bf2f(sx[rr * kFP + k]), gW1[k]);
+        }
+        __syncthreads();
+    }
+    float *out = partial + (uint64_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int k = 0; k < kF; ++k) out[kOffW1 + lane * kF + k] = gW1[k];
+    out[kOffB1 + lane] = gB1;
+#pragma unroll
+    for (int i = 0; i < kH; ++i) out[kOffW2 + lane * kH + i] = gW2[i];
+    out[kOffB2 + lane] = gB2;
+    out[kOffW3 + lane] = gW3;
+    if (lane == 0) out[kOffB3] = gB3;
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) partial_loss[blockIdx.x] = loss;
+}
+
+// grad[p] = sum over waves in order; optional Adam step on P
+__global__ void k_mlp_reduce(const float *partial, const float *partial_loss, uint32_t waves, float *grad,
+                             float *loss_out, float *P, float *m, float *v, float lr, float b1, float b2, float eps,
+                             float bc1, float bc2) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && loss_out) {
+        float l = 0.0f;
+        for (uint32_t w = 0; w < waves; ++w) l += partial_loss[w];
+        *loss_out = l;
+    }
+    if (p >= (uint32_t)kParams) return;
+    float gsum = 0.0f;
+    for (uint32_t w = 0; w < waves; ++w) gsum += partial[(uint64_t)w * kParams + p];
+    if (grad) grad[p] = gsum;
+    if (P) {
+        const float mm = b1 * m[p] + (1.0f - b1) * gsum;
+        const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
+        m[p] = mm;
+        v[p] = vv;
+        P[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    }
+}
+
+// per-vector residual stats: rms and max of the fp16 block scales, fraction
+// of blocks with a zero or non-finite scale
+__global__ void k_residual_stats(const uint16_t *rscale, uint64_t n, uint32_t nb, float4 *out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s2 = 0.0;
+    float mx = 0.0f;
+    uint32_t zero = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const float s = half_to_float_ref(rscale[i * nb + b]);
+        if (s == 0.0f || !__builtin_isfinite(s)) { ++zero; continue; }
+        s2 += (double)s * (double)s;
+        mx = fabsf(s) > mx ? fabsf(s) : mx;
+    }
+    out[i] = make_float4((float)sqrt(s2 / nb), mx, (float)zero / (float)nb, 0.0f);
+}
+
+__device__ __forceinline__ float clampf(float v) {
+    if (!__builtin_isfinite(v)) return 0.0f;
+    return v < -1e4f ? -1e4f : (v > 1e4f ? 1e4f : v);
+}
+
+// features [nq][depth][32] (bf16) of the first min(len, depth) pool entries
+__global__ void k_rerank_features(RerankArgs a) {
+    const uint64_t qi = blockIdx.x;
+    const uint32_t i = threadIdx.x;
+    if (qi >= a.nq || i >= a.depth) return;
+    const uint32_t *pid = a.pool_ids + qi * a.L;
+    const float *pd = a.pool_d + qi * a.L;
+    uint32_t len = 0;
+    while (len < a.L && len < a.depth && pid[len] != 0xFFFFFFFFu) ++len;
+    uint16_t *out = a.feats + (qi * a.depth + i) * kFP;
+    if (i >= len) {
+        for (int k = 0; k < kFP; ++k) out[k] = 0;
+        return;
+    }
+    const QScalars s = a.qv.qs[qi];
+    const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
+    const float qn = s.query_norm;
+    const uint32_t id = pid[i];
+    const float e = pd[i];
+    const float e0 = pd[0], e9 = pd[len > 9 ? 9 : len - 1];
+    const float enext = i + 1 < len ? pd[i + 1] : e, eprev = i > 0 ? pd[i - 1] : e;
+    const float elo = pd[i >= 5 ? i - 5 : 0], ehi = pd[i + 5 < len ? i + 5 : len - 1];
+    const float2 h = a.ix.hdr[id];
+    const float2 sf = a.ix.sfac[id];
+    const float2 rf = a.rfac[id];
+    const float4 rs = a.rstats[id];
+    const float sqcd = sqrtf((float)a.cd);
+    const float nbk = (float)(a.cd / 16);
+    const float bound = sf.y * qn;
+    float f[kF];
+    f[0] = e / S;
+    f[1] = (e - e0) / S;
+    f[2] = (e - e9) / S;
+    f[3] = (float)i / 100.0f;
+    f[4] = logf(1.0f + (float)i) / logf(101.0f);
+    f[5] = (enext - e) / S;
+    f[6] = (e - eprev) / S;
+    f[7] = bound / S;
+    f[8] = h.x / S;
+    f[9] = sqrtf(h.x > 0.0f ? h.x : 0.0f) * qn / S;
+    f[10] = h.y * qn / S;
+    f[11] = sf.x * qn / S;
+    f[12] = e0 / S;
+    f[13] = e9 / S;
+    f[14] = rf.x / S;
+    f[15] = rf.y / S;
+    f[16] = rs.x * qn * sqcd / S;
+    f[17] = rs.y * qn * sqcd / S;
+    f[18] = rs.z;
+    f[19] = 4.0f * qn * rs.x * sqrtf(nbk) / S;
+    f[20] = (float)i / (float)len;
+    f[21] = (ehi - elo) / S;
+    f[22] = e / (e9 > 0.0f ? e9 : 1.0f);
+    f[23] = (e - e9) / (bound + 1e-12f);
+#pragma unroll
+    for (int k = 0; k < kF; ++k) out[k] = f2bf(clampf(f[k]));
+#pragma unroll
+    for (int k = kF; k < kFP; ++k) out[k] = 0;
+}
+
+// Per-query depth from the predicted errors, then the residual-4 rerank of
+// that many pool entries (the same per-candidate arithmetic as the search
+// kernel's rerank) and the top-k by (estimate, id). One wavefront per query.
+__global__ void __launch_bounds__(64) k_learned_rerank(RerankArgs a) {
+    __shared__ float r_est[128];
+    __shared__ uint32_t r_id[128];
+    const uint64_t qi = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t *pid = a.pool_ids + qi * a.L;
+    const float *pd = a.pool_d + qi * a.L;
+    // rerank count as the search computes it: min(|pool|, max(rerank, k))
+    uint32_t len = 0;
+    for (uint32_t i = lane; i < a.L; i += 64) len += pid[i] != 0xFFFFFFFFu ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) len += (uint32_t)__shfl_xor((int)len, off);
+    const uint32_t want = a.depth > a.k ? a.depth : a.k;
+    const uint32_t rc = len < want ? len : want;
+    const QScalars s = a.qv.qs[qi];
+    uint32_t depth = rc;
+    if (a.preds && rc > 0) {
+        const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
+        const uint32_t top = rc < a.min_depth ? rc : a.min_depth;
+        float ub = -__builtin_huge_valf();
+        for (uint32_t j = lane; j < top; j += 64) {
+            const float pr = a.preds[qi * a.depth + j];
+            ub = fmaxf(ub, pd[j] + a.kmult * (pr > 0.0f ? pr : 0.0f) * S);
+        }
+        for (int off = 32; off > 0; off >>= 1) ub = fmaxf(ub, __shfl_xor(ub, off));
+        uint32_t last = 0;
+        for (uint32_t j = lane; j < rc; j += 64) {
+            const float pr = a.preds[qi * a.depth + j];
+            const float lb = pd[j] - a.kmult * (pr > 0.0f ? pr : 0.0f) * S;
+            if (lb <= ub) last = j + 1 > last ? j + 1 : last;
+        }
+        for (int off = 32; off > 0; off >>= 1) last = max(last, (uint32_t)__shfl_xor((int)last, off));
+        depth = last < top ? top : last;
+    }
+    if (lane == 0 && a.out_depth) a.out_depth[qi] = depth;
+    const float *q = a.qv.q + qi * a.cd;
+    for (uint32_t i = lane; i < depth; i += 64) {
+        const uint32_t id = pid[i];
+        float ld = pd[i];
+        if (a.codec == 3) ld = asym_one(q, a.ix.codes + (uint64_t)id * (a.cd / 2), a.cd, a.ix.hdr[id], s);
+        const Interval iv = residual_one(q, a.ix.rscale + (uint64_t)id * (a.cd / 16),
+                                         a.ix.rcodes + (uint64_t)id * (a.cd / 2), a.cd, s, ld);
+        r_est[i] = iv.estimate;
+        r_id[i] = id;
+    }
+    __syncthreads();
+    const uint32_t kout = depth < a.k ? depth : a.k;
+    for (uint32_t r = 0; r < a.k; ++r) {
+        if (r >= kout) {
+            if (lane == 0) { a.out_ids[qi * a.k + r] = 0xFFFFFFFFu; a.out_d[qi * a.k + r] = __builtin_huge_valf(); }
+            continue;
+        }
+        float bd = __builtin_huge_valf();
+        uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
+        for (uint32_t i = lane; i < depth; i += 64) {
+            if (r_id[i] == 0xFFFFFFFFu) continue;
+            if (bpos == 0xFFFFFFFFu || r_est[i] < bd || (r_est[i] == bd && r_id[i] < bid)) {
+                bd = r_est[i]; bid = r_id[i]; bpos = i;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float od = __shfl_xor(bd, off);
+            const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+            const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
+            const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || od < bd || (od == bd && oid < bid));
+            if (take) { bd = od; bid = oid; bpos = opos; }
+        }
+        if (lane == 0) {
+            a.out_ids[qi * a.k + r] = bid;
+            a.out_d[qi * a.k + r] = bd;
+            r_id[bpos] = 0xFFFFFFFFu;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace qg
+
+extern "C" {
+
+int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st) {
+    if (n == 0) return 0;
+    const uint64_t tiles = (n + 15) / 16;
+    hipLaunchKernelGGL(qg::k_mlp_forward, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, P, X, n, y);
+    return (int)hipGetLastError();
+}
+
+uint32_t qg_mlp_grad_waves(uint64_t n) {
+    const uint64_t tiles = (n + 15) / 16;
+    return (uint32_t)((tiles + qg::kTilesPerWave - 1) / qg::kTilesPerWave);
+}
+
+int qg_launch_mlp_grad(const float *P, const uint16_t *X, const float *t, uint64_t n, float tau, float *partial,
+                       float *partial_loss, hipStream_t st) {
+    const uint32_t waves = qg_mlp_grad_waves(n);
+    if (waves == 0) return 0;
+    hipLaunchKernelGGL(qg::k_mlp_grad, dim3(waves), dim3(64), 0, st, P, X, t, n, tau, 1.0f / (float)n, partial,
+                       partial_loss);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_mlp_reduce(const float *partial, const float *partial_loss, uint32_t waves, float *grad,
+                         float *loss_out, float *P, float *m, float *v, float lr, float b1, float b2, float eps,
+                         float bc1, float bc2, hipStream_t st) {
+    hipLaunchKernelGGL(qg::k_mlp_reduce, dim3((qg::kParams + 255) / 256), dim3(256), 0, st, partial, partial_loss,
+                       waves, grad, loss_out, P, m, v, lr, b1, b2, eps, bc1, bc2);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_residual_stats(const uint16_t *rscale, uint64_t n, uint32_t nb, float4 *out, hipStream_t st) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(qg::k_residual_stats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rscale, n, nb, out);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_rerank_features(const qg::RerankArgs *a, hipStream_t st) {
+    if (a->nq == 0) return 0;
+    hipLaunchKernelGGL(qg::k_rerank_features, dim3((unsigned)a->nq), dim3(128), 0, st, *a);
+    return (int)hipGetLastError();
+}
+
+int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st) {
+    if (a->nq == 0) return 0;
+    hipLaunchKernelGGL(qg::k_learned_rerank, dim3((unsigned)a->nq), dim3(64), 0, st, *a);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -190,6 +190,8 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             }
         }
 
+        if (a.pool_only) goto finish;
+        {
         // rerank: first rc pool entries, FP32 long distance, residual-4
         const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
         const uint32_t rc = len < want ? len : want;
@@ -230,12 +232,14 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             }
             __syncthreads();
         }
+        }
+    finish:
         if (lane == 0 && a.stats) {
             uint64_t *st = a.stats + (uint64_t)qi * 4;
             st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
         }
-        if (a.pool_ids && lane == 0) {
-            for (uint32_t i = 0; i < L; ++i) {
+        if (a.pool_ids) {
+            for (uint32_t i = lane; i < L; i += kSearchLanes) {
                 a.pool_ids[(uint64_t)qi * L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
                 a.pool_d[(uint64_t)qi * L + i] = i < len ? pd[i] : __builtin_huge_valf();
             }
