@@ -154,6 +154,16 @@ def make_powerlaw(torch, n, dim, nq, seed, exponent=0.8, qnoise=0.1, chunk=13107
     return x, q.contiguous()
 
 
+def more_queries(torch, x, nq, seed, qnoise=0.1):
+    """Further queries of the power-law workload: base points plus the same
+    N(0, (qnoise/sqrt(dim))^2) noise, re-normalised, from their own seed."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    pick = torch.randint(0, x.shape[0], (nq,), generator=g, device="cuda")
+    noise = (qnoise / x.shape[1] ** 0.5) * torch.randn((nq, x.shape[1]), generator=g, device="cuda")
+    return torch.nn.functional.normalize(x[pick] + noise, dim=1).contiguous()
+
+
 def search_bytes_per_query(cd, gates, dists, reranked, int8):
     """Algorithmic HBM bytes of one query's search: gate reads (MSB plane +
     20-byte factors + id + estimate), full 4-bit distance reads (codes +
@@ -233,8 +243,6 @@ def run_graph(args, torch, np, qg):
     index = sp.encode(x, batch=131072)
     torch.cuda.synchronize()
     encode_s = time.perf_counter() - t0
-    del x
-    torch.cuda.empty_cache()
     log("encoded %d x %d in %.2f s" % (n, dim, encode_s))
     t0 = time.perf_counter()
     deg, edges, evals = sp.build_graph(index, r=args.R, l_build=args.l_build, alpha=1.2, batch=4096,
@@ -279,6 +287,11 @@ def run_graph(args, torch, np, qg):
     ok = [r for r in full if r["recall_at_10"] >= args.target_recall]
     pick = min(ok, key=lambda r: r["ms_per_step"]) if ok else max(full, key=lambda r: r["recall_at_10"])
     head, ids = timed(pick["L"], "full", args.steps, args.warmup)
+    learned_res = None
+    if args.train_queries:
+        learned_res = run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream)
+    del x
+    torch.cuda.empty_cache()
     reranked = min(100, head["L"])
     bpq = search_bytes_per_query(cd, head["gate_evals"], head["distances"], reranked, False)
     achieved = nq * bpq / (head["kernel_ms"] / 1000.0) / 1e9
@@ -326,8 +339,70 @@ def run_graph(args, torch, np, qg):
                      "kernel_ms": head["kernel_ms"], "bytes_per_unit": bpq, "unit_of_work": "query",
                      "units_per_launch": nq},
         "cpu_baseline": cpu, "cpu_build_baseline": cpu_build, "gpu_vs_reference_id_parity": parity,
-        "setup_seconds": time.perf_counter() - t_setup,
+        "learned": learned_res, "setup_seconds": time.perf_counter() - t_setup,
     }
+
+
+def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
+    """Learned rerank depth on the headline search: train on (estimate,
+    exact) pairs from fresh queries, calibrate the error multiplier on
+    held-out queries, then time the pipeline (traversal -> features -> MLP
+    -> learned-depth rerank) on the same test queries as the plain search."""
+    from quantization_graph_amd import learned
+    L, nq = head["L"], q.shape[0]
+    t0 = time.perf_counter()
+    rstats = learned.residual_stats(sp, index)
+    tq = more_queries(torch, x, args.train_queries, 41)
+    feats, targets = learned.training_pairs(sp, index, deg, edges, x, tq, L, "full", rstats)
+    del tq
+    torch.cuda.synchronize()
+    pairs_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mlp = learned.MLP(seed=0)
+    hist = mlp.fit(feats, targets, tau=0.95, epochs=args.train_epochs, batch=8192, lr=2e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log("learned: %d training pairs in %.1f s, %d epochs in %.1f s, loss %s"
+        % (feats.shape[0], pairs_s, args.train_epochs, train_s, ["%.5f" % h for h in hist]))
+    del feats, targets
+    vq = more_queries(torch, x, 2000, 43)
+    vgt = exact_topk(torch, x, vq, 10)
+    qv = sp.prepare(vq, "full")
+    pid, pd, _ = learned.search_pool(sp, index, qv, deg, edges, L)
+    preds = mlp.forward(learned.features(sp, index, qv, pid, pd, rstats))
+    kmult, table = learned.calibrate(sp, index, qv, pid, pd, preds, vgt)
+    log("learned: calibrated error multiplier %.2f on 2000 held-out queries" % kmult)
+    model = learned.LearnedRerank(mlp, kmult)
+
+    def step(ev=None):
+        qv = sp.prepare(q, "full")
+        if ev is not None:
+            ev[0].record(stream)
+        out = model.search(sp, index, qv, deg, edges, L, rstats)
+        if ev is not None:
+            ev[1].record(stream)
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ids, _, depths, _ = step(evs[i])
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
+    res = {"qps": nq / (ms / 1000.0), "ms_per_step": ms,
+           "gpu_ms": float(np.mean([a.elapsed_time(b) for a, b in evs])), "recall_at_10": rec,
+           "mean_depth": float(depths.float().mean().item()), "kmult": kmult, "calibration": table,
+           "training_pairs": int(args.train_queries * min(100, L)), "train_epochs": args.train_epochs,
+           "loss_history": hist, "pairs_seconds": pairs_s, "train_seconds": train_s,
+           "plain_qps": head["qps"], "plain_recall_at_10": head["recall_at_10"],
+           "speedup_vs_plain": (nq / (ms / 1000.0)) / head["qps"]}
+    log("learned: %.3f ms/step  %.0f q/s  recall@10 %.4f  mean depth %.1f  (plain: %.0f q/s, recall %.4f)"
+        % (ms, res["qps"], rec, res["mean_depth"], head["qps"], head["recall_at_10"]))
+    return res
 
 
 def run_fixed(args, torch, np, qg):
@@ -464,6 +539,9 @@ def main():
     ap.add_argument("--refine", type=int, default=1)
     ap.add_argument("--prune-cap", type=int, default=256)
     ap.add_argument("--lists", default="100,200,400", help="search list sizes swept")
+    ap.add_argument("--train-queries", type=int, default=50000,
+                    help="queries whose first 100 pool entries train the rerank-depth MLP (0: skip)")
+    ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--fixed-n", type=int, default=1_000_000)
     ap.add_argument("--fixed-dim", type=int, default=960)
     ap.add_argument("--fixed-nq", type=int, default=10_000)
@@ -517,7 +595,7 @@ def main():
                        "base": args.n, "dim": args.dim, "queries": args.nq, "search_list": h["L"],
                        "rerank": 100, "codec": "full", "graph_degree": args.R, "l_build": args.l_build},
             "recall_at_10": h["recall_at_10"], "recall_target_met": h["recall_at_10"] >= args.target_recall,
-            "rerank": "plain top-100 residual4",
+            "rerank": "plain top-100 residual4 (learned adaptive depth under 'learned')",
             "search_sweep": graph["sweep"],
             "encode_seconds": graph["encode_seconds"], "build_seconds": graph["build_seconds"],
             "build_distance_evaluations": graph["build_distance_evaluations"],

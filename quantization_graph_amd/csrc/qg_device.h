@@ -8,6 +8,18 @@
 
 namespace qg {
 
+// bit j of `bits` ? q : +0.0, as the AND of q with the sign-extended bit
+// (the same value; v_bfe_i32 + v_and instead of compare, hazard wait, select)
+__device__ __forceinline__ float masked_q(uint32_t bits, int j, float q) {
+    int m = __builtin_amdgcn_sbfe((int)bits, j, 1);
+    asm("" : "+v"(m));  // keep the mask an integer: no fold back into compare + select
+    return __uint_as_float(__float_as_uint(q) & (uint32_t)m);
+}
+
+// four bits b0..b3 -> bytes 0x0b0 .. 0x0b3 (b * 0x204081 places bit i at
+// position 7i + i without carries)
+__device__ __forceinline__ uint32_t nibble_bytes(uint32_t b4) { return (b4 * 0x204081u) & 0x01010101u; }
+
 // C1: FP32 query x 4-bit code (queryDistanceLong + longCodeIpsAvx512,
 // space_rabitq.h:1551-1607).
 __device__ __forceinline__ float asym_one(const float *q, const uint8_t *code_row, uint32_t cd, float2 h,
@@ -113,7 +125,7 @@ __device__ __forceinline__ PaperEstimate gate_fp32_one(const float *q, const uin
                 const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
                                       q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
 #pragma unroll
-                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + (((bits >> j) & 1u) ? qv[j] : 0.0f);
+                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + masked_q(bits, j, qv[j]);
             }
         }
     }
@@ -139,9 +151,8 @@ __device__ __forceinline__ PaperEstimate gate_int8_one(const int8_t *x8, const u
         for (int k = 0; k < 4; ++k) {
 #pragma unroll
             for (int nib = 0; nib < 8; ++nib) {
-                const uint32_t bits = (ws[k] >> (4 * nib)) & 0xFu;
-                const uint32_t mask = (bits & 1u) | ((bits & 2u) << 7) | ((bits & 4u) << 14) | ((bits & 8u) << 21);
-                sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[32 * c + 8 * k + nib], sel, false);
+                sel = __builtin_amdgcn_sdot4((int)nibble_bytes((ws[k] >> (4 * nib)) & 0xFu),
+                                             (int)xg[32 * c + 8 * k + nib], sel, false);
             }
         }
     }
@@ -184,6 +195,129 @@ __device__ __forceinline__ Interval residual_one(const float *q, const uint16_t 
     iv.lower_bound = dist - err;
     iv.upper_bound = dist + err;
     return iv;
+}
+
+// ---------------------------------------------------------------------------
+// 16-lane group forms: lane j (= lane % 16) of an aligned group of 16 owns
+// accumulator j of the reference's 16-lane vector and runs its chain in the
+// reference order; the 8/4/2/1 tree of reduce16 is the xor-8/4/2/1
+// butterfly over the group. Every lane of the group returns the same value,
+// bit-identical to the one-lane forms above. Integer sums are exact and are
+// split freely. Rows are first copied to the group's LDS staging area with
+// coalesced 16-byte loads (stage_row_group); LDS operations of one
+// wavefront complete in order, so a wave-level scheduling barrier orders
+// the copy before the strided reads.
+__device__ __forceinline__ float group_reduce16(float v) {
+    v = v + __shfl_xor(v, 8);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 2);
+    return v + __shfl_xor(v, 1);
+}
+
+__device__ __forceinline__ int group_sum_int(int v) {
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 2);
+    return v + __shfl_xor(v, 1);
+}
+
+// copy `bytes` (a multiple of 8) from global memory to the group's LDS area
+__device__ __forceinline__ void stage_row_group(uint4 *dst, const uint8_t *src, uint32_t bytes, int j) {
+    if ((bytes & 15u) == 0) {
+        const uint4 *s4 = (const uint4 *)src;
+        for (uint32_t i = j; i < bytes / 16; i += 16) dst[i] = s4[i];
+    } else {
+        const uint2 *s2 = (const uint2 *)src;
+        uint2 *d2 = (uint2 *)dst;
+        for (uint32_t i = j; i < bytes / 8; i += 16) d2[i] = s2[i];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// C1 on a staged code row: accumulator j sums dims 2(16c + j), 2(16c + j) + 1
+__device__ __forceinline__ float asym_group(const float *q, const uint8_t *row_lds, uint32_t cd, float2 h,
+                                           const QScalars &s, int j) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    float sh = 0.0f, rm = 0.0f;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint32_t byte = row_lds[16 * c + j];
+        const float2 qq = *(const float2 *)(q + 32 * c + 2 * j);
+        const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
+        rm = fmaf((float)(lo & 7u), qq.x, rm);
+        rm = fmaf((float)(hi & 7u), qq.y, rm);
+        if (lo > 7u) sh = sh + qq.x;
+        if (hi > 7u) sh = sh + qq.y;
+    }
+    const float short_ip = group_reduce16(sh) - s.half_sum;
+    return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, group_reduce16(rm));
+}
+
+// C3 on a staged code row
+__device__ __forceinline__ float rem_fp32_group(const float *q, const uint8_t *row_lds, uint32_t cd, float2 h,
+                                               const QScalars &s, float short_ip, int j) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    float rm = 0.0f;
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint32_t byte = row_lds[16 * c + j];
+        const float2 qq = *(const float2 *)(q + 32 * c + 2 * j);
+        rm = fmaf((float)(byte & 7u), qq.x, rm);
+        rm = fmaf((float)((byte >> 4) & 7u), qq.y, rm);
+    }
+    return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, group_reduce16(rm));
+}
+
+// C6 on a staged code row: code dwords d = j, j + 16, ...
+__device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t *row_lds, uint32_t cd, float2 h,
+                                               const QScalars &s, float short_ip, int j) {
+    if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
+    const uint32_t *row = (const uint32_t *)row_lds;
+    const uint32_t *xg = (const uint32_t *)x8p;
+    int acc = 0;
+    for (uint32_t d = j; d < cd / 8; d += 16) {
+        const uint32_t w = row[d];
+        acc = __builtin_amdgcn_sdot4((int)(w & 0x07070707u), (int)xg[2 * d], acc, false);
+        acc = __builtin_amdgcn_sdot4((int)((w >> 4) & 0x07070707u), (int)xg[2 * d + 1], acc, false);
+    }
+    acc = group_sum_int(acc);
+    const double centered = (double)s.int8_scale * ((double)acc - 3.5 * (double)s.int8_code_sum);
+    const double signed_ip = 8.0 * (double)short_ip + centered;
+    return h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
+}
+
+// C4 on a staged MSB row: accumulator j sums dims 16m + j
+__device__ __forceinline__ PaperEstimate gate_fp32_group(const float *q, const uint8_t *msb_lds, uint32_t cd, float4 f,
+                                                        uint8_t fv, const QScalars &s, float eps0, int j) {
+    const uint32_t *mw = (const uint32_t *)msb_lds;
+    float acc = 0.0f;
+    for (uint32_t w = 0; w < cd / 32; ++w) {
+        const uint32_t bits = mw[w];
+        acc = acc + masked_q(bits, j, q[32 * w + j]);
+        acc = acc + masked_q(bits, 16 + j, q[32 * w + 16 + j]);
+    }
+    return finish_gate(group_reduce16(acc) - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
+}
+
+// C5 on a staged MSB row: 4-dim groups g4 = j, j + 16, ...
+__device__ __forceinline__ PaperEstimate gate_int8_group(const int8_t *x8, const uint8_t *msb_lds, uint32_t cd,
+                                                        float4 f, uint8_t fv, const QScalars &s, float eps0, int j) {
+    if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) {
+        PaperEstimate r;
+        r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
+        r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
+        return r;
+    }
+    const uint32_t *xg = (const uint32_t *)x8;
+    const uint32_t *mw = (const uint32_t *)msb_lds;
+    int sel = 0;
+    for (uint32_t g4 = j; g4 < cd / 4; g4 += 16) {
+        sel = __builtin_amdgcn_sdot4((int)nibble_bytes((mw[g4 >> 3] >> (4 * (g4 & 7u))) & 0xFu), (int)xg[g4], sel,
+                                     false);
+    }
+    sel = group_sum_int(sel);
+    const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);
+    return finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, eps0);
 }
 
 // C8: symmetric code<->code distance (distanceBetweenEncoded /

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_fp32(ScanArgs a) {
                 const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
                                       q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
 #pragma unroll
-                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + (((bits >> j) & 1u) ? qv[j] : 0.0f);
+                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + masked_q(bits, j, qv[j]);
             }
         }
     }
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
             for (int nib = 0; nib < 8; ++nib) {  // 4 dims per MSB nibble -> one dot4
                 const uint32_t bits = (ws[k] >> (4 * nib)) & 0xFu;
                 // expand 4 bits to a byte mask 0x01 per selected dim
-                const uint32_t mask = (bits & 1u) | ((bits & 2u) << 7) | ((bits & 4u) << 14) | ((bits & 8u) << 21);
+                const uint32_t mask = nibble_bytes(bits);
                 const uint32_t d4 = 128 * c + 32 * k + 4 * nib;  // first dim of this group
                 sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[d4 / 4], sel, false);
             }

@@ -46,7 +46,11 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
     __shared__ float r_est[kMaxL];
     __shared__ uint32_t r_id[kMaxL];
     __shared__ uint32_t s_qi;
+    __shared__ uint4 stg_all[4][64];  // per-group code row staging (cd/2 <= 1024 bytes)
     const int lane = threadIdx.x;
+    const int grp = lane >> 4, j16 = lane & 15;
+    uint4 *stg = stg_all[grp];
+    const uint8_t *stg8 = (const uint8_t *)stg;
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
 
     for (;;) {
@@ -143,17 +147,18 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             const uint32_t srank = (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
             if (survive) surv[srank] = (uint32_t)lane;
             __syncthreads();
-            // distances of the survivors
-            if ((uint32_t)lane < ns) {
-                const uint32_t slot = surv[lane];
+            // distances of the survivors: a 16-lane group per candidate (lane j
+            // runs the reference's accumulator j), four candidates per pass
+            for (uint32_t b = grp; b < ns; b += 4) {
+                const uint32_t slot = surv[b];
                 const uint32_t id = fresh[slot];
-                const uint8_t *row = a.ix.codes + (uint64_t)id * (cd / 2);
+                stage_row_group(stg, a.ix.codes + (uint64_t)id * (cd / 2), cd / 2, j16);
                 const float2 h = a.ix.hdr[id];
                 float d;
-                if (!f_valid[slot]) d = asym_one(q, row, cd, h, s);
-                else if (a.codec == 3) d = rem_int8_one(x8p, row, cd, h, s, f_sip[slot]);
-                else d = rem_fp32_one(q, row, cd, h, s, f_sip[slot]);
-                s_d[lane] = d;
+                if (!f_valid[slot]) d = asym_group(q, stg8, cd, h, s, j16);
+                else if (a.codec == 3) d = rem_int8_group(x8p, stg8, cd, h, s, f_sip[slot], j16);
+                else d = rem_fp32_group(q, stg8, cd, h, s, f_sip[slot], j16);
+                if (j16 == 0) s_d[b] = d;
             }
             dists += ns;
             __syncthreads();
