@@ -96,7 +96,7 @@ typedef struct qg_query_scalars {
 } qg_query_scalars;
 
 /* Query coarse codecs (QueryCoarseCodec, space_rabitq.h:36-44). */
-enum { QG_CODEC_FULL = 0, QG_CODEC_INT8 = 3 };
+enum { QG_CODEC_FULL = 0, QG_CODEC_B1 = 1, QG_CODEC_INT8 = 3, QG_CODEC_B1_MAIN = 4 };
 
 /* Estimator kernels for qg_scan. */
 enum {
@@ -107,7 +107,11 @@ enum {
     QG_SCAN_GATE_FP32 = 4, /* C4: FP32 1-bit gate -> RabitqPaperEstimate                      */
     QG_SCAN_GATE_INT8 = 5, /* C5: INT8 1-bit gate -> RabitqPaperEstimate                      */
     QG_SCAN_RESIDUAL = 6,  /* E1: residual-4 interval from long_in -> RabitqDistanceInterval  */
-    QG_SCAN_SYMMETRIC = 7  /* C8: code<->code distance, pairs (qidx = lhs row, ids = rhs row)  */
+    QG_SCAN_SYMMETRIC = 7, /* C8: code<->code distance, pairs (qidx = lhs row, ids = rhs row)  */
+    QG_SCAN_PAPER_RECORD = 8, /* gate from the compact record (compute_paper_prune_estimate,
+                                 space_rabitq.h:3926-3959) -> RabitqPaperEstimate              */
+    QG_SCAN_B1 = 9         /* 4-bit code x B1 sign code (query_distance_b1, space_rabitq.h:
+                              3742-3755); queries prepared with QG_CODEC_B1                    */
 };
 
 /* -------------------------------------------------------- device batch API */
@@ -226,6 +230,16 @@ int qg_build_graph(const void *space, const qg_index_desc *ix, uint32_t r, uint3
                    uint32_t early_stop_hops, uint32_t *degrees, uint32_t *edges, uint64_t *distance_evaluations,
                    void *workspace, size_t workspace_bytes, void *stream);
 
+/* Lloyd k-means of rabitq_train_centroids (rabitq_bridge.cpp:101-172) over
+ * device samples [sample_count][dim] in draw order: centroid c starts at
+ * sample c % sample_count, then `iters` rounds of first-minimum assignment
+ * (d = fma(diff, diff, d) in dimension order) and in-order member sums.
+ * centroids: device [k][dim] output; k <= 256.
+ * Workspace: qg_kmeans_workspace_bytes(sample_count, dim, k). */
+size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k);
+int qg_train_centroids(const float *samples, uint64_t sample_count, uint64_t dim, uint32_t k, int iters,
+                       float *centroids, void *workspace, size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------------------ bridge API */
 /* Host-memory drop-ins for rabitq_bridge.cpp; see that file for semantics. */
 
@@ -233,6 +247,10 @@ void *rabitq_space_new(size_t dim, uint32_t seed, size_t centroid_count, size_t 
                        size_t residual_block_size, bool residual_mse, bool residual_fp16); /* :42  */
 void rabitq_space_delete(void *space);                                                 /* :76  */
 bool rabitq_space_set_center(void *space, const float *center);                       /* :80  */
+/* centroid_count must equal the space's (1): the K > 1 codec is not built. */
+bool rabitq_space_set_centroids(void *space, const float *centroids, size_t centroid_count); /* :89 */
+bool rabitq_train_centroids(const float *data, size_t count, size_t dim, size_t k, size_t sample_count,
+                            uint32_t seed, int iters, float *out);                    /* :101 */
 size_t rabitq_full_record_bytes(const void *space);                                    /* :174 */
 size_t rabitq_compact_record_bytes(const void *space);                                 /* :178 */
 size_t rabitq_residual_record_bytes(const void *space);                                /* :183 */
@@ -257,6 +275,38 @@ const void *rabitq_prepare_query(const void *space, const float *query);        
 void rabitq_set_query_coarse_codec(void *space, int codec);                           /* :364 */
 void rabitq_release_query(const void *space, const void *prepared);                   /* :372 */
 float rabitq_query_distance(const void *space, const void *prepared, const void *encoded); /* :380 */
+/* B1 sign-code distance; needs a query prepared under codec 1 or 4, else +inf. */
+float rabitq_query_distance_b1(const void *space, const void *prepared, const void *encoded); /* :389 */
+bool rabitq_query_distance_b1_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                    const void *encoded_base, size_t encoded_stride,
+                                    float *out_distances);                            /* :398 */
+RabitqPaperEstimate rabitq_paper_estimate(const void *space, const void *prepared, const void *encoded,
+                                          float epsilon0);                            /* :425 */
+RabitqPaperEstimate rabitq_paper_estimate_sidecar(const void *space, const void *prepared,
+                                                  const uint8_t *msb_code, const void *factors,
+                                                  float epsilon0);                    /* :448 */
+RabitqPaperEstimate rabitq_paper_estimate_codec(const void *space, const void *prepared, const void *encoded,
+                                                const uint8_t *msb_code, const void *factors,
+                                                float epsilon0);                      /* :475 */
+bool rabitq_paper_estimate_codec_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                       size_t count, const void *encoded_base, size_t encoded_stride,
+                                       const uint8_t *msb_base, size_t msb_stride, const void *factors_base,
+                                       size_t factors_stride, float epsilon0,
+                                       RabitqPaperEstimate *out);                     /* :506 */
+float rabitq_query_distance_with_paper(const void *space, const void *prepared, const void *encoded,
+                                       float short_ip);                               /* :595 */
+float rabitq_query_distance_with_quantized_paper(const void *space, const void *prepared, const void *encoded,
+                                                 float short_ip);                     /* :609 */
+/* The bridge's spaces keep residuals in separate records, so this returns
+ * {long, long, long}, the reference's failure value for that case. */
+RabitqDistanceInterval rabitq_residual_distance(const void *space, const void *prepared, const void *encoded,
+                                                float long_distance);                 /* :624 */
+RabitqDistanceInterval rabitq_residual_distance_from_record(const void *space, const void *prepared,
+                                                            const void *encoded, const void *residual_record,
+                                                            float long_distance);     /* :644 */
+bool rabitq_paper_estimate_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                 const void *encoded_base, size_t encoded_stride, float epsilon0,
+                                 RabitqPaperEstimate *out);                           /* :665 */
 bool rabitq_query_distance_batch(const void *space, const void *prepared, const uint32_t *ids,
                                  const uint8_t *modes, size_t count, const void *encoded_base,
                                  size_t encoded_stride, const float *short_ips, float *out); /* :885 */
@@ -268,6 +318,12 @@ bool rabitq_residual_distance_batch(const void *space, const void *prepared, con
                                     size_t count, const void *encoded_base, size_t encoded_stride,
                                     const void *residual_base, size_t residual_stride,
                                     const float *long_distances, RabitqDistanceInterval *out); /* :972 */
+/* Mode-0 distance then residual interval per id (bit-identical to the two
+ * calls in sequence, space_rabitq.h:5557-5778). */
+bool rabitq_full_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                         size_t count, const void *encoded_base, size_t encoded_stride,
+                                         const void *residual_base, size_t residual_stride,
+                                         RabitqDistanceInterval *out);                /* :1044 */
 
 #ifdef __cplusplus
 }

@@ -1221,3 +1221,115 @@ int ro_build_vamana(const ro_space *s, const uint8_t *cbase, size_t cstride, siz
     if (evals_out) *evals_out = evals;
     return rc;
 }
+
+/* ------------------------------------------------ record gate, B1, k-means */
+/* compute_paper_prune_estimate (space_rabitq.h:3926-3959). short_ip is
+ * shortCodeIpFromFullCodeAvx512 (:1307-1329): lane j of 16 takes pairs j,
+ * j+16, ..., even dim then odd dim, then the _mm512_reduce_add_ps tree. */
+ro_estimate ro_paper_estimate_record(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0) {
+    ro_estimate r;
+    memset(&r, 0, sizeof(r));
+    float hdr[4];
+    memcpy(hdr, compact, 16);  /* norm_sqr, long_scale, short_scale, error_scale */
+    const float norm_sqr = hdr[0], short_scale = hdr[2];
+    if (s->code_dim <= 1 || eps0 < 0.0f || !isfinite(eps0) || norm_sqr <= 0.0f || q->query_norm <= 0.0f ||
+        short_scale <= 0.0f || !isfinite(short_scale))
+        return r;
+    const float data_norm = sqrtf(norm_sqr);
+    const float alpha_raw = 4.0f * data_norm / (short_scale * sqrtf((float)s->code_dim));
+    if (!(alpha_raw > 0.0f) || !isfinite(alpha_raw)) return r;
+    const float alpha = (alpha_raw < 1.0f) ? alpha_raw : 1.0f; /* std::min(1.0f, a) */
+    const uint8_t *code = compact + 16;
+    float sh[16] = {0};
+    for (size_t p0 = 0; p0 < s->code_dim / 2; p0 += 16)
+        for (int j = 0; j < 16; ++j) {
+            const uint8_t b = code[p0 + j];
+            if ((b >> 3) & 1u) sh[j] = sh[j] + q->q[2 * (p0 + j)];
+            if ((b >> 7) & 1u) sh[j] = sh[j] + q->q[2 * (p0 + j) + 1];
+        }
+    const float short_ip = reduce16(sh) - q->half_sum;
+    const float cross = short_ip * short_scale;
+    const float two_norms = 2.0f * data_norm * q->query_norm;
+    const float ip_hat = cross / two_norms;
+    const float alpha2 = alpha * alpha;
+    const float t = (1.0f - alpha2) / alpha2;
+    const float eb = sqrtf(t > 0.0f ? t : 0.0f) * eps0 / sqrtf((float)(s->code_dim - 1));
+    float up = ip_hat + eb;
+    up = (up < 1.0f) ? up : 1.0f;    /* std::min(1.0f, v)  */
+    up = (-1.0f < up) ? up : -1.0f;  /* std::max(-1.0f, v) */
+    const float base = norm_sqr + q->query_norm_sqr;
+    r.lower_bound = SITE(RO_FMA_PAPER_LOWER) ? fmaf(-two_norms, up, base) : base - two_norms * up;
+    r.short_ip = short_ip;
+    r.alpha = alpha;
+    r.ip_hat = ip_hat;
+    r.error_bound = eb;
+    r.valid = isfinite(r.lower_bound) ? 1 : 0;
+    return r;
+}
+
+/* The B1 query code (buildCentroidQuery, space_rabitq.h:3493-3511) and
+ * quantizedLongCodeDotDispatch against it: exact integers. */
+int64_t ro_b1_dot(const ro_space *s, const ro_query *q, const uint8_t *compact, int64_t *signed_sum) {
+    int64_t dot = 0, sum = 0;
+    for (size_t i = 0; i < s->code_dim; ++i) {
+        const int x = q->q[i] > 0.0f ? 1 : -1;
+        sum += x;
+        dot += (int64_t)get_nibble(compact + 16, i) * x;
+    }
+    if (signed_sum) *signed_sum = sum;
+    return dot;
+}
+
+float ro_b1_distance(const ro_space *s, const ro_query *q, const uint8_t *compact) {
+    float hdr[2];
+    memcpy(hdr, compact, 8);
+    if (!long_scale_ok(hdr[1])) return hdr[0] + q->query_norm_sqr;
+    int64_t sum = 0;
+    const int64_t dot = ro_b1_dot(s, q, compact, &sum);
+    const double signed_long_ip = (double)dot - 7.5 * (double)sum;
+    const double est_ip = (double)hdr[1] * signed_long_ip;
+    return hdr[0] + q->query_norm_sqr - (float)est_ip;
+}
+
+int ro_train_centroids(const float *data, size_t count, size_t dim, size_t k, size_t sample_count, uint32_t seed,
+                       int iters, float *out) {
+    if (!data || !out || count == 0 || dim == 0 || k == 0 || k > 256 || sample_count == 0) return -1;
+    if (sample_count > count) sample_count = count;
+    mt32 g;
+    mt32_seed(&g, seed);
+    size_t *sample = (size_t *)malloc(sample_count * sizeof(size_t));
+    for (size_t i = 0; i < sample_count; ++i) sample[i] = (size_t)mt32_next(&g) % count;
+    float *cen = (float *)malloc(k * dim * sizeof(float));
+    float *next = (float *)malloc(k * dim * sizeof(float));
+    size_t *cnt = (size_t *)malloc(k * sizeof(size_t));
+    for (size_t c = 0; c < k; ++c) memcpy(cen + c * dim, data + sample[c % sample_count] * dim, dim * sizeof(float));
+    for (int it = 0; it < iters; ++it) {
+        memset(next, 0, k * dim * sizeof(float));
+        memset(cnt, 0, k * sizeof(size_t));
+        for (size_t i = 0; i < sample_count; ++i) {
+            const float *v = data + sample[i] * dim;
+            size_t best = 0;
+            float best_d = INFINITY;
+            for (size_t c = 0; c < k; ++c) {
+                const float *cc = cen + c * dim;
+                float d = 0.0f;
+                for (size_t j = 0; j < dim; ++j) {
+                    const float diff = v[j] - cc[j];
+                    d = madd_f(RO_FMA_KMEANS_DIST, diff, diff, d);
+                }
+                if (d < best_d) { best_d = d; best = c; }
+            }
+            float *dst = next + best * dim;
+            for (size_t j = 0; j < dim; ++j) dst[j] += v[j];
+            ++cnt[best];
+        }
+        for (size_t c = 0; c < k; ++c) {
+            if (cnt[c] == 0) continue;
+            const float inv = 1.0f / (float)cnt[c];
+            for (size_t j = 0; j < dim; ++j) cen[c * dim + j] = next[c * dim + j] * inv;
+        }
+    }
+    memcpy(out, cen, k * dim * sizeof(float));
+    free(sample); free(cen); free(next); free(cnt);
+    return 0;
+}

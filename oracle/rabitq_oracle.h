@@ -41,10 +41,12 @@ extern "C" {
 #define RO_FMA_RES_HI      (1u << 9)  /* residual: products = fma(hi, odd, lo*even)       */
 #define RO_FMA_RES_LO      (1u << 10) /* residual: products = fma(lo, even, hi*odd)       */
 #define RO_FMA_RES_ACC     (1u << 11) /* residual: ips += scale*block_ip                  */
+#define RO_FMA_PAPER_LOWER (1u << 12) /* record gate: (n + q) - two_norms*ip_upper         */
+#define RO_FMA_KMEANS_DIST (1u << 13) /* k-means: d += diff*diff                          */
 /* The pattern the reference compiler produces (g++ 11, -O3, AVX-512). */
 #define RO_FMA_REFERENCE (RO_FMA_FQA_T | RO_FMA_ENC_OOBAR | RO_FMA_ENC_DECNORM | RO_FMA_ENC_X4IP | \
                           RO_FMA_C1_LANE | RO_FMA_C1_SUB7 | RO_FMA_C1_EST | RO_FMA_RES_HI | \
-                          RO_FMA_RES_ACC)
+                          RO_FMA_RES_ACC | RO_FMA_PAPER_LOWER | RO_FMA_KMEANS_DIST)
 
 extern unsigned ro_fma_mask;
 void ro_set_fma_mask(unsigned mask);
@@ -149,6 +151,18 @@ ro_estimate ro_int_codec_estimate(const ro_space *s, const ro_query *q, const ui
 int64_t ro_symmetric_ip4(const ro_space *s, const uint8_t *lhs, const uint8_t *rhs);
 float ro_symmetric_distance(const ro_space *s, const uint8_t *lhs, const uint8_t *rhs);
 ro_interval ro_residual_interval(const ro_space *s, const ro_query *q, const uint8_t *residual, float long_distance);
+/* compute_paper_prune_estimate (space_rabitq.h:3926-3959): the gate from a
+ * compact record, short_ip over the nibble MSBs in pair-lane order. */
+ro_estimate ro_paper_estimate_record(const ro_space *s, const ro_query *q, const uint8_t *compact, float eps0);
+/* query_distance_b1 (space_rabitq.h:3742-3755): 4-bit code against the
+ * query's sign code (+1 where q_i > 0, else -1). */
+float ro_b1_distance(const ro_space *s, const ro_query *q, const uint8_t *compact);
+int64_t ro_b1_dot(const ro_space *s, const ro_query *q, const uint8_t *compact, int64_t *signed_sum);
+/* rabitq_train_centroids (rabitq_bridge.cpp:101-172): mt19937 sample with
+ * replacement, first k samples as seeds, `iters` Lloyd rounds. Returns 0,
+ * or -1 for the arguments the reference rejects. */
+int ro_train_centroids(const float *data, size_t count, size_t dim, size_t k, size_t sample_count, uint32_t seed,
+                       int iters, float *out);
 double ro_residual_estimate_f64(const ro_space *s, const ro_query *q, const uint8_t *residual, double long_distance);
 
 /* Beam search with the 1-bit gate + residual rerank

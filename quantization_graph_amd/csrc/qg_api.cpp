@@ -35,6 +35,9 @@ int qg_launch_mlp_reduce(const float *partial, const float *partial_loss, uint32
 int qg_launch_residual_stats(const uint16_t *rscale, uint64_t n, uint32_t nb, float4 *out, hipStream_t st);
 int qg_launch_rerank_features(const qg::RerankArgs *a, hipStream_t st);
 int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st);
+size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k);
+int qg_launch_kmeans(const float *samples, uint64_t sc, uint32_t dim, uint32_t k, int iters, float *centroids,
+                     void *workspace, hipStream_t stream);
 }
 
 namespace qg {
@@ -294,7 +297,8 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
                        void *ws, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || (!queries && nq) || !out || (!ws && nq)) return fail("null argument to qg_prepare_queries");
-    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
+    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8 && codec != QG_CODEC_B1 && codec != QG_CODEC_B1_MAIN)
+        return fail("unsupported query codec %d", codec);
     if (!out->q || !out->x8 || !out->x8p || !out->qs) return fail("query output view has a null array");
     out->nq = nq;
     if (nq == 0) return 0;
@@ -331,11 +335,11 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
             RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !ix || !ids || (!qv && kernel != QG_SCAN_SYMMETRIC)) return fail("null argument to qg_scan");
-    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_SYMMETRIC) return fail("unknown scan kernel %d", kernel);
+    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_B1) return fail("unknown scan kernel %d", kernel);
     if (!fixed && !qidx) return fail("pair scans need qidx");
     if (fixed && kernel != QG_SCAN_INT8_FULL && kernel != QG_SCAN_ASYM && kernel != QG_SCAN_RESIDUAL)
         return fail("kernel %d has no fixed-list form", kernel);
-    const bool gate = kernel == QG_SCAN_GATE_FP32 || kernel == QG_SCAN_GATE_INT8;
+    const bool gate = kernel == QG_SCAN_GATE_FP32 || kernel == QG_SCAN_GATE_INT8 || kernel == QG_SCAN_PAPER_RECORD;
     if (gate && !out_est) return fail("gate kernels need out_est");
     if (kernel == QG_SCAN_RESIDUAL && (!out_iv || !long_in)) return fail("residual kernel needs long_in and out_iv");
     if (!gate && kernel != QG_SCAN_RESIDUAL && !out_f) return fail("kernel needs out_f");
@@ -534,6 +538,19 @@ int qg_learned_rerank(const void *space, const qg_index_desc *ix, const qg_query
     a.out_d = out_dists;
     a.out_depth = out_depth;
     return check_hip(qg_launch_learned_rerank(&a, (hipStream_t)stream), "learned rerank launch");
+}
+
+int qg_train_centroids(const float *samples, uint64_t sample_count, uint64_t dim, uint32_t k, int iters,
+                       float *centroids, void *workspace, size_t workspace_bytes, void *stream) {
+    if (!samples || !centroids || !workspace) return fail("null argument to qg_train_centroids");
+    if (sample_count == 0 || dim == 0 || dim > 0xFFFFFFFFull) return fail("sample_count and dim must be positive");
+    if (k == 0 || k > 256) return fail("k must be in [1, 256]");
+    if (workspace_bytes < qg_kmeans_workspace_bytes(sample_count, dim, k))
+        return fail("k-means workspace too small: %zu < %zu", workspace_bytes,
+                    qg_kmeans_workspace_bytes(sample_count, dim, k));
+    return check_hip(qg_launch_kmeans(samples, sample_count, (uint32_t)dim, k, iters, centroids, workspace,
+                                      (hipStream_t)stream),
+                     "k-means launch");
 }
 
 }  // extern "C"

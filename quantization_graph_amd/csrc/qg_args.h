@@ -10,7 +10,7 @@ struct PrepParams {
     const float *raw;          // [nq][dim]
     uint64_t nq;
     uint32_t dim, cd;
-    int codec;                 // 0 Full, 3 Int8
+    int codec;                 // 0 Full, 1 / 4 B1 sign code, 3 Int8
     int identity;              // seed == 0: no rotation
     const float *signs;        // [cd]
     const float *rot_center;   // [cd]
@@ -103,7 +103,7 @@ struct SearchArgs {
     IndexView ix;
     QueryView qv;
     uint32_t cd;
-    int codec;                 // 0 Full, 3 Int8
+    int codec;                 // 0 Full, 1 / 4 B1 sign code, 3 Int8
     uint32_t L, k, rerank;
     float eps0;
     const uint32_t *degrees;   // [n]
@@ -206,7 +206,7 @@ struct RerankArgs {
     const float4 *rstats;      // [n] {rms scale, max scale, zero-scale fraction, 0}
     QueryView qv;
     uint32_t cd;
-    int codec;                 // 0 Full, 3 Int8
+    int codec;                 // 0 Full, 1 / 4 B1 sign code, 3 Int8
     uint64_t nq;
     uint32_t L;                // pool row stride
     uint32_t depth;            // largest rerank depth (feature rows per query)

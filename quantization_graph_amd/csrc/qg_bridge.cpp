@@ -10,9 +10,11 @@
 // the reference (space_rabitq.h:3700-3706).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <random>
 #include <vector>
 
 #include "qg_args.h"
@@ -149,6 +151,35 @@ uint32_t *upload_zeros_u32(Arena *a, size_t count) {
     uint32_t *d = static_cast<uint32_t *>(a->take(count * 4 + 4));
     if (d && count) (void)hipMemset(d, 0, count * 4);
     return d;
+}
+
+// Stage the compact records ids[] (strided host rows) and run one pair scan
+// of `kernel` for the prepared query; out_f / out_est copied back.
+bool scan_compact(Space *s, const BridgeQuery *bq, int kernel, const uint32_t *ids, size_t count,
+                  const void *base, size_t stride, float eps0, float *out_f, RabitqPaperEstimate *out_est) {
+    std::vector<uint8_t> comp;
+    gather(base, stride, s->compact_bytes, ids, count, &comp);
+    Arena a;
+    if (!arena_for(s, count * (s->compact_bytes + 2 * s->cd + 128) + 4096, &a)) return false;
+    qg_index_desc d;
+    if (!upload_records(s, &a, comp, nullptr, count, &d)) return false;
+    uint32_t *rows = upload_iota(&a, count);
+    uint32_t *qidx = upload_zeros_u32(&a, count);
+    float *df = static_cast<float *>(a.take(count * 4 + 4));
+    RabitqPaperEstimate *de = static_cast<RabitqPaperEstimate *>(a.take(count * 24 + 24));
+    if (!rows || !qidx || !df || !de) return false;
+    const qg_query_desc qd = bq->desc();
+    if (qg_scan(s, kernel, 0, &d, &qd, rows, qidx, 0, count, eps0, nullptr, nullptr, out_f ? df : nullptr, nullptr,
+                out_est ? de : nullptr, nullptr, nullptr, nullptr) != 0)
+        return false;
+    if (out_f && hipMemcpy(out_f, df, count * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    if (out_est && hipMemcpy(out_est, de, count * 24, hipMemcpyDeviceToHost) != hipSuccess) return false;
+    return true;
+}
+
+const BridgeQuery *query_of(const Space *s, const void *prepared) {
+    const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
+    return (s && bq && bq->owner == s) ? bq : nullptr;
 }
 
 }  // namespace
@@ -371,7 +402,10 @@ const void *rabitq_prepare_query(const void *space, const float *query) {
     bq.codec = s->codec;
     if (hipMemcpy(bq.raw, query, s->dim * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     qg_query_desc d = bq.desc();
-    const int codec = (s->codec == QG_CODEC_INT8) ? QG_CODEC_INT8 : QG_CODEC_FULL;
+    // The codecs this library prepares: Int8 (dithered code), B1 / B1Main
+    // (sign code); every other codec reads the FP32 residual only.
+    const int codec = (s->codec == QG_CODEC_INT8 || s->codec == QG_CODEC_B1 || s->codec == QG_CODEC_B1_MAIN)
+                          ? s->codec : QG_CODEC_FULL;
     if (qg_prepare_queries(s, bq.raw, 1, codec, &d, bq.ws, nullptr) != 0) return nullptr;
     if (hipDeviceSynchronize() != hipSuccess) return nullptr;
     return &bq;
@@ -446,7 +480,10 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s || (count && (!ids || !msb_base || !factors_base || !out))) return false;
     if (count == 0) return true;
-    if (bq->codec != QG_CODEC_INT8 && bq->codec != QG_CODEC_FULL) return fail("query codec %d has no gate", bq->codec) == 0;
+    // compute_paper_prune_estimate_sidecar (space_rabitq.h:4519-4529): Int8
+    // reads the dithered code, B1 and Int4 have gates of their own (not built
+    // here), every other codec takes the FP32 gate.
+    if (bq->codec == QG_CODEC_B1 || bq->codec == 2) return fail("query codec %d has no gate here", bq->codec) == 0;
     const size_t mb = s->cd / 8;
     std::vector<uint8_t> msb;
     gather(msb_base, msb_stride, mb, ids, count, &msb);
@@ -514,6 +551,195 @@ bool rabitq_residual_distance_batch(const void *space, const void *prepared, con
         return true;
     }
     return hipMemcpy(out, dout, count * 12, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+// ------------------------------------------- single-record and remaining ---
+
+bool rabitq_space_set_centroids(void *space, const float *centroids, size_t centroid_count) {
+    // setCentroids (space_rabitq.h:2918-2936): the count must match the
+    // space's centroid count, which is 1 here.
+    if (!space || !centroids || centroid_count != 1) return false;
+    return rabitq_space_set_center(space, centroids);
+}
+
+bool rabitq_train_centroids(const float *data, size_t count, size_t dim, size_t k, size_t sample_count,
+                            uint32_t seed, int iters, float *out) {
+    if (data == nullptr || out == nullptr || count == 0 || dim == 0 || k == 0 || k > 256 || sample_count == 0)
+        return false;
+    sample_count = std::min(sample_count, count);
+    // The sample draw (std::mt19937(seed)() % count, with replacement) is the
+    // reference's index stream; the rows are staged in draw order.
+    std::mt19937 rng(seed);
+    std::vector<float> rows(sample_count * dim);
+    for (size_t i = 0; i < sample_count; ++i) {
+        const size_t id = static_cast<size_t>(rng()) % count;
+        std::memcpy(rows.data() + i * dim, data + id * dim, dim * sizeof(float));
+    }
+    const size_t ws_bytes = qg_kmeans_workspace_bytes(sample_count, dim, (uint32_t)k);
+    float *d_rows = nullptr, *d_cen = nullptr;
+    void *d_ws = nullptr;
+    bool ok = hipMalloc(&d_rows, rows.size() * sizeof(float)) == hipSuccess &&
+              hipMalloc(&d_cen, k * dim * sizeof(float)) == hipSuccess && hipMalloc(&d_ws, ws_bytes) == hipSuccess;
+    ok = ok && hipMemcpy(d_rows, rows.data(), rows.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && qg_train_centroids(d_rows, sample_count, dim, (uint32_t)k, iters, d_cen, d_ws, ws_bytes, nullptr) == 0;
+    ok = ok && hipMemcpy(out, d_cen, k * dim * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+    if (d_rows) (void)hipFree(d_rows);
+    if (d_cen) (void)hipFree(d_cen);
+    if (d_ws) (void)hipFree(d_ws);
+    return ok;
+}
+
+bool rabitq_query_distance_b1_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                    const void *encoded_base, size_t encoded_stride, float *out_distances) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = query_of(s, prepared);
+    if (!bq || (count && (!ids || !encoded_base || !out_distances))) return false;
+    const bool b1 = bq->codec == QG_CODEC_B1 || bq->codec == QG_CODEC_B1_MAIN;
+    if (!b1 || !scan_compact(s, bq, QG_SCAN_B1, ids, count, encoded_base, encoded_stride, 0.0f, out_distances,
+                             nullptr))
+        for (size_t i = 0; i < count; ++i) out_distances[i] = std::numeric_limits<float>::infinity();
+    return true;
+}
+
+float rabitq_query_distance_b1(const void *space, const void *prepared, const void *encoded) {
+    const Space *s = static_cast<const Space *>(space);
+    const uint32_t id = 0;
+    float out = std::numeric_limits<float>::infinity();
+    if (!s || !encoded) return out;
+    if (!rabitq_query_distance_b1_batch(space, prepared, &id, 1, encoded, s->compact_bytes, &out))
+        return std::numeric_limits<float>::infinity();
+    return out;
+}
+
+bool rabitq_paper_estimate_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                 const void *encoded_base, size_t encoded_stride, float epsilon0,
+                                 RabitqPaperEstimate *out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = query_of(s, prepared);
+    if (!bq || (count && (!ids || !encoded_base || !out))) return false;
+    if (count == 0) return true;
+    if (!scan_compact(s, bq, QG_SCAN_PAPER_RECORD, ids, count, encoded_base, encoded_stride, epsilon0, nullptr, out))
+        std::memset(out, 0, count * sizeof(RabitqPaperEstimate));
+    return true;
+}
+
+RabitqPaperEstimate rabitq_paper_estimate(const void *space, const void *prepared, const void *encoded,
+                                          float epsilon0) {
+    RabitqPaperEstimate r;
+    std::memset(&r, 0, sizeof(r));
+    const Space *s = static_cast<const Space *>(space);
+    const uint32_t id = 0;
+    if (!s || !encoded || !rabitq_paper_estimate_batch(space, prepared, &id, 1, encoded, s->compact_bytes, epsilon0, &r))
+        std::memset(&r, 0, sizeof(r));
+    return r;
+}
+
+bool rabitq_paper_estimate_codec_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                       size_t count, const void *encoded_base, size_t encoded_stride,
+                                       const uint8_t *msb_base, size_t msb_stride, const void *factors_base,
+                                       size_t factors_stride, float epsilon0, RabitqPaperEstimate *out) {
+    // The codec comparison path always reads the 1-bit sidecar; the compact
+    // records are not used (rabitq_bridge.cpp:506-593).
+    (void)encoded_base;
+    (void)encoded_stride;
+    return rabitq_paper_estimate_batch_sidecar(space, prepared, ids, count, msb_base, msb_stride, factors_base,
+                                               factors_stride, epsilon0, out);
+}
+
+RabitqPaperEstimate rabitq_paper_estimate_sidecar(const void *space, const void *prepared, const uint8_t *msb_code,
+                                                  const void *factors, float epsilon0) {
+    RabitqPaperEstimate r;
+    std::memset(&r, 0, sizeof(r));
+    const Space *s = static_cast<const Space *>(space);
+    const uint32_t id = 0;
+    if (!s || !msb_code || !factors ||
+        !rabitq_paper_estimate_batch_sidecar(space, prepared, &id, 1, msb_code, s->cd / 8, factors, 20, epsilon0, &r))
+        std::memset(&r, 0, sizeof(r));
+    return r;
+}
+
+RabitqPaperEstimate rabitq_paper_estimate_codec(const void *space, const void *prepared, const void *encoded,
+                                                const uint8_t *msb_code, const void *factors, float epsilon0) {
+    (void)encoded;
+    return rabitq_paper_estimate_sidecar(space, prepared, msb_code, factors, epsilon0);
+}
+
+static float single_mode_distance(const void *space, const void *prepared, const void *encoded, float short_ip,
+                                  uint8_t mode) {
+    const Space *s = static_cast<const Space *>(space);
+    const uint32_t id = 0;
+    float out = std::numeric_limits<float>::infinity();
+    if (!s || !encoded ||
+        !rabitq_query_distance_batch(space, prepared, &id, &mode, 1, encoded, s->compact_bytes, &short_ip, &out))
+        return std::numeric_limits<float>::infinity();
+    return out;
+}
+
+float rabitq_query_distance_with_paper(const void *space, const void *prepared, const void *encoded,
+                                       float short_ip) {
+    return single_mode_distance(space, prepared, encoded, short_ip, 1);
+}
+
+float rabitq_query_distance_with_quantized_paper(const void *space, const void *prepared, const void *encoded,
+                                                 float short_ip) {
+    return single_mode_distance(space, prepared, encoded, short_ip, 2);
+}
+
+RabitqDistanceInterval rabitq_residual_distance(const void *space, const void *prepared, const void *encoded,
+                                                float long_distance) {
+    // Spaces made by rabitq_space_new store residuals externally, and the
+    // inline-record form throws for them (space_rabitq.h:1706-1712); the
+    // bridge turns that into {long, long, long} (rabitq_bridge.cpp:637-641).
+    (void)space;
+    (void)prepared;
+    (void)encoded;
+    return RabitqDistanceInterval{long_distance, long_distance, long_distance};
+}
+
+RabitqDistanceInterval rabitq_residual_distance_from_record(const void *space, const void *prepared,
+                                                            const void *encoded, const void *residual_record,
+                                                            float long_distance) {
+    const Space *s = static_cast<const Space *>(space);
+    const uint32_t id = 0;
+    RabitqDistanceInterval iv{long_distance, long_distance, long_distance};
+    if (!s || !encoded || !residual_record ||
+        !rabitq_residual_distance_batch(space, prepared, &id, 1, encoded, s->compact_bytes, residual_record,
+                                        s->residual_bytes, &long_distance, &iv))
+        return RabitqDistanceInterval{long_distance, long_distance, long_distance};
+    return iv;
+}
+
+bool rabitq_full_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids,
+                                         size_t count, const void *encoded_base, size_t encoded_stride,
+                                         const void *residual_base, size_t residual_stride,
+                                         RabitqDistanceInterval *out) {
+    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const BridgeQuery *bq = query_of(s, prepared);
+    if (!bq || (count && (!ids || !encoded_base || !residual_base || !out))) return false;
+    if (count == 0) return true;
+    const float inf = std::numeric_limits<float>::infinity();
+    std::vector<uint8_t> comp, res;
+    gather(encoded_base, encoded_stride, s->compact_bytes, ids, count, &comp);
+    gather(residual_base, residual_stride, s->residual_bytes, ids, count, &res);
+    Arena a;
+    bool ok = arena_for(s, count * (s->compact_bytes + s->residual_bytes + 2 * s->cd + 128) + 4096, &a);
+    qg_index_desc d;
+    ok = ok && upload_records(s, &a, comp, &res, count, &d);
+    uint32_t *rows = ok ? upload_iota(&a, count) : nullptr;
+    uint32_t *qidx = ok ? upload_zeros_u32(&a, count) : nullptr;
+    float *dl = ok ? static_cast<float *>(a.take(count * 4)) : nullptr;
+    RabitqDistanceInterval *dout = ok ? static_cast<RabitqDistanceInterval *>(a.take(count * 12)) : nullptr;
+    ok = rows && qidx && dl && dout;
+    const qg_query_desc qd = bq->desc();
+    // mode-0 long distance, then the residual interval from it, on the device
+    ok = ok && qg_scan(s, QG_SCAN_ASYM, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, nullptr, dl, nullptr,
+                       nullptr, nullptr, nullptr, nullptr) == 0;
+    ok = ok && qg_scan(s, QG_SCAN_RESIDUAL, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, dl, nullptr, nullptr,
+                       nullptr, dout, nullptr, nullptr) == 0;
+    ok = ok && hipMemcpy(out, dout, count * 12, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok)
+        for (size_t i = 0; i < count; ++i) out[i] = RabitqDistanceInterval{inf, inf, inf};
+    return true;
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 //   INT8: scale = max|q|/128, x = clamp(floor(q/scale + u), -128, 127) with u
 //   from mt19937_64 seeded by FNV-1a over the raw bytes (:3540-3659), two
 //   draws per dimension, the first used for INT8.
+//   B1 / B1Main: sign code x = +1 where q_i > 0 else -1, and its sum.
 // Serial recurrences (the float sums, FNV-1a, the mt19937_64 seeding) run on
 // single lanes in the reference order; everything else is data-parallel.
 #include "qg_args.h"
@@ -187,6 +188,19 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
             for (uint32_t i = 0; i < cd; ++i) a = fma(err[i], err[i], a);
             s_err = a;
         }
+    } else if (p.codec == 1 || p.codec == 4) {
+        // B1 / B1Main sign code (:3493-3511): +1 where q_i > 0, else -1; the
+        // signed sum goes where the INT8 code sum goes.
+        long long csum = 0;
+        for (uint32_t i = t; i < cd; i += blockDim.x) {
+            const int x = (v[i] > 0.0f) ? 1 : -1;
+            p.x8_out[qi * cd + i] = (int8_t)x;
+            const uint32_t g = i >> 3, w = i & 7u;
+            p.x8p_out[qi * cd + 8 * g + ((w & 1u) ? 4 : 0) + (w >> 1)] = (int8_t)x;
+            csum += x;
+        }
+        for (int off = 32; off > 0; off >>= 1) csum += __shfl_xor(csum, off);
+        if ((t & 63) == 0) atomicAdd((unsigned long long *)&s_csum, (unsigned long long)csum);
     } else {
         for (uint32_t i = t; i < cd; i += blockDim.x) { p.x8_out[qi * cd + i] = 0; p.x8p_out[qi * cd + i] = 0; }
     }
@@ -201,7 +215,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
         s.int8_scale = scale;
         s.int8_error_norm = int8 ? (float)sqrt(s_err) : 0.0f;
         s.max_abs_bits = __float_as_int(max_abs);
-        s.int8_code_sum = int8 ? (int64_t)s_csum : 0;
+        s.int8_code_sum = (int8 || p.codec == 1 || p.codec == 4) ? (int64_t)s_csum : 0;
         s.pad = 0;
         p.qs_out[qi] = s;
     }

@@ -511,12 +511,108 @@ __global__ void __launch_bounds__(kScanThreads) k_residual_grouped(ScanArgs a) {
     }
 }
 
+// ------------------------------------------------------- record gate -----
+// compute_paper_prune_estimate (space_rabitq.h:3926-3959): the gate read
+// from the compact record. short_ip over the nibble MSBs in the pair-lane
+// order of shortCodeIpFromFullCodeAvx512 (:1307-1329); the lower bound is
+// (norm_sqr + |q|^2) - two_norms * ip_upper with the product contracted,
+// as the reference compiler emits it.
+__global__ void __launch_bounds__(kScanThreads) k_paper_record(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    const float short_scale = a.ix.sfac[id].x;
+    PaperEstimate r;
+    r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
+    r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    if (a.cd <= 1 || a.eps0 < 0.0f || !__builtin_isfinite(a.eps0) || h.x <= 0.0f || s.query_norm <= 0.0f ||
+        short_scale <= 0.0f || !__builtin_isfinite(short_scale)) { a.out_est[item] = r; return; }
+    const float data_norm = sqrtf(h.x);
+    const float alpha_raw = 4.0f * data_norm / (short_scale * sqrtf((float)a.cd));
+    if (!(alpha_raw > 0.0f) || !__builtin_isfinite(alpha_raw)) { a.out_est[item] = r; return; }
+    const float alpha = (alpha_raw < 1.0f) ? alpha_raw : 1.0f;
+    const float *qg = a.qv.q + qi * a.cd;
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    float sh[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sh[j] = 0.0f;
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+        const float4 *qp = (const float4 *)(qg + 32 * c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q0 = qp[2 * k], q1 = qp[2 * k + 1];
+            const float qe[4] = {q0.x, q0.z, q1.x, q1.z};
+            const float qo[4] = {q0.y, q0.w, q1.y, q1.w};
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t bits = ws[k] >> (8 * b);
+                sh[j] = sh[j] + masked_q(bits, 3, qe[b]);
+                sh[j] = sh[j] + masked_q(bits, 7, qo[b]);
+            }
+        }
+    }
+    const float short_ip = reduce16(sh) - s.half_sum;
+    const float cross = short_ip * short_scale;
+    const float two_norms = 2.0f * data_norm * s.query_norm;
+    const float ip_hat = cross / two_norms;
+    const float alpha2 = alpha * alpha;
+    const float t = (1.0f - alpha2) / alpha2;
+    const float eb = sqrtf(t > 0.0f ? t : 0.0f) * a.eps0 / sqrtf((float)(a.cd - 1));
+    float up = ip_hat + eb;
+    up = (up < 1.0f) ? up : 1.0f;
+    up = (-1.0f < up) ? up : -1.0f;
+    r.lower_bound = fmaf(-two_norms, up, h.x + s.query_norm_sqr);
+    r.short_ip = short_ip;
+    r.alpha = alpha;
+    r.ip_hat = ip_hat;
+    r.error_bound = eb;
+    r.valid = __builtin_isfinite(r.lower_bound) ? 1 : 0;
+    a.out_est[item] = r;
+}
+
+// ------------------------------------------------------------------ B1 -----
+// query_distance_b1 (space_rabitq.h:3742-3755): the full 4-bit code against
+// the query's +/-1 sign code (prepared in the INT8 slots), exact integer dot,
+// est_ip = long_scale * (dot - 7.5 * sum) in double.
+__global__ void __launch_bounds__(kScanThreads) k_b1(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const float2 h = a.ix.hdr[id];
+    if (!long_scale_ok(h.y)) { a.out_f[item] = h.x + s.query_norm_sqr; return; }
+    const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
+    const uint32_t *xg = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+    int acc = 0;
+    for (uint32_t c = 0; c < a.cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t d = 4 * c + k;
+            acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x0F0F0F0Fu), (int)xg[2 * d], acc, false);
+            acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x0F0F0F0Fu), (int)xg[2 * d + 1], acc, false);
+        }
+    }
+    const double signed_long_ip = (double)acc - 7.5 * (double)s.int8_code_sum;
+    const double est_ip = (double)h.y * signed_long_ip;
+    a.out_f[item] = h.x + s.query_norm_sqr - (float)est_ip;
+    if (a.out_dot) a.out_dot[item] = acc;
+}
+
 }  // namespace qg
 
 extern "C" {
 
 enum { QG_K_INT8_FULL = 0, QG_K_ASYM = 1, QG_K_REM_FP32 = 2, QG_K_REM_INT8 = 3, QG_K_GATE_FP32 = 4,
-       QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7 };
+       QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7, QG_K_PAPER_RECORD = 8, QG_K_B1 = 9 };
 
 int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t stream) {
     const dim3 block(qg::kScanThreads);
@@ -555,6 +651,8 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
             case QG_K_GATE_INT8: hipLaunchKernelGGL(qg::k_gate_int8, grid, block, 0, stream, *a); break;
             case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<false>, grid, block, 0, stream, *a); break;
             case QG_K_SYMMETRIC: hipLaunchKernelGGL(qg::k_symmetric, grid, block, 0, stream, *a); break;
+            case QG_K_PAPER_RECORD: hipLaunchKernelGGL(qg::k_paper_record, grid, block, 0, stream, *a); break;
+            case QG_K_B1: hipLaunchKernelGGL(qg::k_b1, grid, block, 0, stream, *a); break;
             default: return (int)hipErrorInvalidValue;
         }
     }

@@ -11,7 +11,12 @@ The fixture holds inputs and the reference's outputs only:
     distances, residual-4 intervals;
   * symmetric distances for 64 record pairs;
   * a Vamana graph built by rabitq_build_vamana_graph on 300 vectors
-    (R = 16, L_build = 64, alpha = 1.2).
+    (R = 16, L_build = 64, alpha = 1.2);
+  * rabitq_paper_estimate (the gate from the compact record) for the Full
+    and Int8 codecs, and rabitq_query_distance_b1 under the B1 codec;
+  * rabitq_train_centroids on 2000 x 37 random rows (k = 1, 16, 256) and on
+    a three-row near tie whose assignment depends on the contraction of
+    d += diff * diff.
 
 Run: python tests/golden/make_golden.py
 """
@@ -45,6 +50,7 @@ def main():
         full = ref.encode_full(x)
         comp, res = ref.split(full)
         msb, fac = ref.sidecar(comp)
+        fac["pad"] = 0  # the reference leaves the struct padding uninitialised
         p = "d%d_" % dim
         out.update({p + "x": x, p + "q": q, p + "center": c, p + "full": full, p + "compact": comp,
                     p + "residual": res, p + "msb": msb, p + "factors": fac.view(np.uint8).reshape(-1, 20)})
@@ -67,6 +73,16 @@ def main():
             out[p + name + "_gate"] = np.stack(est)
             out[p + name + "_rem"] = np.stack(rem)
             out[p + name + "_residual"] = np.stack(ivs)
+        for codec, name in ((0, "full"), (3, "int8"), (1, "b1")):
+            ref.set_codec(codec)
+            rec = []
+            for i in range(q.shape[0]):
+                pq = ref.prepare(q[i])
+                if codec == 1:
+                    rec.append(ref.b1_batch(pq, comp, ids))
+                else:
+                    rec.append(ref.paper_estimate_batch(pq, comp, ids, 1.9).view(np.uint8).reshape(-1, 24))
+            out[p + name + ("_dist" if codec == 1 else "_record_gate")] = np.stack(rec)
         rng = np.random.default_rng(7)
         pairs = rng.integers(0, x.shape[0], size=(64, 2)).astype(np.uint32)
         out[p + "sym_pairs"] = pairs
@@ -80,6 +96,15 @@ def main():
     deg, edges, msb, fac, evals = ref.build_graph(comp, 16, 64, 1.2, beam=1, batch=64, refine=1, cap=128)
     out.update({"g_x": x, "g_center": c, "g_compact": comp, "g_degrees": deg, "g_edges": edges,
                 "g_evals": np.array([evals], np.uint64)})
+    # k-means (rabitq_train_centroids)
+    ref = Ref(64, 100)
+    km_x = np.random.default_rng(5).standard_normal((2000, 37)).astype(np.float32)
+    out["km_x"] = km_x
+    for k, sc, it in ((1, 2000, 3), (16, 1000, 5), (256, 1500, 2)):
+        out["km_k%d" % k] = ref.train_centroids(km_x, k, sc, 11, it)
+    tie = np.array([[0.5413387, 1.6302696], [0.76907164, 1.5360193], [0.0, 0.0]], np.float32)
+    out["km_tie_x"] = tie
+    out["km_tie"] = ref.train_centroids(tie, 2, 3, 33, 1)
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_vectors.npz"))
 

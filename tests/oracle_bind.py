@@ -21,7 +21,8 @@ REF_SO = os.path.join(ROOT, "oracle", "_ref", "librabitq_ref.so")
 # tests/test_oracle_vs_ref.py (bit names follow oracle/rabitq_oracle.h).
 FMA = dict(ENC_NORM=1 << 0, FQA_T=1 << 1, ENC_OOBAR=1 << 2, ENC_DECNORM=1 << 3,
            ENC_X4IP=1 << 4, C1_LANE=1 << 5, C1_SUB7=1 << 6, C1_EST=1 << 7,
-           GATE_UPPER=1 << 8, RES_HI=1 << 9, RES_LO=1 << 10, RES_ACC=1 << 11)
+           GATE_UPPER=1 << 8, RES_HI=1 << 9, RES_LO=1 << 10, RES_ACC=1 << 11,
+           PAPER_LOWER=1 << 12, KMEANS_DIST=1 << 13)
 
 
 class ROSpace(C.Structure):
@@ -114,6 +115,14 @@ class Oracle:
         L.ro_residual_interval.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.ro_residual_estimate_f64.restype = C.c_double
         L.ro_residual_estimate_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
+        L.ro_paper_estimate_record.restype = Estimate
+        L.ro_paper_estimate_record.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_b1_distance.restype = C.c_float
+        L.ro_b1_distance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ro_b1_dot.restype = C.c_int64
+        L.ro_b1_dot.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+        L.ro_train_centroids.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.c_uint32, C.c_int, C.c_void_p]
         L.ro_search.restype = C.c_size_t
         L.ro_search.argtypes = [C.c_void_p, C.c_void_p, C.c_int,
                                 C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
@@ -260,6 +269,28 @@ class Oracle:
             out[i] = (e.lower_bound, e.short_ip, e.alpha, e.ip_hat, e.error_bound, e.valid, (0, 0, 0))
         return out
 
+    def paper_record(self, q, comp, eps0=1.9):
+        out = np.zeros(comp.shape[0], ESTIMATE_DTYPE)
+        for i in range(comp.shape[0]):
+            e = self.lib.ro_paper_estimate_record(self.sp, C.byref(q), _p(comp[i]), C.c_float(eps0))
+            out[i] = (e.lower_bound, e.short_ip, e.alpha, e.ip_hat, e.error_bound, e.valid, (0, 0, 0))
+        return out
+
+    def b1(self, q, comp):
+        d = np.array([self.lib.ro_b1_distance(self.sp, C.byref(q), _p(comp[i])) for i in range(comp.shape[0])],
+                     np.float32)
+        ssum = C.c_int64()
+        dots = np.array([self.lib.ro_b1_dot(self.sp, C.byref(q), _p(comp[i]), C.byref(ssum))
+                         for i in range(comp.shape[0])], np.int64)
+        return d, dots, ssum.value
+
+    def train_centroids(self, data, k, sample_count, seed, iters):
+        data = np.ascontiguousarray(data, np.float32)
+        out = np.zeros((k, data.shape[1]), np.float32)
+        rc = self.lib.ro_train_centroids(_p(data), data.shape[0], data.shape[1], k, sample_count, seed, iters,
+                                         _p(out))
+        return out if rc == 0 else None
+
     def masked_sum(self, q, msb):
         return np.array([self.lib.ro_masked_sum(self.sp, C.byref(q), _p(msb[i])) for i in range(msb.shape[0])],
                         np.int64)
@@ -367,6 +398,8 @@ class Ref:
         L.rabitq_residual_distance_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                                      C.c_void_p, C.c_size_t, C.c_void_p,
                                                      C.c_size_t, C.c_void_p, C.c_void_p]
+        if hasattr(L, "rabitq_train_centroids"):
+            self._bind_singles()
         if not hasattr(L, "rabitq_build_vamana_graph"):
             self._init_space(dim, seed, centroid_count, residual_bits, block, mse, fp16)
             return
@@ -377,6 +410,102 @@ class Ref:
                                                 C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         self._init_space(dim, seed, centroid_count, residual_bits, block, mse, fp16)
+
+    def _bind_singles(self):
+        """The bridge's single-record and remaining batch entry points
+        (rabitq_bridge.cpp:89-172, 380-700, 1044-1084)."""
+        L, V, Z, F = self.lib, C.c_void_p, C.c_size_t, C.c_float
+        sigs = {
+            "rabitq_space_set_centroids": (C.c_bool, [V, V, Z]),
+            "rabitq_train_centroids": (C.c_bool, [V, Z, Z, Z, Z, C.c_uint32, C.c_int, V]),
+            "rabitq_query_distance_b1": (F, [V, V, V]),
+            "rabitq_query_distance_b1_batch": (C.c_bool, [V, V, V, Z, V, Z, V]),
+            "rabitq_paper_estimate": (Estimate, [V, V, V, F]),
+            "rabitq_paper_estimate_sidecar": (Estimate, [V, V, V, V, F]),
+            "rabitq_paper_estimate_codec": (Estimate, [V, V, V, V, V, F]),
+            "rabitq_paper_estimate_batch": (C.c_bool, [V, V, V, Z, V, Z, F, V]),
+            "rabitq_paper_estimate_codec_batch": (C.c_bool, [V, V, V, Z, V, Z, V, Z, V, Z, F, V]),
+            "rabitq_query_distance_with_paper": (F, [V, V, V, F]),
+            "rabitq_query_distance_with_quantized_paper": (F, [V, V, V, F]),
+            "rabitq_residual_distance": (Interval, [V, V, V, F]),
+            "rabitq_residual_distance_from_record": (Interval, [V, V, V, V, F]),
+            "rabitq_full_residual_distance_batch": (C.c_bool, [V, V, V, Z, V, Z, V, Z, V]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+
+    @staticmethod
+    def _est_row(e):
+        return (e.lower_bound, e.short_ip, e.alpha, e.ip_hat, e.error_bound, e.valid, (0, 0, 0))
+
+    def set_centroids(self, c, count):
+        c = np.ascontiguousarray(c, np.float32)
+        return bool(self.lib.rabitq_space_set_centroids(self.h, _p(c), count))
+
+    def train_centroids(self, data, k, sample_count, seed, iters):
+        data = np.ascontiguousarray(data, np.float32)
+        out = np.zeros((k, data.shape[1]), np.float32)
+        ok = self.lib.rabitq_train_centroids(_p(data), data.shape[0], data.shape[1], k, sample_count, seed, iters,
+                                             _p(out))
+        return out if ok else None
+
+    def b1(self, pq, comp_row):
+        return self.lib.rabitq_query_distance_b1(self.h, pq, _p(comp_row))
+
+    def b1_batch(self, pq, comp, ids):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros(ids.size, np.float32)
+        assert self.lib.rabitq_query_distance_b1_batch(self.h, pq, _p(ids), ids.size, _p(comp), comp.shape[1],
+                                                       _p(out))
+        return out
+
+    def paper_estimate(self, pq, comp_row, eps0):
+        return self._est_row(self.lib.rabitq_paper_estimate(self.h, pq, _p(comp_row), C.c_float(eps0)))
+
+    def paper_estimate_sidecar(self, pq, msb_row, fac, i, eps0):
+        return self._est_row(self.lib.rabitq_paper_estimate_sidecar(
+            self.h, pq, _p(msb_row), C.c_void_p(fac.ctypes.data + 20 * i), C.c_float(eps0)))
+
+    def paper_estimate_codec(self, pq, comp_row, msb_row, fac, i, eps0):
+        return self._est_row(self.lib.rabitq_paper_estimate_codec(
+            self.h, pq, _p(comp_row), _p(msb_row), C.c_void_p(fac.ctypes.data + 20 * i), C.c_float(eps0)))
+
+    def paper_estimate_batch(self, pq, comp, ids, eps0):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros(ids.size, ESTIMATE_DTYPE)
+        assert self.lib.rabitq_paper_estimate_batch(self.h, pq, _p(ids), ids.size, _p(comp), comp.shape[1],
+                                                    C.c_float(eps0), _p(out))
+        return out
+
+    def paper_estimate_codec_batch(self, pq, comp, msb, fac, ids, eps0):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros(ids.size, ESTIMATE_DTYPE)
+        assert self.lib.rabitq_paper_estimate_codec_batch(self.h, pq, _p(ids), ids.size, _p(comp), comp.shape[1],
+                                                          _p(msb), msb.shape[1], _p(fac), 20, C.c_float(eps0),
+                                                          _p(out))
+        return out
+
+    def with_paper(self, pq, comp_row, short_ip, quantized=False):
+        fn = self.lib.rabitq_query_distance_with_quantized_paper if quantized else \
+            self.lib.rabitq_query_distance_with_paper
+        return fn(self.h, pq, _p(comp_row), C.c_float(short_ip))
+
+    def residual_single(self, pq, row, long_d):
+        iv = self.lib.rabitq_residual_distance(self.h, pq, _p(row), C.c_float(long_d))
+        return (iv.estimate, iv.lower_bound, iv.upper_bound)
+
+    def residual_from_record(self, pq, comp_row, res_row, long_d):
+        iv = self.lib.rabitq_residual_distance_from_record(self.h, pq, _p(comp_row), _p(res_row), C.c_float(long_d))
+        return (iv.estimate, iv.lower_bound, iv.upper_bound)
+
+    def full_residual_batch(self, pq, comp, res, ids):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        out = np.zeros(ids.size, INTERVAL_DTYPE)
+        assert self.lib.rabitq_full_residual_distance_batch(self.h, pq, _p(ids), ids.size, _p(comp), comp.shape[1],
+                                                            _p(res), res.shape[1], _p(out))
+        return out
 
     def _init_space(self, dim, seed, centroid_count, residual_bits, block, mse, fp16):
         L = self.lib

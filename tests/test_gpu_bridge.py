@@ -127,3 +127,100 @@ def test_bridge_build_rejects_what_the_reference_rejects(gpu):
         ours.build_graph(comp, 16, 48, 0.5)       # alpha < 1
     with pytest.raises(AssertionError):
         ours.build_graph(comp, 16, 0, 1.2)        # L_build = 0
+
+
+EST_FIELDS = ("lower_bound", "short_ip", "alpha", "ip_hat", "error_bound", "valid")
+
+
+def _est_bytes(rows):
+    a = np.array(rows, ESTIMATE_DTYPE) if isinstance(rows, list) else rows
+    return {f: a[f].tobytes() for f in EST_FIELDS}
+
+
+@pytest.mark.parametrize("dim", [960, 1536])
+def test_bridge_single_record_entry_points(gpu, golden, dim):
+    """The single-record and remaining batch symbols (rabitq_bridge.cpp:380-700,
+    1044-1084) against the reference's outputs: the golden fixture for the
+    record gate and B1 distances, and the live reference bridge for the rest."""
+    p = "d%d_" % dim
+    ours = Ours(dim, 100)
+    ours.set_center(golden[p + "center"])
+    comp, res = golden[p + "compact"], golden[p + "residual"]
+    msb = golden[p + "msb"]
+    fac = golden[p + "factors"].reshape(-1).view(FACTOR_DTYPE).copy()
+    q = golden[p + "q"]
+    ids = np.arange(comp.shape[0], dtype=np.uint32)[::-1].copy()
+    print("case: dim", dim, "records", comp.shape[0], flush=True)
+    for codec, name in ((0, "full"), (3, "int8")):
+        ours.set_codec(codec)
+        for i in range(q.shape[0]):
+            pq = ours.prepare(q[i])
+            g = golden[p + name + "_record_gate"][i].reshape(-1).view(ESTIMATE_DTYPE)
+            eb = ours.paper_estimate_batch(pq, comp, ids, 1.9)
+            assert _est_bytes(eb) == _est_bytes(g[ids])
+            es = [ours.paper_estimate(pq, comp[j], 1.9) for j in range(0, comp.shape[0], 7)]
+            assert _est_bytes(es) == _est_bytes(g[::7])
+            gate = golden[p + name + "_gate"][i].reshape(-1).view(ESTIMATE_DTYPE)
+            cb = ours.paper_estimate_codec_batch(pq, comp, msb, fac, ids, 1.9)
+            assert _est_bytes(cb) == _est_bytes(gate[ids])
+            one = [ours.paper_estimate_sidecar(pq, msb[j], fac, j, 1.9) for j in (0, 5, 47)]
+            two = [ours.paper_estimate_codec(pq, comp[j], msb[j], fac, j, 1.9) for j in (0, 5, 47)]
+            assert _est_bytes(one) == _est_bytes(gate[[0, 5, 47]]) == _est_bytes(two)
+            rem = golden[p + name + "_rem"][i]
+            d = np.array([ours.with_paper(pq, comp[j], gate["short_ip"][j], quantized=codec == 3)
+                          for j in (1, 9, 30)], np.float32)
+            assert d.tobytes() == rem[[1, 9, 30]].tobytes()
+            dist = golden[p + name + "_dist"][i]
+            iv = golden[p + name + "_residual"][i].reshape(-1).view(INTERVAL_DTYPE)
+            fr = ours.full_residual_batch(pq, comp, res, ids)
+            for f in ("estimate", "lower_bound", "upper_bound"):
+                assert fr[f].tobytes() == iv[f][ids].tobytes(), f
+            for j in (2, 40):
+                got = ours.residual_from_record(pq, comp[j], res[j], dist[j])
+                assert np.array(got, np.float32).tobytes() == \
+                    np.array([iv["estimate"][j], iv["lower_bound"][j], iv["upper_bound"][j]], np.float32).tobytes()
+                assert ours.residual_single(pq, comp[j], dist[j]) == (dist[j], dist[j], dist[j])
+    ours.set_codec(1)
+    for i in range(q.shape[0]):
+        pq = ours.prepare(q[i])
+        b = ours.b1_batch(pq, comp, ids)
+        assert b.tobytes() == golden[p + "b1_dist"][i][ids].tobytes()
+        assert np.float32(ours.b1(pq, comp[3])).tobytes() == golden[p + "b1_dist"][i][3].tobytes()
+    for codec in (4, 5, 6):  # B1Main, FullStaged, FullRerankScalar gate with the FP32 query
+        ours.set_codec(codec)
+        pq = ours.prepare(q[1])
+        gate = golden[p + "full_gate"][1].reshape(-1).view(ESTIMATE_DTYPE)
+        assert _est_bytes(ours.gate_batch(pq, msb, fac, ids, 1.9)) == _est_bytes(gate[ids])
+    ours.set_codec(0)
+    pq = ours.prepare(q[0])
+    assert np.isinf(ours.b1_batch(pq, comp, ids[:4])).all()  # no sign code under the Full codec
+
+
+def test_bridge_train_centroids(gpu, golden):
+    """rabitq_train_centroids on the device: bit for bit against the
+    reference's outputs (random rows, the contraction-sensitive near tie) and
+    against the live reference bridge on a larger draw; the reference's
+    argument checks."""
+    ours = Ours(64, 100)
+    x = golden["km_x"]
+    for k, sc, it in ((1, 2000, 3), (16, 1000, 5), (256, 1500, 2)):
+        print("case: kmeans k", k, "samples", sc, "iters", it, flush=True)
+        assert np.array_equal(ours.train_centroids(x, k, sc, 11, it), golden["km_k%d" % k])
+    assert np.array_equal(ours.train_centroids(golden["km_tie_x"], 2, 3, 33, 1), golden["km_tie"])
+    assert ours.train_centroids(x, 0, 10, 1, 1) is None
+    assert ours.train_centroids(x, 257, 10, 1, 1) is None
+    assert ours.train_centroids(x, 4, 0, 1, 1) is None
+    assert np.array_equal(ours.train_centroids(x[:3], 5, 3, 7, 0), ours.train_centroids(x[:3], 5, 3, 7, 0))
+    if Ref.available():
+        ref = Ref(64, 100)
+        big = np.random.default_rng(9).standard_normal((20000, 300)).astype(np.float32)
+        print("case: kmeans 20000 x 300 k 64 iters 4 vs reference", flush=True)
+        assert np.array_equal(ours.train_centroids(big, 64, 12000, 5, 4), ref.train_centroids(big, 64, 12000, 5, 4))
+
+
+def test_bridge_set_centroids(gpu):
+    ours = Ours(64, 100)
+    c = np.random.default_rng(0).standard_normal((2, 64)).astype(np.float32)
+    assert ours.set_centroids(c[0], 1)
+    assert not ours.set_centroids(c, 2)  # the space was made with one centroid
+    assert not ours.set_centroids(c, 0)

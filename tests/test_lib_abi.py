@@ -40,3 +40,28 @@ def test_errors_surface_without_a_device():
     assert b"centroid_count" in L.qg_last_error()
     assert L.qg_space_new(960, 100, 1, 8, 16, True, True) is None
     assert qg.code_dim_for(960) == 1024 and qg.code_dim_for(1536) == 2048 and qg.code_dim_for(100) == 128
+
+
+# The 37 extern "C" entry points of experiments/02_diskann_fair/native/rabitq_bridge.cpp.
+REFERENCE_BRIDGE = """
+rabitq_space_new rabitq_space_delete rabitq_space_set_center rabitq_space_set_centroids rabitq_train_centroids
+rabitq_full_record_bytes rabitq_compact_record_bytes rabitq_residual_record_bytes rabitq_residual_block_size
+rabitq_residual_bits rabitq_encode_full rabitq_copy_compact rabitq_copy_residual_record rabitq_build_vamana_graph
+rabitq_paper_msb_code_bytes rabitq_paper_factor_bytes rabitq_export_paper_sidecar rabitq_symmetric_distance
+rabitq_prepare_query rabitq_set_query_coarse_codec rabitq_release_query rabitq_query_distance
+rabitq_query_distance_b1 rabitq_query_distance_b1_batch rabitq_paper_estimate rabitq_paper_estimate_sidecar
+rabitq_paper_estimate_codec rabitq_paper_estimate_codec_batch rabitq_query_distance_with_paper
+rabitq_query_distance_with_quantized_paper rabitq_residual_distance rabitq_residual_distance_from_record
+rabitq_paper_estimate_batch rabitq_paper_estimate_batch_sidecar rabitq_query_distance_batch
+rabitq_residual_distance_batch rabitq_full_residual_distance_batch
+""".split()
+
+
+def test_library_exports_the_whole_reference_bridge():
+    assert len(REFERENCE_BRIDGE) == 37
+    declared = set(declared_functions())
+    assert not [n for n in REFERENCE_BRIDGE if n not in declared]
+    if not os.path.exists(LIB):
+        pytest.skip("libqgraph_hip.so not built")
+    lib = C.CDLL(LIB)
+    assert not [n for n in REFERENCE_BRIDGE if not hasattr(lib, n)]

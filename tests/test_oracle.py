@@ -325,3 +325,63 @@ def test_search_matches_reference_kernels():
             assert np.array_equal(ids[i, :len(o_ids)], o_ids), (codec, i)
             assert np.array_equal(ds[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
             o.free_query(oq)
+
+
+@pytest.mark.parametrize("dim", [960, 1536])
+def test_record_gate_and_b1_match_reference_fixture(golden, dim):
+    """rabitq_paper_estimate (gate from the compact record) and
+    rabitq_query_distance_b1, bit for bit against the reference's outputs;
+    the B1 dot against brute force and the record gate against its FP64
+    form (short_ip, alpha)."""
+    p = "d%d_" % dim
+    o = Oracle(dim, 100)
+    o.set_center(golden[p + "center"])
+    comp = golden[p + "compact"]
+    hdr = comp[:, :16].copy().view(np.float32).reshape(-1, 4)
+    codes = _unpack(comp[:, 16:16 + o.cd // 2])
+    q = golden[p + "q"]
+    for codec, name in ((0, "full"), (3, "int8")):
+        for i in range(q.shape[0]):
+            oq = o.prepare(q[i], codec)
+            e = o.paper_record(oq, comp, 1.9)
+            g = golden[p + name + "_record_gate"][i].reshape(-1).view(ESTIMATE_DTYPE)
+            for f in ("lower_bound", "short_ip", "alpha", "ip_hat", "error_bound", "valid"):
+                assert e[f].tobytes() == g[f].tobytes(), (name, f)
+            qv = o.qvec(oq).astype(np.float64)
+            sip = ((codes >= 8) * qv).sum(1) - 0.5 * qv.sum()
+            assert np.allclose(e["short_ip"], sip, rtol=1e-4, atol=1e-4 * np.abs(qv).sum())
+            alpha = np.minimum(1.0, 4.0 * np.sqrt(hdr[:, 0].astype(np.float64)) /
+                               (hdr[:, 2].astype(np.float64) * np.sqrt(o.cd)))
+            assert np.allclose(e["alpha"], alpha, rtol=1e-6)
+            o.free_query(oq)
+    for i in range(q.shape[0]):
+        oq = o.prepare(q[i], 0)
+        d, dots, ssum = o.b1(oq, comp)
+        sign = np.where(o.qvec(oq) > 0, 1, -1).astype(np.int64)
+        assert np.array_equal(dots, codes @ sign) and ssum == int(sign.sum())
+        assert np.array_equal(d.view(np.uint32), golden[p + "b1_dist"][i].view(np.uint32))
+        o.free_query(oq)
+
+
+def test_train_centroids_matches_reference_fixture(golden):
+    """rabitq_train_centroids: bit for bit against the reference (random rows
+    and the near tie that fixes the contraction of the distance), and against
+    what the algorithm fixes: no iterations return the sampled rows (mt19937
+    draws mod count), one iteration returns the means of the assignment."""
+    o = Oracle(64, 100)
+    x = golden["km_x"]
+    for k, sc, it in ((1, 2000, 3), (16, 1000, 5), (256, 1500, 2)):
+        assert np.array_equal(o.train_centroids(x, k, sc, 11, it), golden["km_k%d" % k])
+    assert np.array_equal(o.train_centroids(golden["km_tie_x"], 2, 3, 33, 1), golden["km_tie"])
+    n = x.shape[0]
+    sample = np.array([o.lib.ro_mt32_nth(11, i + 1) % n for i in range(300)])
+    assert np.array_equal(o.train_centroids(x, 7, 300, 11, 0), x[sample[np.arange(7) % 300]])
+    c0 = x[sample[:7]].astype(np.float64)
+    xs = x[sample].astype(np.float64)
+    assign = np.argmin(((xs[:, None] - c0[None]) ** 2).sum(-1), 1)
+    expect = np.stack([xs[assign == c].mean(0) if (assign == c).any() else c0[c] for c in range(7)])
+    assert np.allclose(o.train_centroids(x, 7, 300, 11, 1), expect, rtol=1e-5, atol=1e-5)
+    for bad in (dict(k=0), dict(k=257), dict(sc=0)):
+        kw = dict(k=4, sc=10)
+        kw.update(bad)
+        assert o.train_centroids(x, kw["k"], kw["sc"], 1, 1) is None
