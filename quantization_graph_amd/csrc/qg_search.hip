@@ -30,6 +30,31 @@ __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint
     return ad < bd || (ad == bd && aid < bid);
 }
 
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Copy rows ids[0..count) (ids in LDS) of a row-major device table (row_u4
+// uint4 per row, a power of two) to LDS rows 0..count) with gfx950's direct
+// global-to-LDS loads: every 16-byte piece of the batch is in flight at once
+// and no registers hold the data. Lanes past the end re-read the last piece
+// into stage slots below 1024 that nobody reads. count * row_u4 <= 1024.
+__device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
+                                           const uint4 *__restrict__ src, uint32_t lane) {
+    const uint32_t total = count * row_u4;
+    const uint32_t sh = (uint32_t)__builtin_ctz(row_u4);
+    for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+        const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
+        const uint4 *g = src + (uint64_t)ids[e >> sh] * row_u4 + (e & (row_u4 - 1u));
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                         (__attribute__((address_space(3))) void *)(dst + k0), 16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the batch has landed in LDS
+    wave_sync_lds();
+}
+
 __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
     extern __shared__ unsigned char smem[];
     const uint32_t cd = a.cd;
@@ -40,17 +65,16 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
     float *pd = (float *)(smem + 6 * cd);                       // L
     uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * L);        // L
     uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * L);         // L (+pad)
-    __shared__ uint32_t fresh[64], surv[64];
+    __shared__ uint32_t fresh[64], surv[64], sid[64];
     __shared__ float f_lb[64], f_sip[64], s_d[64];
+    __shared__ float2 s_h[64];
     __shared__ uint8_t f_valid[64];
     __shared__ float r_est[kMaxL];
     __shared__ uint32_t r_id[kMaxL];
     __shared__ uint32_t s_qi;
-    __shared__ uint4 stg_all[4][64];  // per-group code row staging (cd/2 <= 1024 bytes)
+    __shared__ uint4 stage[1024];  // 16 KB: a hop's MSB rows, or up to 1024 / (cd/32) code rows
     const int lane = threadIdx.x;
     const int grp = lane >> 4, j16 = lane & 15;
-    uint4 *stg = stg_all[grp];
-    const uint8_t *stg8 = (const uint8_t *)stg;
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
 
     for (;;) {
@@ -87,17 +111,21 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
         for (;;) {
             // first unexpanded entry
             uint32_t f = 0xFFFFFFFFu;
-            for (uint32_t i = lane; i < len; i += kSearchLanes)
-                if (!pexp[i]) { f = i; break; }
-            for (int off = 32; off > 0; off >>= 1) f = min(f, (uint32_t)__shfl_xor((int)f, off));
+            for (uint32_t c0 = 0; c0 < len; c0 += kSearchLanes) {
+                const uint32_t i = c0 + (uint32_t)lane;
+                const uint64_t m = __ballot(i < len && !pexp[i]);
+                if (m) { f = c0 + (uint32_t)__builtin_ctzll(m); break; }
+            }
             if (f == 0xFFFFFFFFu) break;
             const uint32_t node = pid[f];
             __syncthreads();
             if (lane == 0) pexp[f] = 1;
             ++hops;
+            // degree and the whole adjacency row are loaded together
             const uint32_t deg = a.degrees[node];
-            uint32_t nb = 0xFFFFFFFFu;
-            if ((uint32_t)lane < deg) nb = a.edges[(uint64_t)node * a.edge_stride + lane];
+            const uint32_t raw_nb = (uint32_t)lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
+                                                                    : 0xFFFFFFFFu;
+            const uint32_t nb = (uint32_t)lane < deg ? raw_nb : 0xFFFFFFFFu;
             // first occurrence within the list, then test-and-set visited
             bool first_occ = (uint32_t)lane < deg;
             for (int j = 0; j < 63; ++j) {
@@ -120,16 +148,19 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             const uint32_t rank = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
             if (is_fresh) fresh[rank] = nb;
             __syncthreads();
-            // gate every fresh neighbour
+            // gate every fresh neighbour from its MSB row staged in LDS
             PaperEstimate e;
             e.valid = 0;
             e.lower_bound = 0.0f;
             e.short_ip = 0.0f;
+            const uint32_t gid = (uint32_t)lane < nf ? fresh[lane] : 0u;
+            float4 fac = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint8_t fv = 0;
+            if ((uint32_t)lane < nf) { fac = a.ix.factors[gid]; fv = a.ix.fvalid[gid]; }
+            const uint32_t mch = cd / 128;  // uint4 per MSB row
+            stage_rows(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, (uint32_t)lane);
             if ((uint32_t)lane < nf) {
-                const uint32_t id = fresh[lane];
-                const float4 fac = a.ix.factors[id];
-                const uint8_t fv = a.ix.fvalid[id];
-                const uint8_t *mrow = a.ix.msb + (uint64_t)id * (cd / 8);
+                const uint8_t *mrow = (const uint8_t *)(stage + (uint32_t)lane * mch);
                 e = a.codec == 3 ? gate_int8_one(x8, mrow, cd, fac, fv, s, a.eps0)
                                  : gate_fp32_one(q, mrow, cd, fac, fv, s, a.eps0);
                 f_lb[lane] = e.lower_bound;
@@ -145,20 +176,35 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             const uint32_t ns = (uint32_t)__popcll(sm);
             pruned += nf - ns;
             const uint32_t srank = (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
-            if (survive) surv[srank] = (uint32_t)lane;
+            if (survive) { surv[srank] = (uint32_t)lane; sid[srank] = gid; }
             __syncthreads();
-            // distances of the survivors: a 16-lane group per candidate (lane j
-            // runs the reference's accumulator j), four candidates per pass
-            for (uint32_t b = grp; b < ns; b += 4) {
-                const uint32_t slot = surv[b];
-                const uint32_t id = fresh[slot];
-                stage_row_group(stg, a.ix.codes + (uint64_t)id * (cd / 2), cd / 2, j16);
-                const float2 h = a.ix.hdr[id];
-                float d;
-                if (!f_valid[slot]) d = asym_group(q, stg8, cd, h, s, j16);
-                else if (a.codec == 3) d = rem_int8_group(x8p, stg8, cd, h, s, f_sip[slot], j16);
-                else d = rem_fp32_group(q, stg8, cd, h, s, f_sip[slot], j16);
-                if (j16 == 0) s_d[b] = d;
+            // headers of the survivors: issued ahead of the code rows, so both
+            // arrive in the same round trip
+            float2 hreg = make_float2(0.0f, 0.0f);
+            if ((uint32_t)lane < ns) hreg = a.ix.hdr[sid[lane]];
+            // distances of the survivors: code rows staged in LDS in batches,
+            // then a 16-lane group per candidate (lane j runs the reference's
+            // accumulator j), four candidates per pass
+            const uint32_t cch = cd / 32;  // uint4 per code row
+            const uint32_t per = 1024u / cch;
+            for (uint32_t b0 = 0; b0 < ns; b0 += per) {
+                const uint32_t nb_rows = ns - b0 < per ? ns - b0 : per;
+                stage_rows(stage, sid + b0, nb_rows, cch, (const uint4 *)a.ix.codes, (uint32_t)lane);
+                if (b0 == 0) {
+                    if ((uint32_t)lane < ns) s_h[lane] = hreg;
+                    wave_sync_lds();
+                }
+                for (uint32_t b = grp; b < nb_rows; b += 4) {
+                    const uint32_t slot = surv[b0 + b];
+                    const uint8_t *row8 = (const uint8_t *)(stage + b * cch);
+                    const float2 h = s_h[b0 + b];
+                    float d;
+                    if (!f_valid[slot]) d = asym_group(q, row8, cd, h, s, j16);
+                    else if (a.codec == 3) d = rem_int8_group(x8p, row8, cd, h, s, f_sip[slot], j16);
+                    else d = rem_fp32_group(q, row8, cd, h, s, f_sip[slot], j16);
+                    if (j16 == 0) s_d[b0 + b] = d;
+                }
+                __syncthreads();
             }
             dists += ns;
             __syncthreads();
@@ -171,10 +217,11 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
                 if (full && f_valid[slot] && f_lb[slot] > pd[len - 1]) { ++pruned; continue; }
                 if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
                 // position = number of entries ordered before (d, id)
-                uint32_t cnt = 0;
-                for (uint32_t i = lane; i < len; i += kSearchLanes) cnt += cand_less(pd[i], pid[i], d, id) ? 1u : 0u;
-                for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
-                const uint32_t pos = cnt;
+                uint32_t pos = 0;
+                for (uint32_t c0 = 0; c0 < len; c0 += kSearchLanes) {
+                    const uint32_t i = c0 + (uint32_t)lane;
+                    pos += (uint32_t)__popcll(__ballot(i < len && cand_less(pd[i], pid[i], d, id)));
+                }
                 const uint32_t last = full ? L - 1 : len;  // entries [pos, last) move one slot right
                 // move in descending chunks of 64: each chunk is read before it is
                 // written, and lower chunks only overwrite entries already moved
