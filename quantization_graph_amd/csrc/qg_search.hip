@@ -1,6 +1,7 @@
 // qg_search.hip -- batched greedy beam search over a Vamana graph with the
 // 1-bit gate, INT8 / FP32 remaining-bit distances and the residual-4 rerank,
-// one wavefront per query in a persistent work-queue launch.
+// one workgroup of kSearchWaves wavefronts per query in a persistent
+// work-queue launch.
 //
 // Semantics follow search_ours_paper_active_inner + flush_distance_batch
 // (experiments/02_diskann_fair/src/ours_diskann.rs:1539-1986) with beam
@@ -18,12 +19,19 @@
 //        residual-4 interval; the answer is the top-k by (estimate, id).
 // Degrees are at most 64, so a hop's fresh neighbours form one distance
 // batch (DIST_BATCH_SIZE = 64), as in the reference.
+//
+// Work split: the serial steps of a hop (neighbour list, visited marks, the
+// prune decision, the ordered insertion) run on wave 0; the gate and the
+// survivors' distances run on 16-lane groups (lane j owns the reference's
+// accumulator j), 4 * kSearchWaves candidates at a time, from MSB and code
+// rows staged in LDS by gfx950's global-to-LDS loads, one memory round trip
+// per batch.
 #include "qg_args.h"
 #include "qg_device.h"
 
 namespace qg {
 
-constexpr int kSearchLanes = 64;
+constexpr int kSearchWaves = 4;
 constexpr uint32_t kMaxL = 1024;
 
 __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
@@ -38,24 +46,28 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // Copy rows ids[0..count) (ids in LDS) of a row-major device table (row_u4
 // uint4 per row, a power of two) to LDS rows 0..count) with gfx950's direct
-// global-to-LDS loads: every 16-byte piece of the batch is in flight at once
-// and no registers hold the data. Lanes past the end re-read the last piece
-// into stage slots below 1024 that nobody reads. count * row_u4 <= 1024.
+// global-to-LDS loads, the W waves of the group taking interleaved 1 KB
+// pieces; every piece is in flight at once and no registers hold the data.
+// Lanes past the end re-read the last piece into stage slots below 1024 that
+// nobody reads. count * row_u4 <= 1024. The caller's barrier publishes the
+// rows to the other waves.
+template <int W>
 __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
-                                           const uint4 *__restrict__ src, uint32_t lane) {
+                                           const uint4 *__restrict__ src, uint32_t wv, uint32_t lane) {
     const uint32_t total = count * row_u4;
     const uint32_t sh = (uint32_t)__builtin_ctz(row_u4);
-    for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+    for (uint32_t k0 = 64u * wv; k0 < total; k0 += 64u * W) {
         const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
         const uint4 *g = src + (uint64_t)ids[e >> sh] * row_u4 + (e & (row_u4 - 1u));
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
                                          (__attribute__((address_space(3))) void *)(dst + k0), 16, 0, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the batch has landed in LDS
-    wave_sync_lds();
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's pieces are in LDS
 }
 
-__global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
+    constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ unsigned char smem[];
     const uint32_t cd = a.cd;
     const uint32_t L = a.L;
@@ -65,33 +77,35 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
     float *pd = (float *)(smem + 6 * cd);                       // L
     uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * L);        // L
     uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * L);         // L (+pad)
+    __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank arrays after the traversal
     __shared__ uint32_t fresh[64], surv[64], sid[64];
+    __shared__ float4 f_fac[64];
     __shared__ float f_lb[64], f_sip[64], s_d[64];
     __shared__ float2 s_h[64];
-    __shared__ uint8_t f_valid[64];
-    __shared__ float r_est[kMaxL];
-    __shared__ uint32_t r_id[kMaxL];
-    __shared__ uint32_t s_qi;
-    __shared__ uint4 stage[1024];  // 16 KB: a hop's MSB rows, or up to 1024 / (cd/32) code rows
-    const int lane = threadIdx.x;
-    const int grp = lane >> 4, j16 = lane & 15;
+    __shared__ uint8_t f_fv[64], f_valid[64];
+    __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
+    float *r_est = (float *)stage;                   // kMaxL
+    uint32_t *r_id = (uint32_t *)stage + kMaxL;      // kMaxL
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
+    const int j16 = (int)(t & 15u);
+    const uint32_t mch = cd / 128;  // uint4 per MSB row
+    const uint32_t cch = cd / 32;   // uint4 per code row
+    const uint32_t per = 1024u / cch;
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
 
     for (;;) {
-        if (lane == 0) s_qi = atomicAdd(a.counter, 1u);
+        if (t == 0) s_qi = atomicAdd(a.counter, 1u);
         __syncthreads();
         const uint32_t qi = s_qi;
         __syncthreads();
         if (qi >= a.qv.nq) break;
         // stage the prepared query, clear this slot's visited bits
-        for (uint32_t i = lane; i < cd / 4; i += kSearchLanes) {
+        for (uint32_t i = t; i < cd / 4; i += T) {
             ((float4 *)q)[i] = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
-        }
-        for (uint32_t i = lane; i < cd / 4; i += kSearchLanes) {
             ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
             ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
         }
-        for (uint32_t i = lane; i < a.visited_words; i += kSearchLanes) visited[i] = 0u;
+        for (uint32_t i = t; i < a.visited_words; i += T) visited[i] = 0u;
         const QScalars s = a.qv.qs[qi];
         __threadfence();
         __syncthreads();
@@ -99,102 +113,113 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
         uint32_t len = 0;
         uint64_t hops = 0, gates = 0, dists = 1, pruned = 0;
         // start node 0 with its full FP32 distance
-        if (lane == 0) {
-            atomicOr(&visited[0], 1u);
-            pd[0] = asym_one(q, a.ix.codes, cd, a.ix.hdr[0], s);
-            pid[0] = 0;
-            pexp[0] = 0;
+        if (a.n > 0 && L > 0 && wv == 0) {
+            if (lane == 0) sid[0] = 0u;
+            wave_sync_lds();
+            stage_rows<1>(stage, sid, 1, cch, (const uint4 *)a.ix.codes, 0u, lane);
+            wave_sync_lds();
+            if (grp == 0) {
+                const float d0 = asym_group(q, (const uint8_t *)stage, cd, a.ix.hdr[0], s, j16);
+                if (j16 == 0) {
+                    atomicOr(&visited[0], 1u);
+                    pd[0] = d0;
+                    pid[0] = 0;
+                    pexp[0] = 0;
+                }
+            }
         }
         len = (a.n > 0 && L > 0) ? 1 : 0;
         __syncthreads();
 
         for (;;) {
-            // first unexpanded entry
+            // first unexpanded entry (every wave reads the same pool)
             uint32_t f = 0xFFFFFFFFu;
-            for (uint32_t c0 = 0; c0 < len; c0 += kSearchLanes) {
-                const uint32_t i = c0 + (uint32_t)lane;
+            for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
+                const uint32_t i = c0 + lane;
                 const uint64_t m = __ballot(i < len && !pexp[i]);
                 if (m) { f = c0 + (uint32_t)__builtin_ctzll(m); break; }
             }
             if (f == 0xFFFFFFFFu) break;
             const uint32_t node = pid[f];
             __syncthreads();
-            if (lane == 0) pexp[f] = 1;
             ++hops;
-            // degree and the whole adjacency row are loaded together
-            const uint32_t deg = a.degrees[node];
-            const uint32_t raw_nb = (uint32_t)lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
-                                                                    : 0xFFFFFFFFu;
-            const uint32_t nb = (uint32_t)lane < deg ? raw_nb : 0xFFFFFFFFu;
-            // first occurrence within the list, then test-and-set visited
-            bool first_occ = (uint32_t)lane < deg;
-            for (int j = 0; j < 63; ++j) {
-                const uint32_t other = (uint32_t)__shfl((int)nb, j);
-                if (j < lane && other == nb) first_occ = false;
+            if (wv == 0) {
+                if (lane == 0) pexp[f] = 1;
+                // degree and the whole adjacency row are loaded together
+                const uint32_t deg = a.degrees[node];
+                const uint32_t raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
+                                                             : 0xFFFFFFFFu;
+                const uint32_t nb = lane < deg ? raw_nb : 0xFFFFFFFFu;
+                // first occurrence within the list, then test-and-set visited
+                bool first_occ = lane < deg;
+                for (int jj = 0; jj < 63; ++jj) {
+                    const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb, jj);
+                    if ((uint32_t)jj < lane && other == nb) first_occ = false;
+                }
+                bool is_fresh = false;
+                if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
+                    atomicOr(a.err, 1u);
+                    first_occ = false;
+                }
+                if (first_occ) {
+                    const uint32_t bit = 1u << (nb & 31u);
+                    const uint32_t old = atomicOr(&visited[nb >> 5], bit);
+                    is_fresh = (old & bit) == 0u;
+                }
+                const uint64_t fm = __ballot(is_fresh);
+                if (is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
+                if (lane == 0) s_nf = (uint32_t)__popcll(fm);
             }
-            bool is_fresh = false;
-            if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
-                atomicOr(a.err, 1u);
-                first_occ = false;
-            }
-            if (first_occ) {
-                const uint32_t bit = 1u << (nb & 31u);
-                const uint32_t old = atomicOr(&visited[nb >> 5], bit);
-                is_fresh = (old & bit) == 0u;
-            }
-            const uint64_t fm = __ballot(is_fresh);
-            const uint32_t nf = (uint32_t)__popcll(fm);
-            if (nf == 0) { __syncthreads(); continue; }
-            const uint32_t rank = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
-            if (is_fresh) fresh[rank] = nb;
             __syncthreads();
-            // gate every fresh neighbour from its MSB row staged in LDS
-            PaperEstimate e;
-            e.valid = 0;
-            e.lower_bound = 0.0f;
-            e.short_ip = 0.0f;
-            const uint32_t gid = (uint32_t)lane < nf ? fresh[lane] : 0u;
-            float4 fac = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            uint8_t fv = 0;
-            if ((uint32_t)lane < nf) { fac = a.ix.factors[gid]; fv = a.ix.fvalid[gid]; }
-            const uint32_t mch = cd / 128;  // uint4 per MSB row
-            stage_rows(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, (uint32_t)lane);
-            if ((uint32_t)lane < nf) {
-                const uint8_t *mrow = (const uint8_t *)(stage + (uint32_t)lane * mch);
-                e = a.codec == 3 ? gate_int8_one(x8, mrow, cd, fac, fv, s, a.eps0)
-                                 : gate_fp32_one(q, mrow, cd, fac, fv, s, a.eps0);
-                f_lb[lane] = e.lower_bound;
-                f_sip[lane] = e.short_ip;
-                f_valid[lane] = e.valid;
-            }
+            const uint32_t nf = s_nf;
+            if (nf == 0) continue;
             gates += nf;
-            // prune against the pool as it stood at the start of the hop
-            const bool full0 = len >= L;
-            const float back0 = full0 ? pd[len - 1] : 3.402823466e38f;
-            const bool survive = (uint32_t)lane < nf && !(full0 && e.valid && e.lower_bound > back0);
-            const uint64_t sm = __ballot(survive);
-            const uint32_t ns = (uint32_t)__popcll(sm);
-            pruned += nf - ns;
-            const uint32_t srank = (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
-            if (survive) { surv[srank] = (uint32_t)lane; sid[srank] = gid; }
+            // gate: factors and MSB rows of the fresh neighbours in one round trip
+            if (t < nf) {
+                const uint32_t id = fresh[t];
+                f_fac[t] = a.ix.factors[id];
+                f_fv[t] = a.ix.fvalid[id];
+            }
+            stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
             __syncthreads();
-            // headers of the survivors: issued ahead of the code rows, so both
-            // arrive in the same round trip
+            for (uint32_t c = grp; c < nf; c += G) {
+                const uint8_t *mrow = (const uint8_t *)(stage + c * mch);
+                const PaperEstimate e = a.codec == 3
+                    ? gate_int8_group(x8, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16)
+                    : gate_fp32_group(q, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16);
+                if (j16 == 0) {
+                    f_lb[c] = e.lower_bound;
+                    f_sip[c] = e.short_ip;
+                    f_valid[c] = e.valid;
+                }
+            }
+            __syncthreads();
+            // prune against the pool as it stood at the start of the hop
+            if (wv == 0) {
+                const bool full0 = len >= L;
+                const float back0 = full0 ? pd[len - 1] : 3.402823466e38f;
+                const bool survive = lane < nf && !(full0 && f_valid[lane] && f_lb[lane] > back0);
+                const uint64_t sm = __ballot(survive);
+                if (survive) {
+                    const uint32_t r = (uint32_t)__popcll(sm & ((1ull << lane) - 1ull));
+                    surv[r] = lane;
+                    sid[r] = fresh[lane];
+                }
+                if (lane == 0) s_ns = (uint32_t)__popcll(sm);
+            }
+            __syncthreads();
+            const uint32_t ns = s_ns;
+            pruned += nf - ns;
+            // survivors' distances: headers and code rows in one round trip per
+            // batch of `per` rows, one 16-lane group per candidate
             float2 hreg = make_float2(0.0f, 0.0f);
-            if ((uint32_t)lane < ns) hreg = a.ix.hdr[sid[lane]];
-            // distances of the survivors: code rows staged in LDS in batches,
-            // then a 16-lane group per candidate (lane j runs the reference's
-            // accumulator j), four candidates per pass
-            const uint32_t cch = cd / 32;  // uint4 per code row
-            const uint32_t per = 1024u / cch;
+            if (t < ns) hreg = a.ix.hdr[sid[t]];
             for (uint32_t b0 = 0; b0 < ns; b0 += per) {
                 const uint32_t nb_rows = ns - b0 < per ? ns - b0 : per;
-                stage_rows(stage, sid + b0, nb_rows, cch, (const uint4 *)a.ix.codes, (uint32_t)lane);
-                if (b0 == 0) {
-                    if ((uint32_t)lane < ns) s_h[lane] = hreg;
-                    wave_sync_lds();
-                }
-                for (uint32_t b = grp; b < nb_rows; b += 4) {
+                stage_rows<W>(stage, sid + b0, nb_rows, cch, (const uint4 *)a.ix.codes, wv, lane);
+                if (b0 == 0 && t < ns) s_h[t] = hreg;
+                __syncthreads();
+                for (uint32_t b = grp; b < nb_rows; b += G) {
                     const uint32_t slot = surv[b0 + b];
                     const uint8_t *row8 = (const uint8_t *)(stage + b * cch);
                     const float2 h = s_h[b0 + b];
@@ -207,39 +232,43 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
                 __syncthreads();
             }
             dists += ns;
-            __syncthreads();
-            // ordered insertion with the re-check against the current back
-            for (uint32_t b = 0; b < ns; ++b) {
-                const uint32_t slot = surv[b];
-                const uint32_t id = fresh[slot];
-                const float d = s_d[b];
-                const bool full = len >= L;
-                if (full && f_valid[slot] && f_lb[slot] > pd[len - 1]) { ++pruned; continue; }
-                if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
-                // position = number of entries ordered before (d, id)
-                uint32_t pos = 0;
-                for (uint32_t c0 = 0; c0 < len; c0 += kSearchLanes) {
-                    const uint32_t i = c0 + (uint32_t)lane;
-                    pos += (uint32_t)__popcll(__ballot(i < len && cand_less(pd[i], pid[i], d, id)));
+            // ordered insertion with the re-check against the current back (wave 0)
+            if (wv == 0) {
+                for (uint32_t b = 0; b < ns; ++b) {
+                    const uint32_t slot = surv[b];
+                    const uint32_t id = sid[b];
+                    const float d = s_d[b];
+                    const bool full = len >= L;
+                    if (full && f_valid[slot] && f_lb[slot] > pd[len - 1]) { ++pruned; continue; }
+                    if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
+                    // position = number of entries ordered before (d, id)
+                    uint32_t pos = 0;
+                    for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
+                        const uint32_t i = c0 + lane;
+                        pos += (uint32_t)__popcll(__ballot(i < len && cand_less(pd[i], pid[i], d, id)));
+                    }
+                    const uint32_t last = full ? L - 1 : len;  // entries [pos, last) move one slot right
+                    // move in descending chunks of 64: each chunk is read before it is
+                    // written, and lower chunks only overwrite entries already moved
+                    for (uint32_t c = 0; c + pos < last; c += 64u) {
+                        const uint32_t idx = last - 1 - c - lane;
+                        const bool mv = c + lane < last - pos;
+                        float md = 0.0f;
+                        uint32_t mi = 0;
+                        uint8_t me = 0;
+                        if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
+                        wave_sync_lds();
+                        if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
+                        wave_sync_lds();
+                    }
+                    if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
+                    if (!full) ++len;
+                    wave_sync_lds();
                 }
-                const uint32_t last = full ? L - 1 : len;  // entries [pos, last) move one slot right
-                // move in descending chunks of 64: each chunk is read before it is
-                // written, and lower chunks only overwrite entries already moved
-                for (uint32_t c = 0; c + pos < last; c += kSearchLanes) {
-                    const uint32_t idx = last - 1 - c - (uint32_t)lane;
-                    const bool mv = c + (uint32_t)lane < last - pos;
-                    float md = 0.0f;
-                    uint32_t mi = 0;
-                    uint8_t me = 0;
-                    if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
-                    __syncthreads();
-                    if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
-                    __syncthreads();
-                }
-                if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
-                if (!full) ++len;
-                __syncthreads();
+                if (lane == 0) s_len = len;
             }
+            __syncthreads();
+            len = s_len;
         }
 
         if (a.pool_only) goto finish;
@@ -247,7 +276,7 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
         // rerank: first rc pool entries, FP32 long distance, residual-4
         const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
         const uint32_t rc = len < want ? len : want;
-        for (uint32_t i = lane; i < rc; i += kSearchLanes) {
+        for (uint32_t i = t; i < rc; i += T) {
             const uint32_t id = pid[i];
             float ld = pd[i];
             if (a.codec == 3) ld = asym_one(q, a.ix.codes + (uint64_t)id * (cd / 2), cd, a.ix.hdr[id], s);
@@ -257,41 +286,46 @@ __global__ void __launch_bounds__(kSearchLanes) k_search(SearchArgs a) {
             r_id[i] = id;
         }
         __syncthreads();
-        // top-k by (estimate, id): k rounds of a wave-wide argmin
-        const uint32_t kout = rc < a.k ? rc : a.k;
-        for (uint32_t r = 0; r < a.k; ++r) {
-            if (r >= kout) {
-                if (lane == 0) { a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu; a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf(); }
-                continue;
+        // top-k by (estimate, id): k rounds of a wave-wide argmin on wave 0
+        if (wv == 0) {
+            const uint32_t kout = rc < a.k ? rc : a.k;
+            for (uint32_t r = 0; r < a.k; ++r) {
+                if (r >= kout) {
+                    if (lane == 0) {
+                        a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu;
+                        a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf();
+                    }
+                    continue;
+                }
+                float bd = __builtin_huge_valf();
+                uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
+                for (uint32_t i = lane; i < rc; i += 64u) {
+                    if (r_id[i] == 0xFFFFFFFFu) continue;
+                    if (bpos == 0xFFFFFFFFu || cand_less(r_est[i], r_id[i], bd, bid)) { bd = r_est[i]; bid = r_id[i]; bpos = i; }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float od = __shfl_xor(bd, off);
+                    const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+                    const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
+                    const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || cand_less(od, oid, bd, bid));
+                    if (take) { bd = od; bid = oid; bpos = opos; }
+                }
+                if (lane == 0) {
+                    a.out_ids[(uint64_t)qi * a.k + r] = bid;
+                    a.out_d[(uint64_t)qi * a.k + r] = bd;
+                    r_id[bpos] = 0xFFFFFFFFu;
+                }
+                wave_sync_lds();
             }
-            float bd = __builtin_huge_valf();
-            uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
-            for (uint32_t i = lane; i < rc; i += kSearchLanes) {
-                if (r_id[i] == 0xFFFFFFFFu) continue;
-                if (bpos == 0xFFFFFFFFu || cand_less(r_est[i], r_id[i], bd, bid)) { bd = r_est[i]; bid = r_id[i]; bpos = i; }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                const float od = __shfl_xor(bd, off);
-                const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
-                const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
-                const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || cand_less(od, oid, bd, bid));
-                if (take) { bd = od; bid = oid; bpos = opos; }
-            }
-            if (lane == 0) {
-                a.out_ids[(uint64_t)qi * a.k + r] = bid;
-                a.out_d[(uint64_t)qi * a.k + r] = bd;
-                r_id[bpos] = 0xFFFFFFFFu;
-            }
-            __syncthreads();
         }
         }
     finish:
-        if (lane == 0 && a.stats) {
+        if (t == 0 && a.stats) {
             uint64_t *st = a.stats + (uint64_t)qi * 4;
             st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
         }
         if (a.pool_ids) {
-            for (uint32_t i = lane; i < L; i += kSearchLanes) {
+            for (uint32_t i = t; i < L; i += T) {
                 a.pool_ids[(uint64_t)qi * L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
                 a.pool_d[(uint64_t)qi * L + i] = i < len ? pd[i] : __builtin_huge_valf();
             }
@@ -311,6 +345,6 @@ extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStre
     if (a->qv.nq == 0) return 0;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
     const size_t smem = (size_t)a->cd * 6 + (size_t)a->L * 9 + 16;
-    hipLaunchKernelGGL(qg::k_search, dim3(slots), dim3(qg::kSearchLanes), smem, stream, *a);
+    hipLaunchKernelGGL(qg::k_search<qg::kSearchWaves>, dim3(slots), dim3(64 * qg::kSearchWaves), smem, stream, *a);
     return (int)hipGetLastError();
 }
