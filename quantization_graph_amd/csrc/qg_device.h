@@ -237,18 +237,21 @@ __device__ __forceinline__ void stage_row_group(uint4 *dst, const uint8_t *src, 
 }
 
 // C1 on a staged code row: accumulator j sums dims 2(16c + j), 2(16c + j) + 1
+// (uniform trip count, unrolled so the LDS reads of 8 steps are in flight
+// together)
 __device__ __forceinline__ float asym_group(const float *q, const uint8_t *row_lds, uint32_t cd, float2 h,
                                            const QScalars &s, int j) {
     if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
     float sh = 0.0f, rm = 0.0f;
+#pragma unroll 8
     for (uint32_t c = 0; c < cd / 32; ++c) {
         const uint32_t byte = row_lds[16 * c + j];
         const float2 qq = *(const float2 *)(q + 32 * c + 2 * j);
         const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
         rm = fmaf((float)(lo & 7u), qq.x, rm);
         rm = fmaf((float)(hi & 7u), qq.y, rm);
-        if (lo > 7u) sh = sh + qq.x;
-        if (hi > 7u) sh = sh + qq.y;
+        sh = sh + masked_q(byte, 3, qq.x);
+        sh = sh + masked_q(byte, 7, qq.y);
     }
     const float short_ip = group_reduce16(sh) - s.half_sum;
     return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, group_reduce16(rm));
@@ -259,6 +262,7 @@ __device__ __forceinline__ float rem_fp32_group(const float *q, const uint8_t *r
                                                const QScalars &s, float short_ip, int j) {
     if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
     float rm = 0.0f;
+#pragma unroll 8
     for (uint32_t c = 0; c < cd / 32; ++c) {
         const uint32_t byte = row_lds[16 * c + j];
         const float2 qq = *(const float2 *)(q + 32 * c + 2 * j);
@@ -273,12 +277,15 @@ __device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t
                                                const QScalars &s, float short_ip, int j) {
     if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
     const uint32_t *row = (const uint32_t *)row_lds;
-    const uint32_t *xg = (const uint32_t *)x8p;
+    const uint2 *xg = (const uint2 *)x8p;
     int acc = 0;
-    for (uint32_t d = j; d < cd / 8; d += 16) {
+#pragma unroll 8
+    for (uint32_t i = 0; i < cd / 128; ++i) {
+        const uint32_t d = (uint32_t)j + 16u * i;
         const uint32_t w = row[d];
-        acc = __builtin_amdgcn_sdot4((int)(w & 0x07070707u), (int)xg[2 * d], acc, false);
-        acc = __builtin_amdgcn_sdot4((int)((w >> 4) & 0x07070707u), (int)xg[2 * d + 1], acc, false);
+        const uint2 x = xg[d];
+        acc = __builtin_amdgcn_sdot4((int)(w & 0x07070707u), (int)x.x, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)((w >> 4) & 0x07070707u), (int)x.y, acc, false);
     }
     acc = group_sum_int(acc);
     const double centered = (double)s.int8_scale * ((double)acc - 3.5 * (double)s.int8_code_sum);
@@ -286,20 +293,32 @@ __device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t
     return h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
 }
 
-// C4 on a staged MSB row: accumulator j sums dims 16m + j
-__device__ __forceinline__ PaperEstimate gate_fp32_group(const float *q, const uint8_t *msb_lds, uint32_t cd, float4 f,
-                                                        uint8_t fv, const QScalars &s, float eps0, int j) {
-    const uint32_t *mw = (const uint32_t *)msb_lds;
+// C4 on a staged MSB row: accumulator j sums dims 16m + j in order. qp2
+// holds the query pair-interleaved: qp2[16w + j] = (q[32w + j],
+// q[32w + 16 + j]), the two dims of MSB word w that lane j owns.
+__device__ __forceinline__ PaperEstimate gate_fp32_group(const float2 *qp2, const uint8_t *msb_lds, uint32_t cd,
+                                                        float4 f, uint8_t fv, const QScalars &s, float eps0, int j) {
+    const uint4 *m4 = (const uint4 *)msb_lds;
+    const int j2 = 16 + j;
     float acc = 0.0f;
-    for (uint32_t w = 0; w < cd / 32; ++w) {
-        const uint32_t bits = mw[w];
-        acc = acc + masked_q(bits, j, q[32 * w + j]);
-        acc = acc + masked_q(bits, 16 + j, q[32 * w + 16 + j]);
+#pragma unroll 2
+    for (uint32_t u = 0; u < cd / 128; ++u) {
+        const uint4 mw = m4[u];
+        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
+        const float2 q2 = qp2[64 * u + 32 + j], q3 = qp2[64 * u + 48 + j];
+        acc = acc + masked_q(mw.x, j, q0.x);
+        acc = acc + masked_q(mw.x, j2, q0.y);
+        acc = acc + masked_q(mw.y, j, q1.x);
+        acc = acc + masked_q(mw.y, j2, q1.y);
+        acc = acc + masked_q(mw.z, j, q2.x);
+        acc = acc + masked_q(mw.z, j2, q2.y);
+        acc = acc + masked_q(mw.w, j, q3.x);
+        acc = acc + masked_q(mw.w, j2, q3.y);
     }
     return finish_gate(group_reduce16(acc) - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
 }
 
-// C5 on a staged MSB row: 4-dim groups g4 = j, j + 16, ...
+// C5 on a staged MSB row: 4-dim groups g4 = j + 16i (uniform trip count)
 __device__ __forceinline__ PaperEstimate gate_int8_group(const int8_t *x8, const uint8_t *msb_lds, uint32_t cd,
                                                         float4 f, uint8_t fv, const QScalars &s, float eps0, int j) {
     if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) {
@@ -310,10 +329,13 @@ __device__ __forceinline__ PaperEstimate gate_int8_group(const int8_t *x8, const
     }
     const uint32_t *xg = (const uint32_t *)x8;
     const uint32_t *mw = (const uint32_t *)msb_lds;
+    const uint32_t sh = 4u * ((uint32_t)j & 7u);
+    const uint32_t half = (uint32_t)j >> 3;
     int sel = 0;
-    for (uint32_t g4 = j; g4 < cd / 4; g4 += 16) {
-        sel = __builtin_amdgcn_sdot4((int)nibble_bytes((mw[g4 >> 3] >> (4 * (g4 & 7u))) & 0xFu), (int)xg[g4], sel,
-                                     false);
+#pragma unroll 8
+    for (uint32_t i = 0; i < cd / 64; ++i) {
+        const uint32_t g4 = (uint32_t)j + 16u * i;  // dims 4 g4 .. 4 g4 + 3: word 2i + j/8, nibble j % 8
+        sel = __builtin_amdgcn_sdot4((int)nibble_bytes((mw[2u * i + half] >> sh) & 0xFu), (int)xg[g4], sel, false);
     }
     sel = group_sum_int(sel);
     const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);

@@ -65,18 +65,22 @@ __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's pieces are in LDS
 }
 
+// 4 waves per SIMD (<= 128 VGPRs): 1024 workgroups fit the chip at once.
 template <int W>
-__global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_search(SearchArgs a) {
     constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ unsigned char smem[];
     const uint32_t cd = a.cd;
     const uint32_t L = a.L;
+    // query: FP32 residual, then one codec-specific region of 4 cd bytes:
+    // the pair-interleaved FP32 query of the gate, or the INT8 codes
     float *q = (float *)smem;                                   // cd
-    int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd
-    int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd
-    float *pd = (float *)(smem + 6 * cd);                       // L
-    uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * L);        // L
-    uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * L);         // L (+pad)
+    float2 *qp2 = (float2 *)(smem + 4 * cd);                    // cd / 2 (FP32 codec)
+    int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd     (INT8 codec)
+    int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd     (INT8 codec)
+    float *pd = (float *)(smem + 8 * cd);                       // L
+    uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * L);        // L
+    uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * L);         // L (+pad)
     __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank arrays after the traversal
     __shared__ uint32_t fresh[64], surv[64], sid[64];
     __shared__ float4 f_fac[64];
@@ -100,10 +104,18 @@ __global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
         __syncthreads();
         if (qi >= a.qv.nq) break;
         // stage the prepared query, clear this slot's visited bits
-        for (uint32_t i = t; i < cd / 4; i += T) {
+        for (uint32_t i = t; i < cd / 4; i += T)
             ((float4 *)q)[i] = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
-            ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
-            ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
+        if (a.codec == 3) {
+            for (uint32_t i = t; i < cd / 4; i += T) {
+                ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
+                ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
+            }
+        } else {
+            for (uint32_t i = t; i < cd / 2; i += T) {
+                const float *qq = a.qv.q + (uint64_t)qi * cd + 32 * (i >> 4) + (i & 15u);
+                qp2[i] = make_float2(qq[0], qq[16]);
+            }
         }
         for (uint32_t i = t; i < a.visited_words; i += T) visited[i] = 0u;
         const QScalars s = a.qv.qs[qi];
@@ -144,6 +156,9 @@ __global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
             __syncthreads();
             ++hops;
             if (wv == 0) {
+                // the serial steps of a hop hold up the whole group: let this
+                // wave issue ahead of the other groups' waves on its SIMD
+                __builtin_amdgcn_s_setprio(3);
                 if (lane == 0) pexp[f] = 1;
                 // degree and the whole adjacency row are loaded together
                 const uint32_t deg = a.degrees[node];
@@ -169,6 +184,7 @@ __global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
                 const uint64_t fm = __ballot(is_fresh);
                 if (is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
                 if (lane == 0) s_nf = (uint32_t)__popcll(fm);
+                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
             const uint32_t nf = s_nf;
@@ -186,7 +202,7 @@ __global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
                 const uint8_t *mrow = (const uint8_t *)(stage + c * mch);
                 const PaperEstimate e = a.codec == 3
                     ? gate_int8_group(x8, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16)
-                    : gate_fp32_group(q, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16);
+                    : gate_fp32_group(qp2, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16);
                 if (j16 == 0) {
                     f_lb[c] = e.lower_bound;
                     f_sip[c] = e.short_ip;
@@ -232,40 +248,101 @@ __global__ void __launch_bounds__(64 * W) k_search(SearchArgs a) {
                 __syncthreads();
             }
             dists += ns;
-            // ordered insertion with the re-check against the current back (wave 0)
+            // ordered insertion with the re-check against the current back
+            // (insert_candidate_sorted in survivor order), on wave 0. The
+            // decisions only need the pool's back, which is the larger of the
+            // last retained pool entry and the last accepted survivor: the
+            // pool's last 64 entries and the accepted survivors (sorted) sit in
+            // lane registers, and the pool is rewritten once at the end.
             if (wv == 0) {
+                __builtin_amdgcn_s_setprio(3);
+                const uint32_t tb = len > 64u ? len - 64u : 0u;  // pool tail window [tb, len)
+                const float bt_d = tb + lane < len ? pd[tb + lane] : 0.0f;
+                const uint32_t bt_id = tb + lane < len ? pid[tb + lane] : 0u;
+                const uint32_t my_slot = lane < ns ? surv[lane] : 0u;
+                const float my_d = lane < ns ? s_d[lane] : 0.0f;
+                const uint32_t my_id = lane < ns ? sid[lane] : 0u;
+                const float my_lb = f_lb[my_slot];
+                const uint32_t my_v = f_valid[my_slot];
+                uint32_t e = len, na = 0;  // retained pool entries, accepted survivors
+                float a_d = 0.0f;
+                uint32_t a_id = 0xFFFFFFFFu;
                 for (uint32_t b = 0; b < ns; ++b) {
-                    const uint32_t slot = surv[b];
-                    const uint32_t id = sid[b];
-                    const float d = s_d[b];
-                    const bool full = len >= L;
-                    if (full && f_valid[slot] && f_lb[slot] > pd[len - 1]) { ++pruned; continue; }
-                    if (full && !cand_less(d, id, pd[len - 1], pid[len - 1])) continue;
-                    // position = number of entries ordered before (d, id)
-                    uint32_t pos = 0;
-                    for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
-                        const uint32_t i = c0 + lane;
-                        pos += (uint32_t)__popcll(__ballot(i < len && cand_less(pd[i], pid[i], d, id)));
+                    const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_d), (int)b));
+                    const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)my_id, (int)b);
+                    if (e + na >= L) {
+                        float bk_d = 0.0f;
+                        uint32_t bk_id = 0u;
+                        bool from_a = e == 0;
+                        if (e > 0) {
+                            bk_d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bt_d), (int)(e - 1 - tb)));
+                            bk_id = (uint32_t)__builtin_amdgcn_readlane((int)bt_id, (int)(e - 1 - tb));
+                        }
+                        if (na > 0) {
+                            const float ad = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), (int)(na - 1)));
+                            const uint32_t aid = (uint32_t)__builtin_amdgcn_readlane((int)a_id, (int)(na - 1));
+                            if (e == 0 || cand_less(bk_d, bk_id, ad, aid)) { bk_d = ad; bk_id = aid; from_a = true; }
+                        }
+                        const float lb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_lb), (int)b));
+                        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)my_v, (int)b);
+                        if (v && lb > bk_d) { ++pruned; continue; }
+                        if (!cand_less(d, id, bk_d, bk_id)) continue;
+                        if (from_a) --na; else --e;
                     }
-                    const uint32_t last = full ? L - 1 : len;  // entries [pos, last) move one slot right
-                    // move in descending chunks of 64: each chunk is read before it is
-                    // written, and lower chunks only overwrite entries already moved
-                    for (uint32_t c = 0; c + pos < last; c += 64u) {
-                        const uint32_t idx = last - 1 - c - lane;
-                        const bool mv = c + lane < last - pos;
-                        float md = 0.0f;
-                        uint32_t mi = 0;
-                        uint8_t me = 0;
-                        if (mv) { md = pd[idx]; mi = pid[idx]; me = pexp[idx]; }
-                        wave_sync_lds();
-                        if (mv) { pd[idx + 1] = md; pid[idx + 1] = mi; pexp[idx + 1] = me; }
-                        wave_sync_lds();
+                    // sorted insert into the accepted list: lanes past the slot take
+                    // their left neighbour (DPP wave_shr:1)
+                    const uint32_t pos = (uint32_t)__popcll(__ballot(lane < na && cand_less(a_d, a_id, d, id)));
+                    const int sd = __builtin_amdgcn_update_dpp(__float_as_int(a_d), __float_as_int(a_d), 0x138, 0xF, 0xF, false);
+                    const int si = __builtin_amdgcn_update_dpp((int)a_id, (int)a_id, 0x138, 0xF, 0xF, false);
+                    if (lane > pos && lane <= na) { a_d = __int_as_float(sd); a_id = (uint32_t)si; }
+                    if (lane == pos) { a_d = d; a_id = id; }
+                    ++na;
+                }
+                // merge pool[0, e) with the accepted list into pool[0, e + na):
+                // entries before the first accepted one keep their slot; the rest
+                // go through the stage area (free at this point)
+                if (na > 0) {
+                    float *m_d = (float *)stage;
+                    uint32_t *m_id = (uint32_t *)stage + 1024;
+                    uint8_t *m_x = (uint8_t *)((uint32_t *)stage + 2048);
+                    // accepted entry i lands at i + #(pool entries before it)
+                    uint32_t lo = 0, hi = e;
+                    if (lane < na) {
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (cand_less(pd[mid], pid[mid], a_d, a_id)) lo = mid + 1; else hi = mid;
+                        }
                     }
-                    if (lane == 0) { pd[pos] = d; pid[pos] = id; pexp[pos] = 0; }
-                    if (!full) ++len;
+                    const uint32_t first = (uint32_t)__builtin_amdgcn_readlane((int)lo, 0);
+                    if (lane < na) {
+                        const uint32_t dst = lane + lo - first;
+                        m_d[dst] = a_d; m_id[dst] = a_id; m_x[dst] = 0;
+                    }
+                    // retained entry k >= first lands at k + #(accepted entries before it)
+                    for (uint32_t k0 = first; k0 < e; k0 += 64u) {
+                        const uint32_t k = k0 + lane;
+                        if (k < e) {
+                            const float kd = pd[k];
+                            const uint32_t kid = pid[k];
+                            uint32_t c = 0;
+                            for (uint32_t i = 0; i < na; ++i) {
+                                const float ad = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), (int)i));
+                                const uint32_t aid = (uint32_t)__builtin_amdgcn_readlane((int)a_id, (int)i);
+                                c += cand_less(ad, aid, kd, kid) ? 1u : 0u;
+                            }
+                            const uint32_t dst = k + c - first;
+                            m_d[dst] = kd; m_id[dst] = kid; m_x[dst] = pexp[k];
+                        }
+                    }
+                    wave_sync_lds();
+                    for (uint32_t i = first + lane; i < e + na; i += 64u) {
+                        pd[i] = m_d[i - first]; pid[i] = m_id[i - first]; pexp[i] = m_x[i - first];
+                    }
                     wave_sync_lds();
                 }
+                len = e + na;
                 if (lane == 0) s_len = len;
+                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
             len = s_len;
@@ -344,7 +421,7 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream) {
     if (a->qv.nq == 0) return 0;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
-    const size_t smem = (size_t)a->cd * 6 + (size_t)a->L * 9 + 16;
+    const size_t smem = (size_t)a->cd * 8 + (size_t)a->L * 9 + 16;
     hipLaunchKernelGGL(qg::k_search<qg::kSearchWaves>, dim3(slots), dim3(64 * qg::kSearchWaves), smem, stream, *a);
     return (int)hipGetLastError();
 }
