@@ -335,7 +335,7 @@ def run_graph(args, torch, np, qg):
         "build_distance_evaluations": evals, "build_evals_per_s": evals / build_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "qg::k_search (persistent batched graph search, one wavefront per query)",
+                     "kernel": "qg::k_search (persistent batched graph search, one 4-wave workgroup per query)",
                      "kernel_ms": head["kernel_ms"], "bytes_per_unit": bpq, "unit_of_work": "query",
                      "units_per_launch": nq},
         "cpu_baseline": cpu, "cpu_build_baseline": cpu_build, "gpu_vs_reference_id_parity": parity,
@@ -603,6 +603,7 @@ def main():
             "roofline": graph["roofline"], "cpu_baseline": graph["cpu_baseline"],
             "cpu_build_baseline": graph["cpu_build_baseline"],
             "gpu_vs_reference_id_parity": graph["gpu_vs_reference_id_parity"],
+            "learned": graph["learned"],
             "fixed_candidates": fixed_res,
         }
     else:
