@@ -168,7 +168,8 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
 
 /* Batched graph search: greedy beam search (beam width 1, start node 0)
  * with the 1-bit gate, remaining-bit distances and the residual-4 rerank of
- * the first max(rerank, k) pool entries, one wavefront per query.
+ * the first max(rerank, k) pool entries, one 4-wavefront workgroup per
+ * query.
  * Replaces the per-query loop search_ours_paper_active_inner +
  * flush_distance_batch + rerank (ours_diskann.rs:1539-1986) and its bridge
  * calls. Graph: degrees [n], edges [n][edge_stride], edge_stride <= 64.
@@ -176,7 +177,8 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
  * optional stats [nq][4] = hops, gate evaluations, distances, pruned and
  * the final pool [nq][max(l_search, k)]. Workspace:
  * qg_search_workspace_bytes(n, slots) of device memory (per-slot visited
- * bitsets and the work-queue head); slots = concurrent queries. */
+ * bitsets and the work-queue head); slots = concurrent queries.
+ * max(l_search, k) <= 4096 and max(rerank, k) <= 1024. */
 size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots);
 int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
               const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
@@ -185,6 +187,22 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
               void *stream);
 /* With out_ids == out_dists == NULL, qg_search stops after the traversal
  * and only writes the pools (and stats): the input of the learned rerank. */
+
+/* qg_search with uneven lists: l_per_query (device, [nq], may be NULL) gives
+ * each query its own search list size, clamped to [k, max(l_search, k)]
+ * (l_search is then the pool capacity and the pool outputs' row stride);
+ * waves (2 or 4) sets the wavefronts per query workgroup. One launch serves
+ * every query through the work queue; it enqueues no allocation, copy or
+ * memset, so it can be captured in a HIP graph. This batched form has no
+ * single counterpart in the reference, whose runner loops over queries
+ * (diskann_runner.rs:360-439). Dynamic LDS per workgroup:
+ * qg_search_lds_bytes(code_dim, max(l_search, k)) plus about 19 KB static. */
+int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                 const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+                 const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots, uint32_t waves,
+                 uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists,
+                 void *workspace, size_t workspace_bytes, void *stream);
+size_t qg_search_lds_bytes(uint32_t cd, uint32_t L);
 
 /* ------------------------------------- learned adaptive rerank depth
  * No counterpart in the reference. A 24-64-64-1 MLP (BF16 MFMA, FP32

@@ -67,6 +67,13 @@ def lib():
     L.qg_search_workspace_bytes.argtypes = [u64, u32]
     L.qg_search.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, f32,
                             u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.qg_search_ex.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, vp,
+                               f32, u32, u32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
+    L.qg_search_lds_bytes.restype = sz
+    L.qg_search_lds_bytes.argtypes = [u32, u32]
+    L.qg_kmeans_workspace_bytes.restype = sz
+    L.qg_kmeans_workspace_bytes.argtypes = [u64, u64, u32]
+    L.qg_train_centroids.argtypes = [vp, u64, u64, u32, C.c_int, vp, vp, sz, vp]
     L.qg_build_workspace_bytes.restype = sz
     L.qg_build_workspace_bytes.argtypes = [u64, u32, u32, u32, u32]
     L.qg_build_graph.argtypes = [vp, C.POINTER(IndexDesc), u32, u32, f32, u32, u32, u32, u32, u32, vp, vp,

@@ -24,7 +24,7 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
 int qg_launch_topk(const qg::SelectArgs *a, hipStream_t stream);
 int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream);
 int qg_launch_pack(const qg::PackArgs *a, uint64_t n, int unpack, hipStream_t stream);
-int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream);
+int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream);
 int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st);
 uint32_t qg_mlp_grad_waves(uint64_t n);
 int qg_launch_mlp_grad(const float *P, const uint16_t *X, const float *t, uint64_t n, float tau, float *partial,
@@ -384,11 +384,11 @@ size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots) {
     return (size_t)slots * ((n + 32) / 32) * 4 + 256;
 }
 
-int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
-              const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
-              float eps0, uint32_t rerank, uint32_t slots, uint32_t *out_ids, float *out_dists,
-              uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
-              void *stream) {
+int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                 const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+                 const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots, uint32_t waves,
+                 uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists,
+                 void *workspace, size_t workspace_bytes, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     const bool pool_only = out_ids == nullptr && out_dists == nullptr;
     if (!s || !ix || !qv || !degrees || !edges || !workspace || (!pool_only && (!out_ids || !out_dists)))
@@ -399,7 +399,9 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
     if (edge_stride == 0 || edge_stride > 64) return fail("graph degree bound must be in [1, 64]");
     if (ix->n == 0 || ix->n > 0xFFFFFFFEull) return fail("index must hold 1 .. 2^32-2 points");
     const uint32_t L = l_search > k ? l_search : k;
-    if (L > 1024) return fail("search list size %u exceeds 1024", L);
+    if (L > 4096) return fail("search list size %u exceeds 4096", L);
+    if ((rerank > k ? rerank : k) > 1024) return fail("rerank count exceeds 1024");
+    if (waves != 2 && waves != 4) return fail("waves per query must be 2 or 4");
     if ((pool_ids == nullptr) != (pool_dists == nullptr)) return fail("pool outputs come in pairs");
     if (slots == 0) slots = 1;
     if (workspace_bytes < qg_search_workspace_bytes(ix->n, slots))
@@ -429,7 +431,17 @@ int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *q
     a.pool_ids = pool_ids;
     a.pool_d = pool_dists;
     a.pool_only = pool_only ? 1 : 0;
-    return check_hip(qg_launch_search(&a, slots, (hipStream_t)stream), "search launch");
+    a.lq = l_per_query;
+    return check_hip(qg_launch_search(&a, slots, waves, (hipStream_t)stream), "search launch");
+}
+
+int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+              const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+              float eps0, uint32_t rerank, uint32_t slots, uint32_t *out_ids, float *out_dists,
+              uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *workspace, size_t workspace_bytes,
+              void *stream) {
+    return qg_search_ex(space, ix, qv, codec, degrees, edges, edge_stride, k, l_search, nullptr, eps0, rerank, slots,
+                        4, out_ids, out_dists, out_stats, pool_ids, pool_dists, workspace, workspace_bytes, stream);
 }
 
 // ------------------------------------------------- learned rerank depth

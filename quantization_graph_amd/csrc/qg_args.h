@@ -120,6 +120,7 @@ struct SearchArgs {
     uint32_t *pool_ids;        // [nq][L] final pool (optional)
     int pool_only;             // skip the rerank (out_ids / out_d unused)
     float *pool_d;             // [nq][L]
+    const uint32_t *lq;        // [nq] per-query search list sizes (optional; L is then the pool capacity)
 };
 
 // Vamana build (qg_build.hip)

@@ -1,7 +1,8 @@
 // qg_search.hip -- batched greedy beam search over a Vamana graph with the
 // 1-bit gate, INT8 / FP32 remaining-bit distances and the residual-4 rerank,
-// one workgroup of kSearchWaves wavefronts per query in a persistent
-// work-queue launch.
+// one workgroup of W wavefronts (2 or 4, chosen at launch) per query in a
+// persistent work-queue launch; each query may carry its own search list
+// size.
 //
 // Semantics follow search_ours_paper_active_inner + flush_distance_batch
 // (experiments/02_diskann_fair/src/ours_diskann.rs:1539-1986) with beam
@@ -23,7 +24,7 @@
 // Work split: the serial steps of a hop (neighbour list, visited marks, the
 // prune decision, the ordered insertion) run on wave 0; the gate and the
 // survivors' distances run on 16-lane groups (lane j owns the reference's
-// accumulator j), 4 * kSearchWaves candidates at a time, from MSB and code
+// accumulator j), 4 * W candidates at a time, from MSB and code
 // rows staged in LDS by gfx950's global-to-LDS loads, one memory round trip
 // per batch.
 #include "qg_args.h"
@@ -31,8 +32,8 @@
 
 namespace qg {
 
-constexpr int kSearchWaves = 4;
-constexpr uint32_t kMaxL = 1024;
+constexpr uint32_t kMaxL = 4096;       // pool capacity (search list size)
+constexpr uint32_t kMaxRerank = 1024;  // rerank candidates per query
 
 __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
     return ad < bd || (ad == bd && aid < bid);
@@ -71,16 +72,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ unsigned char smem[];
     const uint32_t cd = a.cd;
-    const uint32_t L = a.L;
     // query: FP32 residual, then one codec-specific region of 4 cd bytes:
     // the pair-interleaved FP32 query of the gate, or the INT8 codes
     float *q = (float *)smem;                                   // cd
     float2 *qp2 = (float2 *)(smem + 4 * cd);                    // cd / 2 (FP32 codec)
     int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd     (INT8 codec)
     int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd     (INT8 codec)
-    float *pd = (float *)(smem + 8 * cd);                       // L
-    uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * L);        // L
-    uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * L);         // L (+pad)
+    float *pd = (float *)(smem + 8 * cd);                       // a.L
+    uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * a.L);      // a.L
+    uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * a.L);       // a.L (+pad)
     __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank arrays after the traversal
     __shared__ uint32_t fresh[64], surv[64], sid[64];
     __shared__ float4 f_fac[64];
@@ -88,8 +88,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ float2 s_h[64];
     __shared__ uint8_t f_fv[64], f_valid[64];
     __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
-    float *r_est = (float *)stage;                   // kMaxL
-    uint32_t *r_id = (uint32_t *)stage + kMaxL;      // kMaxL
+    float *r_est = (float *)stage;                   // kMaxRerank
+    uint32_t *r_id = (uint32_t *)stage + kMaxRerank; // kMaxRerank
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
     const uint32_t mch = cd / 128;  // uint4 per MSB row
@@ -103,6 +103,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         const uint32_t qi = s_qi;
         __syncthreads();
         if (qi >= a.qv.nq) break;
+        // this query's search list size: its own (at least k) or the launch's
+        uint32_t L = a.L;
+        if (a.lq) {
+            const uint32_t l = a.lq[qi];
+            L = l < a.k ? a.k : (l > a.L ? a.L : l);
+        }
         // stage the prepared query, clear this slot's visited bits
         for (uint32_t i = t; i < cd / 4; i += T)
             ((float4 *)q)[i] = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
@@ -298,13 +304,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     if (lane == pos) { a_d = d; a_id = id; }
                     ++na;
                 }
-                // merge pool[0, e) with the accepted list into pool[0, e + na):
-                // entries before the first accepted one keep their slot; the rest
-                // go through the stage area (free at this point)
+                // merge pool[0, e) with the accepted list into pool[0, e + na), in
+                // place: retained entries only move right, so they move in
+                // descending chunks of 64 (each read before any write), then
+                // the accepted entries drop into the gaps
                 if (na > 0) {
-                    float *m_d = (float *)stage;
-                    uint32_t *m_id = (uint32_t *)stage + 1024;
-                    uint8_t *m_x = (uint8_t *)((uint32_t *)stage + 2048);
                     // accepted entry i lands at i + #(pool entries before it)
                     uint32_t lo = 0, hi = e;
                     if (lane < na) {
@@ -314,30 +318,28 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         }
                     }
                     const uint32_t first = (uint32_t)__builtin_amdgcn_readlane((int)lo, 0);
-                    if (lane < na) {
-                        const uint32_t dst = lane + lo - first;
-                        m_d[dst] = a_d; m_id[dst] = a_id; m_x[dst] = 0;
-                    }
-                    // retained entry k >= first lands at k + #(accepted entries before it)
-                    for (uint32_t k0 = first; k0 < e; k0 += 64u) {
-                        const uint32_t k = k0 + lane;
-                        if (k < e) {
-                            const float kd = pd[k];
-                            const uint32_t kid = pid[k];
-                            uint32_t c = 0;
+                    // retained entry k >= first moves right by #(accepted entries before it)
+                    for (uint32_t top = e; top > first;) {
+                        const uint32_t base = top - first > 64u ? top - 64u : first;
+                        const uint32_t k = base + lane;
+                        const bool mv = k < top;
+                        float kd = 0.0f;
+                        uint32_t kid = 0, c = 0;
+                        uint8_t kx = 0;
+                        if (mv) {
+                            kd = pd[k]; kid = pid[k]; kx = pexp[k];
                             for (uint32_t i = 0; i < na; ++i) {
                                 const float ad = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), (int)i));
                                 const uint32_t aid = (uint32_t)__builtin_amdgcn_readlane((int)a_id, (int)i);
                                 c += cand_less(ad, aid, kd, kid) ? 1u : 0u;
                             }
-                            const uint32_t dst = k + c - first;
-                            m_d[dst] = kd; m_id[dst] = kid; m_x[dst] = pexp[k];
                         }
+                        wave_sync_lds();
+                        if (mv && c) { pd[k + c] = kd; pid[k + c] = kid; pexp[k + c] = kx; }
+                        wave_sync_lds();
+                        top = base;
                     }
-                    wave_sync_lds();
-                    for (uint32_t i = first + lane; i < e + na; i += 64u) {
-                        pd[i] = m_d[i - first]; pid[i] = m_id[i - first]; pexp[i] = m_x[i - first];
-                    }
+                    if (lane < na) { pd[lane + lo] = a_d; pid[lane + lo] = a_id; pexp[lane + lo] = 0; }
                     wave_sync_lds();
                 }
                 len = e + na;
@@ -402,9 +404,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
         }
         if (a.pool_ids) {
-            for (uint32_t i = t; i < L; i += T) {
-                a.pool_ids[(uint64_t)qi * L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
-                a.pool_d[(uint64_t)qi * L + i] = i < len ? pd[i] : __builtin_huge_valf();
+            for (uint32_t i = t; i < a.L; i += T) {
+                a.pool_ids[(uint64_t)qi * a.L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
+                a.pool_d[(uint64_t)qi * a.L + i] = i < len ? pd[i] : __builtin_huge_valf();
             }
         }
         __syncthreads();
@@ -418,10 +420,24 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 
 }  // namespace qg
 
-extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, hipStream_t stream) {
+template <int W>
+static int launch_search_w(const qg::SearchArgs *a, uint32_t slots, size_t smem, hipStream_t stream) {
+    static size_t raised = 0;  // dynamic LDS above the 64 KB default needs the attribute once per size
+    if (smem > raised) {
+        const hipError_t e = hipFuncSetAttribute((const void *)qg::k_search<W>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return (int)e;
+        raised = smem;
+    }
+    hipLaunchKernelGGL(qg::k_search<W>, dim3(slots), dim3(64 * W), smem, stream, *a);
+    return (int)hipGetLastError();
+}
+
+extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)cd * 8 + (size_t)L * 9 + 16; }
+
+extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream) {
     if (a->qv.nq == 0) return 0;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
-    const size_t smem = (size_t)a->cd * 8 + (size_t)a->L * 9 + 16;
-    hipLaunchKernelGGL(qg::k_search<qg::kSearchWaves>, dim3(slots), dim3(64 * qg::kSearchWaves), smem, stream, *a);
-    return (int)hipGetLastError();
+    const size_t smem = qg_search_lds_bytes(a->cd, a->L);
+    return waves == 2 ? launch_search_w<2>(a, slots, smem, stream) : launch_search_w<4>(a, slots, smem, stream);
 }

@@ -294,10 +294,18 @@ class RabitqSpace:
 
     # ------------------------------------------------------------ search
     def search(self, index, queries, degrees, edges, k=10, l_search=100, eps0=1.9, rerank=100,
-               slots=None, want_pool=False, events=None):
+               slots=None, want_pool=False, events=None, l_per_query=None, waves=4, check_graph=True,
+               workspace=None):
         """Batched graph search (search_ours_paper_active_inner, ours_diskann.rs:1539-1911)
         over a graph given as degrees [n] and edges [n, R] (R <= 64), start
         node 0, with the query codec the queries were prepared with.
+
+        l_per_query (int32 [nq] on the device) gives each query its own
+        search list size, clamped to [k, l_search]; l_search is then the pool
+        capacity. waves (2 or 4) sets the wavefronts per query workgroup.
+        With check_graph=False and a preallocated `workspace` the call
+        enqueues kernels only (no synchronisation), so it can be captured in a
+        HIP graph; read the error flag later with `search_error(workspace)`.
 
         Returns (ids [nq, k] int32, estimates [nq, k], stats [nq, 4] int64 =
         hops, gate evaluations, distances, pruned[, pool ids, pool
@@ -311,6 +319,11 @@ class RabitqSpace:
         L = max(l_search, k)
         if slots is None:
             slots = max(1, min(nq, 4096))
+        lq = None
+        if l_per_query is not None:
+            lq = _dev(l_per_query, torch.int32).reshape(-1)
+            if lq.numel() != nq:
+                raise ValueError("l_per_query needs one entry per query")
         ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
         dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
         stats = torch.zeros((nq, 4), dtype=torch.int64, device="cuda")
@@ -318,21 +331,28 @@ class RabitqSpace:
         pool_d = torch.empty((nq, L), dtype=torch.float32, device="cuda") if want_pool else None
         L_ = _lib.lib()
         ws_bytes = L_.qg_search_workspace_bytes(index.n, slots)
-        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        ws = workspace if workspace is not None else torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        if ws.numel() < ws_bytes:
+            raise ValueError("search workspace too small: need %d bytes" % ws_bytes)
         di, dq = index.desc(), queries.desc()
         if events is not None:
             events[0].record(torch.cuda.current_stream())
-        check(L_.qg_search(self.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
-                           edges.shape[1], k, l_search, C.c_float(eps0), rerank, slots, _ptr(ids), _ptr(dists),
-                           _ptr(stats), _ptr(pool_ids), _ptr(pool_d), _ptr(ws), ws_bytes, _stream()))
+        check(L_.qg_search_ex(self.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
+                              edges.shape[1], k, l_search, _ptr(lq), C.c_float(eps0), rerank, slots, waves,
+                              _ptr(ids), _ptr(dists), _ptr(stats), _ptr(pool_ids), _ptr(pool_d), _ptr(ws),
+                              ws.numel(), _stream()))
         if events is not None:
             events[1].record(torch.cuda.current_stream())
-        err = ws[4:8].view(torch.int32).item()
-        if err:
-            raise _lib.QGError("graph contains an out-of-range neighbour")
+        if check_graph:
+            search_error(ws)
         if want_pool:
             return ids, dists, stats, pool_ids, pool_d
         return ids, dists, stats
+
+    @staticmethod
+    def search_workspace(n, slots):
+        """Device workspace of a search over n points with `slots` concurrent queries."""
+        return torch.empty(_lib.lib().qg_search_workspace_bytes(n, slots), dtype=torch.uint8, device="cuda")
 
     def build_graph(self, index, r=32, l_build=400, alpha=1.2, beam_width=1, batch=4096, refine_passes=0,
                     prune_cap=0, early_stop_hops=0):
@@ -357,6 +377,13 @@ class RabitqSpace:
                                 C.byref(evals), _ptr(ws), ws_bytes, _stream()))
         del ws
         return degrees, edges, evals.value
+
+
+def search_error(workspace):
+    """Raise if the last search that used `workspace` met an out-of-range
+    neighbour (the flag the kernel sets; synchronises)."""
+    if workspace[4:8].view(torch.int32).item():
+        raise _lib.QGError("graph contains an out-of-range neighbour")
 
 
 def topk(values, ids, k):

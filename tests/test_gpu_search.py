@@ -73,3 +73,79 @@ def test_search_rejects_bad_graph(gpu, graph_case):
         sp.search(ix, qv, g["deg"].astype(np.int32), edges.astype(np.int32), k=10, l_search=32)
     with pytest.raises(Exception):
         sp.search(ix, qv, g["deg"].astype(np.int32), np.zeros((g["x"].shape[0], 65), np.int32), k=10)
+
+
+PER_QUERY_L = [10, 16, 37, 64, 100, 200, 600, 1200, 2500, 4096, 23, 77]
+
+
+@pytest.mark.parametrize("waves", [4, 2])
+@pytest.mark.parametrize("codec", [3, 0])
+def test_search_per_query_lists(gpu, graph_case, codec, waves):
+    """Uneven search lists in one launch (pool capacity 4096, above the 64 KB
+    LDS default): every query equals the oracle's search at its own L, for
+    two- and four-wave workgroups."""
+    import quantization_graph_amd as qg
+    g = graph_case
+    sp = qg.RabitqSpace(960, 100)
+    sp.set_center(g["c"])
+    ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
+    print("case: per-query L", PER_QUERY_L, "codec", codec, "waves", waves, flush=True)
+    qv = sp.prepare(gpu.from_numpy(g["q"]).cuda(), codec)
+    lq = gpu.tensor(PER_QUERY_L, dtype=gpu.int32, device="cuda")
+    ids, dists, stats, pool, pool_d = sp.search(ix, qv, g["deg"].astype(np.int32), g["edges"].astype(np.int32),
+                                                k=10, l_search=4096, l_per_query=lq, waves=waves, rerank=100,
+                                                want_pool=True, slots=5)
+    ids, dists, stats = ids.cpu().numpy(), dists.cpu().numpy(), stats.cpu().numpy()
+    pool, pool_d = pool.cpu().numpy(), pool_d.cpu().numpy()
+    orc = g["orc"]
+    for i, L in enumerate(PER_QUERY_L):
+        oq = orc.prepare(g["q"][i], codec)
+        o_ids, o_d, o_pool, o_pd, st = orc.search(oq, codec, g["comp"], g["res"], g["msb"], g["fac"], g["deg"],
+                                                  g["edges"], k=10, L=L, eps0=1.9, rerank=100)
+        m = len(o_pool)
+        assert np.array_equal(pool[i, :m].astype(np.uint32), o_pool), "pool ids (query %d, L %d)" % (i, L)
+        assert np.all(pool[i, m:] == -1)
+        assert np.array_equal(pool_d[i, :m].view(np.uint32), o_pd.view(np.uint32))
+        assert np.array_equal(ids[i, :len(o_ids)].astype(np.uint32), o_ids)
+        assert np.array_equal(dists[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
+        assert tuple(stats[i]) == (st.hops, st.gate_evals, st.full_dists, st.pruned)
+        orc.free_query(oq)
+
+
+def test_search_graph_capture_replays(gpu, graph_case):
+    """Query preparation + search captured in one HIP graph: every replay,
+    with eager searches and synchronisations in between, equals the eager
+    result."""
+    import quantization_graph_amd as qg
+    g = graph_case
+    sp = qg.RabitqSpace(960, 100)
+    sp.set_center(g["c"])
+    ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
+    qx = gpu.from_numpy(g["q"]).cuda()
+    deg = gpu.from_numpy(g["deg"].astype(np.int32)).cuda()
+    edges = gpu.from_numpy(g["edges"].astype(np.int32)).cuda()
+    lq = gpu.tensor(PER_QUERY_L, dtype=gpu.int32, device="cuda")
+    ws = sp.search_workspace(ix.n, 8)
+    print("case: graph capture of prepare + search, 12 queries", flush=True)
+    e_ids, e_d, e_st = sp.search(ix, sp.prepare(qx, "int8"), deg, edges, k=10, l_search=4096, l_per_query=lq,
+                                 slots=8, workspace=ws)
+    side = gpu.cuda.Stream()
+    side.wait_stream(gpu.cuda.current_stream())
+    with gpu.cuda.stream(side):  # warm the allocator on the capture stream
+        sp.search(ix, sp.prepare(qx, "int8"), deg, edges, k=10, l_search=4096, l_per_query=lq, slots=8,
+                  workspace=ws)
+    gpu.cuda.current_stream().wait_stream(side)
+    gpu.cuda.synchronize()
+    graph = gpu.cuda.CUDAGraph()
+    with gpu.cuda.graph(graph):
+        g_ids, g_d, g_st = sp.search(ix, sp.prepare(qx, "int8"), deg, edges, k=10, l_search=4096, l_per_query=lq,
+                                     slots=8, workspace=ws, check_graph=False)
+    for rep in range(3):
+        graph.replay()
+        gpu.cuda.synchronize()
+        qg.space.search_error(ws)
+        assert gpu.equal(g_ids, e_ids) and gpu.equal(g_d.view(gpu.int32), e_d.view(gpu.int32)), rep
+        assert gpu.equal(g_st, e_st)
+        # eager work between replays
+        sp.search(ix, sp.prepare(qx, "full"), deg, edges, k=10, l_search=64, slots=8, workspace=ws)
+        gpu.cuda.synchronize()
