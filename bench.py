@@ -405,6 +405,106 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
     return res
 
 
+def run_batch(args, torch, np, qg):
+    """Batching form: 20,000 queries with uneven search lists in one
+    persistent, HIP-graph-captured launch whose shape (waves per query,
+    concurrent slots, queue order) comes from the autotuner, against one
+    launch per query. 1M x 1024 normalised power-law vectors (seed 3),
+    Vamana M=64 / L_build=400 built on the GPU; per-query lists lognormal
+    with median 400 and log-sigma 0.6 (seed 3), clipped to [64, 4096]; FP32
+    codec, rerank 100."""
+    from quantization_graph_amd import autotune
+    n, dim, nq = args.batch_n, 1024, args.batch_nq
+    x, q = make_powerlaw(torch, n, dim, nq, 3)
+    sp = qg.RabitqSpace(dim, 100)
+    sp.set_center(x.double().mean(0).float().cpu().numpy())
+    index = sp.encode(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    deg, edges, _ = sp.build_graph(index, r=args.R, l_build=args.l_build, alpha=1.2, beam_width=1, batch=4096,
+                                   refine_passes=args.refine, prune_cap=args.prune_cap)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    log("batch: built %d x %d graph in %.1f s" % (n, dim, build_s))
+    lq = autotune.lognormal_lists(nq, 400, 0.6, 64, 4096, 3)
+    lmax = int(lq.max().item())
+    gt = exact_topk(torch, x, q, 10)
+    tuner = autotune.SearchTuner()
+    best = tuner.tune(sp, index, q, "full", deg, edges, k=10, l_search=lmax, l_per_query=lq, rerank=100)
+    log("batch: tuner picked waves %d, slots %d, order %s (%.2f ms); %d candidates"
+        % (best["waves"], best["slots"], best["order"], best["kernel_ms"], len(best["table"])))
+    perm = tuner.order_of(lq, best["order"])
+    qx = q if perm is None else q[perm].contiguous()
+    lqx = lq if perm is None else lq[perm].contiguous()
+    ws = sp.search_workspace(index.n, best["slots"])
+
+    def batch_step():
+        qv = sp.prepare(qx, "full")
+        return sp.search(index, qv, deg, edges, k=10, l_search=lmax, l_per_query=lqx, rerank=100,
+                         slots=best["slots"], waves=best["waves"], workspace=ws, check_graph=False)
+
+    e_ids, e_d, e_st = batch_step()
+    torch.cuda.synchronize()
+    qg.space.search_error(ws)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        batch_step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        g_ids, g_d, g_st = batch_step()
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / args.batch_steps
+    qg.space.search_error(ws)
+    same = bool(torch.equal(g_ids, e_ids) and torch.equal(g_d.view(torch.int32), e_d.view(torch.int32)))
+    ids = torch.empty_like(g_ids)
+    ids[perm if perm is not None else torch.arange(nq, device="cuda")] = g_ids
+    rec = (ids.long().unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
+    hops = g_st[:, 0].double().mean().item()
+    log("batch: %d queries in %.2f ms per graph replay: %.0f q/s, recall@10 %.4f, mean hops %.0f, replay == eager %s"
+        % (nq, ms, nq / (ms / 1000.0), rec, hops, same))
+    # one launch per query (prepare + search, nothing batched), a sample
+    ns = min(args.batch_single, nq)
+    ws1 = sp.search_workspace(index.n, 1)
+    for i in range(2):
+        sp.search(index, sp.prepare(q[i:i + 1], "full"), deg, edges, k=10, l_search=int(lq[i].item()), rerank=100,
+                  slots=1, workspace=ws1, check_graph=False)
+    torch.cuda.synchronize()
+    lhost = lq[:ns].cpu().tolist()
+    t0 = time.perf_counter()
+    for i in range(ns):
+        sp.search(index, sp.prepare(q[i:i + 1], "full"), deg, edges, k=10, l_search=lhost[i], rerank=100,
+                  slots=1, workspace=ws1, check_graph=False)
+    torch.cuda.synchronize()
+    single_s = time.perf_counter() - t0
+    qg.space.search_error(ws1)
+    single_qps = ns / single_s
+    log("batch: per-query launches: %.0f q/s over %d queries (%.2f ms each)" % (single_qps, ns, 1000 * single_s / ns))
+    v = lq.float()
+    return {
+        "config": {"workload": "batched search of %d queries with lognormal search lists (median 400, log-sigma "
+                               "0.6, clipped to [64, 4096], seed 3) over %d x %d normalised power-law vectors "
+                               "(seed 3), Vamana M=%d / L_build=%d built on the GPU, FP32 codec, rerank 100, one "
+                               "HIP-graph-captured persistent launch" % (nq, n, dim, args.R, args.l_build)},
+        "lists": {"median": float(v.median().item()), "mean": float(v.mean().item()),
+                  "min": int(v.min().item()), "max": lmax},
+        "tuner": {k: best[k] for k in ("waves", "slots", "order", "kernel_ms")}, "tuner_table": best["table"],
+        "batched_ms_per_step": ms, "batched_qps": nq / (ms / 1000.0), "recall_at_10": rec, "mean_hops": hops,
+        "graph_replay_equals_eager": same,
+        "per_query_launch_qps": single_qps, "per_query_launch_sample": ns,
+        "speedup_vs_per_query_launches": (nq / (ms / 1000.0)) / single_qps,
+        "build_seconds": build_s,
+    }
+
+
 def run_fixed(args, torch, np, qg):
     from quantization_graph_amd import fixed
     t_setup = time.perf_counter()
@@ -530,7 +630,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("all", "graph", "fixed"), default="all")
+    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch"), default="all")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=1536)
     ap.add_argument("--nq", type=int, default=1000)
@@ -553,6 +653,11 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--cpu-build-records", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-n", type=int, default=1_000_000)
+    ap.add_argument("--batch-nq", type=int, default=20_000)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-single", type=int, default=256, help="queries timed with one launch each")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching workload in --workload all")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (a plain copy kernel of known size, "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -578,6 +683,9 @@ def main():
 
     graph = run_graph(args, torch, np, qg) if args.workload in ("all", "graph") else None
     fixed_res = run_fixed(args, torch, np, qg) if args.workload in ("all", "fixed") else None
+    batch_res = None
+    if args.workload == "batch" or (args.workload == "all" and not args.no_batch):
+        batch_res = run_batch(args, torch, np, qg)
 
     metric = "QPS at Recall@10>=0.95, learned vs plain rerank; Gdist/s vs HBM roofline"
     if graph is not None:
@@ -605,6 +713,16 @@ def main():
             "gpu_vs_reference_id_parity": graph["gpu_vs_reference_id_parity"],
             "learned": graph["learned"],
             "fixed_candidates": fixed_res,
+            "batching": batch_res,
+        }
+    elif fixed_res is None:
+        b = batch_res
+        out = {
+            "metric": "queries/s of one persistent batched launch (uneven search lists)", "value": b["batched_qps"],
+            "unit": "queries/s", "n_gpus": 1, "steps": args.batch_steps, "warmup": args.warmup,
+            "ms_per_step": b["batched_ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": b["config"],
+            "recall_at_10": b["recall_at_10"], "roofline": None, "cpu_baseline": None, "batching": b,
         }
     else:
         f = fixed_res
@@ -613,7 +731,7 @@ def main():
             "warmup": args.warmup, "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "int8", "data": "synthetic", "config": f["config"],
             "recall_at_10": f["recall_at_10"], "roofline": f["roofline"], "cpu_baseline": f["cpu_baseline"],
-            "fixed_candidates": f,
+            "fixed_candidates": f, "batching": batch_res,
         }
     print(json.dumps(out), flush=True)
 
