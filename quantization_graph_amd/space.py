@@ -55,7 +55,10 @@ def _ptr(t):
 
 def _dev(t, dtype):
     if not isinstance(t, torch.Tensor):
-        t = torch.as_tensor(np.asarray(t))
+        a = np.asarray(t)
+        if not a.flags.writeable:  # read-only views (memory-mapped files): torch needs its own copy
+            a = a.copy()
+        t = torch.as_tensor(a)
     return t.to(device="cuda", dtype=dtype).contiguous()
 
 
