@@ -505,6 +505,99 @@ def run_batch(args, torch, np, qg):
     }
 
 
+def run_learned_fixed(args, torch, np, sp, index, x, q, cand, gt, g, plain_ms, plain_recall, plain_depth):
+    """Learned rerank depth on the fixed-candidate pipeline (1M x 960): train
+    on (estimate, exact) pairs of the first-stage top-100 pools of separate
+    training queries, calibrate on held-out queries, then time both
+    pipelines on the workload's queries: plain = score -> top-100 ->
+    residual rerank of all 100; learned = score -> top-100 -> features ->
+    MLP -> residual rerank of the predicted depth."""
+    from quantization_graph_amd import fixed, learned
+    nt = min(args.train_queries // 2, 20000)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(41)
+    def queries_of(count):
+        pick = torch.randint(0, x.shape[0], (count,), generator=gen, device="cuda")
+        return (x[pick] + 0.05 * torch.randn((count, x.shape[1]), generator=gen, device="cuda")).contiguous()
+    tq, vq = queries_of(nt), queries_of(2000)
+    t_top = exact_topk(torch, x, tq, 2000)
+    t_cand = make_candidates(torch, t_top, args.per, gen)
+    del t_top
+    v_top = exact_topk(torch, x, vq, 2000)
+    v_gt = v_top[:, :10].contiguous()
+    v_cand = make_candidates(torch, v_top, args.per, gen)
+    del v_top
+    rstats = learned.residual_stats(sp, index)
+    t0 = time.perf_counter()
+    feats, targets = learned.training_pairs_fixed(sp, index, x, tq, t_cand, "int8", rstats)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    mlp = learned.MLP(seed=0)
+    t0 = time.perf_counter()
+    losses = mlp.fit(feats, targets, tau=0.95, epochs=args.train_epochs)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    log("learned (fixed): %d training pairs in %.1f s, %d epochs in %.1f s"
+        % (targets.numel(), t_pairs, args.train_epochs, t_fit))
+    del feats, targets
+    vqv = sp.prepare(vq, "int8")
+    vpid, vpd = learned.fixed_pool(sp, index, vqv, v_cand)
+    vpreds = mlp.forward(learned.features(sp, index, vqv, vpid, vpd, rstats))
+    kmult, table = learned.calibrate(sp, index, vqv, vpid, vpd, vpreds, v_gt)
+    model = learned.LearnedRerank(mlp, kmult)
+    stream = torch.cuda.current_stream()
+
+    def plain_step(ev=None):
+        qv = sp.prepare(q, "int8")
+        pid, pd = learned.fixed_pool(sp, index, qv, cand)
+        if ev is not None:
+            ev[0].record(stream)
+        ids, _, _ = learned.rerank(sp, index, qv, pid, pd)
+        if ev is not None:
+            ev[1].record(stream)
+        return ids, None
+
+    def learned_step(ev=None):
+        qv = sp.prepare(q, "int8")
+        pid, pd = learned.fixed_pool(sp, index, qv, cand)
+        if ev is not None:
+            ev[0].record(stream)
+        preds = model.predict(sp, index, qv, pid, pd, rstats)
+        ids, _, depths = learned.rerank(sp, index, qv, pid, pd, preds, model.kmult, model.depth, model.min_depth)
+        if ev is not None:
+            ev[1].record(stream)
+        return ids, depths
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ids, depths = fn(evs[i])
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+        stage = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+        return ms, stage, fixed.recall_at_k(ids, gt, 10), depths
+
+    p_ms, p_stage, p_rec, _ = timed(plain_step)
+    l_ms, l_stage, l_rec, depths = timed(learned_step)
+    nq = q.shape[0]
+    log("learned (fixed): plain %.3f ms/step (rerank stage %.3f ms) recall %.4f | learned %.3f ms/step "
+        "(features + MLP + rerank %.3f ms) recall %.4f, mean depth %.1f: %.2fx end to end, %.2fx stage"
+        % (p_ms, p_stage, p_rec, l_ms, l_stage, l_rec, float(depths.float().mean().item()), p_ms / l_ms,
+           p_stage / l_stage))
+    return {"training_pairs": int(nt * 100), "train_queries": nt, "fit_seconds": t_fit, "loss": losses,
+            "kmult": kmult, "calibration": table,
+            "plain": {"ms_per_step": p_ms, "qps": nq / (p_ms / 1000.0), "rerank_stage_ms": p_stage,
+                      "recall_at_10": p_rec, "depth": 100},
+            "learned": {"ms_per_step": l_ms, "qps": nq / (l_ms / 1000.0), "rerank_stage_ms": l_stage,
+                        "recall_at_10": l_rec, "mean_depth": float(depths.float().mean().item())},
+            "speedup_end_to_end": p_ms / l_ms, "speedup_rerank_stage": p_stage / l_stage,
+            "fixed_pipeline": {"depth": plain_depth, "ms_per_step": plain_ms, "recall_at_10": plain_recall}}
+
+
 def run_fixed(args, torch, np, qg):
     from quantization_graph_amd import fixed
     t_setup = time.perf_counter()
@@ -578,6 +671,10 @@ def run_fixed(args, torch, np, qg):
         with open(pmc) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
+    learned_res = None
+    if args.train_queries:
+        learned_res = run_learned_fixed(args, torch, np, sp, index, x, q, cand, gt, g, ms_per_step, recall, depth)
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -617,7 +714,7 @@ def run_fixed(args, torch, np, qg):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "qg::k_int8_full_grouped<1024> (INT8 query x 4-bit code, fixed lists)",
                      "kernel_ms": kernel_ms, "bytes_per_unit": unit_bytes, "units_per_launch": units},
-        "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity,
+        "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity, "learned": learned_res,
         "setup_seconds": time.perf_counter() - t_setup,
     }
     del index, x, q, cand, gt

@@ -525,6 +525,7 @@ int qg_rerank_features(const void *space, const qg_index_desc *ix, const qg_quer
     qg::RerankArgs a;
     if (rerank_args(static_cast<const Space *>(space), ix, qv, pool_ids, pool_dists, L, depth, &a)) return -1;
     if (!rstats || !features) return fail("null residual stats or feature output");
+    if (reinterpret_cast<uintptr_t>(features) & 15u) return fail("feature output must be 16-byte aligned");
     a.rstats = reinterpret_cast<const float4 *>(rstats);
     a.feats = features;
     return check_hip(qg_launch_rerank_features(&a, (hipStream_t)stream), "rerank features launch");

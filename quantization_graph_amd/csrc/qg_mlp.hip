@@ -118,19 +118,92 @@ __device__ __forceinline__ void tile_forward(const uint16_t *xrow, bool xvalid, 
     }
 }
 
-// y[i] for X [n][32] bf16; one wave per 16-sample tile, 4 tiles per block
+// y[i] for X [n][32] bf16 (inference): each wave loads the bf16 weights
+// into registers once and runs kFwdTiles consecutive 16-sample tiles with
+// the MFMA sequence of tile_forward (same operands, same order, so the same
+// y bit for bit).
+constexpr int kFwdTiles = 8;
+
 __global__ void __launch_bounds__(256) k_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y) {
-    __shared__ uint16_t sh1[4][16 * kH];
+    __shared__ uint16_t sh1_all[4][16 * kH];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-    const uint64_t tile = (uint64_t)blockIdx.x * 4 + wv;
-    const uint64_t row = tile * 16 + c;
-    TileOut o;
-    tile_forward(X + row * kFP, row < n, P, sh1[wv], o);
-    if (c == 0) {
+    uint16_t *sh1 = sh1_all[wv];
+    bf16x8_t B1[4], B2[4][2];
+    float b1[4], b2[4], w3[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int oc = 16 * nt + c;
+        uint16_t wb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            wb[j] = k < kF ? f2bf(P[kOffW1 + oc * kF + k]) : (uint16_t)0;
+        }
+        B1[nt] = pack8(wb);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wb[j] = f2bf(P[kOffW2 + oc * kH + 32 * ks + 8 * g + j]);
+            B2[nt][ks] = pack8(wb);
+        }
+        b1[nt] = P[kOffB1 + oc];
+        b2[nt] = P[kOffB2 + oc];
+        w3[nt] = P[kOffW3 + oc];
+    }
+    const float b3 = P[kOffB3];
+    const uint64_t tile0 = ((uint64_t)blockIdx.x * 4 + wv) * kFwdTiles;
+    for (int tt = 0; tt < kFwdTiles; ++tt) {
+        const uint64_t tile = tile0 + tt;
+        if (tile * 16 >= n) break;
+        const uint64_t row = tile * 16 + c;
+        uint16_t xa[8];
+        {
+            const uint4 v = row < n ? *(const uint4 *)(X + row * kFP + 8 * g) : make_uint4(0, 0, 0, 0);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { xa[2 * j] = (uint16_t)(w[j] & 0xFFFF); xa[2 * j + 1] = (uint16_t)(w[j] >> 16); }
+        }
+        const bf16x8_t A1 = pack8(xa);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B1[nt], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float z = acc[r] + b1[nt];
+                sh1[(4 * g + r) * kH + 16 * nt + c] = f2bf(z > 0.0f ? z : 0.0f);
+            }
+        }
+        // sh1 belongs to this wave alone: LDS operations of one wavefront
+        // complete in order, so a wave-level barrier orders the writes above
+        // before the reads below (and those reads before the next tile's writes)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        bf16x8_t A2[2];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) A2[ks] = pack8(sh1 + c * kH + 32 * ks + 8 * g);
+        float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[ks], B2[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float z = acc[r] + b2[nt];
+                part[r] = fmaf(z > 0.0f ? z : 0.0f, w3[nt], part[r]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+            float v = part[r];
+            for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
             const uint64_t rr = tile * 16 + 4 * g + r;
-            if (rr < n) y[rr] = o.y[r];
+            if (c == 0 && rr < n) y[rr] = v + b3;
         }
     }
 }
@@ -282,17 +355,24 @@ __device__ __forceinline__ float clampf(float v) {
 }
 
 // features [nq][depth][32] (bf16) of the first min(len, depth) pool entries
-__global__ void k_rerank_features(RerankArgs a) {
+// (one thread per pool entry; len = the first empty slot within the depth)
+__global__ void __launch_bounds__(128) k_rerank_features(RerankArgs a) {
+    __shared__ uint32_t s_len;
     const uint64_t qi = blockIdx.x;
     const uint32_t i = threadIdx.x;
-    if (qi >= a.nq || i >= a.depth) return;
     const uint32_t *pid = a.pool_ids + qi * a.L;
     const float *pd = a.pool_d + qi * a.L;
-    uint32_t len = 0;
-    while (len < a.L && len < a.depth && pid[len] != 0xFFFFFFFFu) ++len;
-    uint16_t *out = a.feats + (qi * a.depth + i) * kFP;
+    const uint32_t lim = a.L < a.depth ? a.L : a.depth;
+    if (i == 0) s_len = lim;
+    __syncthreads();
+    if (i < lim && pid[i] == 0xFFFFFFFFu) atomicMin(&s_len, i);
+    __syncthreads();
+    const uint32_t len = s_len;
+    if (i >= a.depth) return;
+    uint4 *out = (uint4 *)(a.feats + (qi * a.depth + i) * kFP);
     if (i >= len) {
-        for (int k = 0; k < kFP; ++k) out[k] = 0;
+#pragma unroll
+        for (int k = 0; k < kFP / 8; ++k) out[k] = make_uint4(0, 0, 0, 0);
         return;
     }
     const QScalars s = a.qv.qs[qi];
@@ -335,87 +415,161 @@ __global__ void k_rerank_features(RerankArgs a) {
     f[21] = (ehi - elo) / S;
     f[22] = e / (e9 > 0.0f ? e9 : 1.0f);
     f[23] = (e - e9) / (bound + 1e-12f);
+    uint32_t w[kFP / 2];
 #pragma unroll
-    for (int k = 0; k < kF; ++k) out[k] = f2bf(clampf(f[k]));
+    for (int k = 0; k < kFP / 2; ++k)
+        w[k] = 2 * k < kF ? (uint32_t)f2bf(clampf(f[2 * k])) | ((uint32_t)f2bf(clampf(f[2 * k + 1])) << 16) : 0u;
 #pragma unroll
-    for (int k = kF; k < kFP; ++k) out[k] = 0;
+    for (int k = 0; k < kFP / 8; ++k) out[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
 // Per-query depth from the predicted errors, then the residual-4 rerank of
-// that many pool entries (the same per-candidate arithmetic as the search
-// kernel's rerank) and the top-k by (estimate, id). One wavefront per query.
-__global__ void __launch_bounds__(64) k_learned_rerank(RerankArgs a) {
-    __shared__ float r_est[128];
-    __shared__ uint32_t r_id[128];
+// that many pool entries and the top-k by (estimate, id). One 256-thread
+// workgroup per query, bit-identical to the search kernel's rerank:
+//  * INT8 queries: the FP32 long distance of each candidate on a 16-lane
+//    group over an LDS-staged code row (asym_group);
+//  * residual-4: every (candidate, block) pair's 8-lane product sum on its
+//    own thread (coalesced 8-byte loads of consecutive blocks), kept in LDS;
+//    one thread per candidate then runs the order-dependent recurrence
+//    ips = fma(scale, block_ip, ips) over the blocks in index order (the
+//    rerank needs the estimate only, not the error bound);
+//  * top-k: each candidate's rank under (estimate, id, position) on its own
+//    thread; ranks below k are written out.
+constexpr int kRrThreads = 256;
+constexpr uint32_t kRrMaxDepth = 128;
+constexpr uint32_t kRrWork = kRrMaxDepth * 65;  // block sums of 128 candidates at 64 blocks (+1 pad) per row
+
+__device__ __forceinline__ float rank_key(float v) { return v == v ? v : __builtin_huge_valf(); }
+
+__global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
+    extern __shared__ __align__(16) float qs[];        // [cd]
+    __shared__ __align__(16) float work[kRrWork];      // staged code rows, then block sums
+    __shared__ float r_ld[kRrMaxDepth], r_est[kRrMaxDepth];
+    __shared__ uint32_t r_id[kRrMaxDepth];
+    __shared__ uint32_t s_depth;
     const uint64_t qi = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t cd = a.cd, nb = cd / 16;
     const uint32_t *pid = a.pool_ids + qi * a.L;
     const float *pd = a.pool_d + qi * a.L;
-    // rerank count as the search computes it: min(|pool|, max(rerank, k))
-    uint32_t len = 0;
-    for (uint32_t i = lane; i < a.L; i += 64) len += pid[i] != 0xFFFFFFFFu ? 1u : 0u;
-    for (int off = 32; off > 0; off >>= 1) len += (uint32_t)__shfl_xor((int)len, off);
-    const uint32_t want = a.depth > a.k ? a.depth : a.k;
-    const uint32_t rc = len < want ? len : want;
     const QScalars s = a.qv.qs[qi];
-    uint32_t depth = rc;
-    if (a.preds && rc > 0) {
-        const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
-        const uint32_t top = rc < a.min_depth ? rc : a.min_depth;
-        float ub = -__builtin_huge_valf();
-        for (uint32_t j = lane; j < top; j += 64) {
-            const float pr = a.preds[qi * a.depth + j];
-            ub = fmaxf(ub, pd[j] + a.kmult * (pr > 0.0f ? pr : 0.0f) * S);
-        }
-        for (int off = 32; off > 0; off >>= 1) ub = fmaxf(ub, __shfl_xor(ub, off));
-        uint32_t last = 0;
-        for (uint32_t j = lane; j < rc; j += 64) {
-            const float pr = a.preds[qi * a.depth + j];
-            const float lb = pd[j] - a.kmult * (pr > 0.0f ? pr : 0.0f) * S;
-            if (lb <= ub) last = j + 1 > last ? j + 1 : last;
-        }
-        for (int off = 32; off > 0; off >>= 1) last = max(last, (uint32_t)__shfl_xor((int)last, off));
-        depth = last < top ? top : last;
+    {
+        const float4 *src = (const float4 *)(a.qv.q + qi * cd);
+        for (uint32_t i = t; i < cd / 4; i += kRrThreads) ((float4 *)qs)[i] = src[i];
     }
-    if (lane == 0 && a.out_depth) a.out_depth[qi] = depth;
-    const float *q = a.qv.q + qi * a.cd;
-    for (uint32_t i = lane; i < depth; i += 64) {
-        const uint32_t id = pid[i];
-        float ld = pd[i];
-        if (a.codec == 3) ld = asym_one(q, a.ix.codes + (uint64_t)id * (a.cd / 2), a.cd, a.ix.hdr[id], s);
-        const Interval iv = residual_one(q, a.ix.rscale + (uint64_t)id * (a.cd / 16),
-                                         a.ix.rcodes + (uint64_t)id * (a.cd / 2), a.cd, s, ld);
-        r_est[i] = iv.estimate;
-        r_id[i] = id;
-    }
-    __syncthreads();
-    const uint32_t kout = depth < a.k ? depth : a.k;
-    for (uint32_t r = 0; r < a.k; ++r) {
-        if (r >= kout) {
-            if (lane == 0) { a.out_ids[qi * a.k + r] = 0xFFFFFFFFu; a.out_d[qi * a.k + r] = __builtin_huge_valf(); }
-            continue;
-        }
-        float bd = __builtin_huge_valf();
-        uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
-        for (uint32_t i = lane; i < depth; i += 64) {
-            if (r_id[i] == 0xFFFFFFFFu) continue;
-            if (bpos == 0xFFFFFFFFu || r_est[i] < bd || (r_est[i] == bd && r_id[i] < bid)) {
-                bd = r_est[i]; bid = r_id[i]; bpos = i;
+    if (t < 64) {
+        // rerank count as the search computes it: min(|pool|, max(rerank, k)),
+        // |pool| = the first empty slot (so every reranked id is a real one)
+        uint32_t len = a.L;
+        for (uint32_t i = lane; i < a.L; i += 64)
+            if (pid[i] == 0xFFFFFFFFu) { len = i; break; }
+        for (int off = 32; off > 0; off >>= 1) len = min(len, (uint32_t)__shfl_xor((int)len, off));
+        const uint32_t want = a.depth > a.k ? a.depth : a.k;
+        const uint32_t rc = len < want ? len : want;
+        uint32_t depth = rc;
+        if (a.preds && rc > 0) {
+            const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
+            const uint32_t top = rc < a.min_depth ? rc : a.min_depth;
+            float ub = -__builtin_huge_valf();
+            for (uint32_t j = lane; j < top; j += 64) {
+                const float pr = a.preds[qi * a.depth + j];
+                ub = fmaxf(ub, pd[j] + a.kmult * (pr > 0.0f ? pr : 0.0f) * S);
             }
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            const float od = __shfl_xor(bd, off);
-            const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
-            const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
-            const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || od < bd || (od == bd && oid < bid));
-            if (take) { bd = od; bid = oid; bpos = opos; }
+            for (int off = 32; off > 0; off >>= 1) ub = fmaxf(ub, __shfl_xor(ub, off));
+            uint32_t last = 0;
+            for (uint32_t j = lane; j < rc; j += 64) {
+                const float pr = a.preds[qi * a.depth + j];
+                const float lb = pd[j] - a.kmult * (pr > 0.0f ? pr : 0.0f) * S;
+                if (lb <= ub) last = j + 1 > last ? j + 1 : last;
+            }
+            for (int off = 32; off > 0; off >>= 1) last = max(last, (uint32_t)__shfl_xor((int)last, off));
+            depth = last < top ? top : last;
         }
         if (lane == 0) {
-            a.out_ids[qi * a.k + r] = bid;
-            a.out_d[qi * a.k + r] = bd;
-            r_id[bpos] = 0xFFFFFFFFu;
+            s_depth = depth;
+            if (a.out_depth) a.out_depth[qi] = depth;
+        }
+    }
+    __syncthreads();
+    const uint32_t depth = s_depth;
+    for (uint32_t c = t; c < depth; c += kRrThreads) r_id[c] = pid[c];
+    if (a.codec == 3) {
+        // 16 groups of 16 lanes, one code row (cd / 2 bytes) each in `work`
+        const int g = t >> 4, j = t & 15;
+        uint4 *row = (uint4 *)work + g * (cd / 32);
+        for (uint32_t c = g; c < depth; c += kRrThreads / 16) {
+            const uint32_t id = pid[c];
+            stage_row_group(row, a.ix.codes + (uint64_t)id * (cd / 2), cd / 2, j);
+            const float ld = asym_group(qs, (const uint8_t *)row, cd, a.ix.hdr[id], s, j);
+            if (j == 0) r_ld[c] = ld;
+        }
+    } else {
+        for (uint32_t c = t; c < depth; c += kRrThreads) r_ld[c] = pd[c];
+    }
+    __syncthreads();
+    const uint32_t stride = nb + 1;
+    const uint32_t per = kRrWork / stride;
+    const int lg = __builtin_ctz(nb);
+    for (uint32_t c0 = 0; c0 < depth; c0 += per) {
+        const uint32_t cnt = depth - c0 < per ? depth - c0 : per;
+        for (uint32_t it = t; it < cnt * nb; it += kRrThreads) {
+            const uint32_t c = it >> lg, b = it & (nb - 1);
+            const uint32_t id = r_id[c0 + c];
+            const uint2 w = ((const uint2 *)(a.ix.rcodes + (uint64_t)id * (cd / 2)))[b];
+            const float4 *qp = (const float4 *)(qs + 16 * b);
+            const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+            const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                                  q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+            float bip = 0.0f;
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                const uint32_t byte = ((l < 4 ? w.x : w.y) >> (8 * (l & 3))) & 0xFFu;
+                int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
+                lo -= (lo > 7) ? 16 : 0;
+                hi -= (hi > 7) ? 16 : 0;
+                const float pl = (float)lo * qv[2 * l];
+                bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+            }
+            work[c * stride + b] = bip;
         }
         __syncthreads();
+        for (uint32_t c = t; c < cnt; c += kRrThreads) {
+            const uint32_t id = r_id[c0 + c];
+            const uint2 *srow = (const uint2 *)(a.ix.rscale + (uint64_t)id * nb);
+            const float *bp = work + c * stride;
+            float ips = 0.0f;
+            for (uint32_t b4 = 0; b4 < nb / 4; ++b4) {
+                const uint2 sv = srow[b4];
+                const uint32_t sw[2] = {sv.x, sv.y};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float scale = half_to_float_ref((uint16_t)(sw[e >> 1] >> (16 * (e & 1))));
+                    if (scale == 0.0f || !__builtin_isfinite(scale)) continue;
+                    ips = fmaf(scale, bp[4 * b4 + e], ips);
+                }
+            }
+            r_est[c0 + c] = r_ld[c0 + c] - 2.0f * ips;
+        }
+        __syncthreads();
+    }
+    const uint32_t kout = depth < a.k ? depth : a.k;
+    for (uint32_t c = t; c < depth; c += kRrThreads) {
+        const float e = rank_key(r_est[c]);
+        const uint32_t id = r_id[c];
+        uint32_t rank = 0;
+        for (uint32_t o = 0; o < depth; ++o) {
+            const float eo = rank_key(r_est[o]);
+            const uint32_t io = r_id[o];
+            rank += (eo < e || (eo == e && (io < id || (io == id && o < c)))) ? 1u : 0u;
+        }
+        if (rank < kout) {
+            a.out_ids[qi * a.k + rank] = id;
+            a.out_d[qi * a.k + rank] = r_est[c];
+        }
+    }
+    for (uint32_t r = kout + t; r < a.k; r += kRrThreads) {
+        a.out_ids[qi * a.k + r] = 0xFFFFFFFFu;
+        a.out_d[qi * a.k + r] = __builtin_huge_valf();
     }
 }
 
@@ -426,7 +580,9 @@ extern "C" {
 int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st) {
     if (n == 0) return 0;
     const uint64_t tiles = (n + 15) / 16;
-    hipLaunchKernelGGL(qg::k_mlp_forward, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, P, X, n, y);
+    const uint64_t per_block = 4 * qg::kFwdTiles;
+    hipLaunchKernelGGL(qg::k_mlp_forward, dim3((unsigned)((tiles + per_block - 1) / per_block)), dim3(256), 0, st, P,
+                       X, n, y);
     return (int)hipGetLastError();
 }
 
@@ -466,7 +622,7 @@ int qg_launch_rerank_features(const qg::RerankArgs *a, hipStream_t st) {
 
 int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st) {
     if (a->nq == 0) return 0;
-    hipLaunchKernelGGL(qg::k_learned_rerank, dim3((unsigned)a->nq), dim3(64), 0, st, *a);
+    hipLaunchKernelGGL(qg::k_learned_rerank, dim3((unsigned)a->nq), dim3(qg::kRrThreads), (size_t)a->cd * 4, st, *a);
     return (int)hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ import torch
 
 from . import _lib
 from ._lib import check
+from . import space as S
 from .space import _dev, _ptr, _stream
 
 N_FEATURES = 24
@@ -162,29 +163,60 @@ def query_norm_sqr(queries):
     return queries.qs[:, 4:8].contiguous().view(torch.float32).reshape(-1)
 
 
+def pairs_from_pool(sp, index, x, q, qv, pid, pd, rstats, depth=100):
+    """(features, targets) of the first `depth` entries of prepared queries'
+    pools (ids pid / estimates pd, sorted): target = |exact squared L2 -
+    estimate| / ||q - c||^2, exact from the raw fp32 vectors x [n, dim]."""
+    f = features(sp, index, qv, pid, pd, rstats, depth)
+    d = min(depth, pid.shape[1])
+    ids = pid[:, :d].long()
+    valid = ids >= 0
+    safe = torch.where(valid, ids, torch.zeros_like(ids))
+    exact = ((x[safe] - q[:, None, :]) ** 2).sum(-1)
+    err = (exact - pd[:, :d]).abs() / query_norm_sqr(qv).clamp_min(1e-30)[:, None]
+    t = torch.zeros((q.shape[0], depth), dtype=torch.float32, device="cuda")
+    t[:, :d] = torch.where(valid, err, torch.zeros_like(err))
+    keep = torch.zeros((q.shape[0], depth), dtype=torch.bool, device="cuda")
+    keep[:, :d] = valid
+    keep = keep.reshape(-1)
+    return f[keep], t.reshape(-1)[keep]
+
+
 def training_pairs(sp, index, degrees, edges, x, raw_queries, l_search, codec, rstats, depth=100, chunk=2048):
-    """(features, targets) for the first `depth` pool entries of every query:
-    target = |exact squared L2 - traversal estimate| / ||q - c||^2, exact from
-    the raw fp32 vectors x [n, dim] (device)."""
+    """Training pairs from graph searches: the first `depth` entries of each
+    query's traversal pool."""
     feats, targets = [], []
     for s0 in range(0, raw_queries.shape[0], chunk):
         q = raw_queries[s0:s0 + chunk]
         qv = sp.prepare(q, codec)
         pid, pd, _ = search_pool(sp, index, qv, degrees, edges, l_search)
-        f = features(sp, index, qv, pid, pd, rstats, depth)
-        d = min(depth, pid.shape[1])
-        ids = pid[:, :d].long()
-        valid = ids >= 0
-        safe = torch.where(valid, ids, torch.zeros_like(ids))
-        exact = ((x[safe] - q[:, None, :]) ** 2).sum(-1)
-        err = (exact - pd[:, :d]).abs() / query_norm_sqr(qv).clamp_min(1e-30)[:, None]
-        t = torch.zeros((q.shape[0], depth), dtype=torch.float32, device="cuda")
-        t[:, :d] = torch.where(valid, err, torch.zeros_like(err))
-        keep = torch.zeros((q.shape[0], depth), dtype=torch.bool, device="cuda")
-        keep[:, :d] = valid
-        keep = keep.reshape(-1)
-        feats.append(f[keep])
-        targets.append(t.reshape(-1)[keep])
+        f, t = pairs_from_pool(sp, index, x, q, qv, pid, pd, rstats, depth)
+        feats.append(f)
+        targets.append(t)
+    return torch.cat(feats), torch.cat(targets)
+
+
+def fixed_pool(sp, index, qv, cand_ids, depth=100, codec="int8"):
+    """The first-stage pool of fixed-candidate scoring: each query's `depth`
+    candidates with the smallest first-stage estimates, in (estimate, id)
+    order."""
+    from . import fixed
+    first = fixed.score(sp, index, qv, cand_ids, codec=codec)
+    pid, pd, _ = S.topk(first, cand_ids, min(depth, cand_ids.shape[1]))
+    return pid, pd
+
+
+def training_pairs_fixed(sp, index, x, raw_queries, cand_ids, codec, rstats, depth=100, chunk=2048):
+    """Training pairs from fixed-candidate scoring: the first `depth`
+    entries of each query's first-stage pool."""
+    feats, targets = [], []
+    for s0 in range(0, raw_queries.shape[0], chunk):
+        q = raw_queries[s0:s0 + chunk]
+        qv = sp.prepare(q, codec)
+        pid, pd = fixed_pool(sp, index, qv, cand_ids[s0:s0 + chunk], depth, codec)
+        f, t = pairs_from_pool(sp, index, x, q, qv, pid, pd, rstats, depth)
+        feats.append(f)
+        targets.append(t)
     return torch.cat(feats), torch.cat(targets)
 
 
@@ -209,8 +241,11 @@ class LearnedRerank:
         return ids, dists, depths, st
 
 
-def calibrate(sp, index, qv, pool_ids, pool_d, preds, gt, budget=0.001, grid=(0.5, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0,
-                                                                               4.0, 6.0), depth=100, min_depth=10):
+CALIBRATION_GRID = (0.25, 0.35, 0.5, 0.6, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0, 4.0, 6.0)
+
+
+def calibrate(sp, index, qv, pool_ids, pool_d, preds, gt, budget=0.001, grid=CALIBRATION_GRID, depth=100,
+              min_depth=10):
     """Smallest multiplier whose recall is within `budget` of the plain
     rerank's on these (held-out) queries; returns (kmult, table)."""
     plain, _, _ = rerank(sp, index, qv, pool_ids, pool_d, None, 1.0, depth, min_depth)

@@ -166,3 +166,64 @@ def test_learned_depth_saves_rerank_work(gpu, powerlaw_case):
     print("kmult %.2f plain %.4f learned %.4f mean depth %.1f" % (kmult, r_plain, r_learned, depth.float().mean()))
     assert r_learned >= r_plain - 0.002
     assert float(depth.float().mean()) < 60.0
+
+
+def _query_norms(qv):
+    qs = qv.qs.cpu().numpy().view(np.float32)
+    return qs[:, 0], qs[:, 1]
+
+
+def test_features_match_oracle(gpu, powerlaw_case):
+    """Feature rows of full pools, pools cut short (empty slots inside the
+    depth) and an empty pool, against the FP64 restatement: each bf16 value
+    within one bf16 ulp."""
+    import torch
+    from quantization_graph_amd import learned
+    c = powerlaw_case
+    sp, ix = c["sp"], c["ix"]
+    qv = sp.prepare(c["q"][:160], "full")
+    pid, pd, _ = learned.search_pool(sp, ix, qv, c["deg"], c["edges"], 128)
+    pid, pd = pid.clone(), pd.clone()
+    for r in range(40):  # pools of 3 .. 120 entries
+        pid[r, 3 * r + 3:] = -1
+        pd[r, 3 * r + 3:] = float("inf")
+    pid[40, :] = -1
+    pd[40, :] = float("inf")
+    f = learned.features(sp, ix, qv, pid, pd, c["rstats"])
+    got = mo.bits_to_f32(f.cpu().numpy().view(np.uint16).reshape(160, 100, 32))
+    qn, qns = _query_norms(qv)
+    ref = mo.features(pid.cpu().numpy().view(np.uint32), pd.cpu().numpy(), qn, qns, ix.hdr.cpu().numpy(),
+                      ix.sfac.cpu().numpy(), ix.rfac.cpu().numpy(), c["rstats"].cpu().numpy(), sp.code_dim)
+    assert (got[:, :, 24:] == 0).all()
+    diff = np.abs(got[:, :, :24] - ref)
+    assert (diff <= 2.0 ** -7 * np.abs(ref) + 1e-30).all(), np.unravel_index(np.argmax(diff), diff.shape)
+    assert (got[40] == 0).all() and (got[0, 3:] == 0).all()
+
+
+@pytest.mark.parametrize("codec", ["full", "int8"])
+def test_learned_depth_rule_and_rerank(gpu, powerlaw_case, codec):
+    """Per-query depths from given predictions equal the oracle's rule, and
+    the learned rerank's ids and estimates equal, bit for bit, the plain
+    rerank of pools cut at those depths."""
+    import torch
+    from quantization_graph_amd import learned
+    c = powerlaw_case
+    sp, ix = c["sp"], c["ix"]
+    qv = sp.prepare(c["q"][:300], codec)
+    pid, pd, _ = learned.search_pool(sp, ix, qv, c["deg"], c["edges"], 128)
+    rng = np.random.default_rng(11)
+    preds_np = (rng.normal(size=(300, 100)) * 0.01 + 0.004).astype(np.float32)
+    preds = torch.from_numpy(preds_np).cuda()
+    ids, dists, depth = learned.rerank(sp, ix, qv, pid, pd, preds, 1.5)
+    _, qns = _query_norms(qv)
+    want = mo.rerank_depth(pid.cpu().numpy().view(np.uint32), pd.cpu().numpy(), preds_np, qns, 1.5)
+    got = depth.cpu().numpy()
+    assert (got == want).all(), (got[got != want][:5], want[got != want][:5])
+    assert 10 <= got.min() and got.max() <= 100 and got.min() < got.max()
+    cut = torch.arange(pid.shape[1], device="cuda")[None, :] >= depth.long()[:, None]
+    pid_t = torch.where(cut, torch.full_like(pid, -1), pid)
+    pd_t = torch.where(cut, torch.full_like(pd, float("inf")), pd)
+    ids2, dists2, depth2 = learned.rerank(sp, ix, qv, pid_t, pd_t, None)
+    assert (depth2.cpu() == depth.cpu()).all()
+    assert (ids.cpu() == ids2.cpu()).all()
+    assert (dists.cpu().view(torch.int32) == dists2.cpu().view(torch.int32)).all()
