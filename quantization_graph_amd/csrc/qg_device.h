@@ -197,6 +197,83 @@ __device__ __forceinline__ Interval residual_one(const float *q, const uint16_t 
     return iv;
 }
 
+// E1 split in two for block-parallel reranks: the 8-lane product sum of one
+// 16-dimension block (q16: the block's query values, w: its 8 code bytes),
+// and the order-dependent recurrence ips = fma(scale, block_ip, ips) over
+// the blocks in index order (blocks with a zero or non-finite scale
+// skipped). residual_chain(scales, {residual_block_ip(b)}, ...) equals
+// residual_one(...).estimate bit for bit.
+__device__ __forceinline__ float residual_block_ip(const float *q16, uint2 w) {
+    const float4 *qp = (const float4 *)q16;
+    const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
+    const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
+                          q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
+    float bip = 0.0f;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const uint32_t byte = ((l < 4 ? w.x : w.y) >> (8 * (l & 3))) & 0xFFu;
+        int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
+        lo -= (lo > 7) ? 16 : 0;
+        hi -= (hi > 7) ? 16 : 0;
+        const float pl = (float)lo * qv[2 * l];
+        bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+    }
+    return bip;
+}
+
+// Block sums of candidates 0..cnt) into work[c * stride + b] by the T
+// threads of a workgroup (t = thread index); id_of(c) names candidate c's
+// residual code row. Each thread issues U blocks' 8-byte loads before it
+// computes any, so U memory round trips overlap.
+template <uint32_t T, uint32_t U, typename IdOf>
+__device__ __forceinline__ void residual_block_sums(float *work, uint32_t stride, const float *q,
+                                                    const uint8_t *__restrict__ rcodes, uint32_t cd, uint32_t cnt,
+                                                    IdOf id_of, uint32_t t) {
+    const uint32_t nb = cd / 16, lg = (uint32_t)__builtin_ctz(nb), total = cnt * nb;
+    for (uint32_t it0 = t; it0 < total; it0 += U * T) {
+        uint2 w[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t it = it0 + u * T;
+            w[u] = it < total ? ((const uint2 *)(rcodes + (uint64_t)id_of(it >> lg) * (cd / 2)))[it & (nb - 1u)]
+                              : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t it = it0 + u * T;
+            if (it < total) {
+                const uint32_t b = it & (nb - 1u);
+                work[(it >> lg) * stride + b] = residual_block_ip(q + 16 * b, w[u]);
+            }
+        }
+    }
+}
+
+// nb (= cd / 16) is a multiple of 4; scale_row is 8-byte aligned. The fp16
+// scales are widened by the hardware conversion, which equals
+// half_to_float_ref on every finite value (subnormals included: f16
+// denormals are preserved); the non-finite ones, where the two may differ
+// in a NaN payload, are skipped either way.
+__device__ __forceinline__ float residual_chain(const uint16_t *scale_row, const float *bip, uint32_t nb,
+                                               float long_d) {
+    const uint2 *srow = (const uint2 *)scale_row;
+    float ips = 0.0f;
+#pragma unroll 4
+    for (uint32_t b4 = 0; b4 < nb / 4; ++b4) {
+        const uint2 sv = srow[b4];
+        const uint32_t sw[2] = {sv.x, sv.y};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float scale =
+                (float)__builtin_bit_cast(_Float16, (uint16_t)(sw[e >> 1] >> (16 * (e & 1))));
+            const bool use = scale != 0.0f && __builtin_isfinite(scale);
+            const float next = fmaf(scale, bip[4 * b4 + e], ips);
+            ips = use ? next : ips;
+        }
+    }
+    return long_d - 2.0f * ips;
+}
+
 // ---------------------------------------------------------------------------
 // 16-lane group forms: lane j (= lane % 16) of an aligned group of 16 owns
 // accumulator j of the reference's 16-lane vector and runs its chain in the

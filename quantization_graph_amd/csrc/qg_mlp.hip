@@ -427,7 +427,8 @@ __global__ void __launch_bounds__(128) k_rerank_features(RerankArgs a) {
 // that many pool entries and the top-k by (estimate, id). One 256-thread
 // workgroup per query, bit-identical to the search kernel's rerank:
 //  * INT8 queries: the FP32 long distance of each candidate on a 16-lane
-//    group over an LDS-staged code row (asym_group);
+//    group over an LDS-staged code row (asym_group), the next row's loads
+//    in flight meanwhile;
 //  * residual-4: every (candidate, block) pair's 8-lane product sum on its
 //    own thread (coalesced 8-byte loads of consecutive blocks), kept in LDS;
 //    one thread per candidate then runs the order-dependent recurrence
@@ -495,13 +496,34 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
     for (uint32_t c = t; c < depth; c += kRrThreads) r_id[c] = pid[c];
     if (a.codec == 3) {
         // 16 groups of 16 lanes, one code row (cd / 2 bytes) each in `work`
-        const int g = t >> 4, j = t & 15;
-        uint4 *row = (uint4 *)work + g * (cd / 32);
-        for (uint32_t c = g; c < depth; c += kRrThreads / 16) {
+        // (cd <= 2048: a row is at most 4 uint4 per lane; the next row's
+        // loads are in flight while the current one is scored)
+        const uint32_t g = t >> 4, j = t & 15, cch = cd / 32;
+        uint4 *row = (uint4 *)work + g * cch;
+        uint4 buf[4];
+        float2 h = make_float2(0.0f, 0.0f);
+        auto fetch = [&](uint32_t c) {
             const uint32_t id = pid[c];
-            stage_row_group(row, a.ix.codes + (uint64_t)id * (cd / 2), cd / 2, j);
-            const float ld = asym_group(qs, (const uint8_t *)row, cd, a.ix.hdr[id], s, j);
+            const uint4 *src = (const uint4 *)(a.ix.codes + (uint64_t)id * (cd / 2));
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) buf[k] = j + 16u * k < cch ? src[j + 16u * k] : make_uint4(0u, 0u, 0u, 0u);
+            h = a.ix.hdr[id];
+        };
+        if (g < depth) fetch(g);
+        for (uint32_t c = g; c < depth; c += kRrThreads / 16) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+                if (j + 16u * k < cch) row[j + 16u * k] = buf[k];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const float2 hc = h;
+            if (c + kRrThreads / 16 < depth) fetch(c + kRrThreads / 16);
+            const float ld = asym_group(qs, (const uint8_t *)row, cd, hc, s, (int)j);
             if (j == 0) r_ld[c] = ld;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     } else {
         for (uint32_t c = t; c < depth; c += kRrThreads) r_ld[c] = pd[c];
@@ -509,47 +531,14 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
     __syncthreads();
     const uint32_t stride = nb + 1;
     const uint32_t per = kRrWork / stride;
-    const int lg = __builtin_ctz(nb);
     for (uint32_t c0 = 0; c0 < depth; c0 += per) {
         const uint32_t cnt = depth - c0 < per ? depth - c0 : per;
-        for (uint32_t it = t; it < cnt * nb; it += kRrThreads) {
-            const uint32_t c = it >> lg, b = it & (nb - 1);
-            const uint32_t id = r_id[c0 + c];
-            const uint2 w = ((const uint2 *)(a.ix.rcodes + (uint64_t)id * (cd / 2)))[b];
-            const float4 *qp = (const float4 *)(qs + 16 * b);
-            const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
-            const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
-                                  q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
-            float bip = 0.0f;
-#pragma unroll
-            for (int l = 0; l < 8; ++l) {
-                const uint32_t byte = ((l < 4 ? w.x : w.y) >> (8 * (l & 3))) & 0xFFu;
-                int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
-                lo -= (lo > 7) ? 16 : 0;
-                hi -= (hi > 7) ? 16 : 0;
-                const float pl = (float)lo * qv[2 * l];
-                bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
-            }
-            work[c * stride + b] = bip;
-        }
+        residual_block_sums<kRrThreads, 8>(work, stride, qs, a.ix.rcodes, cd, cnt,
+                                        [&](uint32_t c) { return r_id[c0 + c]; }, t);
         __syncthreads();
-        for (uint32_t c = t; c < cnt; c += kRrThreads) {
-            const uint32_t id = r_id[c0 + c];
-            const uint2 *srow = (const uint2 *)(a.ix.rscale + (uint64_t)id * nb);
-            const float *bp = work + c * stride;
-            float ips = 0.0f;
-            for (uint32_t b4 = 0; b4 < nb / 4; ++b4) {
-                const uint2 sv = srow[b4];
-                const uint32_t sw[2] = {sv.x, sv.y};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float scale = half_to_float_ref((uint16_t)(sw[e >> 1] >> (16 * (e & 1))));
-                    if (scale == 0.0f || !__builtin_isfinite(scale)) continue;
-                    ips = fmaf(scale, bp[4 * b4 + e], ips);
-                }
-            }
-            r_est[c0 + c] = r_ld[c0 + c] - 2.0f * ips;
-        }
+        for (uint32_t c = t; c < cnt; c += kRrThreads)
+            r_est[c0 + c] = residual_chain(a.ix.rscale + (uint64_t)r_id[c0 + c] * nb, work + c * stride, nb,
+                                           r_ld[c0 + c]);
         __syncthreads();
     }
     const uint32_t kout = depth < a.k ? depth : a.k;

@@ -33,7 +33,6 @@
 namespace qg {
 
 constexpr uint32_t kMaxL = 4096;       // pool capacity (search list size)
-constexpr uint32_t kMaxRerank = 1024;  // rerank candidates per query
 
 __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint32_t bid) {
     return ad < bd || (ad == bd && aid < bid);
@@ -81,15 +80,13 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     float *pd = (float *)(smem + 8 * cd);                       // a.L
     uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * a.L);      // a.L
     uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * a.L);       // a.L (+pad)
-    __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank arrays after the traversal
+    __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank block sums after the traversal
     __shared__ uint32_t fresh[64], surv[64], sid[64];
     __shared__ float4 f_fac[64];
     __shared__ float f_lb[64], f_sip[64], s_d[64];
     __shared__ float2 s_h[64];
     __shared__ uint8_t f_fv[64], f_valid[64];
     __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
-    float *r_est = (float *)stage;                   // kMaxRerank
-    uint32_t *r_id = (uint32_t *)stage + kMaxRerank; // kMaxRerank
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
     const uint32_t mch = cd / 128;  // uint4 per MSB row
@@ -350,64 +347,106 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             len = s_len;
         }
 
-        if (a.pool_only) goto finish;
-        {
-        // rerank: first rc pool entries, FP32 long distance, residual-4
-        const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
-        const uint32_t rc = len < want ? len : want;
-        for (uint32_t i = t; i < rc; i += T) {
-            const uint32_t id = pid[i];
-            float ld = pd[i];
-            if (a.codec == 3) ld = asym_one(q, a.ix.codes + (uint64_t)id * (cd / 2), cd, a.ix.hdr[id], s);
-            const Interval iv = residual_one(q, a.ix.rscale + (uint64_t)id * (cd / 16),
-                                             a.ix.rcodes + (uint64_t)id * (cd / 2), cd, s, ld);
-            r_est[i] = iv.estimate;
-            r_id[i] = id;
-        }
-        __syncthreads();
-        // top-k by (estimate, id): k rounds of a wave-wide argmin on wave 0
-        if (wv == 0) {
-            const uint32_t kout = rc < a.k ? rc : a.k;
-            for (uint32_t r = 0; r < a.k; ++r) {
-                if (r >= kout) {
-                    if (lane == 0) {
-                        a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu;
-                        a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf();
-                    }
-                    continue;
-                }
-                float bd = __builtin_huge_valf();
-                uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
-                for (uint32_t i = lane; i < rc; i += 64u) {
-                    if (r_id[i] == 0xFFFFFFFFu) continue;
-                    if (bpos == 0xFFFFFFFFu || cand_less(r_est[i], r_id[i], bd, bid)) { bd = r_est[i]; bid = r_id[i]; bpos = i; }
-                }
-                for (int off = 32; off > 0; off >>= 1) {
-                    const float od = __shfl_xor(bd, off);
-                    const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
-                    const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
-                    const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || cand_less(od, oid, bd, bid));
-                    if (take) { bd = od; bid = oid; bpos = opos; }
-                }
-                if (lane == 0) {
-                    a.out_ids[(uint64_t)qi * a.k + r] = bid;
-                    a.out_d[(uint64_t)qi * a.k + r] = bd;
-                    r_id[bpos] = 0xFFFFFFFFu;
-                }
-                wave_sync_lds();
-            }
-        }
-        }
-    finish:
-        if (t == 0 && a.stats) {
-            uint64_t *st = a.stats + (uint64_t)qi * 4;
-            st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
-        }
+        // the traversal pool goes out before the rerank reuses its arrays
         if (a.pool_ids) {
             for (uint32_t i = t; i < a.L; i += T) {
                 a.pool_ids[(uint64_t)qi * a.L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
                 a.pool_d[(uint64_t)qi * a.L + i] = i < len ? pd[i] : __builtin_huge_valf();
             }
+            __syncthreads();
+        }
+        if (!a.pool_only) {
+            // rerank: the first rc pool entries; their estimates replace pd[0..rc)
+            const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
+            const uint32_t rc = len < want ? len : want;
+            if (a.codec == 3) {
+                // FP32 long distance (recomputed for the INT8 codec): one
+                // candidate per 16-lane group over its code row staged in LDS
+                // (cd <= 2048: a row is at most 4 uint4 per lane; the next
+                // row's loads are in flight while the current one is scored)
+                uint4 *row = stage + grp * cch;
+                uint4 buf[4];
+                float2 h = make_float2(0.0f, 0.0f);
+                auto fetch = [&](uint32_t c) {
+                    const uint32_t id = pid[c];
+                    const uint4 *src = (const uint4 *)(a.ix.codes + (uint64_t)id * (cd / 2));
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) {
+                        const uint32_t i = (uint32_t)j16 + 16u * k;
+                        buf[k] = i < cch ? src[i] : make_uint4(0u, 0u, 0u, 0u);
+                    }
+                    h = a.ix.hdr[id];
+                };
+                if (grp < rc) fetch(grp);
+                for (uint32_t c = grp; c < rc; c += G) {
+#pragma unroll
+                    for (uint32_t k = 0; k < 4; ++k) {
+                        const uint32_t i = (uint32_t)j16 + 16u * k;
+                        if (i < cch) row[i] = buf[k];
+                    }
+                    wave_sync_lds();
+                    const float2 hc = h;
+                    if (c + G < rc) fetch(c + G);
+                    const float ld = asym_group(q, (const uint8_t *)row, cd, hc, s, j16);
+                    if (j16 == 0) pd[c] = ld;
+                    wave_sync_lds();
+                }
+                __syncthreads();
+            }
+            // residual-4: each (candidate, block) product sum on its own
+            // thread into the stage (coalesced loads of consecutive blocks),
+            // then one thread per candidate runs the block recurrence
+            {
+                float *work = (float *)stage;
+                const uint32_t nbk = cd / 16, stride = nbk + 1;
+                const uint32_t pp = 4096u / stride;
+                for (uint32_t c0 = 0; c0 < rc; c0 += pp) {
+                    const uint32_t cnt = rc - c0 < pp ? rc - c0 : pp;
+                    residual_block_sums<T, 4>(work, stride, q, a.ix.rcodes, cd, cnt,
+                                           [&](uint32_t c) { return pid[c0 + c]; }, t);
+                    __syncthreads();
+                    for (uint32_t c = t; c < cnt; c += T)
+                        pd[c0 + c] = residual_chain(a.ix.rscale + (uint64_t)pid[c0 + c] * nbk, work + c * stride,
+                                                    nbk, pd[c0 + c]);
+                    __syncthreads();
+                }
+            }
+            // top-k by (estimate, id): k rounds of a wave-wide argmin on wave 0
+            if (wv == 0) {
+                const uint32_t kout = rc < a.k ? rc : a.k;
+                for (uint32_t r = 0; r < a.k; ++r) {
+                    if (r >= kout) {
+                        if (lane == 0) {
+                            a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu;
+                            a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf();
+                        }
+                        continue;
+                    }
+                    float bd = __builtin_huge_valf();
+                    uint32_t bid = 0xFFFFFFFFu, bpos = 0xFFFFFFFFu;
+                    for (uint32_t i = lane; i < rc; i += 64u) {
+                        if (pid[i] == 0xFFFFFFFFu) continue;
+                        if (bpos == 0xFFFFFFFFu || cand_less(pd[i], pid[i], bd, bid)) { bd = pd[i]; bid = pid[i]; bpos = i; }
+                    }
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const float od = __shfl_xor(bd, off);
+                        const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+                        const uint32_t opos = (uint32_t)__shfl_xor((int)bpos, off);
+                        const bool take = opos != 0xFFFFFFFFu && (bpos == 0xFFFFFFFFu || cand_less(od, oid, bd, bid));
+                        if (take) { bd = od; bid = oid; bpos = opos; }
+                    }
+                    if (lane == 0) {
+                        a.out_ids[(uint64_t)qi * a.k + r] = bid;
+                        a.out_d[(uint64_t)qi * a.k + r] = bd;
+                        pid[bpos] = 0xFFFFFFFFu;
+                    }
+                    wave_sync_lds();
+                }
+            }
+        }
+        if (t == 0 && a.stats) {
+            uint64_t *st = a.stats + (uint64_t)qi * 4;
+            st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
         }
         __syncthreads();
     }
