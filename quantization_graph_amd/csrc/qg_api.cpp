@@ -307,7 +307,10 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
     double *nrm = reinterpret_cast<double *>(w + nq * 8);
     double *dot = reinterpret_cast<double *>(w + nq * 16);
     hipStream_t st = (hipStream_t)stream;
-    if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center, seed, nrm, dot, st), "row seed"))
+    // the FNV-1a seed feeds only the INT8 dither
+    if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center, codec == QG_CODEC_INT8 ? seed : nullptr,
+                                     nrm, dot, st),
+                  "row seed"))
         return -1;
     PrepParams p;
     p.raw = queries;

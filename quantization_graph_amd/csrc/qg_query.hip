@@ -10,7 +10,8 @@
 //   ||q-c||^2 = float(||raw||^2 + ||c||^2 - 2<raw,c>)   (doubles, :3447-3454)
 //   INT8: scale = max|q|/128, x = clamp(floor(q/scale + u), -128, 127) with u
 //   from mt19937_64 seeded by FNV-1a over the raw bytes (:3540-3659), two
-//   draws per dimension, the first used for INT8.
+//   draws per dimension, the first used for INT8 (the other codecs draw
+//   nothing, so their rows are not hashed).
 //   B1 / B1Main: sign code x = +1 where q_i > 0 else -1, and its sum.
 // Serial recurrences (the float sums, FNV-1a, the mt19937_64 seeding) run on
 // single lanes in the reference order; everything else is data-parallel.
@@ -20,31 +21,59 @@ namespace qg {
 
 constexpr int kPrepThreads = 256;
 
-// One thread per row: FNV-1a over the row's bytes, ||row||^2 and <row, c> in
+// One thread per row: FNV-1a over the row's bytes (only when seed_out is
+// given: the INT8 dither is its one consumer), ||row||^2 and <row, c> in
 // double, accumulated in index order exactly as the reference loops do.
-__global__ void k_row_seed(const float *__restrict__ rows, uint64_t n, uint32_t dim,
-                           const float *__restrict__ center, uint64_t *__restrict__ seed_out,
-                           double *__restrict__ norm_out, double *__restrict__ dot_out) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const float *x = rows + r * dim;
-    uint64_t h = 1469598103934665603ull;
-    double nrm = 0.0, dot = 0.0;
-    for (uint32_t i = 0; i < dim; ++i) {
-        const float v = x[i];
+// The recurrences are serial per row, so rows go one per lane; each lane
+// walks its row in chunks of kSeedChunk values and issues the next chunk's
+// loads before it folds the current one in.
+constexpr uint32_t kSeedChunk = 32;
+
+template <bool HASH>
+__device__ __forceinline__ void seed_fold(float v, float c, uint64_t &h, double &nrm, double &dot) {
+    if (HASH) {
         const uint32_t b = __float_as_uint(v);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             h ^= (uint64_t)((b >> (8 * k)) & 0xFFu);
             h *= 1099511628211ull;
         }
-        const double dv = (double)v;
-        nrm += dv * dv;  // exact product: ordering of the add is what matters
-        if (center) dot += dv * (double)center[i];
     }
-    if (seed_out) seed_out[r] = h;
+    const double dv = (double)v;
+    nrm += dv * dv;  // exact product: ordering of the add is what matters
+    dot += dv * (double)c;
+}
+
+template <bool HASH>
+__global__ void __launch_bounds__(64) k_row_seed(const float *__restrict__ rows, uint64_t n, uint32_t dim,
+                                                 const float *__restrict__ center, uint64_t *__restrict__ seed_out,
+                                                 double *__restrict__ norm_out, double *__restrict__ dot_out) {
+    const uint64_t r = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    const bool live = r < n;
+    const float *x = rows + (live ? r : 0) * dim;  // idle lanes walk row 0
+    uint64_t h = 1469598103934665603ull;
+    double nrm = 0.0, dot = 0.0;
+    const uint32_t full = dim / kSeedChunk * kSeedChunk;
+    float cur[kSeedChunk], nxt[kSeedChunk];
+    if (full) {
+#pragma unroll
+        for (uint32_t u = 0; u < kSeedChunk; ++u) cur[u] = x[u];
+    }
+    for (uint32_t base = 0; base < full; base += kSeedChunk) {
+        if (base + kSeedChunk < full) {
+#pragma unroll
+            for (uint32_t u = 0; u < kSeedChunk; ++u) nxt[u] = x[base + kSeedChunk + u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kSeedChunk; ++u) seed_fold<HASH>(cur[u], center ? center[base + u] : 0.0f, h, nrm, dot);
+#pragma unroll
+        for (uint32_t u = 0; u < kSeedChunk; ++u) cur[u] = nxt[u];
+    }
+    for (uint32_t i = full; i < dim; ++i) seed_fold<HASH>(x[i], center ? center[i] : 0.0f, h, nrm, dot);
+    if (!live) return;
+    if (HASH) seed_out[r] = h;
     if (norm_out) norm_out[r] = nrm;
-    if (dot_out) dot_out[r] = dot;
+    if (dot_out) dot_out[r] = center ? dot : 0.0;
 }
 
 // In-LDS Walsh-Hadamard butterflies, stage step 1, 2, 4, ... (hadamard,
@@ -242,9 +271,11 @@ extern "C" {
 int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float *center,
                        uint64_t *seed, double *norm, double *dot, hipStream_t stream) {
     if (n == 0) return 0;
-    const unsigned blocks = (unsigned)((n + 127) / 128);
-    hipLaunchKernelGGL(qg::k_row_seed, dim3(blocks), dim3(128), 0, stream, rows, n, dim, center, seed,
-                       norm, dot);
+    const dim3 grid((unsigned)((n + 63) / 64));
+    if (seed)
+        hipLaunchKernelGGL(qg::k_row_seed<true>, grid, dim3(64), 0, stream, rows, n, dim, center, seed, norm, dot);
+    else
+        hipLaunchKernelGGL(qg::k_row_seed<false>, grid, dim3(64), 0, stream, rows, n, dim, center, seed, norm, dot);
     return (int)hipGetLastError();
 }
 
