@@ -254,9 +254,14 @@ def run_graph(args, torch, np, qg):
     stream = torch.cuda.current_stream()
     cd = sp.code_dim
 
+    # one search workspace for every step; each launch still sets the graph
+    # error flag, read once after a timed loop (no host sync per step)
+    ws = sp.search_workspace(index.n, max(1, min(nq, 4096)))
+
     def step(L, codec, ev=None):
         qv = sp.prepare(q, codec)
-        return sp.search(index, qv, deg, edges, k=10, l_search=L, eps0=1.9, rerank=100, events=ev)
+        return sp.search(index, qv, deg, edges, k=10, l_search=L, eps0=1.9, rerank=100, events=ev,
+                         check_graph=False, workspace=ws)
 
     def timed(L, codec, steps, warmup):
         for _ in range(warmup):
@@ -268,6 +273,7 @@ def run_graph(args, torch, np, qg):
             ids, _, st = step(L, codec, evs[i])
         torch.cuda.synchronize()
         ms = 1000.0 * (time.perf_counter() - t0) / steps
+        qg.space.search_error(ws)
         kms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
         rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
         mean = st.double().mean(0).tolist()
@@ -374,11 +380,13 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
     log("learned: calibrated error multiplier %.2f on 2000 held-out queries" % kmult)
     model = learned.LearnedRerank(mlp, kmult)
 
+    ws = sp.search_workspace(index.n, max(1, min(nq, 4096)))
+
     def step(ev=None):
         qv = sp.prepare(q, "full")
         if ev is not None:
             ev[0].record(stream)
-        out = model.search(sp, index, qv, deg, edges, L, rstats)
+        out = model.search(sp, index, qv, deg, edges, L, rstats, workspace=ws, check_graph=False)
         if ev is not None:
             ev[1].record(stream)
         return out
@@ -392,6 +400,8 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
         ids, _, depths, _ = step(evs[i])
     torch.cuda.synchronize()
     ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    from quantization_graph_amd.space import search_error
+    search_error(ws)
     rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
     res = {"qps": nq / (ms / 1000.0), "ms_per_step": ms,
            "gpu_ms": float(np.mean([a.elapsed_time(b) for a, b in evs])), "recall_at_10": rec,

@@ -111,8 +111,12 @@ def residual_stats(sp, index):
     return out
 
 
-def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=None):
-    """Traversal only (qg_search without outputs): pools [nq, L] and stats."""
+def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=None, workspace=None,
+                check_graph=True):
+    """Traversal only (qg_search without outputs): pools [nq, L] and stats.
+    With check_graph=False and a preallocated `workspace` (sized for `slots`)
+    nothing synchronises; read the error flag later with
+    space.search_error(workspace)."""
     degrees = _dev(degrees, torch.int32).reshape(-1)
     edges = _dev(edges, torch.int32)
     nq = queries.nq
@@ -122,13 +126,16 @@ def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=No
     pool_d = torch.empty((nq, L), dtype=torch.float32, device="cuda")
     stats = torch.zeros((nq, 4), dtype=torch.int64, device="cuda")
     Lb = _lib.lib()
-    ws = torch.empty(Lb.qg_search_workspace_bytes(index.n, slots), dtype=torch.uint8, device="cuda")
+    ws_bytes = Lb.qg_search_workspace_bytes(index.n, slots)
+    ws = workspace if workspace is not None else torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    if ws.numel() < ws_bytes:
+        raise ValueError("search workspace too small: need %d bytes" % ws_bytes)
     di, dq = index.desc(), queries.desc()
     check(Lb.qg_search(sp.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
                        edges.shape[1], 10, L, C.c_float(eps0), 100, slots, None, None, _ptr(stats), _ptr(pool_ids),
                        _ptr(pool_d), _ptr(ws), ws.numel(), _stream()))
-    if ws[4:8].view(torch.int32).item():
-        raise _lib.QGError("graph contains an out-of-range neighbour")
+    if check_graph:
+        S.search_error(ws)
     return pool_ids, pool_d, stats
 
 
@@ -234,8 +241,9 @@ class LearnedRerank:
         f = features(sp, index, qv, pool_ids, pool_d, rstats, self.depth)
         return self.mlp.forward(f)
 
-    def search(self, sp, index, qv, degrees, edges, l_search, rstats, k=10):
-        pid, pd, st = search_pool(sp, index, qv, degrees, edges, l_search)
+    def search(self, sp, index, qv, degrees, edges, l_search, rstats, k=10, workspace=None, check_graph=True):
+        pid, pd, st = search_pool(sp, index, qv, degrees, edges, l_search, workspace=workspace,
+                                  check_graph=check_graph)
         preds = self.predict(sp, index, qv, pid, pd, rstats)
         ids, dists, depths = rerank(sp, index, qv, pid, pd, preds, self.kmult, self.depth, self.min_depth, k)
         return ids, dists, depths, st
