@@ -87,6 +87,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ float2 s_h[64];
     __shared__ uint8_t f_fv[64], f_valid[64];
     __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
+    // next hop's neighbour list and visited words, loaded during this hop's
+    // insertion for the predicted next frontier s_sp_node
+    __shared__ uint32_t sp_nb[64], sp_vw[64];
+    __shared__ uint32_t s_sp_node, s_sp_deg, s_spa_id;
+    __shared__ float s_spa_d;
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
     const uint32_t mch = cd / 128;  // uint4 per MSB row
@@ -121,6 +126,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             }
         }
         for (uint32_t i = t; i < a.visited_words; i += T) visited[i] = 0u;
+        if (t == 0) s_sp_node = 0xFFFFFFFFu;
         const QScalars s = a.qv.qs[qi];
         __threadfence();
         __syncthreads();
@@ -163,10 +169,19 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 // wave issue ahead of the other groups' waves on its SIMD
                 __builtin_amdgcn_s_setprio(3);
                 if (lane == 0) pexp[f] = 1;
-                // degree and the whole adjacency row are loaded together
-                const uint32_t deg = a.degrees[node];
-                const uint32_t raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
-                                                             : 0xFFFFFFFFu;
+                // degree and the whole adjacency row: prefetched during the
+                // last hop's insertion when this node was the predicted
+                // frontier, else loaded together now
+                const bool have = s_sp_node == node;
+                uint32_t deg, raw_nb, vw = 0u;
+                if (have) {
+                    deg = s_sp_deg;
+                    raw_nb = sp_nb[lane];
+                    vw = sp_vw[lane];
+                } else {
+                    deg = a.degrees[node];
+                    raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane] : 0xFFFFFFFFu;
+                }
                 const uint32_t nb = lane < deg ? raw_nb : 0xFFFFFFFFu;
                 // first occurrence within the list, then test-and-set visited
                 bool first_occ = lane < deg;
@@ -181,8 +196,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 }
                 if (first_occ) {
                     const uint32_t bit = 1u << (nb & 31u);
-                    const uint32_t old = atomicOr(&visited[nb >> 5], bit);
-                    is_fresh = (old & bit) == 0u;
+                    if (have) {
+                        // the prefetched word is current: only this step of a
+                        // hop writes the visited bits. The mark needs no reply.
+                        is_fresh = (vw & bit) == 0u;
+                        if (is_fresh) (void)atomicOr(&visited[nb >> 5], bit);
+                    } else {
+                        const uint32_t old = atomicOr(&visited[nb >> 5], bit);
+                        is_fresh = (old & bit) == 0u;
+                    }
                 }
                 const uint64_t fm = __ballot(is_fresh);
                 if (is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
@@ -201,20 +223,66 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             }
             stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
             __syncthreads();
-            for (uint32_t c = grp; c < nf; c += G) {
-                const uint8_t *mrow = (const uint8_t *)(stage + c * mch);
-                const PaperEstimate e = a.codec == 3
-                    ? gate_int8_group(x8, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16)
-                    : gate_fp32_group(qp2, mrow, cd, f_fac[c], f_fv[c], s, a.eps0, j16);
-                if (j16 == 0) {
-                    f_lb[c] = e.lower_bound;
-                    f_sip[c] = e.short_ip;
-                    f_valid[c] = e.valid;
+            // each group scores candidates grp, grp + G, grp + 2G, grp + 3G in one
+            // pass (four independent chains; slots past nf re-read a live row
+            // and are dropped)
+            for (uint32_t c0 = grp; c0 < nf; c0 += 4u * G) {
+                uint32_t cc[4];
+                const uint4 *rr[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    cc[k] = c0 + (uint32_t)k * G;
+                    rr[k] = stage + (cc[k] < nf ? cc[k] : c0) * mch;
+                }
+                if (a.codec == 3) {
+                    int sel[4];
+                    gate_int8_group4(x8, (const uint32_t *)rr[0], (const uint32_t *)rr[1], (const uint32_t *)rr[2],
+                                     (const uint32_t *)rr[3], cd, j16, sel);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (cc[k] >= nf) continue;
+                        const PaperEstimate e = gate_int8_finish(group_sum_int(sel[k]), f_fac[cc[k]], f_fv[cc[k]], s,
+                                                                 a.eps0);
+                        if (j16 == 0) {
+                            f_lb[cc[k]] = e.lower_bound;
+                            f_sip[cc[k]] = e.short_ip;
+                            f_valid[cc[k]] = e.valid;
+                        }
+                    }
+                } else {
+                    float acc[4];
+                    gate_fp32_group4(qp2, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (cc[k] >= nf) continue;
+                        const PaperEstimate e = finish_gate(group_reduce16(acc[k]) - s.half_sum, f_fac[cc[k]],
+                                                            f_fv[cc[k]], s.query_norm, s.query_norm_sqr, a.eps0);
+                        if (j16 == 0) {
+                            f_lb[cc[k]] = e.lower_bound;
+                            f_sip[cc[k]] = e.short_ip;
+                            f_valid[cc[k]] = e.valid;
+                        }
+                    }
                 }
             }
             __syncthreads();
-            // prune against the pool as it stood at the start of the hop
+            // prune against the pool as it stood at the start of the hop; wave 1
+            // notes the first unexpanded pool entry for the prediction of the
+            // next frontier (the pool is not written until the insertion)
+            if (wv == 1) {
+                uint32_t fa = 0xFFFFFFFFu;
+                for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
+                    const uint64_t m = __ballot(c0 + lane < len && !pexp[c0 + lane]);
+                    if (m) { fa = c0 + (uint32_t)__builtin_ctzll(m); break; }
+                }
+                if (lane == 0) {
+                    s_spa_id = fa == 0xFFFFFFFFu ? 0xFFFFFFFFu : pid[fa];
+                    s_spa_d = fa == 0xFFFFFFFFu ? 3.402823466e38f : pd[fa];
+                }
+            }
             if (wv == 0) {
+                // this hop's visited marks are complete before any later read
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
                 const bool full0 = len >= L;
                 const float back0 = full0 ? pd[len - 1] : 3.402823466e38f;
                 const bool survive = lane < nf && !(full0 && f_valid[lane] && f_lb[lane] > back0);
@@ -251,6 +319,40 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 __syncthreads();
             }
             dists += ns;
+            // wave 1, while wave 0 inserts: the next frontier is the better of
+            // the first unexpanded entry and the best survivor (a survivor the
+            // insertion prunes by its bound makes this a miss, handled by
+            // loading again); load its adjacency row and visited words
+            if (wv == 1) {
+                float bd = lane < ns ? s_d[lane] : 3.402823466e38f;
+                uint32_t bid = lane < ns ? sid[lane] : 0xFFFFFFFFu;
+                bool bv = lane < ns;
+                for (int off = 32; off > 0; off >>= 1) {
+                    const float od = __shfl_xor(bd, off);
+                    const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+                    const bool ov = __shfl_xor((int)bv, off) != 0;
+                    if (ov && (!bv || cand_less(od, oid, bd, bid))) { bd = od; bid = oid; bv = true; }
+                }
+                const uint32_t sp_aid = s_spa_id;
+                const bool sp_a = sp_aid != 0xFFFFFFFFu;
+                uint32_t nxt = 0xFFFFFFFFu;
+                if (bv && (!sp_a || cand_less(bd, bid, s_spa_d, sp_aid))) nxt = bid;
+                else if (sp_a) nxt = sp_aid;
+                if (nxt != 0xFFFFFFFFu && nxt < a.n) {
+                    const uint32_t deg2 = a.degrees[nxt];
+                    const uint32_t raw2 = lane < a.edge_stride ? a.edges[(uint64_t)nxt * a.edge_stride + lane]
+                                                               : 0xFFFFFFFFu;
+                    const uint32_t nb2 = lane < deg2 ? raw2 : 0xFFFFFFFFu;
+                    uint32_t w2 = 0u;
+                    if (nb2 < a.n)
+                        w2 = __hip_atomic_load(&visited[nb2 >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    sp_nb[lane] = raw2;
+                    sp_vw[lane] = w2;
+                    if (lane == 0) { s_sp_node = nxt; s_sp_deg = deg2; }
+                } else if (lane == 0) {
+                    s_sp_node = 0xFFFFFFFFu;
+                }
+            }
             // ordered insertion with the re-check against the current back
             // (insert_candidate_sorted in survivor order), on wave 0. The
             // decisions only need the pool's back, which is the larger of the
