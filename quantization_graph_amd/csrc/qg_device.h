@@ -424,40 +424,6 @@ __device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 
     acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;
 }
 
-// INT8 gate of four candidates per group in one pass (exact integer sums,
-// any order); gate_int8_finish turns a group's summed selection into the
-// estimate exactly as gate_int8_group does.
-__device__ __forceinline__ void gate_int8_group4(const int8_t *x8, const uint32_t *r0, const uint32_t *r1,
-                                                 const uint32_t *r2, const uint32_t *r3, uint32_t cd, int j,
-                                                 int (&sel)[4]) {
-    const uint32_t *xg = (const uint32_t *)x8;
-    const uint32_t sh = 4u * ((uint32_t)j & 7u);
-    const uint32_t half = (uint32_t)j >> 3;
-    int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-#pragma unroll 1
-    for (uint32_t i = 0; i < cd / 64; ++i) {
-        const int xv = (int)xg[(uint32_t)j + 16u * i];
-        const uint32_t w = 2u * i + half;
-        s0 = __builtin_amdgcn_sdot4((int)nibble_bytes((r0[w] >> sh) & 0xFu), xv, s0, false);
-        s1 = __builtin_amdgcn_sdot4((int)nibble_bytes((r1[w] >> sh) & 0xFu), xv, s1, false);
-        s2 = __builtin_amdgcn_sdot4((int)nibble_bytes((r2[w] >> sh) & 0xFu), xv, s2, false);
-        s3 = __builtin_amdgcn_sdot4((int)nibble_bytes((r3[w] >> sh) & 0xFu), xv, s3, false);
-    }
-    sel[0] = s0; sel[1] = s1; sel[2] = s2; sel[3] = s3;
-}
-
-__device__ __forceinline__ PaperEstimate gate_int8_finish(int sel_group, float4 f, uint8_t fv, const QScalars &s,
-                                                          float eps0) {
-    if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) {
-        PaperEstimate r;
-        r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
-        r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
-        return r;
-    }
-    const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel_group - s.int8_code_sum);
-    return finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, eps0);
-}
-
 // C5 on a staged MSB row: 4-dim groups g4 = j + 16i (uniform trip count)
 __device__ __forceinline__ PaperEstimate gate_int8_group(const int8_t *x8, const uint8_t *msb_lds, uint32_t cd,
                                                         float4 f, uint8_t fv, const QScalars &s, float eps0, int j) {

@@ -79,7 +79,7 @@ __device__ __forceinline__ bool level_member(int k, float a, int c0, double t_en
 }
 
 __global__ void __launch_bounds__(kEncThreads) k_enc_rotate(EncArgs a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd;
     float *v = (float *)smem;                                   // cd
     unsigned long long *key = (unsigned long long *)(smem + 4 * cd);  // cd
@@ -273,7 +273,7 @@ __device__ __forceinline__ int clamp7_round(double v) {
 }
 
 __global__ void __launch_bounds__(kEncThreads) k_enc_codes(EncArgs a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd;
     float *err = (float *)smem;                 // cd
     uint8_t *nib = (uint8_t *)(smem + 4 * cd);  // cd primary nibbles

@@ -126,7 +126,7 @@ __device__ void mt64_twist(uint64_t *mt) {
 }
 
 __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = p.cd;
     float *v = (float *)smem;                          // cd floats
     double *err = (double *)(smem + 4 * cd);           // cd doubles
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
 __global__ void __launch_bounds__(kPrepThreads) k_rotate_rows(const float *rows, uint64_t n, uint32_t dim,
                                                               uint32_t cd, int identity,
                                                               const float *signs, float *out) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     float *v = (float *)smem;
     const float *x = rows + (uint64_t)blockIdx.x * dim;
     for (uint32_t i = threadIdx.x; i < cd; i += blockDim.x)

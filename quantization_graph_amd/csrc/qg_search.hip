@@ -69,7 +69,7 @@ __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint
 template <int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_search(SearchArgs a) {
     constexpr uint32_t T = 64u * W, G = 4u * W;
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd;
     // query: FP32 residual, then one codec-specific region of 4 cd bytes:
     // the pair-interleaved FP32 query of the gate, or the INT8 codes
@@ -89,8 +89,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
     // next hop's neighbour list and visited words, loaded during this hop's
     // insertion for the predicted next frontier s_sp_node
-    __shared__ uint32_t sp_nb[64], sp_vw[64];
-    __shared__ uint32_t s_sp_node, s_sp_deg, s_spa_id;
+    __shared__ uint32_t sp_nb[64];
+    __shared__ unsigned long long s_sp_fresh;  // lanes whose neighbour is fresh (first occurrence, unvisited)
+    __shared__ uint32_t s_sp_node, s_sp_bad, s_spa_id;
     __shared__ float s_spa_d;
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
@@ -169,39 +170,34 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 // wave issue ahead of the other groups' waves on its SIMD
                 __builtin_amdgcn_s_setprio(3);
                 if (lane == 0) pexp[f] = 1;
-                // degree and the whole adjacency row: prefetched during the
-                // last hop's insertion when this node was the predicted
-                // frontier, else loaded together now
-                const bool have = s_sp_node == node;
-                uint32_t deg, raw_nb, vw = 0u;
-                if (have) {
-                    deg = s_sp_deg;
-                    raw_nb = sp_nb[lane];
-                    vw = sp_vw[lane];
-                } else {
-                    deg = a.degrees[node];
-                    raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane] : 0xFFFFFFFFu;
-                }
-                const uint32_t nb = lane < deg ? raw_nb : 0xFFFFFFFFu;
-                // first occurrence within the list, then test-and-set visited
-                bool first_occ = lane < deg;
-                for (int jj = 0; jj < 63; ++jj) {
-                    const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb, jj);
-                    if ((uint32_t)jj < lane && other == nb) first_occ = false;
-                }
+                // the neighbour list with its first occurrences and their visited
+                // test: worked out during the last hop's insertion when this node
+                // was the predicted frontier (the visited bits have not changed
+                // since: only this step of a hop writes them), else here
                 bool is_fresh = false;
-                if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
-                    atomicOr(a.err, 1u);
-                    first_occ = false;
-                }
-                if (first_occ) {
-                    const uint32_t bit = 1u << (nb & 31u);
-                    if (have) {
-                        // the prefetched word is current: only this step of a
-                        // hop writes the visited bits. The mark needs no reply.
-                        is_fresh = (vw & bit) == 0u;
-                        if (is_fresh) (void)atomicOr(&visited[nb >> 5], bit);
-                    } else {
+                uint32_t nb;
+                if (s_sp_node == node) {
+                    nb = sp_nb[lane];
+                    is_fresh = ((s_sp_fresh >> lane) & 1ull) != 0ull;
+                    if (lane == 0 && s_sp_bad) atomicOr(a.err, 1u);  // out-of-range neighbour
+                    if (is_fresh) (void)atomicOr(&visited[nb >> 5], 1u << (nb & 31u));  // no reply needed
+                } else {
+                    const uint32_t deg = a.degrees[node];
+                    const uint32_t raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
+                                                                 : 0xFFFFFFFFu;
+                    nb = lane < deg ? raw_nb : 0xFFFFFFFFu;
+                    // first occurrence within the list, then test-and-set visited
+                    bool first_occ = lane < deg;
+                    for (int jj = 0; jj < 63; ++jj) {
+                        const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb, jj);
+                        if ((uint32_t)jj < lane && other == nb) first_occ = false;
+                    }
+                    if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
+                        atomicOr(a.err, 1u);
+                        first_occ = false;
+                    }
+                    if (first_occ) {
+                        const uint32_t bit = 1u << (nb & 31u);
                         const uint32_t old = atomicOr(&visited[nb >> 5], bit);
                         is_fresh = (old & bit) == 0u;
                     }
@@ -223,9 +219,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             }
             stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
             __syncthreads();
-            // each group scores candidates grp, grp + G, grp + 2G, grp + 3G in one
-            // pass (four independent chains; slots past nf re-read a live row
-            // and are dropped)
+            // each group scores candidates grp, grp + G, grp + 2G, grp + 3G (FP32:
+            // in one pass, four independent chains; slots past nf re-read a
+            // live row and are dropped)
             for (uint32_t c0 = grp; c0 < nf; c0 += 4u * G) {
                 uint32_t cc[4];
                 const uint4 *rr[4];
@@ -235,14 +231,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     rr[k] = stage + (cc[k] < nf ? cc[k] : c0) * mch;
                 }
                 if (a.codec == 3) {
-                    int sel[4];
-                    gate_int8_group4(x8, (const uint32_t *)rr[0], (const uint32_t *)rr[1], (const uint32_t *)rr[2],
-                                     (const uint32_t *)rr[3], cd, j16, sel);
-#pragma unroll
+                    // integer sums: one candidate at a time, unrolled
                     for (int k = 0; k < 4; ++k) {
                         if (cc[k] >= nf) continue;
-                        const PaperEstimate e = gate_int8_finish(group_sum_int(sel[k]), f_fac[cc[k]], f_fv[cc[k]], s,
-                                                                 a.eps0);
+                        const PaperEstimate e = gate_int8_group(x8, (const uint8_t *)rr[k], cd, f_fac[cc[k]],
+                                                                f_fv[cc[k]], s, a.eps0, j16);
                         if (j16 == 0) {
                             f_lb[cc[k]] = e.lower_bound;
                             f_sip[cc[k]] = e.short_ip;
@@ -346,9 +339,18 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     uint32_t w2 = 0u;
                     if (nb2 < a.n)
                         w2 = __hip_atomic_load(&visited[nb2 >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    sp_nb[lane] = raw2;
-                    sp_vw[lane] = w2;
-                    if (lane == 0) { s_sp_node = nxt; s_sp_deg = deg2; }
+                    // first occurrence within the list (while the loads fly)
+                    bool first2 = lane < deg2;
+                    for (int jj = 0; jj < 63; ++jj) {
+                        const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb2, jj);
+                        if ((uint32_t)jj < lane && other == nb2) first2 = false;
+                    }
+                    const bool bad2 = first2 && nb2 >= a.n;
+                    const bool fresh2 = first2 && !bad2 && (w2 & (1u << (nb2 & 31u))) == 0u;
+                    const uint64_t fm2 = __ballot(fresh2);
+                    const uint64_t bm2 = __ballot(bad2);
+                    sp_nb[lane] = nb2;
+                    if (lane == 0) { s_sp_node = nxt; s_sp_fresh = fm2; s_sp_bad = bm2 != 0ull ? 1u : 0u; }
                 } else if (lane == 0) {
                     s_sp_node = 0xFFFFFFFFu;
                 }

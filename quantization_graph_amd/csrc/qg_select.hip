@@ -15,7 +15,7 @@ constexpr int kSelectThreads = 256;
 
 
 __global__ void __launch_bounds__(kSelectThreads) k_topk(SelectArgs a) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t n = a.per_query;
     uint32_t np = 1;
     while (np < n) np <<= 1;
