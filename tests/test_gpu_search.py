@@ -33,24 +33,21 @@ def graph_case():
     return dict(x=x, q=q, c=c, orc=orc, comp=comp, res=res, msb=msb, fac=fac, deg=deg, edges=edges)
 
 
-@pytest.mark.parametrize("codec", [3, 0])
-@pytest.mark.parametrize("L", [16, 64, 200])
-def test_search_matches_oracle(gpu, graph_case, codec, L):
+def _compare_with_oracle(gpu, g, deg, edges, codec, L):
     import quantization_graph_amd as qg
-    g = graph_case
     sp = qg.RabitqSpace(960, 100)
     sp.set_center(g["c"])
     ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
     qv = sp.prepare(gpu.from_numpy(g["q"]).cuda(), codec)
-    ids, dists, stats, pool, pool_d = sp.search(ix, qv, g["deg"].astype(np.int32), g["edges"].astype(np.int32),
+    ids, dists, stats, pool, pool_d = sp.search(ix, qv, deg.astype(np.int32), edges.astype(np.int32),
                                                 k=10, l_search=L, eps0=1.9, rerank=100, want_pool=True)
     ids, dists, stats = ids.cpu().numpy(), dists.cpu().numpy(), stats.cpu().numpy()
     pool, pool_d = pool.cpu().numpy(), pool_d.cpu().numpy()
     orc = g["orc"]
     for i in range(g["q"].shape[0]):
         oq = orc.prepare(g["q"][i], codec)
-        o_ids, o_d, o_pool, o_pd, st = orc.search(oq, codec, g["comp"], g["res"], g["msb"], g["fac"], g["deg"],
-                                                  g["edges"], k=10, L=L, eps0=1.9, rerank=100)
+        o_ids, o_d, o_pool, o_pd, st = orc.search(oq, codec, g["comp"], g["res"], g["msb"], g["fac"], deg,
+                                                  edges, k=10, L=L, eps0=1.9, rerank=100)
         m = len(o_pool)
         assert np.array_equal(pool[i, :m].astype(np.uint32), o_pool), "pool ids (query %d)" % i
         assert np.array_equal(pool_d[i, :m].view(np.uint32), o_pd.view(np.uint32)), "pool distances"
@@ -58,6 +55,31 @@ def test_search_matches_oracle(gpu, graph_case, codec, L):
         assert np.array_equal(dists[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
         assert tuple(stats[i]) == (st.hops, st.gate_evals, st.full_dists, st.pruned)
         orc.free_query(oq)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+@pytest.mark.parametrize("L", [16, 64, 200])
+def test_search_matches_oracle(gpu, graph_case, codec, L):
+    _compare_with_oracle(gpu, graph_case, graph_case["deg"], graph_case["edges"], codec, L)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+def test_search_duplicate_neighbours_match_oracle(gpu, graph_case, codec):
+    """Adjacency lists with repeated entries: only a neighbour's first
+    occurrence is a candidate (the reference marks it visited as it walks the
+    list), whether the kernel prepared the list ahead during the previous
+    hop or loads it on the spot."""
+    g = graph_case
+    deg, edges = g["deg"].copy(), g["edges"].copy()
+    rng = np.random.default_rng(5)
+    for v in range(edges.shape[0]):
+        d = int(deg[v])
+        if d < 4:
+            continue
+        for _ in range(3):  # copy earlier entries over later slots
+            a, b = sorted(rng.choice(d, size=2, replace=False))
+            edges[v, b] = edges[v, a]
+    _compare_with_oracle(gpu, g, deg, edges, codec, 64)
 
 
 def test_search_rejects_bad_graph(gpu, graph_case):
@@ -73,6 +95,24 @@ def test_search_rejects_bad_graph(gpu, graph_case):
         sp.search(ix, qv, g["deg"].astype(np.int32), edges.astype(np.int32), k=10, l_search=32)
     with pytest.raises(Exception):
         sp.search(ix, qv, g["deg"].astype(np.int32), np.zeros((g["x"].shape[0], 65), np.int32), k=10)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+def test_search_rejects_bad_neighbour_past_the_start(gpu, graph_case, codec):
+    """An out-of-range neighbour in lists reached after the first hop (which
+    the kernel prepares ahead of time) is reported as well."""
+    import quantization_graph_amd as qg
+    g = graph_case
+    sp = qg.RabitqSpace(960, 100)
+    sp.set_center(g["c"])
+    ix = qg.DeviceIndex.from_records(sp, g["comp"], g["res"])
+    qv = sp.prepare(gpu.from_numpy(g["q"][:4]).cuda(), codec)
+    edges = g["edges"].astype(np.int64).copy()
+    edges[1:, 3] = g["x"].shape[0] + 7  # every list but the start node's
+    with pytest.raises(qg._lib.QGError):
+        sp.search(ix, qv, g["deg"].astype(np.int32), edges.astype(np.int32), k=10, l_search=32)
+    # the same graph without the bad entries searches cleanly afterwards
+    sp.search(ix, qv, g["deg"].astype(np.int32), g["edges"].astype(np.int32), k=10, l_search=32)
 
 
 PER_QUERY_L = [10, 16, 37, 64, 100, 200, 600, 1200, 2500, 4096, 23, 77]
@@ -110,6 +150,31 @@ def test_search_per_query_lists(gpu, graph_case, codec, waves):
         assert np.array_equal(dists[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
         assert tuple(stats[i]) == (st.hops, st.gate_evals, st.full_dists, st.pruned)
         orc.free_query(oq)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+@pytest.mark.parametrize("L", [16, 64, 200])
+def test_search_matches_oracle(gpu, graph_case, codec, L):
+    _compare_with_oracle(gpu, graph_case, graph_case["deg"], graph_case["edges"], codec, L)
+
+
+@pytest.mark.parametrize("codec", [3, 0])
+def test_search_duplicate_neighbours_match_oracle(gpu, graph_case, codec):
+    """Adjacency lists with repeated entries: only a neighbour's first
+    occurrence is a candidate (the reference marks it visited as it walks the
+    list), whether the kernel prepared the list ahead during the previous
+    hop or loads it on the spot."""
+    g = graph_case
+    deg, edges = g["deg"].copy(), g["edges"].copy()
+    rng = np.random.default_rng(5)
+    for v in range(edges.shape[0]):
+        d = int(deg[v])
+        if d < 4:
+            continue
+        for _ in range(3):  # copy earlier entries over later slots
+            a, b = sorted(rng.choice(d, size=2, replace=False))
+            edges[v, b] = edges[v, a]
+    _compare_with_oracle(gpu, g, deg, edges, codec, 64)
 
 
 def test_search_graph_capture_replays(gpu, graph_case):
