@@ -38,6 +38,11 @@ __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint
     return ad < bd || (ad == bd && aid < bid);
 }
 
+// the same order without short-circuit evaluation (no branches on uniform values)
+__device__ __forceinline__ bool cand_less_nb(float ad, uint32_t aid, float bd, uint32_t bid) {
+    return (ad < bd) | ((ad == bd) & (aid < bid));
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -339,11 +344,25 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     uint32_t w2 = 0u;
                     if (nb2 < a.n)
                         w2 = __hip_atomic_load(&visited[nb2 >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    // first occurrence within the list (while the loads fly)
+                    // first occurrence within the list (while the loads fly): a
+                    // 2^17-bit hash filter in the stage (idle during the
+                    // insertion) clears lists without repeats; a possible repeat
+                    // (or a collision, about 1.5% of 64-entry lists) takes the
+                    // exact readlane pass
                     bool first2 = lane < deg2;
-                    for (int jj = 0; jj < 63; ++jj) {
-                        const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb2, jj);
-                        if ((uint32_t)jj < lane && other == nb2) first2 = false;
+                    {
+                        uint32_t *filt = (uint32_t *)stage;
+                        const uint32_t h = (nb2 * 0x9E3779B1u) >> 15;
+                        if (first2) filt[h >> 5] = 0u;
+                        wave_sync_lds();
+                        bool seen = false;
+                        if (first2) seen = (atomicOr(&filt[h >> 5], 1u << (h & 31u)) & (1u << (h & 31u))) != 0u;
+                        if (__ballot(seen)) {
+                            for (int jj = 0; jj < 63; ++jj) {
+                                const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb2, jj);
+                                if ((uint32_t)jj < lane && other == nb2) first2 = false;
+                            }
+                        }
                     }
                     const bool bad2 = first2 && nb2 >= a.n;
                     const bool fresh2 = first2 && !bad2 && (w2 & (1u << (nb2 & 31u))) == 0u;
@@ -374,37 +393,45 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 uint32_t e = len, na = 0;  // retained pool entries, accepted survivors
                 float a_d = 0.0f;
                 uint32_t a_id = 0xFFFFFFFFu;
+                // Branch-free: every operand is read (lane indices clamped into
+                // range) and the decision, the counters and the accepted-list
+                // update are selects, so the loop body has no data-dependent
+                // branch. When the pool is full, the back is the larger of the
+                // last retained pool entry (window lane e - 1 - tb) and the
+                // last accepted survivor (lane na - 1).
+                uint32_t npruned = 0;
                 for (uint32_t b = 0; b < ns; ++b) {
                     const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_d), (int)b));
                     const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)my_id, (int)b);
-                    if (e + na >= L) {
-                        float bk_d = 0.0f;
-                        uint32_t bk_id = 0u;
-                        bool from_a = e == 0;
-                        if (e > 0) {
-                            bk_d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bt_d), (int)(e - 1 - tb)));
-                            bk_id = (uint32_t)__builtin_amdgcn_readlane((int)bt_id, (int)(e - 1 - tb));
-                        }
-                        if (na > 0) {
-                            const float ad = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), (int)(na - 1)));
-                            const uint32_t aid = (uint32_t)__builtin_amdgcn_readlane((int)a_id, (int)(na - 1));
-                            if (e == 0 || cand_less(bk_d, bk_id, ad, aid)) { bk_d = ad; bk_id = aid; from_a = true; }
-                        }
-                        const float lb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_lb), (int)b));
-                        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)my_v, (int)b);
-                        if (v && lb > bk_d) { ++pruned; continue; }
-                        if (!cand_less(d, id, bk_d, bk_id)) continue;
-                        if (from_a) --na; else --e;
-                    }
+                    const float lb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_lb), (int)b));
+                    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)my_v, (int)b);
+                    const int pi = e > 0 ? (int)(e - 1 - tb) : 0;
+                    const int ai = na > 0 ? (int)(na - 1) : 0;
+                    const float pbd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bt_d), pi));
+                    const uint32_t pbi = (uint32_t)__builtin_amdgcn_readlane((int)bt_id, pi);
+                    const float abd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), ai));
+                    const uint32_t abi = (uint32_t)__builtin_amdgcn_readlane((int)a_id, ai);
+                    const bool from_a = (na > 0) & ((e == 0) | cand_less_nb(pbd, pbi, abd, abi));
+                    const float bk_d = from_a ? abd : pbd;
+                    const uint32_t bk_id = from_a ? abi : pbi;
+                    const bool full = e + na >= L;
+                    const bool prune = full & (v != 0u) & (lb > bk_d);
+                    const bool take = !full | (!prune & cand_less_nb(d, id, bk_d, bk_id));
+                    npruned += prune ? 1u : 0u;
+                    e -= (full & take & !from_a) ? 1u : 0u;
+                    const uint32_t nna = na - ((full & take & from_a) ? 1u : 0u);
                     // sorted insert into the accepted list: lanes past the slot take
                     // their left neighbour (DPP wave_shr:1)
-                    const uint32_t pos = (uint32_t)__popcll(__ballot(lane < na && cand_less(a_d, a_id, d, id)));
+                    const uint32_t pos = (uint32_t)__popcll(__ballot((lane < nna) & cand_less_nb(a_d, a_id, d, id)));
                     const int sd = __builtin_amdgcn_update_dpp(__float_as_int(a_d), __float_as_int(a_d), 0x138, 0xF, 0xF, false);
                     const int si = __builtin_amdgcn_update_dpp((int)a_id, (int)a_id, 0x138, 0xF, 0xF, false);
-                    if (lane > pos && lane <= na) { a_d = __int_as_float(sd); a_id = (uint32_t)si; }
-                    if (lane == pos) { a_d = d; a_id = id; }
-                    ++na;
+                    const bool shift = take & (lane > pos) & (lane <= nna);
+                    const bool put = take & (lane == pos);
+                    a_d = put ? d : (shift ? __int_as_float(sd) : a_d);
+                    a_id = put ? id : (shift ? (uint32_t)si : a_id);
+                    na = nna + (take ? 1u : 0u);
                 }
+                pruned += npruned;
                 // merge pool[0, e) with the accepted list into pool[0, e + na), in
                 // place: retained entries only move right, so they move in
                 // descending chunks of 64 (each read before any write), then
