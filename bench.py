@@ -640,8 +640,7 @@ def run_fixed(args, torch, np, qg):
         first = fixed.score(sp, index, qv, cand, codec="int8")
         if events is not None:
             events[1].record(stream)
-        ids, dists, _, _ = fixed.rerank(sp, index, qv, cand, first, k=10, depth=depth)
-        return ids, dists
+        return fixed.rerank_topk(sp, index, qv, cand, first, k=10, depth=depth)
 
     def timed(depth, steps, warmup):
         for _ in range(warmup):

@@ -9,8 +9,12 @@ with the 4-bit estimator and ranks by (estimate, index)
 distance (INT8 codec: recomputed, :1850-1866) and the residual-4 correction
 (rabitq_residual_distance_batch), then the final top-k by (estimate, id).
 
-Everything runs on the device in five launches for the whole batch:
-score -> top-rerank -> FP32 restore -> residual -> top-k.
+Everything runs on the device for the whole batch: score -> top-rerank ->
+(FP32 restore -> residual -> top-k). With k <= rerank depth <= 128 the
+bracketed part is one launch of the workgroup rerank kernel (one 256-thread
+workgroup per query: block-parallel residual sums, the ordered block
+recurrence per candidate, top-k by (estimate, id)); otherwise three launches
+that also return every reranked candidate's estimate.
 """
 import torch
 
@@ -40,12 +44,26 @@ def rerank(sp, index, queries, cand_ids, first, k=10, depth=100, restore_full=Tr
     return ids, dists, top_ids, est
 
 
+def rerank_topk(sp, index, queries, cand_ids, first, k=10, depth=100):
+    """The final top-k of rerank() (same ids and estimates, bit for bit),
+    without the per-candidate estimates: one workgroup-per-query launch after
+    the top-`depth` selection when k <= depth <= 128. The FP32 long distance
+    is restored for INT8 queries, as rerank(restore_full=True) does."""
+    depth = min(depth, cand_ids.shape[1])
+    if not (k <= depth <= 128):
+        ids, dists, _, _ = rerank(sp, index, queries, cand_ids, first, k=k, depth=depth,
+                                  restore_full=queries.codec == S.CODEC_INT8)
+        return ids, dists
+    from . import learned
+    top_ids, top_first, _ = S.topk(first, cand_ids, depth)
+    ids, dists, _ = learned.rerank(sp, index, queries, top_ids, top_first, None, depth=depth, k=k)
+    return ids, dists
+
+
 def search(sp, index, queries, cand_ids, k=10, depth=100, codec="int8", eps0=1.9):
     """Score + rerank for a prepared query batch; returns (ids, estimates)."""
     first = score(sp, index, queries, cand_ids, codec=codec, eps0=eps0)
-    ids, dists, _, _ = rerank(sp, index, queries, cand_ids, first, k=k, depth=depth,
-                              restore_full=S._CODECS[codec] == S.CODEC_INT8)
-    return ids, dists
+    return rerank_topk(sp, index, queries, cand_ids, first, k=k, depth=depth)
 
 
 def recall_at_k(found, truth, k=10):
