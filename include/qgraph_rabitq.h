@@ -178,7 +178,8 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
  * the final pool [nq][max(l_search, k)]. Workspace:
  * qg_search_workspace_bytes(n, slots) of device memory (per-slot visited
  * bitsets and the work-queue head); slots = concurrent queries.
- * max(l_search, k) <= 4096 and max(rerank, k) <= 1024. */
+ * max(l_search, k) <= 4096, max(rerank, k) <= 1024 and code_dim >= 128
+ * (dimension above 64). */
 size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots);
 int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
               const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,

@@ -398,6 +398,8 @@ int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc
         return fail("null argument to qg_search");
     if (pool_only && !pool_ids) return fail("qg_search without outputs needs the pool outputs");
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
+    // the search stages MSB rows in 16-byte pieces: cd / 8 bytes must fill one
+    if (s->cd < 128) return fail("the batched search needs a code dimension of at least 128 (dimension above 64)");
     if (k == 0) return fail("k must be positive");
     if (edge_stride == 0 || edge_stride > 64) return fail("graph degree bound must be in [1, 64]");
     if (ix->n == 0 || ix->n > 0xFFFFFFFEull) return fail("index must hold 1 .. 2^32-2 points");

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     // insertion for the predicted next frontier s_sp_node
     __shared__ uint32_t sp_nb[64];
     __shared__ unsigned long long s_sp_fresh;  // lanes whose neighbour is fresh (first occurrence, unvisited)
-    __shared__ uint32_t s_sp_node, s_sp_bad, s_spa_id;
+    __shared__ uint32_t s_sp_node, s_sp_bad, s_spa_id, s_hit;
     __shared__ float s_spa_d;
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
@@ -208,8 +208,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     }
                 }
                 const uint64_t fm = __ballot(is_fresh);
-                if (is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
-                if (lane == 0) s_nf = (uint32_t)__popcll(fm);
+                const bool hit = s_sp_node == node;  // fresh list, factors and MSB rows already staged
+                if (!hit && is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
+                if (lane == 0) { s_nf = (uint32_t)__popcll(fm); s_hit = hit ? 1u : 0u; }
                 __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
@@ -217,13 +218,17 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             if (nf == 0) continue;
             gates += nf;
             // gate: factors and MSB rows of the fresh neighbours in one round trip
-            if (t < nf) {
-                const uint32_t id = fresh[t];
-                f_fac[t] = a.ix.factors[id];
-                f_fv[t] = a.ix.fvalid[id];
+            // (already done during the last insertion when the frontier was
+            // predicted)
+            if (!s_hit) {
+                if (t < nf) {
+                    const uint32_t id = fresh[t];
+                    f_fac[t] = a.ix.factors[id];
+                    f_fv[t] = a.ix.fvalid[id];
+                }
+                stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
+                __syncthreads();
             }
-            stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
-            __syncthreads();
             // each group scores candidates grp, grp + G, grp + 2G, grp + 3G (FP32:
             // in one pass, four independent chains; slots past nf re-read a
             // live row and are dropped)
@@ -369,6 +374,17 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     const uint64_t fm2 = __ballot(fresh2);
                     const uint64_t bm2 = __ballot(bad2);
                     sp_nb[lane] = nb2;
+                    // the next hop's gate inputs: fresh list, factors and MSB rows
+                    // (fresh[], f_fac, f_fv and the stage are idle during the
+                    // insertion; the filter above is done with the stage)
+                    if (fresh2) {
+                        const uint32_t r = (uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull));
+                        fresh[r] = nb2;
+                        f_fac[r] = a.ix.factors[nb2];
+                        f_fv[r] = a.ix.fvalid[nb2];
+                    }
+                    wave_sync_lds();
+                    stage_rows<1>(stage, fresh, (uint32_t)__popcll(fm2), mch, (const uint4 *)a.ix.msb, 0u, lane);
                     if (lane == 0) { s_sp_node = nxt; s_sp_fresh = fm2; s_sp_bad = bm2 != 0ull ? 1u : 0u; }
                 } else if (lane == 0) {
                     s_sp_node = 0xFFFFFFFFu;
