@@ -105,61 +105,6 @@ __device__ __forceinline__ float rem_int8_one(const int8_t *x8p, const uint8_t *
     return h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
 }
 
-// C4: FP32 1-bit gate.
-__device__ __forceinline__ PaperEstimate gate_fp32_one(const float *q, const uint8_t *msb_row, uint32_t cd, float4 f,
-                                                       uint8_t fv, const QScalars &s, float eps0) {
-    const uint4 *m = (const uint4 *)msb_row;
-    float acc[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
-    for (uint32_t c = 0; c < cd / 128; ++c) {
-        const uint4 w = m[c];
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const uint32_t bits = (ws[k] >> (16 * half)) & 0xFFFFu;
-                const float4 *qp = (const float4 *)(q + 128 * c + 32 * k + 16 * half);
-                const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
-                const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
-                                      q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
-#pragma unroll
-                for (int j = 0; j < 16; ++j) acc[j] = acc[j] + masked_q(bits, j, qv[j]);
-            }
-        }
-    }
-    return finish_gate(reduce16(acc) - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
-}
-
-// C5: INT8 1-bit gate (exact masked sum).
-__device__ __forceinline__ PaperEstimate gate_int8_one(const int8_t *x8, const uint8_t *msb_row, uint32_t cd, float4 f,
-                                                       uint8_t fv, const QScalars &s, float eps0) {
-    if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) {
-        PaperEstimate r;
-        r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
-        r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
-        return r;
-    }
-    const uint32_t *xg = (const uint32_t *)x8;
-    const uint4 *m = (const uint4 *)msb_row;
-    int sel = 0;
-    for (uint32_t c = 0; c < cd / 128; ++c) {
-        const uint4 w = m[c];
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-            for (int nib = 0; nib < 8; ++nib) {
-                sel = __builtin_amdgcn_sdot4((int)nibble_bytes((ws[k] >> (4 * nib)) & 0xFu),
-                                             (int)xg[32 * c + 8 * k + nib], sel, false);
-            }
-        }
-    }
-    const float short_ip = 0.5f * s.int8_scale * (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);
-    return finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, eps0);
-}
-
 // E1: residual-4 interval for one candidate.
 __device__ __forceinline__ Interval residual_one(const float *q, const uint16_t *scale_row, const uint8_t *rcode_row,
                                                  uint32_t cd, const QScalars &s, float long_d) {
@@ -370,36 +315,13 @@ __device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t
     return h.x + s.query_norm_sqr - (float)((double)h.y * signed_ip);
 }
 
-// C4 on a staged MSB row: accumulator j sums dims 16m + j in order. qp2
-// holds the query pair-interleaved: qp2[16w + j] = (q[32w + j],
-// q[32w + 16 + j]), the two dims of MSB word w that lane j owns.
-__device__ __forceinline__ PaperEstimate gate_fp32_group(const float2 *qp2, const uint8_t *msb_lds, uint32_t cd,
-                                                        float4 f, uint8_t fv, const QScalars &s, float eps0, int j) {
-    const uint4 *m4 = (const uint4 *)msb_lds;
-    const int j2 = 16 + j;
-    float acc = 0.0f;
-#pragma unroll 2
-    for (uint32_t u = 0; u < cd / 128; ++u) {
-        const uint4 mw = m4[u];
-        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
-        const float2 q2 = qp2[64 * u + 32 + j], q3 = qp2[64 * u + 48 + j];
-        acc = acc + masked_q(mw.x, j, q0.x);
-        acc = acc + masked_q(mw.x, j2, q0.y);
-        acc = acc + masked_q(mw.y, j, q1.x);
-        acc = acc + masked_q(mw.y, j2, q1.y);
-        acc = acc + masked_q(mw.z, j, q2.x);
-        acc = acc + masked_q(mw.z, j2, q2.y);
-        acc = acc + masked_q(mw.w, j, q3.x);
-        acc = acc + masked_q(mw.w, j2, q3.y);
-    }
-    return finish_gate(group_reduce16(acc) - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
-}
-
-// The gate of four candidates per 16-lane group in one pass (MSB rows
-// r0..r3 staged in LDS): four independent accumulation chains, each in the
-// order of gate_fp32_group (so each equals it bit for bit), sharing the
-// query loads. Returns the four lanes' partial sums; the caller reduces and
-// finishes each.
+// C4 on staged MSB rows, four candidates per 16-lane group in one pass (rows
+// r0..r3 in LDS): lane j's accumulator for each candidate sums dims 16m + j
+// in order, as the reference's 16-lane loop does; qp2 holds the query
+// pair-interleaved, qp2[16w + j] = (q[32w + j], q[32w + 16 + j]), the two
+// dims of MSB word w that lane j owns. The four chains are independent and
+// share the query loads. Returns the four lanes' partial sums; the caller
+// reduces (reduce16 order) and finishes each.
 __device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 *r0, const uint4 *r1,
                                                  const uint4 *r2, const uint4 *r3, uint32_t cd, int j,
                                                  float (&acc)[4]) {
@@ -410,15 +332,19 @@ __device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 
         const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
         const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
         const float2 q2 = qp2[64 * u + 32 + j], q3 = qp2[64 * u + 48 + j];
-#define QG_GATE_STEP(A, M)                                                         \
-        A = A + masked_q(M.x, j, q0.x); A = A + masked_q(M.x, j2, q0.y);           \
-        A = A + masked_q(M.y, j, q1.x); A = A + masked_q(M.y, j2, q1.y);           \
-        A = A + masked_q(M.z, j, q2.x); A = A + masked_q(M.z, j2, q2.y);           \
-        A = A + masked_q(M.w, j, q3.x); A = A + masked_q(M.w, j2, q3.y);
-        QG_GATE_STEP(a0, m0)
-        QG_GATE_STEP(a1, m1)
-        QG_GATE_STEP(a2, m2)
-        QG_GATE_STEP(a3, m3)
+        // the four chains advance in turn (masked_q's register barrier keeps
+        // source order, so the interleaving has to be written out)
+#define QG_GATE_STEP(W, JJ, QV)                                                    \
+        a0 = a0 + masked_q(m0.W, JJ, QV); a1 = a1 + masked_q(m1.W, JJ, QV);        \
+        a2 = a2 + masked_q(m2.W, JJ, QV); a3 = a3 + masked_q(m3.W, JJ, QV);
+        QG_GATE_STEP(x, j, q0.x)
+        QG_GATE_STEP(x, j2, q0.y)
+        QG_GATE_STEP(y, j, q1.x)
+        QG_GATE_STEP(y, j2, q1.y)
+        QG_GATE_STEP(z, j, q2.x)
+        QG_GATE_STEP(z, j2, q2.y)
+        QG_GATE_STEP(w, j, q3.x)
+        QG_GATE_STEP(w, j2, q3.y)
 #undef QG_GATE_STEP
     }
     acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;

@@ -286,19 +286,19 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_fp32(ScanArgs a) {
     const uint32_t id = a.ids[item];
     const QScalars s = a.qv.qs[qi];
     const float *qg = a.qv.q + qi * a.cd;
-    const uint4 *m = (const uint4 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    const uint2 *m = (const uint2 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
     float acc[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
-    for (uint32_t c = 0; c < a.cd / 128; ++c) {  // 16 bytes of MSB = 128 dims
-        const uint4 w = m[c];
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (uint32_t c = 0; c < a.cd / 64; ++c) {  // 8 bytes of MSB = 64 dims (code_dim >= 64)
+        const uint2 w = m[c];
+        const uint32_t ws[2] = {w.x, w.y};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 2; ++k) {
 #pragma unroll
             for (int half = 0; half < 2; ++half) {  // one 16-dim group per 16 bits
                 const uint32_t bits = (ws[k] >> (16 * half)) & 0xFFFFu;
-                const float4 *qp = (const float4 *)(qg + 128 * c + 32 * k + 16 * half);
+                const float4 *qp = (const float4 *)(qg + 64 * c + 32 * k + 16 * half);
                 const float4 q4[4] = {qp[0], qp[1], qp[2], qp[3]};
                 const float qv[16] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
                                       q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w};
@@ -327,19 +327,19 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
     r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
     if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) { a.out_est[item] = r; return; }
     const uint32_t *xg = (const uint32_t *)(a.qv.x8 + qi * a.cd);
-    const uint4 *m = (const uint4 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    const uint2 *m = (const uint2 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
     int sel = 0;
-    for (uint32_t c = 0; c < a.cd / 128; ++c) {
-        const uint4 w = m[c];
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (uint32_t c = 0; c < a.cd / 64; ++c) {  // 8 bytes of MSB = 64 dims (code_dim >= 64)
+        const uint2 w = m[c];
+        const uint32_t ws[2] = {w.x, w.y};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 2; ++k) {
 #pragma unroll
             for (int nib = 0; nib < 8; ++nib) {  // 4 dims per MSB nibble -> one dot4
                 const uint32_t bits = (ws[k] >> (4 * nib)) & 0xFu;
                 // expand 4 bits to a byte mask 0x01 per selected dim
                 const uint32_t mask = nibble_bytes(bits);
-                const uint32_t d4 = 128 * c + 32 * k + 4 * nib;  // first dim of this group
+                const uint32_t d4 = 64 * c + 32 * k + 4 * nib;  // first dim of this group
                 sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[d4 / 4], sel, false);
             }
         }

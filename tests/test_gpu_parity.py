@@ -15,7 +15,7 @@ from oracle_bind import Oracle, Ref
 
 pytestmark = pytest.mark.gpu
 
-DIMS = [960, 1536, 100]
+DIMS = [960, 1536, 100, 48]
 
 
 def _setup(gpu, dim, n=384, nq=6, seed=0):
@@ -109,7 +109,7 @@ def test_estimators_fixed_lists(gpu, dim):
         orc.free_query(oq)
 
 
-@pytest.mark.parametrize("dim", [960, 1536])
+@pytest.mark.parametrize("dim", [960, 1536, 100, 48])
 @pytest.mark.parametrize("codec", [0, 3])
 def test_gates_and_remaining_pairs(gpu, dim, codec):
     qg, sp, orc, x, q, c = _setup(gpu, dim)
