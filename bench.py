@@ -293,9 +293,7 @@ def run_graph(args, torch, np, qg):
     ok = [r for r in full if r["recall_at_10"] >= args.target_recall]
     pick = min(ok, key=lambda r: r["ms_per_step"]) if ok else max(full, key=lambda r: r["recall_at_10"])
     head, ids = timed(pick["L"], "full", args.steps, args.warmup)
-    learned_res = None
-    if args.train_queries:
-        learned_res = run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream)
+    learned_res = None  # the learned workload has its own base set (run_learned_workload)
     del x
     torch.cuda.empty_cache()
     reranked = min(100, head["L"])
@@ -349,8 +347,64 @@ def run_graph(args, torch, np, qg):
     }
 
 
+def run_learned_workload(args, torch, np, qg):
+    """The learned adaptive rerank workload: 1M x 1536 normalised power-law
+    vectors (seed 4) with a Vamana graph (M=64, L_build=400, refine 1, cap
+    256) built on the GPU and 1000 queries. The plain search (fused top-100
+    residual rerank) runs at the smallest swept L with Recall@10 >= the
+    target; the learned depth is then trained, calibrated and timed there."""
+    n, dim, nq = args.n, args.dim, args.nq
+    x, q = make_powerlaw(torch, n, dim, nq, 4)
+    gt = exact_topk(torch, x, q, 10)
+    sp = qg.RabitqSpace(dim, 100)
+    sp.set_center(x.double().mean(0).float().cpu().numpy())
+    index = sp.encode(x, batch=131072)
+    t0 = time.perf_counter()
+    deg, edges, _ = sp.build_graph(index, r=args.R, l_build=args.l_build, alpha=1.2, batch=4096,
+                                   refine_passes=args.refine, prune_cap=args.prune_cap)
+    torch.cuda.synchronize()
+    log("learned workload (seed 4): graph built in %.1f s" % (time.perf_counter() - t0))
+    stream = torch.cuda.current_stream()
+    ws = sp.search_workspace(index.n, max(1, min(nq, 4096)))
+
+    def plain(L, steps, warmup):
+        def step():
+            qv = sp.prepare(q, "full")
+            return sp.search(index, qv, deg, edges, k=10, l_search=L, rerank=100, check_graph=False, workspace=ws)
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ids, _, _ = step()
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / steps
+        qg.space.search_error(ws)
+        rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
+        return {"L": L, "ms_per_step": ms, "qps": nq / (ms / 1000.0), "recall_at_10": rec}
+
+    lists = [int(v) for v in args.lists.split(",") if v]
+    L = lists[-1]
+    for cand in lists:
+        if plain(cand, 3, 1)["recall_at_10"] >= args.target_recall:
+            L = cand
+            break
+    head = plain(L, args.steps, args.warmup)
+    log("learned workload (seed 4): plain search L=%d: %.3f ms/step  %.0f q/s  recall@10 %.4f"
+        % (L, head["ms_per_step"], head["qps"], head["recall_at_10"]))
+    res = run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream)
+    res["config"] = {"workload": "learned adaptive rerank depth: %d x %d normalised power-law vectors (seed 4), "
+                                 "Vamana M=%d / L_build=%d built on the GPU, %d queries, FP32 codec, L=%d; "
+                                 "MLP 24->64->64->1 (BF16 operands, FP32 accumulation) trained on the GPU"
+                                 % (n, dim, args.R, args.l_build, nq, L),
+                     "base": n, "dim": dim, "queries": nq, "search_list": L, "seed": 4}
+    del x, index, deg, edges, ws
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
-    """Learned rerank depth on the headline search: train on (estimate,
+    """Learned rerank depth on one search workload: train on (estimate,
     exact) pairs from fresh queries, calibrate the error multiplier on
     held-out queries, then time the pipeline (traversal -> features -> MLP
     -> learned-depth rerank) on the same test queries as the plain search."""
@@ -788,6 +842,8 @@ def main():
         return
 
     graph = run_graph(args, torch, np, qg) if args.workload in ("all", "graph") else None
+    if graph is not None and args.train_queries:
+        graph["learned"] = run_learned_workload(args, torch, np, qg)
     fixed_res = run_fixed(args, torch, np, qg) if args.workload in ("all", "fixed") else None
     batch_res = None
     if args.workload == "batch" or (args.workload == "all" and not args.no_batch):
