@@ -76,6 +76,66 @@ __global__ void __launch_bounds__(64) k_row_seed(const float *__restrict__ rows,
     if (dot_out) dot_out[r] = center ? dot : 0.0;
 }
 
+// Without the hash (every codec but INT8) the two recurrences of a row are
+// independent FP64 add chains. They go to two wavefronts: wave 0 sums
+// ||row||^2 and wave 1 sums <row, c> for the same 64 rows, one row per
+// lane, each in index order. The center is widened to double in LDS once
+// per block. This halves each lane's FP64 issue (FP64 runs at half the FP32
+// rate here), and 64-value chunks keep the next loads in flight meanwhile.
+constexpr uint32_t kNormChunk = 64;
+
+__global__ void __launch_bounds__(128) k_row_norms(const float *__restrict__ rows, uint64_t n, uint32_t dim,
+                                                   const float *__restrict__ center, double *__restrict__ norm_out,
+                                                   double *__restrict__ dot_out) {
+    extern __shared__ __align__(16) double cs[];  // [dim] when center is given
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    if (center)
+        for (uint32_t i = t; i < dim; i += 128) cs[i] = (double)center[i];
+    __syncthreads();
+    const uint64_t r = (uint64_t)blockIdx.x * 64 + lane;
+    const bool live = r < n;
+    if (wv == 1 && !center) {
+        if (live && dot_out) dot_out[r] = 0.0;
+        return;
+    }
+    const float *x = rows + (live ? r : 0) * dim;  // idle lanes walk row 0
+    double acc = 0.0;
+    const uint32_t full = dim / kNormChunk * kNormChunk;
+    float cur[kNormChunk], nxt[kNormChunk];
+    if (full) {
+#pragma unroll
+        for (uint32_t u = 0; u < kNormChunk; ++u) cur[u] = x[u];
+    }
+    for (uint32_t base = 0; base < full; base += kNormChunk) {
+        if (base + kNormChunk < full) {
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) nxt[u] = x[base + kNormChunk + u];
+        }
+        if (wv == 0) {
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) {
+                const double dv = (double)cur[u];
+                acc += dv * dv;  // exact product: ordering of the add is what matters
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) acc += (double)cur[u] * cs[base + u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kNormChunk; ++u) cur[u] = nxt[u];
+    }
+    for (uint32_t i = full; i < dim; ++i) {
+        const double dv = (double)x[i];
+        acc += wv == 0 ? dv * dv : dv * cs[i];
+    }
+    if (!live) return;
+    if (wv == 0) {
+        if (norm_out) norm_out[r] = acc;
+    } else if (dot_out) {
+        dot_out[r] = acc;
+    }
+}
+
 // In-LDS Walsh-Hadamard butterflies, stage step 1, 2, 4, ... (hadamard,
 // space_rabitq.h:944-988). Only add/sub: bit-exact in any schedule.
 __device__ void fht_lds(float *v, uint32_t cd) {
@@ -275,7 +335,8 @@ int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float 
     if (seed)
         hipLaunchKernelGGL(qg::k_row_seed<true>, grid, dim3(64), 0, stream, rows, n, dim, center, seed, norm, dot);
     else
-        hipLaunchKernelGGL(qg::k_row_seed<false>, grid, dim3(64), 0, stream, rows, n, dim, center, seed, norm, dot);
+        hipLaunchKernelGGL(qg::k_row_norms, grid, dim3(128), center ? (size_t)dim * 8 : 0, stream, rows, n, dim,
+                           center, norm, dot);
     return (int)hipGetLastError();
 }
 
