@@ -785,12 +785,144 @@ def run_fixed(args, torch, np, qg):
     return res
 
 
+def run_signscan(args, torch, np, qg):
+    """1-bit sign-plane database x four query codecs (the query-codec scan of
+    experiments/04_query_codec_1bit_scan): 1M x 960 vectors (64-cluster
+    Gaussian mixture, std 0.3, seed 0; GIST1M's shape), database = sign plane
+    of each rotated vector (128 B at code dimension 1024). Reference
+    protocol: each of 10 queries scanned alone over the whole database, one
+    launch per (query, codec), every distance written out. Batched form: the
+    10 queries (and 64) in one launch. The 128 MB database fits the 256 MB
+    Infinity Cache, so launches rotate over three copies (384 MB) and every
+    scan streams from HBM."""
+    from quantization_graph_amd.signscan import SignScan, CODECS
+    n, dim, nq = args.scan_n, 960, 10
+    x, q, _ = make_data(torch, n, dim, nq, 64, 0.3, 0, 0.05)
+    sp = qg.RabitqSpace(dim, 100)
+    ss = SignScan(sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    db = ss.encode(x)
+    torch.cuda.synchronize()
+    encode_s = time.perf_counter() - t0
+    dbs = [db] + [db.clone() for _ in range(2)]
+    stream = torch.cuda.current_stream()
+    pq1 = [ss.prepare(q[i:i + 1]) for i in range(nq)]
+    pq = ss.prepare(q)
+    q64 = torch.cat([q] * 7)[:64].contiguous()
+    pq64 = ss.prepare(q64)
+    out1 = torch.empty((1, n), dtype=torch.float32, device="cuda")
+    outb = torch.empty((64, n), dtype=torch.float32, device="cuda")
+    cd = ss.cd
+
+    def timed(codec, launches):
+        """mean ms per launch over HIP events on the launch stream"""
+        for j in range(3):
+            launches(j)
+        torch.cuda.synchronize()
+        reps = max(3, args.scan_rounds)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for r in range(reps):
+            launches(r)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / reps
+
+    res = {"codecs": {}}
+    for name, codec in CODECS.items():
+        # one query per launch: 10 launches per round, database copy j % 3
+        def single(r, codec=codec):
+            for i in range(nq):
+                ss.scan(codec, dbs[(r * nq + i) % 3], n, pq1[i], out=out1)
+        ms1 = timed(codec, single) / nq
+        ms10 = timed(codec, lambda r, codec=codec: ss.scan(codec, dbs[r % 3], n, pq, out=outb[:nq]))
+        ms64 = timed(codec, lambda r, codec=codec: ss.scan(codec, dbs[r % 3], n, pq64, out=outb))
+        bytes1 = n * (cd // 8 + 4)
+        res["codecs"][name] = {
+            "ms_per_query_scan": ms1, "gdist_per_s": n / (ms1 / 1000.0) / 1e9,
+            "gb_per_s": bytes1 / (ms1 / 1000.0) / 1e9,
+            "batched10_ms_per_launch": ms10, "batched10_gdist_per_s": 10 * n / (ms10 / 1000.0) / 1e9,
+            "batched64_ms_per_launch": ms64, "batched64_gdist_per_s": 64 * n / (ms64 / 1000.0) / 1e9}
+        log("sign scan %-4s: %.4f ms per query (%.1f Gdist/s, %.0f GB/s); 10 queries/launch %.3f ms; "
+            "64 queries/launch %.3f ms (%.1f Gdist/s)" % (name, ms1, n / ms1 / 1e6, bytes1 / ms1 / 1e6, ms10, ms64,
+                                                         64 * n / ms64 / 1e6))
+    b1 = res["codecs"]["b1"]
+    unit_bytes = cd // 8 + 4
+    res["roofline"] = {"bound": "hbm", "achieved": b1["gb_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": b1["gb_per_s"] / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "qg::k_ss_scan_bits<1,16> (B1 query, one query per launch)",
+                       "kernel_ms": b1["ms_per_query_scan"], "bytes_per_unit": unit_bytes, "units_per_launch": n}
+    pmc = os.path.join(ROOT, "profiles", "signscan_b1_1Mx960_pmc.json")
+    if os.path.exists(pmc) and n == 1_000_000:
+        with open(pmc) as f:
+            res["roofline"]["traffic"] = json.load(f).get("traffic_bytes_per_launch")
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ss.prepare(q)
+    torch.cuda.synchronize()
+    res["prepare_ms_per_10_queries"] = 1000.0 * (time.perf_counter() - t0) / 5
+    res["encode_seconds"] = encode_s
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = signscan_cpu_baseline(np, torch, db, n, q, pq1, ss, out1)
+    res["cpu_baseline"] = cpu
+    res["config"] = {"workload": "1-bit sign-plane scan, %d x %d GMM(64, std 0.3, seed 0), %d queries, codecs "
+                                 "fp32/b1/int4/int8, one query per launch (reference protocol) and batched" %
+                                 (n, dim, nq), "code_dim": cd}
+    del x, q, db, dbs, outb
+    torch.cuda.empty_cache()
+    return res
+
+
+def signscan_cpu_baseline(np, torch, db, n, q, pq1, ss, out1):
+    """The reference's own scan (oracle/_ref/libsignscan_ref.so, compiled
+    from query_codec_1bit_scan.cpp) on one host thread, as its benchmark
+    runs it: each query over the whole database, per codec; 3 queries x 4
+    codecs after a warm-up scan, median per codec. The first query's four
+    distance rows are compared with the GPU's bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from oracle_bind import RefSignScan
+    except ImportError:
+        return None
+    if not RefSignScan.available():
+        log("sign scan: oracle/_ref/libsignscan_ref.so not present, no CPU baseline")
+        return None
+    ref = RefSignScan(100)
+    db_h = db.cpu().numpy().view(np.uint64)
+    q_h = q.cpu().numpy()
+    per = {}
+    parity = True
+    for qi in range(3):
+        h = ref.prepare(q_h[qi])[0]
+        for codec in range(4):
+            if qi == 0:
+                ref.scan(codec, db_h, n, h)
+            t0 = time.perf_counter()
+            d_ref = ref.scan(codec, db_h, n, h)
+            per.setdefault(codec, []).append(time.perf_counter() - t0)
+            if qi == 0:
+                d_gpu = ss.scan(codec, db, n, pq1[0], out=out1).cpu().numpy()[0]
+                parity &= bool(np.array_equal(d_gpu.view(np.uint32), d_ref.view(np.uint32)))
+        ref.free(h)
+    names = ("fp32", "b1", "int4", "int8")
+    ms = {names[c]: 1000.0 * float(np.median(v)) for c, v in per.items()}
+    log("sign scan CPU reference (1 thread): " + ", ".join("%s %.2f ms" % (k, v) for k, v in ms.items()) +
+        "; GPU == reference bit for bit: %s" % parity)
+    return {"value": n / (ms["b1"] / 1000.0) / 1e9, "unit": "Gdist/s (B1 codec)", "cores": 1, "kind": "reference",
+            "sample": "3 queries x 4 codecs over all %d vectors of this workload, one host thread, median per "
+                      "codec, the reference's scan compiled from source" % n,
+            "ms_per_query_scan": ms, "gpu_equals_reference": parity}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch"), default="all")
+    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch", "signscan"), default="all")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=1536)
     ap.add_argument("--nq", type=int, default=1000)
@@ -818,6 +950,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-single", type=int, default=256, help="queries timed with one launch each")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching workload in --workload all")
+    ap.add_argument("--scan-n", type=int, default=1_000_000, help="vectors of the 1-bit sign-plane scan workload")
+    ap.add_argument("--scan-rounds", type=int, default=20, help="timed rounds per sign-scan configuration")
+    ap.add_argument("--no-signscan", action="store_true", help="skip the sign-scan workload in --workload all")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (a plain copy kernel of known size, "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -848,6 +983,9 @@ def main():
     batch_res = None
     if args.workload == "batch" or (args.workload == "all" and not args.no_batch):
         batch_res = run_batch(args, torch, np, qg)
+    scan_res = None
+    if args.workload == "signscan" or (args.workload == "all" and not args.no_signscan):
+        scan_res = run_signscan(args, torch, np, qg)
 
     metric = "QPS at Recall@10>=0.95, learned vs plain rerank; Gdist/s vs HBM roofline"
     if graph is not None:
@@ -876,6 +1014,16 @@ def main():
             "learned": graph["learned"],
             "fixed_candidates": fixed_res,
             "batching": batch_res,
+            "sign_scan": scan_res,
+        }
+    elif fixed_res is None and batch_res is None:
+        r = scan_res
+        out = {
+            "metric": "Gdist/s of the 1-bit sign-plane scan (B1 codec, one query per launch)",
+            "value": r["codecs"]["b1"]["gdist_per_s"], "unit": "Gdist/s", "n_gpus": 1, "steps": args.scan_rounds,
+            "warmup": 3, "ms_per_step": r["codecs"]["b1"]["ms_per_query_scan"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": r["config"],
+            "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"], "sign_scan": r,
         }
     elif fixed_res is None:
         b = batch_res

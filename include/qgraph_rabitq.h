@@ -259,6 +259,41 @@ size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k
 int qg_train_centroids(const float *samples, uint64_t sample_count, uint64_t dim, uint32_t k, int iters,
                        float *centroids, void *workspace, size_t workspace_bytes, void *stream);
 
+/* ------------------------------------------------ 1-bit sign-plane scan
+ * The fixed 1-bit database scanned with four query codecs of
+ * experiments/04_query_codec_1bit_scan/query_codec_1bit_scan.cpp, general in
+ * the code dimension (the reference fixes 960 -> 1024) and batched over
+ * queries (the reference scans one query at a time, :868-925). Bit i of a
+ * database vector is set when coordinate i of its rotation is > 0; the
+ * rotation is the space's (signs, zero padding, unnormalised Walsh-Hadamard;
+ * no centering, no scaling; seed 0 is the identity). The database is stored
+ * in blocks of 32 vectors, word-major inside a block:
+ * db[(v / 32 * W + w) * 32 + v % 32], W = code_dim / 64 u64 words
+ * (build_or_load_database, :341-384), so n must be a multiple of 32. */
+enum { QG_SIGNSCAN_FP32 = 0, QG_SIGNSCAN_B1 = 1, QG_SIGNSCAN_INT4 = 2, QG_SIGNSCAN_INT8 = 3 }; /* Codec, :735 */
+#define QG_SIGNSCAN_BATCH 32
+#define QG_SIGNSCAN_ROWS 13
+/* Scalars of QueryRepresentations (:405-425). */
+typedef struct qg_signscan_query {
+    float fp32_sum, fp32_norm2, b1_scale, b1_constant;
+    float int4_scale, int4_constant, int8_scale, int8_constant;
+    int64_t int4_sum, int8_sum;
+} qg_signscan_query;
+/* Database build of n vectors (device [n][dim]) into db (device,
+ * n * code_dim / 64 u64); replaces build_or_load_database (:341-384). */
+int qg_signscan_encode(const void *space, const float *x, uint64_t n, uint64_t *db, void *stream);
+/* prepare_query (:458-492) of nq queries (device [nq][dim]): q [nq][code_dim]
+ * the rotated FP32 query, bits [nq][13][W] (row 0 the B1 sign code, rows 1-4
+ * the INT4 planes, rows 5-12 the INT8 planes, plane 0 first, two's
+ * complement), scalars [nq]. */
+int qg_signscan_prepare(const void *space, const float *queries, uint64_t nq, float *q, uint64_t *bits,
+                        qg_signscan_query *scalars, void *stream);
+/* scan_fp32 / scan_b1 / scan_integer<4|8> (:529-648; codec QG_SIGNSCAN_*)
+ * of nq prepared queries against all n database vectors in one launch:
+ * out [nq][n] distances. q is read by the FP32 codec, bits by the others. */
+int qg_signscan(const void *space, int codec, const uint64_t *db, uint64_t n, const float *q, const uint64_t *bits,
+                const qg_signscan_query *scalars, uint64_t nq, float *out, void *stream);
+
 /* ------------------------------------------------------------ bridge API */
 /* Host-memory drop-ins for rabitq_bridge.cpp; see that file for semantics. */
 

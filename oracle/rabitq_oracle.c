@@ -1333,3 +1333,127 @@ int ro_train_centroids(const float *data, size_t count, size_t dim, size_t k, si
     free(sample); free(cen); free(next); free(cnt);
     return 0;
 }
+
+/* ------------------------------------------------ 1-bit sign-plane scan */
+
+/* build_or_load_database, query_codec_1bit_scan.cpp:341-384: bit b of word
+ * w is set when rotated coordinate w*64+b is > 0. */
+int ro_ss_encode(const ro_space *s, const float *x, uint64_t n, uint64_t *db) {
+    const size_t cd = s->code_dim, W = cd / 64;
+    if (cd % 64 != 0 || n % RO_SS_BATCH != 0) return -1;
+    float *rot = (float *)malloc(cd * sizeof(float));
+    for (uint64_t row = 0; row < n; ++row) {
+        ro_rotate(s, x + row * s->dim, rot);
+        const uint64_t block = row / RO_SS_BATCH, lane = row % RO_SS_BATCH;
+        for (size_t w = 0; w < W; ++w) {
+            uint64_t bits = 0;
+            for (size_t b = 0; b < 64; ++b)
+                if (rot[w * 64 + b] > 0.0f) bits |= (uint64_t)1 << b;
+            db[(block * W + w) * RO_SS_BATCH + lane] = bits;
+        }
+    }
+    free(rot);
+    return 0;
+}
+
+/* make_integer_query<Bits>, :427-456: max-abs scale, round-to-nearest-even
+ * codes clamped to +-(2^(Bits-1) - 1), two's-complement bit planes. */
+static void ss_integer_query(const float *q, size_t cd, int bits, uint64_t *planes, float *scale_out,
+                             float *constant_out, int64_t *sum_out) {
+    const int qmax = (1 << (bits - 1)) - 1;
+    const size_t W = cd / 64;
+    float max_abs = 0.0f;
+    for (size_t i = 0; i < cd; ++i) {
+        const float a = fabsf(q[i]);
+        max_abs = max_abs < a ? a : max_abs;
+    }
+    const float scale = max_abs > 0.0f ? max_abs / (float)qmax : 1.0f;
+    memset(planes, 0, (size_t)bits * W * sizeof(uint64_t));
+    int64_t sum = 0, norm2 = 0;
+    for (size_t i = 0; i < cd; ++i) {
+        long c = lrintf(q[i] / scale);
+        if (c < -qmax) c = -qmax;
+        if (c > qmax) c = qmax;
+        sum += c;
+        norm2 += (int64_t)c * c;
+        const uint8_t enc = (uint8_t)(int8_t)c;
+        for (int p = 0; p < bits; ++p)
+            if ((enc >> p) & 1u) planes[(size_t)p * W + i / 64] |= (uint64_t)1 << (i % 64);
+    }
+    *scale_out = scale;
+    *constant_out = madd_f(RO_FMA_SS_INTCONST, scale * scale, (float)norm2, (float)cd);
+    *sum_out = sum;
+}
+
+/* prepare_query, :458-492: the rotation, then in index order the FP32 sum
+ * and squared norm, the B1 sign code with its mean-|q| scale, and the INT4
+ * and INT8 bit planes. */
+void ro_ss_prepare(const ro_space *s, const float *raw, float *q, uint64_t *bits, ro_ss_query *out) {
+    const size_t cd = s->code_dim, W = cd / 64;
+    memset(out, 0, sizeof(*out));
+    ro_rotate(s, raw, q);
+    float sum = 0.0f, norm2 = 0.0f, abs_sum = 0.0f;
+    for (size_t i = 0; i < cd; ++i) {
+        sum += q[i];
+        norm2 = madd_f(RO_FMA_SS_NORM2, q[i], q[i], norm2);
+    }
+    memset(bits, 0, W * sizeof(uint64_t));
+    for (size_t i = 0; i < cd; ++i) {
+        if (q[i] > 0.0f) bits[i / 64] |= (uint64_t)1 << (i % 64);
+        abs_sum += fabsf(q[i]);
+    }
+    out->fp32_sum = sum;
+    out->fp32_norm2 = norm2;
+    out->b1_scale = abs_sum / (float)cd;
+    out->b1_constant = (float)cd * madd_f(RO_FMA_SS_B1CONST, out->b1_scale, out->b1_scale, 1.0f);
+    ss_integer_query(q, cd, 4, bits + W, &out->int4_scale, &out->int4_constant, &out->int4_sum);
+    ss_integer_query(q, cd, 8, bits + 5 * W, &out->int8_scale, &out->int8_constant, &out->int8_sum);
+}
+
+static uint64_t ss_word(const uint64_t *db, size_t W, uint64_t cand, size_t w) {
+    return db[((cand / RO_SS_BATCH) * W + w) * RO_SS_BATCH + cand % RO_SS_BATCH];
+}
+
+/* scan_fp32 / scan_b1 / scan_integer<Bits>, :529-648. FP32: 16 lane
+ * accumulators, lane j summing the query values of the set bits among
+ * dimensions j, j+16, j+32, ... in order, then the _mm512_reduce_add_ps
+ * tree; B1: XOR popcount; INT4/INT8: per-plane AND popcounts weighted by
+ * the two's-complement plane weights (exact integers). */
+void ro_ss_scan(const ro_space *s, int codec, const uint64_t *db, uint64_t n, const float *q,
+                const uint64_t *bits, const ro_ss_query *qs, float *out) {
+    const size_t cd = s->code_dim, W = cd / 64;
+    const uint64_t count = (n / RO_SS_BATCH) * RO_SS_BATCH;
+    for (uint64_t c = 0; c < count; ++c) {
+        if (codec == 0) {
+            float acc[16] = {0};
+            for (size_t w = 0; w < W; ++w) {
+                const uint64_t word = ss_word(db, W, c, w);
+                for (size_t b = 0; b < 64; ++b)
+                    if ((word >> b) & 1u) acc[b % 16] += q[w * 64 + b];
+            }
+            const float selected = reduce16(acc);
+            const float dot = 2.0f * selected - qs->fp32_sum;
+            out[c] = ((float)cd + qs->fp32_norm2) - 2.0f * dot;
+        } else if (codec == 1) {
+            uint64_t mism = 0;
+            for (size_t w = 0; w < W; ++w) mism += (uint64_t)__builtin_popcountll(ss_word(db, W, c, w) ^ bits[w]);
+            const float signed_dot = (float)cd - 2.0f * (float)mism;
+            out[c] = madd_f(RO_FMA_SS_B1DIST, -(2.0f * qs->b1_scale), signed_dot, qs->b1_constant);
+        } else {
+            const int B = codec == 2 ? 4 : 8;
+            const uint64_t *planes = bits + (codec == 2 ? 1 : 5) * W;
+            int32_t selected = 0;
+            for (int p = 0; p < B; ++p) {
+                const int32_t weight = p + 1 == B ? -(1 << (B - 1)) : (1 << p);
+                int32_t cnt = 0;
+                for (size_t w = 0; w < W; ++w)
+                    cnt += __builtin_popcountll(ss_word(db, W, c, w) & planes[(size_t)p * W + w]);
+                selected += cnt * weight;
+            }
+            const int64_t dot = 2LL * selected - (codec == 2 ? qs->int4_sum : qs->int8_sum);
+            const float scale = codec == 2 ? qs->int4_scale : qs->int8_scale;
+            const float constant = codec == 2 ? qs->int4_constant : qs->int8_constant;
+            out[c] = madd_f(RO_FMA_SS_INTDIST, -(2.0f * scale), (float)dot, constant);
+        }
+    }
+}

@@ -43,10 +43,16 @@ extern "C" {
 #define RO_FMA_RES_ACC     (1u << 11) /* residual: ips += scale*block_ip                  */
 #define RO_FMA_PAPER_LOWER (1u << 12) /* record gate: (n + q) - two_norms*ip_upper         */
 #define RO_FMA_KMEANS_DIST (1u << 13) /* k-means: d += diff*diff                          */
+#define RO_FMA_SS_NORM2    (1u << 14) /* sign scan query: norm2 += v*v                    */
+#define RO_FMA_SS_B1CONST  (1u << 15) /* sign scan query: cd*(1 + s*s)                    */
+#define RO_FMA_SS_INTCONST (1u << 16) /* sign scan query: cd + (scale*scale)*code_norm2   */
+#define RO_FMA_SS_B1DIST   (1u << 17) /* sign scan B1: constant - (2*scale)*signed_dot     */
+#define RO_FMA_SS_INTDIST  (1u << 18) /* sign scan INT4/INT8: constant - (2*scale)*dot     */
 /* The pattern the reference compiler produces (g++ 11, -O3, AVX-512). */
 #define RO_FMA_REFERENCE (RO_FMA_FQA_T | RO_FMA_ENC_OOBAR | RO_FMA_ENC_DECNORM | RO_FMA_ENC_X4IP | \
                           RO_FMA_C1_LANE | RO_FMA_C1_SUB7 | RO_FMA_C1_EST | RO_FMA_RES_HI | \
-                          RO_FMA_RES_ACC | RO_FMA_PAPER_LOWER | RO_FMA_KMEANS_DIST)
+                          RO_FMA_RES_ACC | RO_FMA_PAPER_LOWER | RO_FMA_KMEANS_DIST | \
+                          RO_FMA_SS_B1CONST | RO_FMA_SS_INTCONST | RO_FMA_SS_B1DIST | RO_FMA_SS_INTDIST)
 
 extern unsigned ro_fma_mask;
 void ro_set_fma_mask(unsigned mask);
@@ -197,6 +203,29 @@ size_t ro_robust_prune(const ro_space *s, const uint8_t *compact_base, size_t co
 int ro_build_vamana(const ro_space *s, const uint8_t *compact_base, size_t compact_stride, size_t n, size_t r,
                     size_t l_build, float alpha, size_t beam, size_t batch, size_t refine, size_t cap,
                     size_t early_stop, uint32_t *degrees, uint32_t *edges, uint64_t *distance_evaluations);
+
+/* ------------------------------------------------ 1-bit sign-plane scan
+ * experiments/04_query_codec_1bit_scan/query_codec_1bit_scan.cpp, general
+ * in the code dimension (the reference fixes 960 -> 1024). The database is
+ * the sign plane of each vector's rotation (ro_rotate of the space: signs,
+ * padding, Walsh-Hadamard; no center, no scaling) in blocks of 32 vectors,
+ * word-major inside a block: db[(block * W + word) * 32 + lane], W = cd/64.
+ * A query carries its rotation q[cd], 13 bit rows of W words (row 0 the B1
+ * sign code, rows 1-4 the INT4 planes, rows 5-12 the INT8 planes) and the
+ * scalars below. Codecs: 0 FP32, 1 B1, 2 INT4, 3 INT8. */
+typedef struct ro_ss_query {
+    float fp32_sum, fp32_norm2, b1_scale, b1_constant;
+    float int4_scale, int4_constant, int8_scale, int8_constant;
+    int64_t int4_sum, int8_sum;
+} ro_ss_query;
+#define RO_SS_BATCH 32
+#define RO_SS_ROWS 13
+/* n must be a multiple of 32. */
+int ro_ss_encode(const ro_space *s, const float *x, uint64_t n, uint64_t *db);
+void ro_ss_prepare(const ro_space *s, const float *raw, float *q, uint64_t *bits, ro_ss_query *out);
+/* distances of the first (n / 32) * 32 database vectors */
+void ro_ss_scan(const ro_space *s, int codec, const uint64_t *db, uint64_t n, const float *q,
+                const uint64_t *bits, const ro_ss_query *qs, float *out);
 
 #ifdef __cplusplus
 }

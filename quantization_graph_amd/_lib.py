@@ -88,6 +88,9 @@ def lib():
     L.qg_rerank_features.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), vp, vp, u32, u32, vp, vp, vp]
     L.qg_learned_rerank.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, u32, u32, vp,
                                     f32, u32, u32, vp, vp, vp, vp]
+    L.qg_signscan_encode.argtypes = [vp, vp, u64, vp, vp]
+    L.qg_signscan_prepare.argtypes = [vp, vp, u64, vp, vp, vp, vp]
+    L.qg_signscan.argtypes = [vp, i32, vp, u64, vp, vp, vp, u64, vp, vp]
     _lib = L
     return L
 

@@ -38,6 +38,12 @@ int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st);
 size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k);
 int qg_launch_kmeans(const float *samples, uint64_t sc, uint32_t dim, uint32_t k, int iters, float *centroids,
                      void *workspace, hipStream_t stream);
+int qg_launch_ss_encode(const float *x, uint64_t n, uint32_t dim, uint32_t cd, int identity, const float *signs,
+                        uint64_t *db, hipStream_t st);
+int qg_launch_ss_prepare(const float *raw, uint64_t nq, uint32_t dim, uint32_t cd, int identity, const float *signs,
+                         float *q_out, uint64_t *bits_out, qg::SsQuery *qs_out, hipStream_t st);
+int qg_launch_ss_scan(int codec, const uint64_t *db, uint64_t n, uint32_t W, const float *q, const uint64_t *bits,
+                      const qg::SsQuery *qs, uint64_t nq, float *out, hipStream_t st);
 }
 
 namespace qg {
@@ -569,6 +575,43 @@ int qg_train_centroids(const float *samples, uint64_t sample_count, uint64_t dim
     return check_hip(qg_launch_kmeans(samples, sample_count, (uint32_t)dim, k, iters, centroids, workspace,
                                       (hipStream_t)stream),
                      "k-means launch");
+}
+
+static_assert(sizeof(qg_signscan_query) == sizeof(SsQuery), "qg_signscan_query layout");
+
+int qg_signscan_encode(const void *space, const float *x, uint64_t n, uint64_t *db, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !x || !db) return fail("null argument to qg_signscan_encode");
+    if (n % QG_SIGNSCAN_BATCH != 0) return fail("sign-scan databases hold whole blocks of 32 vectors (n = %llu)",
+                                                (unsigned long long)n);
+    if (n / QG_SIGNSCAN_BATCH > 0x7FFFFFFFull / QG_SIGNSCAN_BATCH) return fail("n too large for one launch");
+    return check_hip(qg_launch_ss_encode(x, n, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0, s->d_signs, db,
+                                         (hipStream_t)stream),
+                     "sign-scan encode launch");
+}
+
+int qg_signscan_prepare(const void *space, const float *queries, uint64_t nq, float *q, uint64_t *bits,
+                        qg_signscan_query *scalars, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !queries || !q || !bits || !scalars) return fail("null argument to qg_signscan_prepare");
+    if (nq > 0x7FFFFFFFull) return fail("nq too large for one launch");
+    return check_hip(qg_launch_ss_prepare(queries, nq, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0, s->d_signs,
+                                          q, bits, reinterpret_cast<SsQuery *>(scalars), (hipStream_t)stream),
+                     "sign-scan prepare launch");
+}
+
+int qg_signscan(const void *space, int codec, const uint64_t *db, uint64_t n, const float *q, const uint64_t *bits,
+                const qg_signscan_query *scalars, uint64_t nq, float *out, void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !db || !scalars || !out) return fail("null argument to qg_signscan");
+    if (codec < QG_SIGNSCAN_FP32 || codec > QG_SIGNSCAN_INT8) return fail("unknown sign-scan codec %d", codec);
+    if (codec == QG_SIGNSCAN_FP32 ? !q : !bits) return fail("null query rows for sign-scan codec %d", codec);
+    if (n % QG_SIGNSCAN_BATCH != 0) return fail("sign-scan databases hold whole blocks of 32 vectors (n = %llu)",
+                                                (unsigned long long)n);
+    if (n > 0x7FFFFFFFull * 64ull) return fail("n too large for one launch");
+    return check_hip(qg_launch_ss_scan(codec, db, n, (uint32_t)(s->cd / 64), q, bits,
+                                       reinterpret_cast<const SsQuery *>(scalars), nq, out, (hipStream_t)stream),
+                     "sign-scan launch");
 }
 
 }  // extern "C"

@@ -24,6 +24,15 @@ struct PrepParams {
     QScalars *qs_out;          // [nq]
 };
 
+// Scalars of one prepared sign-scan query (qg_signscan_query; the scalar
+// fields of QueryRepresentations, query_codec_1bit_scan.cpp:405-425).
+struct SsQuery {
+    float fp32_sum, fp32_norm2, b1_scale, b1_constant;
+    float int4_scale, int4_constant, int8_scale, int8_constant;
+    int64_t int4_sum, int8_sum;
+};
+static_assert(sizeof(SsQuery) == 48, "SsQuery layout");
+
 struct ScanArgs {
     IndexView ix;
     QueryView qv;

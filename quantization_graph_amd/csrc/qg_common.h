@@ -73,6 +73,23 @@ struct Interval {
     float upper_bound;
 };
 
+// In-LDS Walsh-Hadamard butterflies of the whole workgroup, stage step 1,
+// 2, 4, ... (hadamard, space_rabitq.h:944-988; the 1-bit scan's copy,
+// query_codec_1bit_scan.cpp:248-270, has the same per-element steps). Only
+// add/sub: bit-exact in any schedule. Ends with a barrier.
+__device__ inline void fht_lds(float *v, uint32_t cd) {
+    for (uint32_t step = 1; step < cd; step <<= 1) {
+        for (uint32_t b = threadIdx.x; b < cd / 2; b += blockDim.x) {
+            const uint32_t blk = (b / step) * (step << 1);
+            const uint32_t i = blk + (b % step);
+            const float a = v[i], c = v[i + step];
+            v[i] = a + c;
+            v[i + step] = a - c;
+        }
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ float reduce16(const float *v) {
     float t8[8], t4[4];
 #pragma unroll

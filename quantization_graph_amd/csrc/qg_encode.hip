@@ -54,19 +54,6 @@ __global__ void k_enc_norm(EncArgs a) {
     a.meta[v].rn = sqrtf(s);
 }
 
-__device__ void fht_lds_enc(float *v, uint32_t cd) {
-    for (uint32_t step = 1; step < cd; step <<= 1) {
-        for (uint32_t b = threadIdx.x; b < cd / 2; b += blockDim.x) {
-            const uint32_t blk = (b / step) * (step << 1);
-            const uint32_t i = blk + (b % step);
-            const float p = v[i], q = v[i + step];
-            v[i] = p + q;
-            v[i + step] = p - q;
-        }
-        __syncthreads();
-    }
-}
-
 __device__ __forceinline__ int init_code(double t_start, float a) {
     int c = (int)fma(t_start, (double)a, kEps);
     return c < 7 ? c : 7;
@@ -103,7 +90,7 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_rotate(EncArgs a) {
     if (t == 0) s_max_bits = 0;
     if (t < 7) { s_start[t] = 0x7FFFFFFF; s_end[t] = 0; }
     __syncthreads();
-    if (!a.identity) fht_lds_enc(v, cd);
+    if (!a.identity) fht_lds(v, cd);
     float m = 0.0f;
     for (uint32_t i = t; i < cd; i += blockDim.x) {
         const float r = v[i];

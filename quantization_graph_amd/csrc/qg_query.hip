@@ -136,21 +136,6 @@ __global__ void __launch_bounds__(128) k_row_norms(const float *__restrict__ row
     }
 }
 
-// In-LDS Walsh-Hadamard butterflies, stage step 1, 2, 4, ... (hadamard,
-// space_rabitq.h:944-988). Only add/sub: bit-exact in any schedule.
-__device__ void fht_lds(float *v, uint32_t cd) {
-    for (uint32_t step = 1; step < cd; step <<= 1) {
-        for (uint32_t b = threadIdx.x; b < cd / 2; b += blockDim.x) {
-            const uint32_t blk = (b / step) * (step << 1);
-            const uint32_t i = blk + (b % step);
-            const float a = v[i], c = v[i + step];
-            v[i] = a + c;
-            v[i + step] = a - c;
-        }
-        __syncthreads();
-    }
-}
-
 __device__ __forceinline__ uint64_t mt64_temper(uint64_t x) {
     x ^= (x >> 29) & 0x5555555555555555ull;
     x ^= (x << 17) & 0x71D67FFFEDA60000ull;

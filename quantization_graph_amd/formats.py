@@ -9,6 +9,13 @@
   frozen start points}, then for every node, frozen points last, a u32
   neighbour count followed by the u32 neighbour ids.
 
+* the 1-bit sign-plane database cache of the query-codec scan
+  (experiments/04_query_codec_1bit_scan/query_codec_1bit_scan.cpp:226-339):
+  a 64-byte header {char magic[16] = "GIST1M1BITBLK1", u32 version = 1,
+  u32 input dim, u32 code dim, u32 batch = 32, u64 vector count, u64 size
+  of the source fvecs file, u64 seed, u64 payload bytes}, then the u64 words
+  in blocks of 32 vectors, word-major inside a block.
+
 Readers memory-map the files, so a 1M x 1536 base set is not copied twice.
 """
 from __future__ import annotations
@@ -20,6 +27,8 @@ from pathlib import Path
 import numpy as np
 
 GRAPH_HEADER_BYTES = 24
+SIGN_CACHE_MAGIC = b"GIST1M1BITBLK1"
+SIGN_CACHE_HEADER = struct.Struct("<16sIIIIQQQQ")
 
 
 def _vecs_shape(path, itemsize=4):
@@ -152,3 +161,40 @@ def graph_edge_count(path):
     """Out-edges over every list of the file (count_diskann_graph_edges,
     diskann_runner.rs:1814-1832)."""
     return int(read_graph(path)[0].astype(np.int64).sum())
+
+
+def write_sign_cache(path, db, input_dim, code_dim, data_count, source_size, seed):
+    """Writes a sign-plane database (u64 words in the blocked layout) as the
+    reference's cache file (save_cache, :314-339)."""
+    words = np.ascontiguousarray(db).view(np.uint64).reshape(-1)
+    if words.size != data_count * (code_dim // 64):
+        raise ValueError("payload holds %d words, expected %d" % (words.size, data_count * (code_dim // 64)))
+    head = SIGN_CACHE_HEADER.pack(SIGN_CACHE_MAGIC, 1, input_dim, code_dim, 32, data_count, source_size, seed,
+                                  words.nbytes)
+    with open(path, "wb") as f:
+        f.write(head)
+        words.tofile(f)
+
+
+def read_sign_cache(path, mmap=True):
+    """(header dict, u64 words) of a sign-plane database cache file; raises
+    on the checks of load_cache (:291-312) that do not need the source
+    file: magic, version, batch and payload size."""
+    with open(path, "rb") as f:
+        raw = f.read(SIGN_CACHE_HEADER.size)
+    if len(raw) != SIGN_CACHE_HEADER.size:
+        raise ValueError("%s: truncated header" % path)
+    magic, version, input_dim, code_dim, batch, count, source_size, seed, payload = SIGN_CACHE_HEADER.unpack(raw)
+    if magic[:15] != SIGN_CACHE_MAGIC + b"\0" or version != 1 or batch != 32:
+        raise ValueError("%s: not a version-1 sign-plane cache" % path)
+    if code_dim % 64 or payload != count * (code_dim // 64) * 8:
+        raise ValueError("%s: payload size does not match the header" % path)
+    if os.path.getsize(path) < SIGN_CACHE_HEADER.size + payload:
+        raise ValueError("%s: truncated payload" % path)
+    head = dict(input_dim=input_dim, code_dim=code_dim, batch=batch, data_count=count, source_size=source_size,
+                seed=seed, payload_bytes=payload)
+    if mmap:
+        words = np.memmap(path, dtype=np.uint64, mode="r", offset=SIGN_CACHE_HEADER.size, shape=(payload // 8,))
+    else:
+        words = np.fromfile(path, dtype=np.uint64, offset=SIGN_CACHE_HEADER.size, count=payload // 8)
+    return head, words

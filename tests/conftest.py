@@ -38,3 +38,10 @@ def gpu():
 def golden():
     """Vectors written by the reference bridge (tests/golden/make_golden.py)."""
     return np.load(os.path.join(ROOT, "tests", "golden", "ref_vectors.npz"))
+
+
+@pytest.fixture(scope="session")
+def golden_ss():
+    """Vectors written by the reference's 1-bit query-codec scan
+    (tests/golden/make_signscan_golden.py)."""
+    return np.load(os.path.join(ROOT, "tests", "golden", "signscan_ref.npz"))

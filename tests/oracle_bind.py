@@ -16,13 +16,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "librabitq_ref.so")
+SIGNSCAN_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsignscan_ref.so")
 
 # Contraction sites of the reference compiler, pinned against oracle/_ref by
 # tests/test_oracle_vs_ref.py (bit names follow oracle/rabitq_oracle.h).
 FMA = dict(ENC_NORM=1 << 0, FQA_T=1 << 1, ENC_OOBAR=1 << 2, ENC_DECNORM=1 << 3,
            ENC_X4IP=1 << 4, C1_LANE=1 << 5, C1_SUB7=1 << 6, C1_EST=1 << 7,
            GATE_UPPER=1 << 8, RES_HI=1 << 9, RES_LO=1 << 10, RES_ACC=1 << 11,
-           PAPER_LOWER=1 << 12, KMEANS_DIST=1 << 13)
+           PAPER_LOWER=1 << 12, KMEANS_DIST=1 << 13, SS_NORM2=1 << 14, SS_B1CONST=1 << 15,
+           SS_INTCONST=1 << 16, SS_B1DIST=1 << 17, SS_INTDIST=1 << 18)
 
 
 class ROSpace(C.Structure):
@@ -42,6 +44,17 @@ class ROQuery(C.Structure):
                 ("half_sum", C.c_float), ("sum", C.c_float), ("positive_sum", C.c_float),
                 ("int8_scale", C.c_float), ("int8_error_norm", C.c_float),
                 ("int8_code_sum", C.c_int64)]
+
+
+class SSQuery(C.Structure):
+    """Scalars of one sign-scan query (ro_ss_query / qg_signscan_query), 48 bytes."""
+    _fields_ = [("fp32_sum", C.c_float), ("fp32_norm2", C.c_float), ("b1_scale", C.c_float),
+                ("b1_constant", C.c_float), ("int4_scale", C.c_float), ("int4_constant", C.c_float),
+                ("int8_scale", C.c_float), ("int8_constant", C.c_float), ("int4_sum", C.c_int64),
+                ("int8_sum", C.c_int64)]
+
+
+SS_ROWS = 13  # bit rows per query: B1 code, 4 INT4 planes, 8 INT8 planes
 
 
 class Estimate(C.Structure):
@@ -226,6 +239,28 @@ class Oracle:
 
     def qx8(self, q):
         return np.ctypeslib.as_array(q.x8, (self.cd,)).copy()
+
+    # ---- 1-bit sign-plane scan (query_codec_1bit_scan.cpp restated)
+    def ss_encode(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.dim)
+        db = np.zeros(X.shape[0] * (self.cd // 64), np.uint64)
+        if self.lib.ro_ss_encode(self.sp, _p(X), C.c_uint64(X.shape[0]), _p(db)) != 0:
+            raise ValueError("ro_ss_encode rejected n=%d" % X.shape[0])
+        return db
+
+    def ss_prepare(self, raw):
+        raw = np.ascontiguousarray(raw, np.float32)
+        q = np.zeros(self.cd, np.float32)
+        bits = np.zeros(SS_ROWS * (self.cd // 64), np.uint64)
+        qs = SSQuery()
+        self.lib.ro_ss_prepare(self.sp, _p(raw), _p(q), _p(bits), C.byref(qs))
+        return q, bits, qs
+
+    def ss_scan(self, codec, db, n, q, bits, qs):
+        out = np.zeros(n, np.float32)
+        self.lib.ro_ss_scan(self.sp, C.c_int(codec), _p(db), C.c_uint64(n), _p(q), _p(bits), C.byref(qs),
+                            _p(out))
+        return out
 
     def asym(self, q, comp):
         return np.array([self.lib.ro_asym_distance(self.sp, C.byref(q), _p(comp[i]))
@@ -652,3 +687,87 @@ class RefSearch:
         if rc != 0:
             raise RuntimeError("reference search failed")
         return out
+
+
+class RefSignScan:
+    """The reference's 1-bit query-codec scan
+    (experiments/04_query_codec_1bit_scan/query_codec_1bit_scan.cpp) compiled
+    from source behind oracle/sign_scan_ref.cpp. Fixed at 960 -> 1024."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(SIGNSCAN_REF_SO)
+
+    def __init__(self, seed=100):
+        L = C.CDLL(SIGNSCAN_REF_SO)
+        L.qcs_input_dim.restype = C.c_size_t
+        L.qcs_code_dim.restype = C.c_size_t
+        L.qcs_signs.argtypes = [C.c_uint32, C.c_void_p]
+        L.qcs_build_database.argtypes = [C.c_char_p, C.c_uint64, C.c_uint32, C.c_void_p]
+        L.qcs_build_database_cached.argtypes = [C.c_char_p, C.c_uint64, C.c_uint32, C.c_char_p, C.c_void_p]
+        L.qcs_prepare.restype = C.c_void_p
+        L.qcs_prepare.argtypes = [C.c_void_p, C.c_uint32]
+        L.qcs_query_fields.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.qcs_query_free.argtypes = [C.c_void_p]
+        L.qcs_scan.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+        L.qcs_validate.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        self.lib, self.seed = L, seed
+        self.dim, self.cd = L.qcs_input_dim(), L.qcs_code_dim()
+        self.W = self.cd // 64
+
+    def signs(self):
+        out = np.zeros(self.cd, np.float32)
+        self.lib.qcs_signs(self.seed, _p(out))
+        return out
+
+    def write_fvecs(self, X, tmpdir):
+        X = np.ascontiguousarray(X, np.float32)
+        path = os.path.join(str(tmpdir), "qcs_base.fvecs")
+        rows = np.empty((X.shape[0], 1 + self.dim), np.float32)
+        rows[:, 0] = np.array([self.dim], np.int32).view(np.float32)[0]
+        rows[:, 1:] = X
+        rows.tofile(path)
+        return path
+
+    def build_database(self, X, tmpdir, cache=None):
+        """The reference's database build over X written as an fvecs file;
+        with cache, through the reference's cache file at that path (loaded
+        when its header matches, written otherwise)."""
+        n = X.shape[0]
+        path = self.write_fvecs(X, tmpdir)
+        db = np.zeros(n * self.W, np.uint64)
+        if cache is None:
+            rc = self.lib.qcs_build_database(path.encode(), n, self.seed, _p(db))
+        else:
+            rc = self.lib.qcs_build_database_cached(path.encode(), n, self.seed, str(cache).encode(), _p(db))
+        if rc != 0:
+            raise RuntimeError("reference database build failed")
+        return db
+
+    def prepare(self, raw):
+        """Returns (handle, q, bits [13*W], scalars dict) for one raw query."""
+        raw = np.ascontiguousarray(raw, np.float32)
+        h = self.lib.qcs_prepare(_p(raw), self.seed)
+        q = np.zeros(self.cd, np.float32)
+        bits = np.zeros(SS_ROWS * self.W, np.uint64)
+        fs = np.zeros(8, np.float32)
+        isum = np.zeros(2, np.int64)
+        W = self.W
+        self.lib.qcs_query_fields(h, _p(q), _p(bits[:W]), C.c_void_p(bits.ctypes.data + 8 * W),
+                                  C.c_void_p(bits.ctypes.data + 8 * 5 * W), _p(fs), _p(isum))
+        names = ("fp32_sum", "fp32_norm2", "b1_scale", "b1_constant", "int4_scale", "int4_constant",
+                 "int8_scale", "int8_constant")
+        sc = {k: np.float32(v) for k, v in zip(names, fs)}
+        sc["int4_sum"], sc["int8_sum"] = int(isum[0]), int(isum[1])
+        return h, q, bits, sc
+
+    def free(self, h):
+        self.lib.qcs_query_free(h)
+
+    def scan(self, codec, db, n, h):
+        out = np.full(n, np.nan, np.float32)
+        self.lib.qcs_scan(codec, _p(db), n, h, _p(out))
+        return out
+
+    def validate(self, db, n, h):
+        return self.lib.qcs_validate(_p(db), n, h) == 0
