@@ -129,6 +129,10 @@ size_t qg_space_code_dim(const void *space);
 /* Device pointers to the rotation signs [cd] and the rotated center [cd]. */
 const float *qg_space_signs(const void *space);
 const float *qg_space_rotated_center(const void *space);
+/* Host copies of the rotation signs [code_dim] and the center [dim] (either
+ * output may be NULL): the random_seed_ / centroids_ / fht_signs_ payload
+ * RaBitQSpace::saveState writes (space_rabitq.h:3190-3192). */
+int qg_space_host_constants(const void *space, float *signs_out, float *center_out);
 
 /* Encode n base vectors (device, row-major [n][dim]) into the SoA index.
  * Replaces rabitq_encode_full + copy_compact + copy_residual_record +

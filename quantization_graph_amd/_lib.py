@@ -50,6 +50,7 @@ def lib():
     L.qg_space_code_dim.argtypes = [vp]
     L.qg_space_signs.restype = vp
     L.qg_space_signs.argtypes = [vp]
+    L.qg_space_host_constants.argtypes = [vp, vp, vp]
     L.qg_space_rotated_center.restype = vp
     L.qg_space_rotated_center.argtypes = [vp]
     L.qg_encode_workspace_bytes.restype = sz

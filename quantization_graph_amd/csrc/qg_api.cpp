@@ -236,6 +236,13 @@ int qg_space_set_center(void *space, const float *center_host) {
 size_t qg_space_dim(const void *space) { return space ? static_cast<const Space *>(space)->dim : 0; }
 size_t qg_space_code_dim(const void *space) { return space ? static_cast<const Space *>(space)->cd : 0; }
 const float *qg_space_signs(const void *space) { return space ? static_cast<const Space *>(space)->d_signs : nullptr; }
+int qg_space_host_constants(const void *space, float *signs_out, float *center_out) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s) return fail("null space");
+    if (signs_out) std::memcpy(signs_out, s->h_signs.data(), s->cd * sizeof(float));
+    if (center_out) std::memcpy(center_out, s->h_center.data(), s->dim * sizeof(float));
+    return 0;
+}
 const float *qg_space_rotated_center(const void *space) {
     return space ? static_cast<const Space *>(space)->d_rot_center : nullptr;
 }

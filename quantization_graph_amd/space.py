@@ -187,6 +187,9 @@ class RabitqSpace:
         self.full_record_bytes = 24 + 2 * self.block_count + self.code_dim + 1
         self.paper_msb_code_bytes = (self.code_dim + 7) // 8
         self.paper_factor_bytes = 20
+        self.residual_block_size = residual_block_size
+        self.residual_mse = bool(residual_mse)
+        self.residual_fp16 = bool(residual_fp16)
         self.codec = CODEC_INT8
 
     @classmethod
@@ -214,6 +217,44 @@ class RabitqSpace:
         if c.size != self.dim:
             raise ValueError("center has %d values, expected %d" % (c.size, self.dim))
         check(_lib.lib().qg_space_set_center(self.handle, c.ctypes.data_as(C.c_void_p)))
+
+    def host_constants(self):
+        """(rotation signs [code_dim], center [dim]) as float32 host arrays."""
+        signs = np.zeros(self.code_dim, np.float32)
+        center = np.zeros(self.dim, np.float32)
+        check(_lib.lib().qg_space_host_constants(self.handle, signs.ctypes.data_as(C.c_void_p),
+                                                 center.ctypes.data_as(C.c_void_p)))
+        return signs, center
+
+    def save_state(self, path):
+        """RaBitQSpace::saveState (space_rabitq.h:3139-3197): the EXRBTQ31
+        file the reference's loadState reads."""
+        from . import formats
+        signs, center = self.host_constants()
+        formats.write_space_state(path, self.dim, self.seed, center, signs, self.residual_block_size,
+                                  self.residual_mse, self.residual_fp16)
+
+    @classmethod
+    def load_state(cls, path):
+        """RaBitQSpace::loadState (space_rabitq.h:3200-3345) of an EXRBTQ31
+        file: a space of the stored dimension, seed, residual settings and
+        center. Raises ValueError when the file holds more than one centroid
+        (this build holds K = 1) or rotation signs other than its seed's."""
+        from . import formats
+        head, seed, centroids, signs = formats.read_space_state(path)
+        flags = head["flags"]
+        if head["centroid_count"] != 1:
+            raise ValueError("%s: %d centroids; this build holds K = 1" % (path, head["centroid_count"]))
+        sp = cls(head["dim"], seed, 1, head["residual_bits"], head["residual_block_size"], bool(flags & 4),
+                 bool(flags & 8))
+        want = formats.space_state_fields(head["dim"], 1, head["residual_block_size"], bool(flags & 4),
+                                          bool(flags & 8))
+        if head != want:
+            raise ValueError("%s: record layout differs from this space's" % path)
+        if not np.array_equal(sp.host_constants()[0], signs):
+            raise ValueError("%s: rotation signs are not those of seed %d" % (path, seed))
+        sp.set_center(centroids[0])
+        return sp
 
     def set_query_coarse_codec(self, codec):
         """RabitqSpace::set_query_coarse_codec (ours_diskann.rs:299-312):

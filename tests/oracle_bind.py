@@ -771,3 +771,69 @@ class RefSignScan:
 
     def validate(self, db, n, h):
         return self.lib.qcs_validate(_p(db), n, h) == 0
+
+
+INDEXIO_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libindexio_ref.so")
+
+
+class RefIndexIO:
+    """The reference's space-state and Vamana-index serialisers behind
+    oracle/index_io_ref.cpp (oracle/_ref/libindexio_ref.so); spaces and
+    records come from the bridge compiled into the same library."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(INDEXIO_REF_SO)
+
+    def __init__(self):
+        L = C.CDLL(INDEXIO_REF_SO)
+        L.iio_save_state.argtypes = [C.c_size_t, C.c_uint32, C.c_bool, C.c_bool, C.c_void_p, C.c_char_p]
+        L.iio_load_save_state.argtypes = [C.c_size_t, C.c_bool, C.c_bool, C.c_char_p, C.c_char_p]
+        L.iio_build_save_index.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float,
+                                           C.c_size_t, C.c_size_t, C.c_size_t, C.c_char_p]
+        L.iio_load_save_index.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
+        L.rabitq_space_new.restype = C.c_void_p
+        L.rabitq_space_new.argtypes = [C.c_size_t, C.c_uint32, C.c_size_t, C.c_size_t, C.c_size_t, C.c_bool,
+                                       C.c_bool]
+        L.rabitq_space_delete.argtypes = [C.c_void_p]
+        L.rabitq_space_set_center.argtypes = [C.c_void_p, C.c_void_p]
+        L.rabitq_encode_full.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.rabitq_copy_compact.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        for f in ("rabitq_full_record_bytes", "rabitq_compact_record_bytes"):
+            getattr(L, f).restype = C.c_size_t
+            getattr(L, f).argtypes = [C.c_void_p]
+        self.lib = L
+
+    def save_state(self, dim, seed, path, center=None, mse=True, fp16=True):
+        c = None if center is None else np.ascontiguousarray(center, np.float32)
+        return self.lib.iio_save_state(dim, seed, mse, fp16, _p(c) if c is not None else None, str(path).encode())
+
+    def load_save_state(self, dim, src, dst, mse=True, fp16=True):
+        return self.lib.iio_load_save_state(dim, mse, fp16, str(src).encode(), str(dst).encode())
+
+    def space(self, dim, seed, center):
+        h = self.lib.rabitq_space_new(dim, seed, 1, 4, 16, True, True)
+        c = np.ascontiguousarray(center, np.float32)
+        self.lib.rabitq_space_set_center(h, _p(c))
+        return h
+
+    def compact_records(self, h, X):
+        L = self.lib
+        X = np.ascontiguousarray(X, np.float32)
+        full = np.zeros(L.rabitq_full_record_bytes(h), np.uint8)
+        comp = np.zeros((X.shape[0], L.rabitq_compact_record_bytes(h)), np.uint8)
+        for i in range(X.shape[0]):
+            L.rabitq_encode_full(h, _p(X[i]), _p(full))
+            L.rabitq_copy_compact(h, _p(full), _p(comp[i]))
+        return comp
+
+    def build_save_index(self, h, comp, r, l_build, alpha, beam, batch, refine, path):
+        comp = np.ascontiguousarray(comp, np.uint8)
+        return self.lib.iio_build_save_index(h, _p(comp), comp.shape[0], r, l_build, C.c_float(alpha), beam, batch,
+                                             refine, str(path).encode())
+
+    def load_save_index(self, h, src, dst):
+        return self.lib.iio_load_save_index(h, str(src).encode(), str(dst).encode())
+
+    def free(self, h):
+        self.lib.rabitq_space_delete(h)
