@@ -662,6 +662,47 @@ def run_learned_fixed(args, torch, np, sp, index, x, q, cand, gt, g, plain_ms, p
             "fixed_pipeline": {"depth": plain_depth, "ms_per_step": plain_ms, "recall_at_10": plain_recall}}
 
 
+def fixed_precision_check(np, torch, qg, sp, index, comp_h, q, cand, center, dim, nq_sample):
+    """The precision form's report on a sample of the fixed-candidate
+    workload: the INT8 query x full 4-bit code kernel's integer dot products
+    against the oracle's exact dots (pinned to the reference), its float
+    estimates bit for bit against the oracle, and the estimates' largest
+    relative error against the FP64 value of the same formula,
+    norm_sqr + |q|^2 - long_scale * scale * (dot - 7.5 * code_sum)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle
+    orc = Oracle(dim, 100)
+    orc.set_center(center)
+    qv = sp.prepare(q[:nq_sample], "int8")
+    ids = cand[:nq_sample]
+    est, _, _, _, dot = sp.scan_fixed(qg.SCAN_INT8_FULL, index, qv, ids, eps0=1.9, want=("f", "f2", "dot"))
+    est, dot, ids_h = est.cpu().numpy(), dot.cpu().numpy(), ids.cpu().numpy()
+    qsc = qv.scalars()
+    hdr = comp_h[:, :8].copy().view(np.float32)
+    q_h = q[:nq_sample].cpu().numpy()
+    dots_equal = est_equal = 0
+    max_rel = 0.0
+    for i in range(nq_sample):
+        oq = orc.prepare(q_h[i], 3)
+        rows = comp_h[ids_h[i]]
+        e_o, _, d_o = orc.int8_full(oq, rows)
+        orc.free_query(oq)
+        dots_equal += int(np.array_equal(dot[i], d_o))
+        est_equal += int(np.array_equal(est[i].view(np.uint32), e_o.view(np.uint32)))
+        h = hdr[ids_h[i]].astype(np.float64)
+        qn2 = float(qsc["query_norm_sqr"][i])
+        exact = h[:, 0] + qn2 - h[:, 1] * float(qsc["int8_scale"][i]) * (
+            d_o.astype(np.float64) - 7.5 * float(qsc["int8_code_sum"][i]))
+        rel = np.abs(est[i].astype(np.float64) - exact) / (np.abs(exact) + qn2)
+        max_rel = max(max_rel, float(rel.max()))
+    res = {"sample": "%d queries x %d candidates of the fixed-candidate workload" % (nq_sample, ids_h.shape[1]),
+           "int_dots_equal_oracle": dots_equal == nq_sample, "estimates_bit_exact": est_equal == nq_sample,
+           "max_rel_err_vs_fp64": max_rel}
+    log("precision form: integer dots equal the oracle: %s; estimates bit-exact: %s; max relative error vs "
+        "FP64 %.3g" % (res["int_dots_equal_oracle"], res["estimates_bit_exact"], max_rel))
+    return res
+
+
 def run_fixed(args, torch, np, qg):
     from quantization_graph_amd import fixed
     t_setup = time.perf_counter()
@@ -740,10 +781,13 @@ def run_fixed(args, torch, np, qg):
 
     cpu = None
     parity = None
+    precision = None
     if not args.no_cpu:
         comp, res = index.to_records()
         comp_h, res_h = comp.cpu().numpy(), res.cpu().numpy()
         del comp, res
+        precision = fixed_precision_check(np, torch, qg, sp, index, comp_h, q, cand, center, args.fixed_dim,
+                                          min(100, nq))
         ns = min(args.cpu_queries, nq)
         cand_h = cand[:ns].cpu().numpy()
         q_h = q[:ns].cpu().numpy()
@@ -778,6 +822,7 @@ def run_fixed(args, torch, np, qg):
                      "kernel": "qg::k_int8_full_grouped<1024> (INT8 query x 4-bit code, fixed lists)",
                      "kernel_ms": kernel_ms, "bytes_per_unit": unit_bytes, "units_per_launch": units},
         "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity, "learned": learned_res,
+        "precision": precision,
         "setup_seconds": time.perf_counter() - t_setup,
     }
     del index, x, q, cand, gt
