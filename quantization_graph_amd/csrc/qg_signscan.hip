@@ -222,6 +222,234 @@ __global__ void __launch_bounds__(kSsThreads) k_ss_scan_bits(const uint64_t *__r
     }
 }
 
+// ------------------------------------------- batched integer codecs on MFMA
+// For a batch of queries the integer codecs are one int8 GEMM: with the
+// database row as 0/1 bytes s_i and the query as int8 codes x_i,
+// G = sum_i s_i x_i is exact in int32, and
+//   INT4 / INT8: integer_dot = 2 G - code_sum        (scan_integer<Bits>)
+//   B1 (x = +-1): mismatches = popcount(q) - G, signed_dot = cd - 2 mism
+// are the reference's integers, so every distance is bit-identical to the
+// popcount kernels (v_mfma_i32_16x16x64_i8, gfx950's 64-deep int8 step). A workgroup stages QT x 16 queries' codes in LDS (the
+// 8 x 8 bit transpose of their planes) and its waves walk 16 VT-vector
+// tiles: each lane holds its vector's words in registers and expands a
+// step's 16 bits to 0/1 bytes once for all query tiles (A operand); the
+// MFMAs read the query codes from LDS (B operand).
+// Lane (g = lane / 16, r = lane % 16) supplies 16 consecutive dimensions
+// 64 k + 16 g .. +15 of step k on both sides (vector r as the A row, query
+// r as the B column); accumulator j holds vector 4 g + j against query r, so
+// a lane stores 4 consecutive distances of one query row.
+typedef int v4i_ss __attribute__((ext_vector_type(4)));
+typedef float v4f_ss __attribute__((ext_vector_type(4)));
+
+// byte j of the result holds bit j of every input byte i at bit i
+__device__ __forceinline__ uint64_t transpose8x8(uint64_t x) {
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x = x ^ t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x = x ^ t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    return x ^ t ^ (t << 28);
+}
+
+// two waves per SIMD (<= 256 VGPRs): the vectors' words stay in registers
+// (2 VT W VGPRs) and each step's bytes are expanded just before its MFMAs
+template <int CODEC, int W, int VT, int QT>
+__global__ void __launch_bounds__(kSsThreads) __attribute__((amdgpu_waves_per_eu(2))) k_ss_scan_mfma(const uint64_t *__restrict__ db, uint64_t n,
+                                                             const uint64_t *__restrict__ bits,
+                                                             const SsQuery *__restrict__ qs, uint64_t nq,
+                                                             float *__restrict__ out) {
+    constexpr int KS = W, CD = 64 * W, QC = 16 * QT;  // one 64-dimension step per database word
+    constexpr int RS = CD + 16;  // LDS row stride: 8 lanes reading 16 bytes at one offset of 8 rows hit 32 banks
+    constexpr int kBits = CODEC == 1 ? 1 : (CODEC == 2 ? 4 : 8);
+    constexpr int kRow0 = CODEC == 1 ? 0 : (CODEC == 2 ? 1 : 5);
+    extern __shared__ __align__(16) unsigned char smem[];
+    int8_t *sq = (int8_t *)smem;  // [QC][RS] query codes
+    __shared__ float s_scale2[QC], s_const[QC];
+    __shared__ int s_off[QC];
+    const uint32_t t = threadIdx.x, lane = t & 63u, g = lane >> 4, rl = lane & 15u;
+    const uint64_t tiles = (n + 16 * VT - 1) / (16 * VT);
+    const uint64_t wave0 = (uint64_t)blockIdx.x * (kSsThreads / 64) + (t >> 6);
+    const uint64_t waves = (uint64_t)gridDim.x * (kSsThreads / 64);
+    for (uint64_t q0 = 0; q0 < nq; q0 += QC) {
+        __syncthreads();
+        for (uint32_t item = t; item < (uint32_t)QC * (CD / 8); item += kSsThreads) {
+            const uint32_t ql = item / (CD / 8), b = item % (CD / 8);
+            const uint64_t q = q0 + ql;
+            uint64_t code = 0;
+            if (q < nq) {
+                const uint64_t *rows = bits + (q * 13 + kRow0) * W + b / 8;
+                const uint32_t sh = 8 * (b % 8);
+                uint64_t x = 0;
+#pragma unroll
+                for (int p = 0; p < kBits; ++p) x |= ((rows[p * W] >> sh) & 0xFFull) << (8 * p);
+                const uint64_t y = transpose8x8(x);
+                if (CODEC == 1) code = ~(y * 0xFEull);                               // 1 -> +1, 0 -> -1
+                else if (CODEC == 2) code = y | (((y >> 3) & 0x0101010101010101ull) * 0xF0ull);  // sign-extend
+                else code = y;
+            }
+            *(uint64_t *)(sq + (uint64_t)ql * RS + 8 * b) = code;
+        }
+        if (t < (uint32_t)QC) {
+            const uint64_t q = q0 + t;
+            if (q < nq) {
+                const SsQuery s = qs[q];
+                if (CODEC == 1) {
+                    int pc = 0;
+                    for (int w = 0; w < W; ++w) pc += __popcll(bits[q * 13 * W + w]);
+                    s_scale2[t] = -(2.0f * s.b1_scale);
+                    s_const[t] = s.b1_constant;
+                    s_off[t] = pc;
+                } else {
+                    s_scale2[t] = -(2.0f * (CODEC == 2 ? s.int4_scale : s.int8_scale));
+                    s_const[t] = CODEC == 2 ? s.int4_constant : s.int8_constant;
+                    s_off[t] = -(int)(CODEC == 2 ? s.int4_sum : s.int8_sum);
+                }
+            }
+        }
+        __syncthreads();
+        // this lane's queries (column rl of each query tile) and their scalars
+        float sc2[QT], cst[QT];
+        int off[QT];
+        float *orow[QT];
+#pragma unroll
+        for (int qt = 0; qt < QT; ++qt) {
+            const uint32_t ql = 16 * qt + rl;
+            sc2[qt] = s_scale2[ql];
+            cst[qt] = s_const[ql];
+            off[qt] = s_off[ql];
+            orow[qt] = q0 + ql < nq ? out + (q0 + ql) * n : nullptr;
+        }
+        // a tile's words are loaded while the previous tile computes (two
+        // register sets in turn); loads are unconditional (past the end they
+        // re-read the last tile, whose results are not stored), so the
+        // compiler waits on exactly the loads a step uses
+        auto load_words = [&](uint64_t tile, uint64_t (&dst)[VT][W]) {
+            const uint64_t tc = tile < tiles ? tile : tiles - 1;
+#pragma unroll
+            for (int vt = 0; vt < VT; ++vt) {
+                uint64_t v = tc * 16 * VT + 16 * vt + rl;
+                v = v < n ? v : n - 1;
+                const uint64_t *row = db + (v / kSsBatch) * W * kSsBatch + v % kSsBatch;
+#pragma unroll
+                for (int w = 0; w < W; ++w) dst[vt][w] = row[(uint64_t)w * kSsBatch];
+            }
+        };
+        auto compute_tile = [&](uint64_t tile, const uint64_t (&word)[VT][W]) {
+            const uint64_t v0 = tile * 16 * VT;
+            v4i_ss acc[VT][QT];
+#pragma unroll
+            for (int vt = 0; vt < VT; ++vt)
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt) acc[vt][qt] = (v4i_ss){0, 0, 0, 0};
+            // the query codes of step k + kAhead are read while step k runs
+            // (a ring of kAhead + 1 register sets); the scheduling barrier
+            // keeps the compiler from hoisting every step's reads, and their
+            // registers, to the top
+            constexpr int kAhead = QT >= 4 ? 1 : 2;
+            const int8_t *arow = sq + rl * RS + 16 * g;
+            const uint32_t sh16 = 16 * (g & 1);  // this lane's 16 bits inside its half of the word
+            v4i_ss ring[kAhead + 1][QT];
+#pragma unroll
+            for (int k = 0; k < kAhead && k < KS; ++k)
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt) ring[k][qt] = *(const v4i_ss *)(arow + 16 * qt * RS + 64 * k);
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                if (k + kAhead < KS) {
+#pragma unroll
+                    for (int qt = 0; qt < QT; ++qt)
+                        ring[(k + kAhead) % (kAhead + 1)][qt] =
+                            *(const v4i_ss *)(arow + 16 * qt * RS + 64 * (k + kAhead));
+                }
+                v4i_ss bx[VT];
+#pragma unroll
+                for (int vt = 0; vt < VT; ++vt) {
+                    const uint32_t half = g >= 2 ? (uint32_t)(word[vt][k] >> 32) : (uint32_t)word[vt][k];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) bx[vt][i] = (int)nibble_bytes(__builtin_amdgcn_ubfe(half, sh16 + 4 * i, 4));
+                }
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt)
+#pragma unroll
+                    for (int vt = 0; vt < VT; ++vt)
+                        acc[vt][qt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bx[vt], ring[k % (kAhead + 1)][qt],
+                                                                            acc[vt][qt], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // acc[vt][qt][j]: vector v0 + 16 vt + 4 g + j against query 16 qt + rl;
+            // n is a multiple of 32, so a 16-vector tile is all in or all out
+#pragma unroll
+            for (int vt = 0; vt < VT; ++vt) {
+                const uint64_t vb = v0 + 16 * vt;
+                if (vb >= n) continue;
+#pragma unroll
+                for (int qt = 0; qt < QT; ++qt) {
+                    if (!orow[qt]) continue;
+                    v4f_ss d;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int gsum = acc[vt][qt][j];
+                        const int dot = CODEC == 1 ? CD - 2 * (off[qt] - gsum) : 2 * gsum + off[qt];
+                        d[j] = fmaf(sc2[qt], (float)dot, cst[qt]);
+                    }
+                    *(v4f_ss *)(orow[qt] + vb + 4 * g) = d;
+                }
+            }
+        };
+        uint64_t wa[VT][W], wb[VT][W];
+        load_words(wave0, wa);
+        for (uint64_t tile = wave0; tile < tiles; tile += 2 * waves) {
+            load_words(tile + waves, wb);
+            compute_tile(tile, wa);
+            if (tile + waves >= tiles) break;
+            load_words(tile + 2 * waves, wa);
+            compute_tile(tile + waves, wb);
+        }
+    }
+}
+
+template <int CODEC, int W, int VT, int QT>
+int launch_mfma_wq(uint64_t n, hipStream_t st, const uint64_t *db, const uint64_t *bits, const SsQuery *qs,
+                   uint64_t nq, float *out) {
+    const uint64_t tiles = (n + 16 * VT - 1) / (16 * VT);
+    uint64_t blocks = (tiles + 3) / 4;
+    if (blocks > 512) blocks = 512;  // two resident workgroups per CU; staging amortised over the tiles
+    const size_t smem = (size_t)16 * QT * (64 * W + 16);
+    static bool raised = false;  // dynamic LDS above the 64 KB default needs the attribute once
+    if (!raised && smem > 65536) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ss_scan_mfma<CODEC, W, VT, QT>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return (int)e;
+        raised = true;
+    }
+    hipLaunchKernelGGL((k_ss_scan_mfma<CODEC, W, VT, QT>), dim3((unsigned)blocks), dim3(kSsThreads), smem, st, db, n,
+                       bits, qs, nq, out);
+    return (int)hipGetLastError();
+}
+
+// query tiles per staged chunk: as many as the batch needs, up to 66 KB of
+// codes (4 tiles up to code dimension 1024, 2 at 2048): two workgroups per CU
+template <int CODEC, int W, int VT, int QTMAX>
+int launch_mfma_w(uint64_t n, hipStream_t st, const uint64_t *db, const uint64_t *bits, const SsQuery *qs,
+                  uint64_t nq, float *out) {
+    if (nq <= 16) return launch_mfma_wq<CODEC, W, VT, 1>(n, st, db, bits, qs, nq, out);
+    if (nq <= 32 || QTMAX == 2) return launch_mfma_wq<CODEC, W, VT, 2>(n, st, db, bits, qs, nq, out);
+    return launch_mfma_wq<CODEC, W, VT, (QTMAX >= 4 ? 4 : 2)>(n, st, db, bits, qs, nq, out);
+}
+
+template <int CODEC>
+int launch_mfma(uint32_t W, hipStream_t st, const uint64_t *db, uint64_t n, const uint64_t *bits, const SsQuery *qs,
+                uint64_t nq, float *out) {
+    switch (W) {
+        case 2: return launch_mfma_w<CODEC, 2, 2, 4>(n, st, db, bits, qs, nq, out);
+        case 4: return launch_mfma_w<CODEC, 4, 2, 4>(n, st, db, bits, qs, nq, out);
+        case 8: return launch_mfma_w<CODEC, 8, 2, 4>(n, st, db, bits, qs, nq, out);
+        case 16: return launch_mfma_w<CODEC, 16, 2, 4>(n, st, db, bits, qs, nq, out);
+        case 32: return launch_mfma_w<CODEC, 32, 1, 2>(n, st, db, bits, qs, nq, out);
+        default: return (int)hipErrorInvalidValue;
+    }
+}
+
 template <int CODEC>
 int launch_bits(uint32_t W, dim3 grid, hipStream_t st, const uint64_t *db, uint64_t n, const uint64_t *bits,
                 const SsQuery *qs, uint64_t nq, float *out) {
@@ -263,9 +491,21 @@ int qg_launch_ss_scan(int codec, const uint64_t *db, uint64_t n, uint32_t W, con
         case 0:
             hipLaunchKernelGGL(qg::k_ss_scan_fp32, grid, dim3(qg::kSsThreads), 0, st, db, n, W, q, qs, nq, out);
             return (int)hipGetLastError();
-        case 1: return qg::launch_bits<1>(W, grid, st, db, n, bits, qs, nq, out);
-        case 2: return qg::launch_bits<2>(W, grid, st, db, n, bits, qs, nq, out);
-        case 3: return qg::launch_bits<3>(W, grid, st, db, n, bits, qs, nq, out);
+        case 1:
+        case 2:
+        case 3:
+            // batches on the matrix cores (code dimension >= 128, 16-byte
+            // aligned output rows): from 8 queries for INT4 / INT8; B1's XOR
+            // popcount costs 70 operations per pair and stays ahead up to
+            // about 48 queries on MI355X
+            if (nq >= (codec == 1 ? 48u : 8u) && W >= 2 && ((uintptr_t)out & 15u) == 0) {
+                return codec == 1 ? qg::launch_mfma<1>(W, st, db, n, bits, qs, nq, out)
+                                  : (codec == 2 ? qg::launch_mfma<2>(W, st, db, n, bits, qs, nq, out)
+                                                : qg::launch_mfma<3>(W, st, db, n, bits, qs, nq, out));
+            }
+            return codec == 1 ? qg::launch_bits<1>(W, grid, st, db, n, bits, qs, nq, out)
+                              : (codec == 2 ? qg::launch_bits<2>(W, grid, st, db, n, bits, qs, nq, out)
+                                            : qg::launch_bits<3>(W, grid, st, db, n, bits, qs, nq, out));
         default: return (int)hipErrorInvalidValue;
     }
 }

@@ -80,6 +80,30 @@ def test_signscan_matches_oracle(gpu, dim):
     assert W == qg_bits.shape[1] // 13
 
 
+@pytest.mark.parametrize("dim,nq", [(100, 70), (960, 9), (960, 70), (1536, 37), (2000, 70)])
+def test_signscan_batched_matches_oracle(gpu, dim, nq):
+    """Batches of 8 or more queries run the integer codecs on the matrix
+    cores (int8 GEMM of query codes x database bits); every distance must
+    still equal the oracle's bit for bit, across query chunks and partial
+    16-query tiles."""
+    print("signscan batched vs oracle: dim %d, %d queries" % (dim, nq), flush=True)
+    sp, ss = _setup(gpu, dim)
+    orc = Oracle(dim, 100)
+    rng = np.random.default_rng(dim + nq)
+    n = 32 * 21
+    x = (rng.standard_normal((n, dim)) * rng.uniform(0.05, 5.0, (n, 1))).astype(np.float32)
+    q = (rng.standard_normal((nq, dim)) * rng.uniform(0.01, 50.0, (nq, 1))).astype(np.float32)
+    q[0] = 0.0
+    db = ss.encode(gpu.from_numpy(x).cuda())
+    db_o = orc.ss_encode(x)
+    pq = ss.prepare(gpu.from_numpy(q).cuda())
+    outs = {c: ss.scan(c, db, n, pq).cpu().numpy() for c in (1, 2, 3)}
+    for i in range(nq):
+        qo, bo, so = orc.ss_prepare(q[i])
+        for codec in (1, 2, 3):
+            assert _f32_equal(outs[codec][i], orc.ss_scan(codec, db_o, n, qo, bo, so)), (i, codec)
+
+
 def test_signscan_rejects_partial_blocks(gpu):
     from quantization_graph_amd._lib import QGError
     sp, ss = _setup(gpu, 100)
@@ -125,6 +149,13 @@ def test_signscan_full_size_self_match(gpu):
     own_fp = out_fp.gather(1, rows[:, None]).squeeze(1).double().cpu().numpy()
     assert np.abs(own_fp - want_fp).max() <= 1e-5 * np.abs(want_fp).max()
     assert np.array_equal(out_fp.argmin(1).cpu().numpy(), rows.cpu().numpy())
+    # the batched (matrix-core) integer scans equal the single-query scans
+    # on all 1M rows: 50 queries whose first 6 are the rows above
+    more = torch.cat([rows, torch.randint(0, n, (44,), generator=g, device="cuda")])
+    pq50 = ss.prepare(x[more])
+    for codec, single in ((1, out_b1), (3, out_i8), (2, ss.scan(2, db, n, pq))):
+        batched = ss.scan(codec, db, n, pq50)[:6]
+        assert torch.equal(batched.view(torch.int32), single.view(torch.int32)), codec
     # INT8: own-row dot = sum |code| (codes of the query's positive / non-positive
     # coordinates meet signs +1 / -1; zero codes add nothing)
     bits = pq.bits.cpu().numpy().view(np.uint64)[:, 5:13, :]
