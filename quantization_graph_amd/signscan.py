@@ -84,6 +84,9 @@ class SignScan:
             raise ValueError("expected [n, %d] rows" % self.dim)
         if out is None:
             out = torch.empty(n * self.words, dtype=torch.int64, device="cuda")
+        elif (out.dtype != torch.int64 or not out.is_cuda or out.numel() < n * self.words
+              or not out.is_contiguous()):
+            raise ValueError("out must be a contiguous int64 CUDA tensor of n * code_dim / 64 words")
         check(_lib.lib().qg_signscan_encode(self.space.handle, _ptr(x), n, _ptr(out), _stream()))
         return out
 
@@ -101,10 +104,13 @@ class SignScan:
         codec = CODECS.get(codec, codec)
         if db.numel() < n * self.words:
             raise ValueError("database holds fewer than %d vectors" % n)
+        if db.dtype != torch.int64 or not db.is_cuda or not db.is_contiguous():
+            raise ValueError("db must be a contiguous int64 CUDA tensor of database words")
         if out is None:
             out = torch.empty((pq.nq, n), dtype=torch.float32, device="cuda")
-        elif out.numel() < pq.nq * n or not out.is_contiguous():
-            raise ValueError("out must be a contiguous [nq, n] f32 tensor")
+        elif (out.dtype != torch.float32 or not out.is_cuda or out.numel() < pq.nq * n
+              or not out.is_contiguous()):
+            raise ValueError("out must be a contiguous [nq, n] float32 CUDA tensor")
         check(_lib.lib().qg_signscan(self.space.handle, int(codec), _ptr(db), n, _ptr(pq.q), _ptr(pq.bits),
                                      _ptr(pq.scalars), pq.nq, _ptr(out), _stream()))
         return out
