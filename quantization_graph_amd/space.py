@@ -430,16 +430,16 @@ def search_error(workspace):
         raise _lib.QGError("graph contains an out-of-range neighbour")
 
 
-def topk(values, ids, k):
+def topk(values, ids, k, want_pos=False):
     """Per-row top-k of values [nq, P] by (value, id); returns (ids, values,
-    positions), each [nq, k]. Rows shorter than k are padded with id
-    0xFFFFFFFF and +inf."""
+    positions), each [nq, k] (positions in the row only with want_pos, else
+    None). Rows shorter than k are padded with id 0xFFFFFFFF and +inf."""
     values = values.contiguous()
     ids = _dev(ids, torch.int32)
     nq, p = values.shape
     out_ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
     out_v = torch.empty((nq, k), dtype=torch.float32, device="cuda")
-    out_pos = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    out_pos = torch.empty((nq, k), dtype=torch.int32, device="cuda") if want_pos else None
     check(_lib.lib().qg_topk(_ptr(values), _ptr(ids), nq, p, k, _ptr(out_ids), _ptr(out_v),
                              _ptr(out_pos), _stream()))
     return out_ids, out_v, out_pos

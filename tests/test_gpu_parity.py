@@ -140,20 +140,29 @@ def test_gates_and_remaining_pairs(gpu, dim, codec):
         orc.free_query(oq)
 
 
-def test_topk_order_and_ties(gpu):
+@pytest.mark.parametrize("per,k", [(1000, 100), (1024, 10), (37, 100), (64, 64), (200, 1), (3000, 100)])
+@pytest.mark.parametrize("want_pos", [False, True])
+def test_topk_order_and_ties(gpu, per, k, want_pos):
+    """Both top-k kernels (the register sort of one wavefront for lists up to
+    1024 without positions, the workgroup sort otherwise) return the first k
+    entries by (value, id), values bit for bit (signed zeros included)."""
     import quantization_graph_amd as qg
-    rng = np.random.default_rng(3)
-    nq, per, k = 5, 1000, 100
-    vals = rng.integers(0, 50, size=(nq, per)).astype(np.float32)  # many exact ties
+    rng = np.random.default_rng(3 + per + k)
+    nq = 9
+    vals = rng.integers(-20, 30, size=(nq, per)).astype(np.float32)  # many exact ties
     vals[0, :10] = -0.0
     vals[0, 10:20] = 0.0
     ids = np.stack([rng.permutation(10 * per)[:per] for _ in range(nq)]).astype(np.int32)
-    oid, ov, opos = qg.topk(gpu.from_numpy(vals).cuda(), gpu.from_numpy(ids).cuda(), k)
+    oid, ov, opos = qg.topk(gpu.from_numpy(vals).cuda(), gpu.from_numpy(ids).cuda(), k, want_pos=want_pos)
     oid, ov = oid.cpu().numpy(), ov.cpu().numpy()
+    kk = min(k, per)
     for i in range(nq):
-        order = sorted(range(per), key=lambda j: (float(vals[i, j]), int(ids[i, j])))[:k]
-        assert np.array_equal(oid[i], ids[i, order])
-        assert np.array_equal(ov[i], vals[i, order])
+        order = sorted(range(per), key=lambda j: (float(vals[i, j]), int(ids[i, j])))[:kk]
+        assert np.array_equal(oid[i, :kk], ids[i, order])
+        assert np.array_equal(ov[i, :kk].view(np.uint32), vals[i, order].view(np.uint32))
+        assert (oid[i, kk:] == -1).all() and np.isinf(ov[i, kk:]).all()
+        if want_pos:
+            assert np.array_equal(opos.cpu().numpy()[i, :kk], order)
 
 
 def test_records_roundtrip(gpu):
