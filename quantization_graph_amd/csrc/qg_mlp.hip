@@ -438,7 +438,8 @@ __global__ void __launch_bounds__(128) k_rerank_features(RerankArgs a) {
 //    thread; ranks below k are written out.
 constexpr int kRrThreads = 256;
 constexpr uint32_t kRrMaxDepth = 128;
-constexpr uint32_t kRrWork = kRrMaxDepth * 65;  // block sums of 128 candidates at 64 blocks (+1 pad) per row
+constexpr uint32_t kRrWork = 64 * 65;  // block sums of 64 candidates at 64 blocks (+1 pad) per row: 17 KB, 6 workgroups per CU
+static_assert(kRrWork * 4 >= 16 * (2048 / 32) * 16, "the INT8 restore stages 16 code rows of code dimension 2048");
 
 __device__ __forceinline__ float rank_key(float v) { return v == v ? v : __builtin_huge_valf(); }
 
