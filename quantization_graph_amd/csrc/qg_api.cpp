@@ -109,19 +109,23 @@ Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t res
         space_destroy(s);
         return nullptr;
     }
-    hipMemcpy(s->d_signs, s->h_signs.data(), s->cd * 4, hipMemcpyHostToDevice);
-    hipMemset(s->d_center, 0, dim * 4);
-    hipMemset(s->d_rot_center, 0, s->cd * 4);
+    if (check_hip(hipMemcpy(s->d_signs, s->h_signs.data(), s->cd * 4, hipMemcpyHostToDevice), "upload signs") ||
+        check_hip(hipMemset(s->d_center, 0, dim * 4), "clear center") ||
+        check_hip(hipMemset(s->d_rot_center, 0, s->cd * 4), "clear rotated center")) {
+        space_destroy(s);
+        return nullptr;
+    }
     s->center_norm_sqr = 0.0f;
     return s;
 }
 
 void space_destroy(Space *s) {
     if (!s) return;
-    if (s->d_signs) hipFree(s->d_signs);
-    if (s->d_center) hipFree(s->d_center);
-    if (s->d_rot_center) hipFree(s->d_rot_center);
-    if (s->d_bridge) hipFree(s->d_bridge);
+    // teardown: a failed free leaves nothing to recover, so its status is dropped
+    if (s->d_signs) (void)hipFree(s->d_signs);
+    if (s->d_center) (void)hipFree(s->d_center);
+    if (s->d_rot_center) (void)hipFree(s->d_rot_center);
+    if (s->d_bridge) (void)hipFree(s->d_bridge);
     delete s;
 }
 
@@ -137,7 +141,7 @@ int space_set_center(Space *s, const float *center_host) {
                                         s->d_signs, s->d_rot_center, 0);
     double norm = 0.0;
     if (!rc) rc = hipMemcpy(&norm, d_norm, sizeof(double), hipMemcpyDeviceToHost);
-    hipFree(d_norm);
+    (void)hipFree(d_norm);
     if (rc) return check_hip(rc, "set center");
     s->center_norm_sqr = static_cast<float>(norm);
     return 0;
