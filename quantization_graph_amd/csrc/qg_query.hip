@@ -174,11 +174,9 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = p.cd;
     float *v = (float *)smem;                          // cd floats
-    double *err = (double *)(smem + 4 * cd);           // cd doubles
-    uint64_t *mt = (uint64_t *)(smem + 12 * cd);       // 312 words
+    uint64_t *mt = (uint64_t *)(smem + 4 * cd);        // 312 words
     __shared__ float s_sum, s_pos, s_max;
     __shared__ long long s_csum;
-    __shared__ double s_err;
     const uint64_t qi = blockIdx.x;
     const float *raw = p.raw + qi * p.dim;
     const uint32_t t = threadIdx.x;
@@ -243,8 +241,6 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
                     if (u >= 1.0) u = 0x1.fffffffffffffp-1;  // nextafter(1.0, 0.0)
                     int x = (int)floor((double)v[i] / (double)scale + u);
                     x = x > 127 ? 127 : (x < -128 ? -128 : x);
-                    const double e = (double)v[i] - (double)scale * (double)x;
-                    err[i] = e;
                     p.x8_out[qi * cd + i] = (int8_t)x;
                     // regroup: per 8 dims, 4 even dims then 4 odd dims
                     const uint32_t g = i >> 3, w = i & 7u;
@@ -256,12 +252,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
         }
         for (int off = 32; off > 0; off >>= 1) csum += __shfl_xor(csum, off);
         if ((t & 63) == 0) atomicAdd((unsigned long long *)&s_csum, (unsigned long long)csum);
-        __syncthreads();
-        if (t == 0) {
-            double a = 0.0;
-            for (uint32_t i = 0; i < cd; ++i) a = fma(err[i], err[i], a);
-            s_err = a;
-        }
+        // the error norm's ordered FP64 chain runs in k_query_err, one lane per query
     } else if (p.codec == 1 || p.codec == 4) {
         // B1 / B1Main sign code (:3493-3511): +1 where q_i > 0, else -1; the
         // signed sum goes where the INT8 code sum goes.
@@ -287,12 +278,58 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
         s.sum = s_sum;
         s.positive_sum = s_pos;
         s.int8_scale = scale;
-        s.int8_error_norm = int8 ? (float)sqrt(s_err) : 0.0f;
+        s.int8_error_norm = 0.0f;  // filled by k_query_err for INT8 queries
         s.max_abs_bits = __float_as_int(max_abs);
         s.int8_code_sum = (int8 || p.codec == 1 || p.codec == 4) ? (int64_t)s_csum : 0;
         s.pad = 0;
         p.qs_out[qi] = s;
     }
+}
+
+// The INT8 error norm (:3592-3600): sqrt of sum_i e_i^2 in index order as
+// one FP64 fma chain, e_i = q_i - scale x_i as the dither computed it. One
+// lane per query, so the 2 cd-long chain of every query runs at once instead
+// of on one lane of its preparation workgroup.
+__global__ void __launch_bounds__(64) k_query_err(const float *q, const int8_t *x8, uint64_t nq, uint32_t cd,
+                                                  QScalars *qs) {
+    const uint64_t qi = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    if (qi >= nq) return;
+    QScalars &s = qs[qi];
+    const float max_abs = __int_as_float(s.max_abs_bits);
+    if (!(max_abs > 0.0f)) return;  // no INT8 code: the norm stays 0
+    const double scale = (double)s.int8_scale;
+    const float4 *qr = (const float4 *)(q + qi * cd);
+    const uint4 *xr = (const uint4 *)(x8 + qi * cd);
+    // 16 dimensions per step; the next step's rows are loaded while this one
+    // is folded in (cd is a multiple of 64)
+    float4 qc[4], qn[4];
+    uint4 xc, xn;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) qc[k] = qr[k];
+    xc = xr[0];
+    double a = 0.0;
+    for (uint32_t c = 0; c < cd / 16; ++c) {
+        if (c + 1 < cd / 16) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qn[k] = qr[4 * (c + 1) + k];
+            xn = xr[c + 1];
+        }
+        const uint32_t xw[4] = {xc.x, xc.y, xc.z, xc.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float qv[4] = {qc[k].x, qc[k].y, qc[k].z, qc[k].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = (int)(int8_t)(xw[k] >> (8 * j));
+                const double e = (double)qv[j] - scale * (double)x;
+                a = fma(e, e, a);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qc[k] = qn[k];
+        xc = xn;
+    }
+    s.int8_error_norm = (float)sqrt(a);
 }
 
 // Rotation only (setCentroids' rotate, :2918-2936): rows -> FHT(signs * x).
@@ -327,8 +364,11 @@ int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float 
 
 int qg_launch_query_prep(const qg::PrepParams *p, hipStream_t stream) {
     if (p->nq == 0) return 0;
-    const size_t smem = (size_t)p->cd * 12 + 312 * 8;
+    const size_t smem = (size_t)p->cd * 4 + 312 * 8;
     hipLaunchKernelGGL(qg::k_query_prep, dim3((unsigned)p->nq), dim3(qg::kPrepThreads), smem, stream, *p);
+    if (p->codec == 3)
+        hipLaunchKernelGGL(qg::k_query_err, dim3((unsigned)((p->nq + 63) / 64)), dim3(64), 0, stream, p->q_out,
+                           p->x8_out, p->nq, p->cd, p->qs_out);
     return (int)hipGetLastError();
 }
 
