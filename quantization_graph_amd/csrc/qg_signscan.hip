@@ -465,7 +465,7 @@ int launch_mfma(uint32_t W, hipStream_t st, const uint64_t *db, uint64_t n, cons
         case 2: return launch_mfma_w<CODEC, 2, 2, 4>(n, st, db, bits, qs, nq, out);
         case 4: return launch_mfma_w<CODEC, 4, 2, 4>(n, st, db, bits, qs, nq, out);
         case 8: return launch_mfma_w<CODEC, 8, 2, 4>(n, st, db, bits, qs, nq, out);
-        case 16: return launch_mfma_w<CODEC, 16, 2, 4>(n, st, db, bits, qs, nq, out);
+        case 16: return launch_mfma_w<CODEC, 16, 1, 4>(n, st, db, bits, qs, nq, out);
         case 32: return launch_mfma_w<CODEC, 32, 1, 2>(n, st, db, bits, qs, nq, out);
         default: return (int)hipErrorInvalidValue;
     }
