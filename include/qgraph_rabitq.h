@@ -166,7 +166,9 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
             const float *short_ips, const float *long_in, float *out_f, float *out_f2,
             RabitqPaperEstimate *out_est, RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream);
 
-/* Per-query top-k by (value, id) over [nq][per_query] lists (per_query <= 4096). */
+/* Per-query top-k by (value, id) over [nq][per_query] lists (per_query <= 4096);
+ * out_values and out_pos may be NULL (lists up to 1024 without positions take
+ * a faster register sort with the same results). */
 int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_query, uint32_t k,
             uint32_t *out_ids, float *out_values, uint32_t *out_pos, void *stream);
 
