@@ -180,8 +180,14 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
  * flush_distance_batch + rerank (ours_diskann.rs:1539-1986) and its bridge
  * calls. Graph: degrees [n], edges [n][edge_stride], edge_stride <= 64.
  * Outputs [nq][k] ids / estimates (missing entries: 0xFFFFFFFF / +inf);
- * optional stats [nq][4] = hops, gate evaluations, distances, pruned and
- * the final pool [nq][max(l_search, k)]. Workspace:
+ * optional stats [nq][6] = hops, gate evaluations, distances computed
+ * (start node included), prunes (first pass and re-check), re-checks made
+ * with the pool full and the gate valid, and re-check prunes; and the final
+ * pool [nq][max(l_search, k)]. The reference's counters follow from these
+ * (ours_diskann.rs:1604-1968): visited_nodes = 1 + gate evaluations,
+ * distance_computations = distances - re-check prunes, paper_checked =
+ * prunes - re-check prunes + re-checks, paper_would_prune = prunes.
+ * Workspace:
  * qg_search_workspace_bytes(n, slots) of device memory (per-slot visited
  * bitsets and the work-queue head); slots = concurrent queries.
  * max(l_search, k) <= 4096, max(rerank, k) <= 1024 and code_dim >= 128

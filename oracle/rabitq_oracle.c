@@ -880,6 +880,14 @@ static void pool_insert(ro_cand *pool, size_t *len, size_t L, float d, uint32_t 
  * stop off; codec 3 = Int8 (gate C5, mode-2 flush) or 0 = Full (gate C4,
  * mode-1 flush). Rerank: top min(|pool|, max(rerank, k)) ids, FP32 long
  * distance recomputed for Int8, residual interval, sort by (estimate, id). */
+static void trace_event(ro_search_event *trace, size_t cap, size_t *len, uint32_t hop, uint32_t id, float lb,
+                        float back, int stage, int full, int valid, int pruned) {
+    if (!trace || *len >= cap) { if (len) ++*len; return; }
+    ro_search_event *e = &trace[(*len)++];
+    e->hop = hop; e->id = id; e->lower_bound = lb; e->back = back;
+    e->stage = (uint8_t)stage; e->full = (uint8_t)full; e->valid = (uint8_t)valid; e->pruned = (uint8_t)pruned;
+}
+
 size_t ro_search(const ro_space *s, const ro_query *q, int codec,
                  const uint8_t *cbase, size_t cstride,
                  const uint8_t *rbase, size_t rstride,
@@ -890,8 +898,28 @@ size_t ro_search(const ro_space *s, const ro_query *q, int codec,
                  uint32_t *out_ids, float *out_d,
                  uint32_t *pool_ids_out, float *pool_d_out, size_t *pool_len_out,
                  ro_search_stats *st) {
+    return ro_search_traced(s, q, codec, cbase, cstride, rbase, rstride, mbase, mstride, factors, degrees, edges,
+                            estride, n, k, L, eps0, rerank, out_ids, out_d, pool_ids_out, pool_d_out, pool_len_out,
+                            st, NULL, 0, NULL);
+}
+
+/* trace (may be NULL) receives up to trace_cap prune decisions; *trace_len
+ * (may be NULL) counts them all. */
+size_t ro_search_traced(const ro_space *s, const ro_query *q, int codec,
+                        const uint8_t *cbase, size_t cstride,
+                        const uint8_t *rbase, size_t rstride,
+                        const uint8_t *mbase, size_t mstride,
+                        const ro_factors *factors,
+                        const uint32_t *degrees, const uint32_t *edges, size_t estride,
+                        size_t n, size_t k, size_t L, float eps0, size_t rerank,
+                        uint32_t *out_ids, float *out_d,
+                        uint32_t *pool_ids_out, float *pool_d_out, size_t *pool_len_out,
+                        ro_search_stats *st, ro_search_event *trace, size_t trace_cap, size_t *trace_len) {
     ro_search_stats local;
+    size_t tl_local = 0;
     if (!st) st = &local;
+    if (!trace_len) trace_len = &tl_local;
+    *trace_len = 0;
     memset(st, 0, sizeof(*st));
     if (k == 0 || n == 0) return 0;
     if (L < k) L = k;
@@ -932,7 +960,10 @@ size_t ro_search(const ro_space *s, const ro_query *q, int codec,
             if (j < nf) {
                 const int full = len >= L;
                 const float lb = full ? pool[len - 1].d : 3.402823466e38f;
-                if (full && est[j].valid && est[j].lower_bound > lb) { st->pruned++; continue; }
+                const int cut = full && est[j].valid && est[j].lower_bound > lb;
+                trace_event(trace, trace_cap, trace_len, (uint32_t)st->hops, fresh[j], est[j].lower_bound, lb, 0,
+                            full, est[j].valid != 0, cut);
+                if (cut) { st->pruned++; continue; }
                 bid[nb] = fresh[j];
                 bslot[nb] = j;
                 nb++;
@@ -949,7 +980,11 @@ size_t ro_search(const ro_space *s, const ro_query *q, int codec,
                 st->full_dists++;
                 const int full = len >= L;
                 const float lb = full ? pool[len - 1].d : 3.402823466e38f;
-                if (full && e->valid && e->lower_bound > lb) { st->pruned++; continue; }
+                const int cut = full && e->valid && e->lower_bound > lb;
+                trace_event(trace, trace_cap, trace_len, (uint32_t)st->hops, id, e->lower_bound, lb, 1, full,
+                            e->valid != 0, cut);
+                if (full && e->valid) st->recheck_checked++;
+                if (cut) { st->pruned++; st->recheck_pruned++; continue; }
                 pool_insert(pool, &len, L, d, id);
             }
             nb = 0;

@@ -176,9 +176,33 @@ double ro_residual_estimate_f64(const ro_space *s, const ro_query *q, const uint
 typedef struct ro_search_stats {
     uint64_t hops;
     uint64_t gate_evals;
-    uint64_t full_dists;
-    uint64_t pruned;
+    uint64_t full_dists;      /* start node + every distance computed */
+    uint64_t pruned;          /* first-pass and re-check prunes */
+    uint64_t recheck_checked; /* re-checks with the pool full and the gate valid */
+    uint64_t recheck_pruned;  /* of those, pruned */
 } ro_search_stats;
+/* One prune decision of the search (trace of ro_search_traced): the
+ * candidate, its gate bound, the pool back it was tested against, and
+ * whether the pool was full, the gate valid and the candidate pruned.
+ * stage 0 = first pass against the pool as it stood at the start of the
+ * hop, 1 = re-check at insertion. */
+typedef struct ro_search_event {
+    uint32_t hop;
+    uint32_t id;
+    float lower_bound;
+    float back;
+    uint8_t stage, full, valid, pruned;
+} ro_search_event;
+size_t ro_search_traced(const ro_space *s, const ro_query *q, int codec,
+                        const uint8_t *compact_base, size_t compact_stride,
+                        const uint8_t *residual_base, size_t residual_stride,
+                        const uint8_t *msb_base, size_t msb_stride,
+                        const ro_factors *factors,
+                        const uint32_t *degrees, const uint32_t *edges, size_t edge_stride,
+                        size_t n_points, size_t k, size_t l_value, float eps0, size_t rerank,
+                        uint32_t *out_ids, float *out_dists,
+                        uint32_t *pool_ids_out, float *pool_dists_out, size_t *pool_len_out,
+                        ro_search_stats *stats, ro_search_event *trace, size_t trace_cap, size_t *trace_len);
 size_t ro_search(const ro_space *s, const ro_query *q, int codec,
                  const uint8_t *compact_base, size_t compact_stride,
                  const uint8_t *residual_base, size_t residual_stride,

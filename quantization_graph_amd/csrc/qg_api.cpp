@@ -125,7 +125,6 @@ void space_destroy(Space *s) {
     if (s->d_signs) (void)hipFree(s->d_signs);
     if (s->d_center) (void)hipFree(s->d_center);
     if (s->d_rot_center) (void)hipFree(s->d_rot_center);
-    if (s->d_bridge) (void)hipFree(s->d_bridge);
     delete s;
 }
 

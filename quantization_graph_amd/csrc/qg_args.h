@@ -125,7 +125,7 @@ struct SearchArgs {
     uint32_t *err;             // set to non-zero on an out-of-range neighbour
     uint32_t *out_ids;         // [nq][k]
     float *out_d;              // [nq][k]
-    uint64_t *stats;           // [nq][4] hops, gate evaluations, distances, pruned (optional)
+    uint64_t *stats;           // [nq][6] hops, gate evaluations, distances, pruned, re-checks, re-check prunes (optional)
     uint32_t *pool_ids;        // [nq][L] final pool (optional)
     int pool_only;             // skip the rerank (out_ids / out_d unused)
     float *pool_d;             // [nq][L]

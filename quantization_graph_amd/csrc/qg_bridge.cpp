@@ -3,9 +3,10 @@
 //
 // Every entry point keeps the reference's signature, argument meaning and
 // failure sentinels (false / nullptr / +inf / zeroed estimates / intervals
-// equal to the long distance). Work is staged through a per-space device
-// arena, computed by the same kernels as the batch API, and copied back
-// synchronously. The prepared-query handle lives in thread-local storage,
+// equal to the long distance). Work is staged through the calling thread's
+// device buffer, computed by the same kernels as the batch API, and copied
+// back synchronously, so threads may share one space as the reference's
+// callers do. The prepared-query handle lives in thread-local storage,
 // valid on the calling thread until its next rabitq_prepare_query, as in
 // the reference (space_rabitq.h:3700-3706).
 #include <hip/hip_runtime.h>
@@ -34,20 +35,38 @@ struct Arena {
     }
 };
 
-bool arena_for(Space *s, size_t bytes, Arena *a) {
+// Device staging memory of the calling thread. The reference's space is
+// shared read-only by many threads (Send + Sync, ours_diskann.rs:250-251)
+// whose workers call the bridge at once (encode workers, :2150-2199), so
+// nothing a call stages lives in the space: each thread grows and frees only
+// its own buffer. Calls go to the null stream, which orders every thread's
+// copies and kernels; each call waits for its own results before returning.
+struct Staging {
+    void *base = nullptr;
+    size_t cap = 0;
+    ~Staging() {
+        if (base) (void)hipFree(base);
+    }
+};
+
+thread_local Staging t_staging;
+
+bool arena_for(const Space *s, size_t bytes, Arena *a) {
+    (void)s;
     bytes += 64 * 256;
-    if (s->bridge_bytes < bytes) {
-        if (s->d_bridge) (void)hipFree(s->d_bridge);
-        s->d_bridge = nullptr;
-        s->bridge_bytes = 0;
+    Staging &st = t_staging;
+    if (st.cap < bytes) {
+        if (st.base) (void)hipFree(st.base);
+        st.base = nullptr;
+        st.cap = 0;
         size_t cap = 1 << 20;
         while (cap < bytes) cap <<= 1;
-        if (hipMalloc(&s->d_bridge, cap) != hipSuccess) return fail("bridge staging allocation failed") == 0;
-        s->bridge_bytes = cap;
+        if (hipMalloc(&st.base, cap) != hipSuccess) return fail("bridge staging allocation failed") == 0;
+        st.cap = cap;
     }
-    a->base = static_cast<char *>(s->d_bridge);
+    a->base = static_cast<char *>(st.base);
     a->used = 0;
-    a->cap = s->bridge_bytes;
+    a->cap = st.cap;
     return true;
 }
 
@@ -123,7 +142,7 @@ void gather(const void *base, size_t stride, size_t row_bytes, const uint32_t *i
 
 // Stage `count` compact records (+ optional residual records) and unpack
 // them to SoA rows 0..count-1.
-bool upload_records(Space *s, Arena *a, const std::vector<uint8_t> &comp, const std::vector<uint8_t> *res,
+bool upload_records(const Space *s, Arena *a, const std::vector<uint8_t> &comp, const std::vector<uint8_t> *res,
                     size_t count, qg_index_desc *d) {
     if (!stage_index(s, a, count, d)) return false;
     void *dc = a->take(comp.size());
@@ -155,7 +174,7 @@ uint32_t *upload_zeros_u32(Arena *a, size_t count) {
 
 // Stage the compact records ids[] (strided host rows) and run one pair scan
 // of `kernel` for the prepared query; out_f / out_est copied back.
-bool scan_compact(Space *s, const BridgeQuery *bq, int kernel, const uint32_t *ids, size_t count,
+bool scan_compact(const Space *s, const BridgeQuery *bq, int kernel, const uint32_t *ids, size_t count,
                   const void *base, size_t stride, float eps0, float *out_f, RabitqPaperEstimate *out_est) {
     std::vector<uint8_t> comp;
     gather(base, stride, s->compact_bytes, ids, count, &comp);
@@ -209,7 +228,7 @@ size_t rabitq_paper_msb_code_bytes(const void *space) { return (static_cast<cons
 size_t rabitq_paper_factor_bytes(void) { return 20; }
 
 bool rabitq_encode_full(const void *space, const float *raw, void *encoded_out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     if (!s || !raw || !encoded_out) return false;
     Arena a;
     const size_t ws_bytes = qg_encode_workspace_bytes(s, 1);
@@ -266,7 +285,7 @@ bool rabitq_copy_residual_record(const void *space, const void *full_encoded, vo
 
 bool rabitq_export_paper_sidecar(const void *space, const void *compact_records_in, size_t record_count,
                                  size_t compact_stride, uint8_t *msb_out, void *factors_out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     if (!s || (!compact_records_in && record_count) || !msb_out || !factors_out) return false;
     if (record_count == 0) return true;
     if (compact_stride < s->compact_bytes) return false;
@@ -372,7 +391,7 @@ bool rabitq_build_vamana_graph(void *space, const void *compact_records_in, size
 }
 
 float rabitq_symmetric_distance(const void *space, const void *lhs, const void *rhs) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const float inf = std::numeric_limits<float>::infinity();
     if (!s || !lhs || !rhs) return inf;
     std::vector<uint8_t> comp(2 * s->compact_bytes);
@@ -424,7 +443,7 @@ void rabitq_release_query(const void *space, const void *prepared) {
 bool rabitq_query_distance_batch(const void *space, const void *prepared, const uint32_t *ids, const uint8_t *modes,
                                  size_t count, const void *encoded_base, size_t encoded_stride,
                                  const float *short_ips, float *out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s || (count && (!ids || !modes || !encoded_base || !out))) return false;
     if (count == 0) return true;
@@ -476,7 +495,7 @@ float rabitq_query_distance(const void *space, const void *prepared, const void 
 bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared, const uint32_t *ids, size_t count,
                                          const uint8_t *msb_base, size_t msb_stride, const void *factors_base,
                                          size_t factors_stride, float epsilon0, RabitqPaperEstimate *out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s || (count && (!ids || !msb_base || !factors_base || !out))) return false;
     if (count == 0) return true;
@@ -525,7 +544,7 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
 bool rabitq_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
                                     const void *encoded_base, size_t encoded_stride, const void *residual_base,
                                     size_t residual_stride, const float *long_distances, RabitqDistanceInterval *out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s ||
         (count && (!ids || !encoded_base || !residual_base || !long_distances || !out)))
@@ -591,7 +610,7 @@ bool rabitq_train_centroids(const float *data, size_t count, size_t dim, size_t 
 
 bool rabitq_query_distance_b1_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
                                     const void *encoded_base, size_t encoded_stride, float *out_distances) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = query_of(s, prepared);
     if (!bq || (count && (!ids || !encoded_base || !out_distances))) return false;
     const bool b1 = bq->codec == QG_CODEC_B1 || bq->codec == QG_CODEC_B1_MAIN;
@@ -614,7 +633,7 @@ float rabitq_query_distance_b1(const void *space, const void *prepared, const vo
 bool rabitq_paper_estimate_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
                                  const void *encoded_base, size_t encoded_stride, float epsilon0,
                                  RabitqPaperEstimate *out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = query_of(s, prepared);
     if (!bq || (count && (!ids || !encoded_base || !out))) return false;
     if (count == 0) return true;
@@ -713,7 +732,7 @@ bool rabitq_full_residual_distance_batch(const void *space, const void *prepared
                                          size_t count, const void *encoded_base, size_t encoded_stride,
                                          const void *residual_base, size_t residual_stride,
                                          RabitqDistanceInterval *out) {
-    Space *s = const_cast<Space *>(static_cast<const Space *>(space));
+    const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = query_of(s, prepared);
     if (!bq || (count && (!ids || !encoded_base || !residual_base || !out))) return false;
     if (count == 0) return true;

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         __syncthreads();
 
         uint32_t len = 0;
-        uint64_t hops = 0, gates = 0, dists = 1, pruned = 0;
+        uint64_t hops = 0, gates = 0, dists = 1, pruned = 0, rchecked = 0, rpruned = 0;
         // start node 0 with its full FP32 distance
         if (a.n > 0 && L > 0 && wv == 0) {
             if (lane == 0) sid[0] = 0u;
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 // branch. When the pool is full, the back is the larger of the
                 // last retained pool entry (window lane e - 1 - tb) and the
                 // last accepted survivor (lane na - 1).
-                uint32_t npruned = 0;
+                uint32_t npruned = 0, nchecked = 0;
                 for (uint32_t b = 0; b < ns; ++b) {
                     const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_d), (int)b));
                     const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)my_id, (int)b);
@@ -434,6 +434,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     const bool prune = full & (v != 0u) & (lb > bk_d);
                     const bool take = !full | (!prune & cand_less_nb(d, id, bk_d, bk_id));
                     npruned += prune ? 1u : 0u;
+                    nchecked += (full & (v != 0u)) ? 1u : 0u;
                     e -= (full & take & !from_a) ? 1u : 0u;
                     const uint32_t nna = na - ((full & take & from_a) ? 1u : 0u);
                     // sorted insert into the accepted list: lanes past the slot take
@@ -448,6 +449,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     na = nna + (take ? 1u : 0u);
                 }
                 pruned += npruned;
+                rpruned += npruned;
+                rchecked += nchecked;
                 // merge pool[0, e) with the accepted list into pool[0, e + na), in
                 // place: retained entries only move right, so they move in
                 // descending chunks of 64 (each read before any write), then
@@ -592,8 +595,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             }
         }
         if (t == 0 && a.stats) {
-            uint64_t *st = a.stats + (uint64_t)qi * 4;
-            st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned;
+            uint64_t *st = a.stats + (uint64_t)qi * 6;
+            st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned; st[4] = rchecked; st[5] = rpruned;
         }
         __syncthreads();
     }

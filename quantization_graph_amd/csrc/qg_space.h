@@ -22,8 +22,6 @@ struct Space {
     float *d_center = nullptr;
     float *d_rot_center = nullptr;
     float center_norm_sqr = 0.0f;
-    void *d_bridge = nullptr;  // staging buffer of the host-memory bridge
-    size_t bridge_bytes = 0;
 };
 
 Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits, size_t block,

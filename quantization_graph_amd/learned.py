@@ -124,7 +124,7 @@ def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=No
     slots = slots or max(1, min(nq, 4096))
     pool_ids = torch.empty((nq, L), dtype=torch.int32, device="cuda")
     pool_d = torch.empty((nq, L), dtype=torch.float32, device="cuda")
-    stats = torch.zeros((nq, 4), dtype=torch.int64, device="cuda")
+    stats = torch.zeros((nq, 6), dtype=torch.int64, device="cuda")
     Lb = _lib.lib()
     ws_bytes = Lb.qg_search_workspace_bytes(index.n, slots)
     ws = workspace if workspace is not None else torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")

@@ -270,7 +270,7 @@ def run_ours_exrabitq4(ds: Dataset, cfg: RunConfig, out_root: Path, progress_log
     """One Ours run (run_ours_exrabitq4, diskann_runner.rs:155-552) on the
     GPU; returns the payload dict the writers take."""
     import torch
-    from .space import RabitqSpace
+    from .space import RabitqSpace, search_counters
     if cfg.centroid_count != 1:
         raise SystemExit("ExRaBitQ centroid count K > 1 is not implemented by this build (K = 1 only)")
 
@@ -372,9 +372,10 @@ def run_ours_exrabitq4(ds: Dataset, cfg: RunConfig, out_root: Path, progress_log
             h_ids = ids.cpu().numpy().astype(np.int64)
             hits += int((h_ids[:, :, None] == gt[:, None, :].astype(np.int64)).any(-1).sum())
             total += K * nq
-            s = st.sum(0).cpu().numpy().astype(np.float64)
-            stats_sum = s if stats_sum is None else stats_sum + s
+            s = {name: float(v.sum()) for name, v in search_counters(st.cpu().numpy()).items()}
+            stats_sum = s if stats_sum is None else {name: stats_sum[name] + v for name, v in s.items()}
         elapsed = time.perf_counter() - start
+        ctr = stats_sum
         acc = accuracy_metrics(h_ids, dists.cpu().numpy(), queries, base, gt)
         qn = float(nq * cfg.repeats)
         lat_sorted = sorted(lat)
@@ -384,12 +385,14 @@ def run_ours_exrabitq4(ds: Dataset, cfg: RunConfig, out_root: Path, progress_log
             latency_p95_us=lat_sorted[int(round((len(lat_sorted) - 1) * 0.95))],
             prepare_us=prep_s * 1e6 / qn, traverse_us=search_s * 1e6 / qn, rerank_us=float("nan"),
             neighbor_fetch_us=float("nan"), visited_mark_us=float("nan"), paper_batch_us=float("nan"),
-            flush_us=float("nan"), visited_nodes=stats_sum[0] / qn, distance_computations=stats_sum[2] / qn,
-            paper_checked=stats_sum[1] / qn, paper_would_prune=stats_sum[3] / qn,
-            paper_msb_kernel_calls=float("nan"), paper_remaining_kernel_calls=float("nan"),
+            flush_us=float("nan"), visited_nodes=ctr["visited_nodes"] / qn,
+            distance_computations=ctr["distance_computations"] / qn,
+            paper_checked=ctr["paper_checked"] / qn, paper_would_prune=ctr["paper_would_prune"] / qn,
+            paper_msb_kernel_calls=ctr["paper_msb_kernel_calls"] / qn,
+            paper_remaining_kernel_calls=ctr["paper_remaining_kernel_calls"] / qn,
             ffi_calls_per_query=0.0, query_coarse_codec=codec, status="done", **acc))
         progress("search", "search_list_size=%d query_coarse_codec=%s recall=%.6f qps=%.3f paper_checked=%d "
-                 "traverse_us=%.3f" % (L, codec, results[-1]["recall"], results[-1]["qps"], int(stats_sum[1]),
+                 "traverse_us=%.3f" % (L, codec, results[-1]["recall"], results[-1]["qps"], int(ctr["paper_checked"]),
                                        results[-1]["traverse_us"]))
         log("L=%d recall@10=%.4f qps=%.1f" % (L, results[-1]["recall"], results[-1]["qps"]))
     progress("search", "done")

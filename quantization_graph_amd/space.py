@@ -351,10 +351,12 @@ class RabitqSpace:
         enqueues kernels only (no synchronisation), so it can be captured in a
         HIP graph; read the error flag later with `search_error(workspace)`.
 
-        Returns (ids [nq, k] int32, estimates [nq, k], stats [nq, 4] int64 =
-        hops, gate evaluations, distances, pruned[, pool ids, pool
-        distances [nq, max(l_search, k)]]). `events` = (start, end) CUDA
-        events recorded on the launch stream around the search kernel."""
+        Returns (ids [nq, k] int32, estimates [nq, k], stats [nq, 6] int64 =
+        hops, gate evaluations, distances computed, prunes, re-checks with
+        the pool full, re-check prunes[, pool ids, pool distances [nq,
+        max(l_search, k)]]); `search_counters` turns stats into the
+        reference's counters. `events` = (start, end) CUDA events recorded on
+        the launch stream around the search kernel."""
         degrees = _dev(degrees, torch.int32).reshape(-1)
         edges = _dev(edges, torch.int32)
         if degrees.numel() != index.n or edges.shape[0] != index.n:
@@ -370,7 +372,7 @@ class RabitqSpace:
                 raise ValueError("l_per_query needs one entry per query")
         ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
         dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
-        stats = torch.zeros((nq, 4), dtype=torch.int64, device="cuda")
+        stats = torch.zeros((nq, 6), dtype=torch.int64, device="cuda")
         pool_ids = torch.empty((nq, L), dtype=torch.int32, device="cuda") if want_pool else None
         pool_d = torch.empty((nq, L), dtype=torch.float32, device="cuda") if want_pool else None
         L_ = _lib.lib()
@@ -421,6 +423,28 @@ class RabitqSpace:
                                 C.byref(evals), _ptr(ws), ws_bytes, _stream()))
         del ws
         return degrees, edges, evals.value
+
+
+def search_counters(stats):
+    """The reference runner's per-query search counters from qg_search's
+    stats rows ([..., 6] integers: hops, gate evaluations, distances
+    computed, prunes, re-checks with the pool full and the gate valid,
+    re-check prunes), following where ours_diskann.rs:1539-1986 counts them:
+    visited_nodes is the start node plus every fresh neighbour (:1604, 1659);
+    distance_computations and paper_remaining_kernel_calls count the
+    distances that pass the re-check at insertion (:1960-1961);
+    paper_checked counts first-pass prunes plus re-checks made with the pool
+    full and the gate valid (:1720, 1947); paper_would_prune and
+    paper_full_saved count prunes at both sites; paper_msb_kernel_calls the
+    gate evaluations. Returns a dict of arrays (or numbers) of those names."""
+    import numpy as np
+    st = np.asarray(stats, dtype=np.int64)
+    hops, gates, dists, pruned, rchecked, rpruned = (st[..., i] for i in range(6))
+    computed = dists - rpruned
+    return dict(hops=hops, visited_nodes=1 + gates, distance_computations=computed,
+                paper_checked=pruned - rpruned + rchecked, paper_would_prune=pruned, paper_full_saved=pruned,
+                paper_not_pruned=rchecked - rpruned, paper_msb_kernel_calls=gates,
+                paper_remaining_kernel_calls=computed)
 
 
 def search_error(workspace):

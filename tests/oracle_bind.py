@@ -70,7 +70,43 @@ class Interval(C.Structure):
 
 class SearchStats(C.Structure):
     _fields_ = [("hops", C.c_uint64), ("gate_evals", C.c_uint64),
-                ("full_dists", C.c_uint64), ("pruned", C.c_uint64)]
+                ("full_dists", C.c_uint64), ("pruned", C.c_uint64),
+                ("recheck_checked", C.c_uint64), ("recheck_pruned", C.c_uint64)]
+
+    def as_tuple(self):
+        return (self.hops, self.gate_evals, self.full_dists, self.pruned, self.recheck_checked,
+                self.recheck_pruned)
+
+    def reference_counters(self):
+        """The reference's counters (ours_diskann.rs:1604-1968, the same
+        values as vamana::SearchStats of search_vamana_paper_active) from
+        these: (visited_nodes, distance_computations, hops, paper_checked,
+        paper_would_prune, paper_not_pruned, paper_msb_kernel_calls,
+        paper_remaining_kernel_calls)."""
+        dc = self.full_dists - self.recheck_pruned
+        first_pruned = self.pruned - self.recheck_pruned
+        return (1 + self.gate_evals, dc, self.hops, first_pruned + self.recheck_checked, self.pruned,
+                self.recheck_checked - self.recheck_pruned, self.gate_evals, dc)
+
+
+class SearchEvent(C.Structure):
+    """ro_search_event: one prune decision of the oracle's search."""
+    _fields_ = [("hop", C.c_uint32), ("id", C.c_uint32), ("lower_bound", C.c_float), ("back", C.c_float),
+                ("stage", C.c_uint8), ("full", C.c_uint8), ("valid", C.c_uint8), ("pruned", C.c_uint8)]
+
+
+SEARCH_EVENT_DTYPE = np.dtype([("hop", "<u4"), ("id", "<u4"), ("lower_bound", "<f4"), ("back", "<f4"),
+                               ("stage", "u1"), ("full", "u1"), ("valid", "u1"), ("pruned", "u1")])
+
+
+class RefStats(C.Structure):
+    """SrStats of oracle/search_ref.cpp: the reference's search counters."""
+    _fields_ = [(n, C.c_uint64) for n in ("visited_nodes", "distance_computations", "hops", "paper_checked",
+                                          "paper_would_prune", "paper_not_pruned", "paper_msb_kernel_calls",
+                                          "paper_remaining_kernel_calls")]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
 
 
 ESTIMATE_DTYPE = np.dtype([("lower_bound", "<f4"), ("short_ip", "<f4"), ("alpha", "<f4"),
@@ -144,6 +180,8 @@ class Oracle:
                                 C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_size_t,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p]
+        L.ro_search_traced.restype = C.c_size_t
+        L.ro_search_traced.argtypes = list(L.ro_search.argtypes) + [C.c_void_p, C.c_size_t, C.c_void_p]
         L.ro_robust_prune.restype = C.c_size_t
         L.ro_robust_prune.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32,
                                       C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
@@ -360,6 +398,27 @@ class Oracle:
                                  n, k, L, C.c_float(eps0), rerank, _p(ids), _p(ds), _p(pool_ids), _p(pool_d),
                                  C.byref(plen), C.byref(st))
         return ids[:got], ds[:got], pool_ids[:plen.value], pool_d[:plen.value], st
+
+    def search_traced(self, q, codec, comp, res, msb, fac, degrees, edges, k=10, L=100, eps0=1.9, rerank=100,
+                      trace_cap=1 << 20):
+        """search() plus the trace of every prune decision (SEARCH_EVENT_DTYPE)."""
+        n = comp.shape[0]
+        ids = np.zeros(k, np.uint32)
+        ds = np.zeros(k, np.float32)
+        pool_ids = np.zeros(max(L, k) + 1, np.uint32)
+        pool_d = np.zeros(max(L, k) + 1, np.float32)
+        plen = C.c_size_t()
+        st = SearchStats()
+        trace = np.zeros(trace_cap, SEARCH_EVENT_DTYPE)
+        tlen = C.c_size_t()
+        got = self.lib.ro_search_traced(self.sp, C.byref(q), codec, _p(comp), comp.shape[1], _p(res), res.shape[1],
+                                        _p(msb), msb.shape[1], _p(fac), _p(degrees), _p(edges), edges.shape[1],
+                                        n, k, L, C.c_float(eps0), rerank, _p(ids), _p(ds), _p(pool_ids),
+                                        _p(pool_d), C.byref(plen), C.byref(st), _p(trace), trace_cap,
+                                        C.byref(tlen))
+        assert tlen.value <= trace_cap, "trace buffer too small"
+        return (ids[:got], ds[:got], pool_ids[:plen.value], pool_d[:plen.value], st,
+                trace[:tlen.value].copy())
 
     def robust_prune(self, comp, source, pool_ids, pool_d, r, alpha, with_evals=False):
         pool_ids = np.ascontiguousarray(pool_ids, np.uint32)
@@ -837,3 +896,81 @@ class RefIndexIO:
 
     def free(self, h):
         self.lib.rabitq_space_delete(h)
+
+
+SEARCH_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libsearch_ref.so")
+
+
+class RefGraphSearch:
+    """The reference's own graph search behind oracle/search_ref.cpp
+    (oracle/_ref/libsearch_ref.so), over a space made by the bridge compiled
+    into the same library: VamanaIndex::searchKnnPlainThenResidualRerank
+    (vamana_index.h:530-636) and vamana::search_vamana_paper_active
+    (vamana_search.h:216-364)."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(SEARCH_REF_SO)
+
+    def __init__(self, dim, seed, center):
+        L = C.CDLL(SEARCH_REF_SO)
+        L.rabitq_space_new.restype = C.c_void_p
+        L.rabitq_space_new.argtypes = [C.c_size_t, C.c_uint32, C.c_size_t, C.c_size_t, C.c_size_t, C.c_bool,
+                                       C.c_bool]
+        L.rabitq_space_delete.argtypes = [C.c_void_p]
+        L.rabitq_space_set_center.argtypes = [C.c_void_p, C.c_void_p]
+        L.rabitq_set_query_coarse_codec.argtypes = [C.c_void_p, C.c_int]
+        L.sr_index_open.restype = C.c_void_p
+        L.sr_index_open.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_size_t, C.c_float]
+        L.sr_index_close.argtypes = [C.c_void_p]
+        L.sr_index_knn.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]
+        L.sr_pool.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_void_p]
+        self.lib = L
+        self.h = L.rabitq_space_new(dim, seed, 1, 4, 16, True, True)
+        self.center = np.ascontiguousarray(center, np.float32)
+        L.rabitq_space_set_center(self.h, _p(self.center))
+        self.ix = None
+
+    def __del__(self):
+        if getattr(self, "ix", None):
+            self.lib.sr_index_close(self.ix)
+        if getattr(self, "h", None):
+            self.lib.rabitq_space_delete(self.h)
+
+    def set_codec(self, codec):
+        self.lib.rabitq_set_query_coarse_codec(self.h, codec)
+
+    def open_index(self, index_path, residual_path, n, eps0=1.9):
+        self.ix = self.lib.sr_index_open(self.h, str(index_path).encode(), str(residual_path).encode(), n,
+                                         C.c_float(eps0))
+        if not self.ix:
+            raise RuntimeError("the reference rejected the index or residual file")
+
+    def knn(self, query, k, L, rerank=100):
+        query = np.ascontiguousarray(query, np.float32)
+        labels = np.zeros(k, np.uint64)
+        d = np.zeros(k, np.float32)
+        cnt = C.c_size_t()
+        st = RefStats()
+        if self.lib.sr_index_knn(self.ix, _p(query), k, L, rerank, _p(labels), _p(d), C.byref(cnt),
+                                 C.byref(st)) != 0:
+            raise RuntimeError("reference search failed")
+        return labels[:cnt.value].astype(np.uint32), d[:cnt.value], st
+
+    def pool(self, query, comp, msb, fac, degrees, edges, L, eps0=1.9):
+        query = np.ascontiguousarray(query, np.float32)
+        n = comp.shape[0]
+        ids = np.zeros(max(L, 1) + 1, np.uint32)
+        d = np.zeros(max(L, 1) + 1, np.float32)
+        plen = C.c_size_t()
+        st = RefStats()
+        degrees = np.ascontiguousarray(degrees, np.uint32)
+        edges = np.ascontiguousarray(edges, np.uint32)
+        if self.lib.sr_pool(self.h, _p(query), _p(comp), comp.shape[1], _p(msb), msb.shape[1], _p(fac),
+                            _p(degrees), _p(edges), edges.shape[1], n, L, C.c_float(eps0), _p(ids), _p(d),
+                            C.byref(plen), C.byref(st)) != 0:
+            raise RuntimeError("reference traversal failed")
+        return ids[:plen.value], d[:plen.value], st

@@ -82,6 +82,64 @@ def test_bridge_matches_reference(gpu, dim):
             np.float32(ours.symmetric(comp[a], comp[b])).tobytes()
 
 
+@pytest.mark.parametrize("dim", [960, 1536])
+def test_bridge_threads_share_one_space(gpu, dim):
+    """Eight host threads call the bridge on one shared space at once, as
+    the reference's encode workers and query threads do (ours_diskann.rs:
+    2150-2199; the space is Send + Sync, :250-251): encode_full, and per
+    prepared query (thread-local) the gate batch, the distance batch and the
+    residual batch, with batch sizes that differ per thread so each thread's
+    staging memory grows on its own. Every output equals the reference
+    bridge's, byte for byte."""
+    from concurrent.futures import ThreadPoolExecutor
+    if not Ref.available():
+        pytest.skip("oracle/_ref not built")
+    x, q = gaussian_mixture(320, dim, 8, 0.3, 17, nq=16)
+    c = x.mean(0).astype(np.float32)
+    ref, ours = Ref(dim, 100), Ours(dim, 100)
+    for s in (ref, ours):
+        s.set_center(c)
+    full = ref.encode_full(x)
+    comp, res = ref.split(full)
+    msb, fac = ref.sidecar(comp)
+    rng = np.random.default_rng(3)
+    perms = [rng.permutation(x.shape[0]).astype(np.uint32)[:40 + 30 * w] for w in range(8)]
+    want = {}
+    for codec in (0, 3):
+        ref.set_codec(codec)
+        for qi in range(q.shape[0]):
+            ids = perms[qi % 8]
+            pr = ref.prepare(q[qi])
+            e = ref.gate_batch(pr, msb, fac, ids, 1.9)
+            d = ref.distance_batch(pr, comp, ids, np.zeros(ids.size, np.uint8))
+            iv = ref.residual_batch(pr, comp, res, ids, d)
+            want[codec, qi] = (e.tobytes(), d.tobytes(), iv.tobytes())
+
+    def worker(w):
+        got = {}
+        rows = np.arange(w, x.shape[0], 8)
+        enc = ours.encode_full(x[rows])
+        assert np.array_equal(enc, full[rows]), "encode_full, thread %d" % w
+        for qi in range(w, q.shape[0], 8):
+            ids = perms[qi % 8]
+            po = ours.prepare(q[qi])
+            e = ours.gate_batch(po, msb, fac, ids, 1.9)
+            d = ours.distance_batch(po, comp, ids, np.zeros(ids.size, np.uint8))
+            iv = ours.residual_batch(po, comp, res, ids, d)
+            got[qi] = (e.tobytes(), d.tobytes(), iv.tobytes())
+        return got
+
+    for codec in (0, 3):
+        ours.set_codec(codec)  # between sweeps, as the runner does (diskann_runner.rs:361)
+        print("case: 8 threads on one space, D %d codec %d" % (dim, codec), flush=True)
+        for rep in range(2):
+            with ThreadPoolExecutor(8) as pool:
+                results = list(pool.map(worker, range(8)))
+            for got in results:
+                for qi, outs in got.items():
+                    assert outs == want[codec, qi], (codec, qi, rep)
+
+
 def test_bridge_error_sentinels(gpu):
     ours = Ours(128, 100)
     L = ours.lib

@@ -109,27 +109,41 @@ def test_training_reduces_pinball_loss(gpu):
     assert 0.8 < float((t <= y).mean()) < 0.97
 
 
-@pytest.fixture(scope="module")
-def powerlaw_case(gpu):
+def _powerlaw(n, dim, nq):
     import torch
     import bench
     import quantization_graph_amd as qg
     from quantization_graph_amd import learned
-    n, dim = 20000, 256
-    x, q = bench.make_powerlaw(torch, n, dim, 3500, 4)
+    print("case: power-law base %d x %d, %d queries" % (n, dim, nq), flush=True)
+    x, q = bench.make_powerlaw(torch, n, dim, nq, 4)
     gt = bench.exact_topk(torch, x, q, 10)
     sp = qg.RabitqSpace(dim, 100)
     sp.set_center(x.double().mean(0).float().cpu().numpy())
     ix = sp.encode(x)
-    deg, edges, _ = sp.build_graph(ix, r=32, l_build=100, alpha=1.2, batch=2048, refine_passes=1, prune_cap=128)
+    r = 32 if dim <= 1024 else 64
+    deg, edges, _ = sp.build_graph(ix, r=r, l_build=100, alpha=1.2, batch=2048, refine_passes=1, prune_cap=128)
     rstats = learned.residual_stats(sp, ix)
-    return dict(x=x, q=q, gt=gt, sp=sp, ix=ix, deg=deg, edges=edges, rstats=rstats)
+    return dict(dim=dim, x=x, q=q, gt=gt, sp=sp, ix=ix, deg=deg, edges=edges, rstats=rstats)
+
+
+@pytest.fixture(scope="module")
+def powerlaw_case(gpu):
+    return _powerlaw(20000, 256, 3500)
+
+
+@pytest.fixture(scope="module", params=[256, 1536], ids=lambda d: "D%d" % d)
+def parity_case(gpu, request, powerlaw_case):
+    """The D=256 case, and D=1536 (code dimension 2048, the learned
+    workload's) on a degree-64 graph."""
+    if request.param == 256:
+        return powerlaw_case
+    return _powerlaw(8000, 1536, 400)
 
 
 @pytest.mark.parametrize("codec", ["full", "int8"])
-def test_plain_rerank_equals_fused_search(gpu, powerlaw_case, codec):
+def test_plain_rerank_equals_fused_search(gpu, parity_case, codec):
     from quantization_graph_amd import learned
-    c = powerlaw_case
+    c = parity_case
     sp, ix = c["sp"], c["ix"]
     qv = sp.prepare(c["q"][:300], codec)
     ids, dists, st = sp.search(ix, qv, c["deg"], c["edges"], k=10, l_search=64, rerank=100)
@@ -173,13 +187,13 @@ def _query_norms(qv):
     return qs[:, 0], qs[:, 1]
 
 
-def test_features_match_oracle(gpu, powerlaw_case):
+def test_features_match_oracle(gpu, parity_case):
     """Feature rows of full pools, pools cut short (empty slots inside the
     depth) and an empty pool, against the FP64 restatement: each bf16 value
     within one bf16 ulp."""
     import torch
     from quantization_graph_amd import learned
-    c = powerlaw_case
+    c = parity_case
     sp, ix = c["sp"], c["ix"]
     qv = sp.prepare(c["q"][:160], "full")
     pid, pd, _ = learned.search_pool(sp, ix, qv, c["deg"], c["edges"], 128)
@@ -201,13 +215,13 @@ def test_features_match_oracle(gpu, powerlaw_case):
 
 
 @pytest.mark.parametrize("codec", ["full", "int8"])
-def test_learned_depth_rule_and_rerank(gpu, powerlaw_case, codec):
+def test_learned_depth_rule_and_rerank(gpu, parity_case, codec):
     """Per-query depths from given predictions equal the oracle's rule, and
     the learned rerank's ids and estimates equal, bit for bit, the plain
     rerank of pools cut at those depths."""
     import torch
     from quantization_graph_amd import learned
-    c = powerlaw_case
+    c = parity_case
     sp, ix = c["sp"], c["ix"]
     qv = sp.prepare(c["q"][:300], codec)
     pid, pd, _ = learned.search_pool(sp, ix, qv, c["deg"], c["edges"], 128)
