@@ -217,6 +217,22 @@ int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc
                  void *workspace, size_t workspace_bytes, void *stream);
 size_t qg_search_lds_bytes(uint32_t cd, uint32_t L);
 
+/* qg_search_ex without the workspace argument; it allocates nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The work-queue head, error flag and
+ * per-slot visited bitsets live in a file-scope device array of
+ * qg_search_noalloc_scratch_bytes() (32 MiB) that each GPU holds until the
+ * process ends; calls whose qg_search_workspace_bytes(n, slots) exceeds it
+ * fail with an error. Every call on one GPU shares that array, so calls that
+ * overlap on one GPU race. Outputs and errors as qg_search_ex. */
+size_t qg_search_noalloc_scratch_bytes(void);
+int qg_search_noalloc(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                      const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
+                      uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots,
+                      uint32_t waves, uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids,
+                      float *pool_dists, void *stream);
+
 /* ------------------------------------- learned adaptive rerank depth
  * No counterpart in the reference. A 24-64-64-1 MLP (BF16 MFMA, FP32
  * accumulation; parameters are qg_mlp_param_count() FP32 values laid out

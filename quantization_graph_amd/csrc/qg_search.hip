@@ -27,8 +27,11 @@
 // accumulator j), 4 * W candidates at a time, from MSB and code
 // rows staged in LDS by gfx950's global-to-LDS loads, one memory round trip
 // per batch.
+#include <mutex>
+
 #include "qg_args.h"
 #include "qg_device.h"
+#include "qg_space.h"
 
 namespace qg {
 
@@ -629,4 +632,58 @@ extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
     const size_t smem = qg_search_lds_bytes(a->cd, a->L);
     return waves == 2 ? launch_search_w<2>(a, slots, smem, stream) : launch_search_w<4>(a, slots, smem, stream);
+}
+
+// ------------------------------------------ search without a workspace
+// qg_search_noalloc takes its work-queue head, error flag and per-slot
+// visited bitsets from `scratch`, a file-scope device array that each GPU
+// holds from the library's load to the end of the process, so the call
+// allocates nothing. Host code never passes `scratch` itself (its host-side
+// name is not a device address): the array's address on each device comes
+// from hipGetSymbolAddress on that device's first call and is kept per
+// device. Every call on one GPU shares that GPU's array; the search kernel
+// clears each slot's bitset before it uses it and a zero-fill kernel clears
+// the queue head and error flag at launch, so leftovers of an earlier call do
+// not matter, but calls that overlap on one GPU race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;  // 32 MiB per GPU
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+extern "C" size_t qg_search_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int qg_search_noalloc(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                                 const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
+                                 uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank,
+                                 uint32_t slots, uint32_t waves, uint32_t *out_ids, float *out_dists,
+                                 uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists, void *stream) {
+    if (!ix) return qg::fail("null argument to qg_search_noalloc");
+    if (slots == 0) slots = 1;
+    const size_t need = qg_search_workspace_bytes(ix->n, slots);
+    if (need > kNoAllocScratchBytes)
+        return qg::fail("qg_search_noalloc: %u slots over %llu points need %zu bytes of scratch, above its "
+                        "%zu-byte maximum (use fewer slots, or qg_search_ex with a workspace)",
+                        slots, (unsigned long long)ix->n, need, kNoAllocScratchBytes);
+    int dev = 0;
+    if (qg::check_hip((int)hipGetDevice(&dev), "qg_search_noalloc: hipGetDevice")) return -1;
+    if (dev < 0 || dev >= kMaxDevices) return qg::fail("qg_search_noalloc: device %d out of range", dev);
+    void *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            if (qg::check_hip((int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                              "qg_search_noalloc: address of the scratch array"))
+                return -1;
+            g_scratch_ptr[dev] = p;
+        }
+        ptr = g_scratch_ptr[dev];
+    }
+    return qg_search_ex(space, ix, qv, codec, degrees, edges, edge_stride, k, l_search, l_per_query, eps0, rerank,
+                        slots, waves, out_ids, out_dists, out_stats, pool_ids, pool_dists, ptr, kNoAllocScratchBytes,
+                        stream);
 }
