@@ -172,6 +172,32 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
 int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_query, uint32_t k,
             uint32_t *out_ids, float *out_values, uint32_t *out_pos, void *stream);
 
+/* ------------------------------------------------ precision mode
+ * A QR rotation in place of the space's sign-flipped Walsh-Hadamard one
+ * (the reference's formal rotation, space_rabitq.h:2900-2912; that remains
+ * the default and the parity mode). No counterpart in the reference's
+ * formal codec; the QR-orthogonalised Gaussian it follows is
+ * experiments/05_disk_system_fair/quantizers.py:383-390.
+ * qg_space_set_rotation_qr takes a Gaussian matrix (host, FP64,
+ * code_dim x code_dim row-major), factors it G = QR in FP64 on the GPU
+ * (classical Gram-Schmidt with reorthogonalisation, diag(R) > 0), and from
+ * then on the space rotates with rotated = Q * pad(x): encode and query
+ * preparation run the product on the BF16 matrix cores with FP32
+ * accumulation, both operands split into bf16_parts BF16 parts (2: hi + lo,
+ * 3 products, about 16 significant bits; 3: hi + mid + lo, 6 products, FP32
+ * class), encode normalises the rotated residual
+ * by its compensated FP32 norm and takes |x - c|, the short and error
+ * factors from compensated FP32 sums; everything else is the codec as
+ * before. Needs code_dim >= 128. The 1-bit sign-plane scan refuses such a
+ * space. qg_space_rotation_qr copies Q (code_dim x code_dim row-major) to
+ * the host; qg_rotate_rows applies the space's rotation to n device rows
+ * (out [n][code_dim]; subtract_center only for QR spaces: Q * pad(x - c),
+ * the product encode uses). */
+int qg_space_set_rotation_qr(void *space, const double *gaussian, int bf16_parts);
+int qg_space_rotation_qr(const void *space, double *q_out);
+int qg_rotate_rows(const void *space, const float *rows, uint64_t n, int subtract_center, float *out,
+                   void *stream);
+
 /* Batched graph search: greedy beam search (beam width 1, start node 0)
  * with the 1-bit gate, remaining-bit distances and the residual-4 rerank of
  * the first max(rerank, k) pool entries, one 4-wavefront workgroup per

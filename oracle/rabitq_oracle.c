@@ -404,7 +404,61 @@ void ro_encode_full(const ro_space *s, const float *x, uint8_t *out) {
     }
     float *rot = (float *)malloc(cd * sizeof(float));
     ro_rotate(s, res, rot);
+    ro_encode_from_unit(s, rn, rot, 0, out);
+    free(res); free(rot);
+}
 
+/* Compensated FP32 sums of the precision mode: Knuth's TwoSum and the fma
+ * TwoProduct (Ogita, Rump and Oishi's Sum2 / Dot2), in the kernels' order. */
+typedef struct { float s, c; } comp_t;
+static void comp_add(comp_t *a, float p) {
+    const float t = a->s + p;
+    const float bp = t - a->s;
+    a->c = a->c + ((a->s - (t - bp)) + (p - bp));
+    a->s = t;
+}
+static void comp_add_sq(comp_t *a, float r) {
+    const float p = r * r;
+    const float e = fmaf(r, r, -p);
+    comp_add(a, p);
+    a->c = a->c + e;
+}
+static void comp_merge(comp_t *a, comp_t b) {
+    comp_add(a, b.s);
+    a->c = a->c + b.c;
+}
+
+/* Precision-mode encode of x given y = Q (x - c), the rotation GEMM's
+ * output: |x - c| by a compensated sum in index order; the unit vector is y
+ * divided by its compensated norm, summed as the encode workgroup does (256
+ * strided partial sums, then a pairwise tree over them); the short and error
+ * factors from a compensated sum of |o| in FP32; the codec as before. */
+void ro_encode_full_rotated(const ro_space *s, const float *x, const float *y, uint8_t *out) {
+    const size_t D = s->dim, cd = s->code_dim;
+    comp_t rc = {0.0f, 0.0f};
+    for (size_t i = 0; i < D; ++i) comp_add_sq(&rc, x[i] - s->center[i]);
+    const float rn = sqrtf(rc.s + rc.c);
+    comp_t part[256];
+    for (size_t t = 0; t < 256; ++t) {
+        part[t].s = 0.0f; part[t].c = 0.0f;
+        for (size_t i = t; i < cd; i += 256) comp_add_sq(&part[t], y[i]);
+    }
+    for (size_t h = 128; h > 0; h >>= 1)
+        for (size_t t = 0; t < h; ++t) comp_merge(&part[t], part[t + h]);
+    const float n2 = part[0].s + part[0].c;
+    const float inv = n2 > 0.0f ? 1.0f / sqrtf(n2) : 0.0f;
+    float *rot = (float *)malloc(cd * sizeof(float));
+    for (size_t i = 0; i < cd; ++i) rot[i] = y[i] * inv;
+    ro_encode_from_unit(s, rn, rot, 1, out);
+    free(rot);
+}
+
+/* Everything after the rotation: fastQuantizeAbs, nibble codes, scales,
+ * residual codes and factors of the rotated unit vector rot of a residual
+ * of norm rn; precision = 1 takes the short and error factors from a
+ * compensated FP32 sum of |rot| instead of the reference's FP64 sums. */
+void ro_encode_from_unit(const ro_space *s, float rn, const float *rot, int precision, uint8_t *out) {
+    const size_t cd = s->code_dim;
     memset(out, 0, s->full_bytes);
     uint8_t *scales = out + 24;
     uint8_t *code = scales + s->scale_bytes;
@@ -429,7 +483,23 @@ void ro_encode_full(const ro_space *s, const float *x, uint8_t *out) {
         o_obar = madd_d(RO_FMA_ENC_OOBAR, (double)rot[i] * sign, inv_sqrt_d, o_obar);
         half_l1 += 0.5 * fabs((double)rot[i]);
     }
-    if (rn > 0.0f && half_l1 > 1e-12) {
+    if (precision) {
+        comp_t l1c = {0.0f, 0.0f};
+        for (size_t i = 0; i < cd; ++i) comp_add(&l1c, fabsf(rot[i]));
+        const float l1 = l1c.s + l1c.c, half = 0.5f * l1;
+        if (rn > 0.0f && half > 1e-12f) {
+            short_scale = 2.0f * rn / half;
+            const float sq = sqrtf((float)cd);
+            float o = l1 / sq;
+            if (!isfinite(o)) o = 0.8f;
+            if (o < 1e-6f) o = 1e-6f;
+            if (o > 0.999999f) o = 0.999999f;
+            const float o2 = o * o;
+            float r = (1.0f - o2) / o2;
+            if (r < 0.0f) r = 0.0f;
+            error_scale = sqrtf(r) * (0.359375f / sq) * 2.0f * rn;
+        }
+    } else if (rn > 0.0f && half_l1 > 1e-12) {
         short_scale = (float)(2.0 * (double)rn / half_l1);
         if (!isfinite(o_obar)) o_obar = 0.8;
         if (o_obar < 1e-6) o_obar = 1e-6;
@@ -472,7 +542,7 @@ void ro_encode_full(const ro_space *s, const float *x, uint8_t *out) {
     memcpy(out, hdr, 16);
     memcpy(out + 16, rf, 8);
     *cid = 0;
-    free(res); free(rot); free(a); free(m); free(qerr); free(rc);
+    free(a); free(m); free(qerr); free(rc);
 }
 
 /* copyCompactPayloadFromFull / copyResidualRecordFromFull, :6005-6029. */
@@ -524,12 +594,20 @@ void ro_query_free(ro_query *q) { free(q->q); free(q->x8); memset(q, 0, sizeof(*
 /* initializePreparedQuery + buildCentroidQuery, space_rabitq.h:3429-3698
  * (K = 1; Full or Int8 codec). */
 void ro_prepare_query(const ro_space *s, const float *raw, int codec, ro_query *q) {
-    const size_t D = s->dim, cd = s->code_dim;
-    float *rq = (float *)malloc(cd * sizeof(float));
+    float *rq = (float *)malloc(s->code_dim * sizeof(float));
     ro_rotate(s, raw, rq);
+    ro_prepare_query_rotated(s, raw, rq, s->rotated_center, codec, q);
+    free(rq);
+}
+
+/* Query preparation from the rotated raw query rq and rotated center rc
+ * (the precision mode passes the rotation GEMM's outputs). */
+void ro_prepare_query_rotated(const ro_space *s, const float *raw, const float *rq, const float *rc, int codec,
+                              ro_query *q) {
+    const size_t D = s->dim, cd = s->code_dim;
     double raw_n2 = 0.0;
     for (size_t i = 0; i < D; ++i) raw_n2 += (double)raw[i] * (double)raw[i];
-    for (size_t i = 0; i < cd; ++i) q->q[i] = rq[i] - s->rotated_center[i];
+    for (size_t i = 0; i < cd; ++i) q->q[i] = rq[i] - rc[i];
     double dot = 0.0;
     for (size_t i = 0; i < D; ++i) dot += (double)raw[i] * (double)s->center[i];
     float qn2 = (float)(raw_n2 + (double)s->center_norm_sqr - 2.0 * dot);
@@ -578,7 +656,6 @@ void ro_prepare_query(const ro_space *s, const float *raw, int codec, ro_query *
         }
         q->int8_error_norm = (float)sqrt(err);
     }
-    free(rq);
 }
 
 /* ------------------------------------------------------------- estimators */

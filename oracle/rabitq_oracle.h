@@ -93,6 +93,14 @@ void ro_rotate(const ro_space *s, const float *x, float *out);
 void ro_fast_quantize_abs(size_t code_dim, const float *abs_unit, uint8_t *code, float *ip_norm);
 uint16_t ro_float_to_half(float v);
 float ro_half_to_float(uint16_t h);
+/* Precision mode (QR rotation): encode of x given y = Q (x - c) from the
+ * rotation GEMM, and query preparation given the rotated raw query rq and
+ * the rotated center rc. */
+void ro_encode_full_rotated(const ro_space *s, const float *x, const float *y, uint8_t *out);
+/* The codec after the rotation, shared by both encodes: records of the
+ * rotated unit vector rot (cd floats) of a residual of norm rn;
+ * precision = 1 takes the short / error factors from compensated FP32 sums. */
+void ro_encode_from_unit(const ro_space *s, float rn, const float *rot, int precision, uint8_t *out);
 void ro_encode_full(const ro_space *s, const float *x, uint8_t *full_out);
 void ro_copy_compact(const ro_space *s, const uint8_t *full, uint8_t *compact_out);
 void ro_copy_residual(const ro_space *s, const uint8_t *full, uint8_t *residual_out);
@@ -125,6 +133,8 @@ void ro_query_alloc(const ro_space *s, ro_query *q);
 void ro_query_free(ro_query *q);
 /* codec: 0 = Full, 3 = Int8 (QueryCoarseCodec, space_rabitq.h:36-44). */
 void ro_prepare_query(const ro_space *s, const float *raw, int codec, ro_query *q);
+void ro_prepare_query_rotated(const ro_space *s, const float *raw, const float *rq, const float *rc, int codec,
+                              ro_query *q);
 
 typedef struct ro_estimate {
     float lower_bound;

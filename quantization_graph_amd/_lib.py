@@ -53,6 +53,9 @@ def lib():
     L.qg_space_host_constants.argtypes = [vp, vp, vp]
     L.qg_space_rotated_center.restype = vp
     L.qg_space_rotated_center.argtypes = [vp]
+    L.qg_space_set_rotation_qr.argtypes = [vp, vp, i32]
+    L.qg_space_rotation_qr.argtypes = [vp, vp]
+    L.qg_rotate_rows.argtypes = [vp, vp, u64, i32, vp, vp]
     L.qg_encode_workspace_bytes.restype = sz
     L.qg_encode_workspace_bytes.argtypes = [vp, u64]
     L.qg_encode.argtypes = [vp, vp, u64, C.POINTER(IndexDesc), vp, sz, vp]

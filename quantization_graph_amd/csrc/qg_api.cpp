@@ -38,6 +38,10 @@ int qg_launch_learned_rerank(const qg::RerankArgs *a, hipStream_t st);
 size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k);
 int qg_launch_kmeans(const float *samples, uint64_t sc, uint32_t dim, uint32_t k, int iters, float *centroids,
                      void *workspace, hipStream_t stream);
+int qg_launch_qr(double *A, uint32_t cd, double *r_scratch, uint32_t *bad, hipStream_t st);
+int qg_launch_qr_split(const double *QT, uint32_t cd, uint32_t dim, uint32_t kpad, uint32_t parts, uint16_t *qp,
+                       hipStream_t st);
+int qg_launch_rot_gemm(const qg::RotGemmArgs *a, hipStream_t st);
 int qg_launch_ss_encode(const float *x, uint64_t n, uint32_t dim, uint32_t cd, int identity, const float *signs,
                         uint64_t *db, hipStream_t st);
 int qg_launch_ss_prepare(const float *raw, uint64_t nq, uint32_t dim, uint32_t cd, int identity, const float *signs,
@@ -125,7 +129,26 @@ void space_destroy(Space *s) {
     if (s->d_signs) (void)hipFree(s->d_signs);
     if (s->d_center) (void)hipFree(s->d_center);
     if (s->d_rot_center) (void)hipFree(s->d_rot_center);
+    if (s->d_qt) (void)hipFree(s->d_qt);
+    if (s->d_qp) (void)hipFree(s->d_qp);
     delete s;
+}
+
+// Rows through the QR rotation's GEMM: out = Q * pad(row - center) (center
+// subtracted when requested).
+int qr_rotate(const Space *s, const float *rows, uint64_t n, bool subtract_center, float *out, hipStream_t st) {
+    RotGemmArgs g;
+    g.x = rows;
+    g.n = n;
+    g.dim = (uint32_t)s->dim;
+    g.cd = (uint32_t)s->cd;
+    g.kpad = s->kpad;
+    g.center = subtract_center ? s->d_center : nullptr;
+    g.row_scale = nullptr;
+    g.q = s->d_qp;
+    g.parts = s->parts;
+    g.out = out;
+    return qg_launch_rot_gemm(&g, st);
 }
 
 // setCentroids for K = 1 (space_rabitq.h:2918-2936).
@@ -136,8 +159,9 @@ int space_set_center(Space *s, const float *center_host) {
     double *d_norm = nullptr;
     if (check_hip(hipMalloc(&d_norm, sizeof(double)), "hipMalloc")) return -1;
     int rc = qg_launch_row_seed(s->d_center, 1, (uint32_t)s->dim, nullptr, nullptr, d_norm, nullptr, 0);
-    if (!rc) rc = qg_launch_rotate_rows(s->d_center, 1, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0,
-                                        s->d_signs, s->d_rot_center, 0);
+    if (!rc && s->rotation == 1) rc = qr_rotate(s, s->d_center, 1, false, s->d_rot_center, 0);
+    else if (!rc) rc = qg_launch_rotate_rows(s->d_center, 1, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0,
+                                             s->d_signs, s->d_rot_center, 0);
     double norm = 0.0;
     if (!rc) rc = hipMemcpy(&norm, d_norm, sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d_norm);
@@ -197,6 +221,7 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.block_count = (uint32_t)s->block_count;
         a.identity = s->seed == 0;
         a.mse = s->mse;
+        a.precision = s->rotation == 1 ? 1 : 0;
         a.signs = s->d_signs;
         a.center = s->d_center;
         a.rot = reinterpret_cast<float *>(w);
@@ -212,6 +237,9 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.rscale = out->rscale + b0 * s->block_count;
         a.rcodes = out->rcodes + b0 * (cd / 2);
         a.rfac = reinterpret_cast<float2 *>(out->rfac) + b0;
+        // the QR rotation of (x - c) on the matrix cores fills rot; the encode
+        // kernels then normalise it
+        if (a.precision && check_hip(qr_rotate(s, a.x, m, true, a.rot, stream), "rotation GEMM launch")) return -1;
         if (check_hip(qg_launch_encode(&a, stream), "encode launch")) return -1;
     }
     return 0;
@@ -234,6 +262,70 @@ void qg_space_delete(void *space) { space_destroy(static_cast<Space *>(space)); 
 
 int qg_space_set_center(void *space, const float *center_host) {
     return space_set_center(static_cast<Space *>(space), center_host);
+}
+
+int qg_space_set_rotation_qr(void *space, const double *gaussian, int bf16_parts) {
+    Space *s = static_cast<Space *>(space);
+    if (!s || !gaussian) return fail("null space or Gaussian matrix");
+    if (bf16_parts != 2 && bf16_parts != 3) return fail("bf16_parts must be 2 or 3");
+    const size_t cd = s->cd;
+    if (cd < 128) return fail("the QR rotation needs a code dimension of at least 128 (dimension above 64)");
+    // A = G^T: row j of A is column j of G
+    std::vector<double> at(cd * cd);
+    for (size_t i = 0; i < cd; ++i)
+        for (size_t j = 0; j < cd; ++j) at[j * cd + i] = gaussian[i * cd + j];
+    if (!s->d_qt && check_hip(hipMalloc(&s->d_qt, cd * cd * 8), "hipMalloc rotation")) return -1;
+    double *r = nullptr;
+    uint32_t *bad = nullptr;
+    if (check_hip(hipMalloc(&r, cd * 8), "hipMalloc") || check_hip(hipMalloc(&bad, 4), "hipMalloc")) {
+        if (r) (void)hipFree(r);
+        return -1;
+    }
+    uint32_t h_bad = 0;
+    int rc = hipMemcpy(s->d_qt, at.data(), cd * cd * 8, hipMemcpyHostToDevice);
+    if (!rc) rc = hipMemset(bad, 0, 4);
+    if (!rc) rc = qg_launch_qr(s->d_qt, (uint32_t)cd, r, bad, 0);
+    if (!rc) rc = hipMemcpy(&h_bad, bad, 4, hipMemcpyDeviceToHost);
+    (void)hipFree(r);
+    (void)hipFree(bad);
+    if (rc) return check_hip(rc, "QR factorisation");
+    if (h_bad) return fail("the Gaussian matrix is numerically rank-deficient");
+    const uint32_t kpad = (uint32_t)((s->dim + 31) / 32 * 32);
+    if (s->d_qp) (void)hipFree(s->d_qp);
+    s->d_qp = nullptr;
+    if (check_hip(hipMalloc(&s->d_qp, (size_t)bf16_parts * cd * kpad * 2), "hipMalloc rotation parts")) return -1;
+    s->kpad = kpad;
+    s->parts = (uint32_t)bf16_parts;
+    if (check_hip(qg_launch_qr_split(s->d_qt, (uint32_t)cd, (uint32_t)s->dim, kpad, s->parts, s->d_qp, 0),
+                  "rotation split"))
+        return -1;
+    s->rotation = 1;
+    if (check_hip(qr_rotate(s, s->d_center, 1, false, s->d_rot_center, 0), "rotated center")) return -1;
+    return check_hip(hipDeviceSynchronize(), "QR rotation setup");
+}
+
+int qg_space_rotation_qr(const void *space, double *q_out) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || !q_out) return fail("null space or output");
+    if (s->rotation != 1) return fail("the space has no QR rotation");
+    const size_t cd = s->cd;
+    std::vector<double> qt(cd * cd);
+    if (check_hip(hipMemcpy(qt.data(), s->d_qt, cd * cd * 8, hipMemcpyDeviceToHost), "download rotation")) return -1;
+    for (size_t i = 0; i < cd; ++i)
+        for (size_t j = 0; j < cd; ++j) q_out[i * cd + j] = qt[j * cd + i];
+    return 0;
+}
+
+int qg_rotate_rows(const void *space, const float *rows, uint64_t n, int subtract_center, float *out,
+                   void *stream) {
+    const Space *s = static_cast<const Space *>(space);
+    if (!s || (n && (!rows || !out))) return fail("null argument to qg_rotate_rows");
+    if (s->rotation == 1) return check_hip(qr_rotate(s, rows, n, subtract_center != 0, out, (hipStream_t)stream),
+                                           "rotation GEMM launch");
+    if (subtract_center) return fail("qg_rotate_rows subtracts the center for QR-rotated spaces only");
+    return check_hip(qg_launch_rotate_rows(rows, n, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0, s->d_signs, out,
+                                           (hipStream_t)stream),
+                     "rotate launch");
 }
 
 size_t qg_space_dim(const void *space) { return space ? static_cast<const Space *>(space)->dim : 0; }
@@ -342,6 +434,8 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
     p.raw_norm = nrm;
     p.raw_dot = dot;
     p.q_out = out->q;
+    p.pre_rotated = s->rotation == 1 ? 1 : 0;
+    if (p.pre_rotated && check_hip(qr_rotate(s, queries, nq, false, out->q, st), "rotation GEMM launch")) return -1;
     p.x8_out = out->x8;
     p.x8p_out = out->x8p;
     p.qs_out = reinterpret_cast<QScalars *>(out->qs);
@@ -592,6 +686,7 @@ static_assert(sizeof(qg_signscan_query) == sizeof(SsQuery), "qg_signscan_query l
 int qg_signscan_encode(const void *space, const float *x, uint64_t n, uint64_t *db, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !x || !db) return fail("null argument to qg_signscan_encode");
+    if (s->rotation != 0) return fail("the 1-bit sign-plane scan uses the Walsh-Hadamard rotation, not a QR one");
     if (n % QG_SIGNSCAN_BATCH != 0) return fail("sign-scan databases hold whole blocks of 32 vectors (n = %llu)",
                                                 (unsigned long long)n);
     if (n / QG_SIGNSCAN_BATCH > 0x7FFFFFFFull / QG_SIGNSCAN_BATCH) return fail("n too large for one launch");
@@ -604,6 +699,7 @@ int qg_signscan_prepare(const void *space, const float *queries, uint64_t nq, fl
                         qg_signscan_query *scalars, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !queries || !q || !bits || !scalars) return fail("null argument to qg_signscan_prepare");
+    if (s->rotation != 0) return fail("the 1-bit sign-plane scan uses the Walsh-Hadamard rotation, not a QR one");
     if (nq > 0x7FFFFFFFull) return fail("nq too large for one launch");
     return check_hip(qg_launch_ss_prepare(queries, nq, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0, s->d_signs,
                                           q, bits, reinterpret_cast<SsQuery *>(scalars), (hipStream_t)stream),

@@ -18,7 +18,8 @@ struct PrepParams {
     const uint64_t *seed;      // [nq] FNV-1a of the raw bytes
     const double *raw_norm;    // [nq]
     const double *raw_dot;     // [nq]
-    float *q_out;              // [nq][cd]
+    float *q_out;              // [nq][cd]; holds the rotated raw rows on entry when pre_rotated
+    int pre_rotated;           // QR rotation: q_out already holds Q * pad(raw)
     int8_t *x8_out;            // [nq][cd]
     int8_t *x8p_out;           // [nq][cd]
     QScalars *qs_out;          // [nq]
@@ -78,6 +79,7 @@ struct EncArgs {
     uint32_t dim, cd, block, block_count;
     int identity;              // seed == 0
     int mse;
+    int precision;             // QR rotation: compensated FP32 norms and factors, rot filled by the GEMM
     const float *signs;        // [cd]
     const float *center;       // [dim]
     float *rot;                // ws [n][cd]
@@ -94,6 +96,20 @@ struct EncArgs {
     uint16_t *rscale;
     uint8_t *rcodes;
     float2 *rfac;
+};
+
+// Rotation GEMM of the QR rotation: out[r][i] = sum_k Q[i][k] * u_r[k] with
+// u_r = (x_r - center) * row_scale[r] (center / row_scale optional), Q as
+// BF16 parts [parts][cd][kpad] (k contiguous, zero past dim).
+struct RotGemmArgs {
+    const float *x;            // [n][dim]
+    uint64_t n;
+    uint32_t dim, cd, kpad;
+    const float *center;       // [dim] or null
+    const float *row_scale;    // [n] or null
+    const uint16_t *q;         // [parts][cd][kpad] bf16 bits: Q rounded, then the rounded remainders
+    uint32_t parts;            // 2 or 3 BF16 parts per operand
+    float *out;                // [n][cd]
 };
 
 struct PackArgs {

@@ -32,6 +32,59 @@ namespace qg {
 constexpr int kEncThreads = 256;
 constexpr double kEps = 1e-5;
 
+// Compensated FP32 arithmetic of the precision mode (QR rotation): Knuth's
+// branch-free TwoSum and the fma TwoProduct carry each rounding error into a
+// second accumulator, so a sum of squares or of magnitudes is accurate to
+// about FP32 rounding of the exact result (Ogita, Rump and Oishi's Sum2 /
+// Dot2).
+struct Comp {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float p) {
+        const float t = s + p;
+        const float bp = t - s;
+        c = c + ((s - (t - bp)) + (p - bp));
+        s = t;
+    }
+    __device__ __forceinline__ void add_sq(float r) {
+        const float p = r * r;
+        const float e = fmaf(r, r, -p);
+        add(p);
+        c = c + e;
+    }
+    __device__ __forceinline__ void merge(Comp o) {
+        add(o.s);
+        c = c + o.c;
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
+// block-wide merge of per-thread compensated sums in a fixed tree order
+__device__ __forceinline__ float comp_block(Comp v, float *red_s, float *red_c) {
+    const uint32_t t = threadIdx.x;
+    red_s[t] = v.s;
+    red_c[t] = v.c;
+    __syncthreads();
+    for (uint32_t h = kEncThreads / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            Comp a;
+            a.s = red_s[t];
+            a.c = red_c[t];
+            Comp b;
+            b.s = red_s[t + h];
+            b.c = red_c[t + h];
+            a.merge(b);
+            red_s[t] = a.s;
+            red_c[t] = a.c;
+        }
+        __syncthreads();
+    }
+    Comp r;
+    r.s = red_s[0];
+    r.c = red_c[0];
+    __syncthreads();
+    return r.value();
+}
+
 
 
 
@@ -44,6 +97,12 @@ __global__ void k_enc_norm(EncArgs a) {
     if (v >= a.n) return;
     const float *x = a.x + v * a.dim;
     const uint32_t D = a.dim;
+    if (a.precision) {  // compensated sum of squares
+        Comp c;
+        for (uint32_t i = 0; i < D; ++i) c.add_sq(x[i] - a.center[i]);
+        a.meta[v].rn = sqrtf(c.value());
+        return;
+    }
     const uint32_t vec_end = D >= 16 ? (D / 16) * 16 + ((D % 16) >= 8 ? 8 : 0) : 0;
     float s = 0.0f;
     for (uint32_t i = 0; i < D; ++i) {
@@ -76,21 +135,37 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_rotate(EncArgs a) {
     const uint32_t t = threadIdx.x;
     const float *x = a.x + vi * a.dim;
     const float rn = a.meta[vi].rn;
-    const bool scale = rn > 0.0f;
-    const float inv = scale ? 1.0f / rn : 1.0f;
-    for (uint32_t i = t; i < cd; i += blockDim.x) {
-        float u = 0.0f;
-        if (i < a.dim) {
-            u = x[i] - a.center[i];
-            if (scale) u = u * inv;
-            if (!a.identity) u = u * a.signs[i];
+    if (a.precision) {
+        // rot holds Q (x - c) from the rotation GEMM: the unit vector is it
+        // divided by its compensated norm, which corrects the GEMM's scale
+        // error as well (Q is orthogonal: the exact norm is |x - c|)
+        __shared__ float red_s[kEncThreads], red_c[kEncThreads];
+        Comp c;
+        for (uint32_t i = t; i < cd; i += blockDim.x) {
+            const float y = a.rot[vi * cd + i];
+            v[i] = y;
+            c.add_sq(y);
         }
-        v[i] = u;
+        const float n2 = comp_block(c, red_s, red_c);
+        const float inv = n2 > 0.0f ? 1.0f / sqrtf(n2) : 0.0f;
+        for (uint32_t i = t; i < cd; i += blockDim.x) v[i] = v[i] * inv;
+    } else {
+        const bool scale = rn > 0.0f;
+        const float inv = scale ? 1.0f / rn : 1.0f;
+        for (uint32_t i = t; i < cd; i += blockDim.x) {
+            float u = 0.0f;
+            if (i < a.dim) {
+                u = x[i] - a.center[i];
+                if (scale) u = u * inv;
+                if (!a.identity) u = u * a.signs[i];
+            }
+            v[i] = u;
+        }
     }
     if (t == 0) s_max_bits = 0;
     if (t < 7) { s_start[t] = 0x7FFFFFFF; s_end[t] = 0; }
     __syncthreads();
-    if (!a.identity) fht_lds(v, cd);
+    if (!a.identity && !a.precision) fht_lds(v, cd);
     float m = 0.0f;
     for (uint32_t i = t; i < cd; i += blockDim.x) {
         const float r = v[i];
@@ -226,6 +301,30 @@ __global__ void k_enc_scan(EncArgs a) {
     }
     const float rn = mt.rn;
     const float long_scale = 2.0f * rn * ip_norm;
+    if (a.precision) {
+        // the factors from a compensated FP32 sum of |o| (no FP64)
+        Comp l1c;
+        for (uint32_t i = 0; i < cd; ++i) l1c.add(fabsf(rot[i]));
+        const float l1 = l1c.value(), half = 0.5f * l1;
+        float short_scale = 0.0f, error_scale = 0.0f;
+        if (rn > 0.0f && half > 1e-12f) {
+            short_scale = 2.0f * rn / half;
+            const float sq = sqrtf((float)cd);
+            float o = l1 / sq;
+            if (!__builtin_isfinite(o)) o = 0.8f;
+            o = (1e-6f < o) ? o : 1e-6f;
+            o = (0.999999f < o) ? 0.999999f : o;
+            const float o2 = o * o;
+            float r = (1.0f - o2) / o2;
+            r = (0.0f < r) ? r : 0.0f;
+            error_scale = sqrtf(r) * (0.359375f / sq) * 2.0f * rn;
+        }
+        a.hdr[vi] = make_float2(rn * rn, long_scale);
+        a.sfac[vi] = make_float2(short_scale, error_scale);
+        a.meta[vi].best_t = best_t;
+        a.meta[vi].ip_norm = ip_norm;
+        return;
+    }
     double o_obar = 0.0, half_l1 = 0.0;
     const double inv_sqrt_d = 1.0 / sqrt((double)cd);
     for (uint32_t i = 0; i < cd; ++i) {

@@ -181,11 +181,15 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
     const float *raw = p.raw + qi * p.dim;
     const uint32_t t = threadIdx.x;
 
-    for (uint32_t i = t; i < cd; i += blockDim.x)
-        v[i] = (i < p.dim) ? (p.identity ? raw[i] : raw[i] * p.signs[i]) : 0.0f;
+    if (p.pre_rotated) {  // QR rotation: q_out holds Q * pad(raw) from the rotation GEMM
+        for (uint32_t i = t; i < cd; i += blockDim.x) v[i] = p.q_out[qi * cd + i];
+    } else {
+        for (uint32_t i = t; i < cd; i += blockDim.x)
+            v[i] = (i < p.dim) ? (p.identity ? raw[i] : raw[i] * p.signs[i]) : 0.0f;
+    }
     if (t == 0) { s_max = 0.0f; s_csum = 0; }
     __syncthreads();
-    if (!p.identity) fht_lds(v, cd);
+    if (!p.identity && !p.pre_rotated) fht_lds(v, cd);
     for (uint32_t i = t; i < cd; i += blockDim.x) {
         const float d = v[i] - p.rot_center[i];
         v[i] = d;

@@ -22,6 +22,13 @@ struct Space {
     float *d_center = nullptr;
     float *d_rot_center = nullptr;
     float center_norm_sqr = 0.0f;
+    // rotation: 0 = sign-flipped Walsh-Hadamard (the reference's), 1 = the
+    // QR rotation of the precision mode (qg_space_set_rotation_qr)
+    int rotation = 0;
+    double *d_qt = nullptr;      // QR: Q^T [cd][cd], FP64
+    uint16_t *d_qp = nullptr;    // QR: BF16 parts of Q [parts][cd][kpad]
+    uint32_t kpad = 0;           // dim rounded up to 32
+    uint32_t parts = 3;          // BF16 parts per GEMM operand (2 or 3)
 };
 
 Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t residual_bits, size_t block,

@@ -191,6 +191,7 @@ class RabitqSpace:
         self.residual_mse = bool(residual_mse)
         self.residual_fp16 = bool(residual_fp16)
         self.codec = CODEC_INT8
+        self.rotation = "fht"
 
     @classmethod
     def new_with_centroids(cls, dim, seed, centroid_count, centroids):
@@ -217,6 +218,40 @@ class RabitqSpace:
         if c.size != self.dim:
             raise ValueError("center has %d values, expected %d" % (c.size, self.dim))
         check(_lib.lib().qg_space_set_center(self.handle, c.ctypes.data_as(C.c_void_p)))
+
+    def set_qr_rotation(self, seed=None, gaussian=None, bf16_parts=3):
+        """Precision mode: rotate with Q of the FP64 QR factorisation (on the
+        GPU) of a Gaussian matrix instead of the sign-flipped Walsh-Hadamard
+        transform. The matrix is `gaussian` (code_dim x code_dim) or else
+        numpy's default_rng(seed).standard_normal((code_dim, code_dim)).
+        Encode and query preparation then run the rotation on the BF16
+        matrix cores with each operand split in bf16_parts parts (2: about 16
+        significant bits, 3: FP32 class) (qg_space_set_rotation_qr)."""
+        if gaussian is None:
+            if seed is None:
+                raise ValueError("give a seed or a Gaussian matrix")
+            gaussian = np.random.default_rng(seed).standard_normal((self.code_dim, self.code_dim))
+        g = np.ascontiguousarray(np.asarray(gaussian, np.float64))
+        if g.shape != (self.code_dim, self.code_dim):
+            raise ValueError("the Gaussian matrix must be %d x %d" % (self.code_dim, self.code_dim))
+        check(_lib.lib().qg_space_set_rotation_qr(self.handle, g.ctypes.data_as(C.c_void_p), int(bf16_parts)))
+        self.rotation = "qr"
+
+    def rotation_matrix(self):
+        """Q of a QR-rotated space (code_dim x code_dim FP64, rotated = Q x)."""
+        q = np.zeros((self.code_dim, self.code_dim), np.float64)
+        check(_lib.lib().qg_space_rotation_qr(self.handle, q.ctypes.data_as(C.c_void_p)))
+        return q
+
+    def rotate(self, rows, subtract_center=False):
+        """The space's rotation of device rows [n, dim] -> [n, code_dim]
+        (subtract_center: Q (x - c), the product encode takes; QR spaces)."""
+        import torch
+        rows = _dev(rows, torch.float32)
+        out = torch.empty((rows.shape[0], self.code_dim), dtype=torch.float32, device="cuda")
+        check(_lib.lib().qg_rotate_rows(self.handle, _ptr(rows), rows.shape[0], 1 if subtract_center else 0,
+                                        _ptr(out), _stream()))
+        return out
 
     def host_constants(self):
         """(rotation signs [code_dim], center [dim]) as float32 host arrays."""

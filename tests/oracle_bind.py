@@ -236,6 +236,36 @@ class Oracle:
             self.lib.ro_encode_full(self.sp, _p(X[i]), _p(out[i]))
         return out
 
+    def encode_full_rotated(self, X, Y):
+        """Precision-mode records of X given Y = Q (X - c), the rotation
+        GEMM's output (ro_encode_full_rotated)."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.dim)
+        Y = np.ascontiguousarray(Y, np.float32).reshape(-1, self.cd)
+        out = np.zeros((X.shape[0], self.full_bytes), np.uint8)
+        for i in range(X.shape[0]):
+            self.lib.ro_encode_full_rotated(self.sp, _p(X[i]), _p(Y[i]), _p(out[i]))
+        return out
+
+    def encode_from_unit(self, rn, rot, precision):
+        """Records of the rotated unit vector rot of a residual of norm rn
+        (ro_encode_from_unit)."""
+        rot = np.ascontiguousarray(rot, np.float32)
+        out = np.zeros(self.full_bytes, np.uint8)
+        self.lib.ro_encode_from_unit(self.sp, C.c_float(rn), _p(rot), int(precision), _p(out))
+        return out
+
+    def prepare_rotated(self, raw, rq, rc, codec=3):
+        """Query preparation from the rotated raw query rq and rotated
+        center rc (ro_prepare_query_rotated)."""
+        q = ROQuery()
+        self.lib.ro_query_alloc(self.sp, C.byref(q))
+        raw = np.ascontiguousarray(raw, np.float32)
+        rq = np.ascontiguousarray(rq, np.float32)
+        rc = np.ascontiguousarray(rc, np.float32)
+        self.lib.ro_prepare_query_rotated(self.sp, _p(raw), _p(rq), _p(rc), codec, C.byref(q))
+        q._raw = raw
+        return q
+
     def split(self, full):
         n = full.shape[0]
         comp = np.zeros((n, self.compact_bytes), np.uint8)
