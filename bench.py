@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix-core peak
 
 
 def log(*a):
@@ -703,6 +704,148 @@ def fixed_precision_check(np, torch, qg, sp, index, comp_h, q, cand, center, dim
     return res
 
 
+def precision_form(args, torch, np, qg, x, q, cand, gt, center, depth, fht_ms, fht_recall):
+    """The precision form on the fixed-candidate workload: the same 1M x 960
+    base set encoded with the QR rotation (FP64 QR of a seeded Gaussian on
+    the GPU, the rotation GEMM on BF16 MFMA with three BF16 parts per operand
+    and FP32 accumulation, compensated FP32 norms and factors), the same INT8
+    queries and candidate lists. Reports the rotation GEMM's matrix-core
+    throughput, the pipeline's QPS and recall beside the Walsh-Hadamard
+    encode's, and on a sample: the integer dots and estimates against the
+    oracle (bit for bit), the largest relative error of the FP32 4-bit
+    estimate against the same estimator in FP64 with the FP64 rotation, and
+    the GEMM's per-row error against the FP64 product."""
+    from quantization_graph_amd import fixed
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle
+    dim, n, nq, per = args.fixed_dim, args.fixed_n, args.fixed_nq, args.per
+    stream = torch.cuda.current_stream()
+    sp = qg.RabitqSpace(dim, 100)
+    sp.set_center(center)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sp.set_qr_rotation(seed=2, bf16_parts=3)
+    qr_s = time.perf_counter() - t0
+    cd = sp.code_dim
+    kpad = (dim + 31) // 32 * 32
+    gemm = {}
+    for parts in (2, 3):
+        sp.set_qr_rotation(seed=2, bf16_parts=parts)
+        out = torch.empty((n, cd), dtype=torch.float32, device="cuda")
+        L_ = qg._lib.lib()
+        from quantization_graph_amd.space import _ptr, _stream
+        def rot():
+            qg._lib.check(L_.qg_rotate_rows(sp.handle, _ptr(x), n, 1, _ptr(out), _stream()))
+        rot()
+        torch.cuda.synchronize()
+        reps = 10
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a_, b_ in ev:
+            a_.record(stream)
+            rot()
+            b_.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([a_.elapsed_time(b_) for a_, b_ in ev]))
+        products = 3 if parts == 2 else 6
+        useful = 2.0 * n * dim * cd
+        issued = useful * products * kpad / dim
+        gemm[parts] = {"kernel_ms": ms, "products_per_element": products,
+                       "useful_tflops": useful / (ms / 1e3) / 1e12,
+                       "mfma_tflops": issued / (ms / 1e3) / 1e12,
+                       "mfma_frac_of_bf16_peak": issued / (ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS,
+                       "hbm_gbs": (n * dim * 4 + n * cd * 4) / (ms / 1e3) / 1e9}
+        log("rotation GEMM (%d BF16 parts): %.3f ms for %d x %d -> %d, %.1f TFLOP/s on the matrix cores "
+            "(%.1f%% of the BF16 peak)" % (parts, ms, n, dim, cd, gemm[parts]["mfma_tflops"],
+                                          100 * gemm[parts]["mfma_frac_of_bf16_peak"]))
+        del out
+    sp.set_qr_rotation(seed=2, bf16_parts=3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = sp.encode(x, batch=262144)
+    torch.cuda.synchronize()
+    encode_s = time.perf_counter() - t0
+
+    def step():
+        qv = sp.prepare(q, "int8")
+        first = fixed.score(sp, index, qv, cand, codec="int8")
+        return fixed.rerank_topk(sp, index, qv, cand, first, k=10, depth=depth)
+    step()
+    torch.cuda.synchronize()
+    steps = max(3, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ids, _ = step()
+    torch.cuda.synchronize()
+    ms_step = 1000.0 * (time.perf_counter() - t0) / steps
+    recall = fixed.recall_at_k(ids, gt, 10)
+    log("precision form: QR %.3f s, encode %.2f s, %.3f ms/step (%.0f q/s), recall@10 %.4f (Walsh-Hadamard "
+        "encode: %.3f ms/step, recall %.4f)" % (qr_s, encode_s, ms_step, nq / (ms_step / 1e3), recall, fht_ms,
+                                              fht_recall))
+    # accuracy on a sample
+    ns = min(100, nq)
+    comp = index.to_records()[0].cpu().numpy()
+    qs = q[:ns]
+    qv8 = sp.prepare(qs, "int8")
+    qvf = sp.prepare(qs, "full")
+    ids_s = cand[:ns]
+    est, _, _, _, dot = sp.scan_fixed(qg.SCAN_INT8_FULL, index, qv8, ids_s, eps0=1.9, want=("f", "f2", "dot"))
+    asym = sp.scan_fixed(qg.SCAN_ASYM, index, qvf, ids_s)[0].cpu().numpy()
+    est, dot, ids_h = est.cpu().numpy(), dot.cpu().numpy(), ids_s.cpu().numpy()
+    rq = sp.rotate(qs).cpu().numpy()
+    rc = sp.rotate(torch.from_numpy(center[None, :]).cuda()).cpu().numpy()[0]
+    Q = sp.rotation_matrix()
+    q_h = qs.cpu().numpy()
+    orc = Oracle(dim, 100)
+    orc.set_center(center)
+    ls = comp[:, 4:8].copy().view(np.float32)[:, 0].astype(np.float64)
+    ns2 = comp[:, :4].copy().view(np.float32)[:, 0].astype(np.float64)
+    dots_ok = est_ok = 0
+    max_rel = max_rel_terms = 0.0
+    for i in range(ns):
+        oq = orc.prepare_rotated(q_h[i], rq[i], rc, 3)
+        e_o, _, d_o = orc.int8_full(oq, comp[ids_h[i]])
+        orc.free_query(oq)
+        dots_ok += int(np.array_equal(dot[i], d_o))
+        est_ok += int(np.array_equal(est[i].view(np.uint32), e_o.view(np.uint32)))
+        rows = comp[ids_h[i]]
+        codes = rows[:, 16:16 + cd // 2]
+        nib = np.stack([codes & 15, codes >> 4], axis=2).reshape(rows.shape[0], cd).astype(np.float64)
+        x4 = 0.5 * ls[ids_h[i]][:, None] * (nib - 7.5)
+        dq = q_h[i].astype(np.float64) - center.astype(np.float64)
+        d64 = ns2[ids_h[i]] + float(dq @ dq) - 2.0 * (x4 @ (Q[:, :dim] @ dq))
+        err = np.abs(asym[i].astype(np.float64) - d64)
+        max_rel = max(max_rel, float((err / np.abs(d64)).max()))
+        max_rel_terms = max(max_rel_terms, float((err / (ns2[ids_h[i]] + float(dq @ dq))).max()))
+    xs = x[:2000]
+    y = sp.rotate(xs, subtract_center=True).cpu().numpy().astype(np.float64)
+    want = (xs.cpu().numpy().astype(np.float64) - center.astype(np.float64)) @ Q[:, :dim].T
+    gemm_err = float((np.linalg.norm(y - want, axis=1) / np.linalg.norm(want, axis=1)).max())
+    log("precision form: integer dots equal the oracle %s, estimates bit-exact %s, max relative error of the "
+        "FP32 estimate vs FP64 %.3g (%.3g of |o|^2 + |q|^2), GEMM per-row error vs FP64 %.3g"
+        % (dots_ok == ns, est_ok == ns, max_rel, max_rel_terms, gemm_err))
+    g3 = gemm[3]
+    res = {
+        "config": {"workload": "precision form: %d x %d GMM(256, std 0.3, seed 2), QR rotation (seed 2), %d INT8 "
+                               "queries x %d fixed candidates, residual-4 top-%d" % (n, dim, nq, per, depth)},
+        "qr_seconds": qr_s, "encode_seconds": encode_s, "ms_per_step": ms_step, "qps": nq / (ms_step / 1e3),
+        "recall_at_10": recall, "fht_ms_per_step": fht_ms, "fht_recall_at_10": fht_recall,
+        "rotation_gemm": {str(k): v for k, v in gemm.items()},
+        "roofline": {"bound": "mfma", "achieved": g3["mfma_tflops"], "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": g3["mfma_frac_of_bf16_peak"], "traffic": None,
+                     "kernel": "qg::k_rot_gemm<3> (rotation GEMM, 3 BF16 parts, 6 products)",
+                     "kernel_ms": g3["kernel_ms"],
+                     "flops_per_unit": 2.0 * dim * cd * 6 * kpad / dim, "units_per_launch": n},
+        "sample": "%d queries x %d candidates; GEMM error over 2000 rows" % (ns, per),
+        "int_dots_equal_oracle": dots_ok == ns, "estimates_bit_exact": est_ok == ns,
+        "max_rel_err_fp32_estimate_vs_fp64": max_rel,
+        "max_err_fp32_estimate_vs_fp64_over_norm_terms": max_rel_terms,
+        "gemm_max_row_rel_err_vs_fp64": gemm_err,
+    }
+    del index
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_fixed(args, torch, np, qg):
     from quantization_graph_amd import fixed
     t_setup = time.perf_counter()
@@ -775,6 +918,10 @@ def run_fixed(args, torch, np, qg):
         with open(pmc) as f:
             traffic = json.load(f).get("traffic_bytes_per_launch")
 
+    precision_mode = None
+    if not args.no_precision:
+        precision_mode = precision_form(args, torch, np, qg, x, q, cand, gt, center, depth, ms_per_step, recall)
+
     learned_res = None
     if args.train_queries:
         learned_res = run_learned_fixed(args, torch, np, sp, index, x, q, cand, gt, g, ms_per_step, recall, depth)
@@ -822,7 +969,7 @@ def run_fixed(args, torch, np, qg):
                      "kernel": "qg::k_int8_full_grouped<1024> (INT8 query x 4-bit code, fixed lists)",
                      "kernel_ms": kernel_ms, "bytes_per_unit": unit_bytes, "units_per_launch": units},
         "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity, "learned": learned_res,
-        "precision": precision,
+        "precision": precision, "precision_form": precision_mode,
         "setup_seconds": time.perf_counter() - t_setup,
     }
     del index, x, q, cand, gt
@@ -990,6 +1137,7 @@ def main():
     ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--cpu-build-records", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the QR-rotation precision form")
     ap.add_argument("--batch-n", type=int, default=1_000_000)
     ap.add_argument("--batch-nq", type=int, default=20_000)
     ap.add_argument("--batch-steps", type=int, default=5)
