@@ -71,6 +71,8 @@ typedef struct qg_index_desc {
     uint8_t *rcodes;    /* [n][cd/2] residual 4-bit codes                           */
     float *rfac;        /* [n][2]    residual_norm_sqr, long_residual_ip            */
     uint64_t n;
+    uint8_t *cid;       /* [n]       centroid id (the compact record's last byte);
+                         * may be NULL for a space of one centroid             */
 } qg_index_desc;
 
 /* Prepared queries on the device (QueryContext, space_rabitq.h:101-132). */
@@ -110,8 +112,11 @@ enum {
     QG_SCAN_SYMMETRIC = 7, /* C8: code<->code distance, pairs (qidx = lhs row, ids = rhs row)  */
     QG_SCAN_PAPER_RECORD = 8, /* gate from the compact record (compute_paper_prune_estimate,
                                  space_rabitq.h:3926-3959) -> RabitqPaperEstimate              */
-    QG_SCAN_B1 = 9         /* 4-bit code x B1 sign code (query_distance_b1, space_rabitq.h:
+    QG_SCAN_B1 = 9,        /* 4-bit code x B1 sign code (query_distance_b1, space_rabitq.h:
                               3742-3755); queries prepared with QG_CODEC_B1                    */
+    QG_SCAN_RESIDUAL_RECORD = 10 /* E1 in the per-record form (computeResidualDistanceInterval
+                              FromRecord, :1723-1735): what single-record calls and spaces of
+                              several centroids take; pairs only                              */
 };
 
 /* -------------------------------------------------------- device batch API */
@@ -151,14 +156,21 @@ int qg_unpack_records(const void *space, const uint8_t *compact, size_t compact_
 
 /* Prepare nq queries (device [nq][dim]). Replaces rabitq_prepare_query +
  * rabitq_set_query_coarse_codec (rabitq_bridge.cpp:355-370). Workspace:
- * qg_prepare_workspace_bytes(nq) bytes of device memory. */
+ * qg_prepare_workspace_bytes(nq) bytes of device memory. A space of K
+ * centroids writes K * nq query rows (out->nq = K * nq), row k * nq + i the
+ * context of query i for centroid k (buildCentroidQuery, space_rabitq.h:
+ * 3429-3663, the INT8 dither seeded with the centroid id). */
 size_t qg_prepare_workspace_bytes(uint64_t nq);
 int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int codec,
                        qg_query_desc *out, void *workspace, void *stream);
 
 /* Estimators over candidate lists. fixed != 0: ids is [nq][per_query];
  * fixed == 0: (qidx[i], ids[i]) pairs, i < n_pairs (for QG_SCAN_SYMMETRIC
- * both are rows of ix and qv is unused). Replaces
+ * both are rows of ix and qv is unused). In a space of K > 1 centroids the
+ * query rows are qg_prepare_queries' K contexts per query (row k * nq + i);
+ * a record is scored with its centroid's context (qidx = cid * nq + i, the
+ * reference's queryForEncoded), the 1-bit sidecar gates with centroid 0's,
+ * and QG_SCAN_SYMMETRIC adds the cross-centroid terms (it reads ix->cid). Replaces
  * rabitq_query_distance_batch (:885), rabitq_paper_estimate_batch_sidecar
  * (:702), rabitq_residual_distance_batch (:972) for many queries at once. */
 int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, const qg_query_desc *qv,

@@ -11,9 +11,11 @@ Public surface:
 """
 from . import fixed, learned  # noqa: F401
 from .space import (CODEC_FULL, CODEC_INT8, SCAN_ASYM, SCAN_GATE_FP32, SCAN_GATE_INT8,
-                    SCAN_INT8_FULL, SCAN_REM_FP32, SCAN_REM_INT8, SCAN_RESIDUAL, DeviceIndex,
+                    SCAN_INT8_FULL, SCAN_REM_FP32, SCAN_REM_INT8, SCAN_RESIDUAL, SCAN_SYMMETRIC,
+                    SCAN_PAPER_RECORD, SCAN_B1, SCAN_RESIDUAL_RECORD, DeviceIndex,
                     DeviceQueries, RabitqSpace, code_dim_for, estimates_np, intervals_np, topk)
 
 __all__ = ["RabitqSpace", "DeviceIndex", "DeviceQueries", "topk", "code_dim_for", "estimates_np",
            "intervals_np", "CODEC_FULL", "CODEC_INT8", "SCAN_ASYM", "SCAN_GATE_FP32",
-           "SCAN_GATE_INT8", "SCAN_INT8_FULL", "SCAN_REM_FP32", "SCAN_REM_INT8", "SCAN_RESIDUAL"]
+           "SCAN_GATE_INT8", "SCAN_INT8_FULL", "SCAN_REM_FP32", "SCAN_REM_INT8", "SCAN_RESIDUAL",
+           "SCAN_SYMMETRIC", "SCAN_PAPER_RECORD", "SCAN_B1", "SCAN_RESIDUAL_RECORD"]

@@ -23,7 +23,7 @@ class IndexDesc(C.Structure):
     _fields_ = [("codes", C.c_void_p), ("hdr", C.c_void_p), ("sfac", C.c_void_p),
                 ("msb", C.c_void_p), ("factors", C.c_void_p), ("fvalid", C.c_void_p),
                 ("rscale", C.c_void_p), ("rcodes", C.c_void_p), ("rfac", C.c_void_p),
-                ("n", C.c_uint64)]
+                ("n", C.c_uint64), ("cid", C.c_void_p)]
 
 
 class QueryDesc(C.Structure):
@@ -53,6 +53,9 @@ def lib():
     L.qg_space_host_constants.argtypes = [vp, vp, vp]
     L.qg_space_rotated_center.restype = vp
     L.qg_space_rotated_center.argtypes = [vp]
+    L.qg_space_set_centroids.argtypes = [vp, vp, sz]
+    L.qg_space_centroid_count.restype = sz
+    L.qg_space_centroid_count.argtypes = [vp]
     L.qg_space_set_rotation_qr.argtypes = [vp, vp, i32]
     L.qg_space_rotation_qr.argtypes = [vp, vp]
     L.qg_rotate_rows.argtypes = [vp, vp, u64, i32, vp, vp]

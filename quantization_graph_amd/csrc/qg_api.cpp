@@ -39,6 +39,8 @@ size_t qg_kmeans_workspace_bytes(uint64_t sample_count, uint64_t dim, uint32_t k
 int qg_launch_kmeans(const float *samples, uint64_t sc, uint32_t dim, uint32_t k, int iters, float *centroids,
                      void *workspace, hipStream_t stream);
 int qg_launch_qr(double *A, uint32_t cd, double *r_scratch, uint32_t *bad, hipStream_t st);
+int qg_launch_centroid_pairs(const float *centroids, const float *rot_centroids, uint32_t K, uint32_t dim,
+                             uint32_t cd, float2 *pairs, hipStream_t st);
 int qg_launch_qr_split(const double *QT, uint32_t cd, uint32_t dim, uint32_t kpad, uint32_t parts, uint16_t *qp,
                        hipStream_t st);
 int qg_launch_rot_gemm(const qg::RotGemmArgs *a, hipStream_t st);
@@ -80,7 +82,6 @@ Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t res
                     bool mse, bool fp16) {
     if (dim == 0) { fail("dimension must be positive"); return nullptr; }
     if (centroid_count == 0 || centroid_count > 256) { fail("centroid_count must be in [1, 256]"); return nullptr; }
-    if (centroid_count != 1) { fail("only centroid_count = 1 (the formal configuration) is implemented"); return nullptr; }
     if (residual_bits != 4) { fail("only 4-bit residual codes are implemented"); return nullptr; }
     if (block != 16) { fail("only residual block size 16 is implemented"); return nullptr; }
     if (!fp16) { fail("only fp16 residual scale storage is implemented"); return nullptr; }
@@ -106,16 +107,21 @@ Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t res
         std::uniform_int_distribution<int> bernoulli(0, 1);
         for (size_t i = 0; i < s->cd; ++i) s->h_signs[i] = (bernoulli(rng) ? 1.0f : -1.0f) * inv_sqrt;
     }
-    s->h_center.assign(dim, 0.0f);
+    const size_t K = centroid_count;
+    s->centroid_count = K;
+    s->h_center.assign(K * dim, 0.0f);
+    s->h_cnorm.assign(K, 0.0f);
     if (check_hip(hipMalloc(&s->d_signs, s->cd * 4), "hipMalloc signs") ||
-        check_hip(hipMalloc(&s->d_center, dim * 4), "hipMalloc center") ||
-        check_hip(hipMalloc(&s->d_rot_center, s->cd * 4), "hipMalloc rotated center")) {
+        check_hip(hipMalloc(&s->d_center, K * dim * 4), "hipMalloc center") ||
+        check_hip(hipMalloc(&s->d_rot_center, K * s->cd * 4), "hipMalloc rotated center") ||
+        check_hip(hipMalloc(&s->d_pair, K * K * sizeof(float2)), "hipMalloc centroid pairs")) {
         space_destroy(s);
         return nullptr;
     }
     if (check_hip(hipMemcpy(s->d_signs, s->h_signs.data(), s->cd * 4, hipMemcpyHostToDevice), "upload signs") ||
-        check_hip(hipMemset(s->d_center, 0, dim * 4), "clear center") ||
-        check_hip(hipMemset(s->d_rot_center, 0, s->cd * 4), "clear rotated center")) {
+        check_hip(hipMemset(s->d_center, 0, K * dim * 4), "clear center") ||
+        check_hip(hipMemset(s->d_rot_center, 0, K * s->cd * 4), "clear rotated center") ||
+        check_hip(hipMemset(s->d_pair, 0, K * K * sizeof(float2)), "clear centroid pairs")) {
         space_destroy(s);
         return nullptr;
     }
@@ -129,6 +135,7 @@ void space_destroy(Space *s) {
     if (s->d_signs) (void)hipFree(s->d_signs);
     if (s->d_center) (void)hipFree(s->d_center);
     if (s->d_rot_center) (void)hipFree(s->d_rot_center);
+    if (s->d_pair) (void)hipFree(s->d_pair);
     if (s->d_qt) (void)hipFree(s->d_qt);
     if (s->d_qp) (void)hipFree(s->d_qp);
     delete s;
@@ -151,23 +158,39 @@ int qr_rotate(const Space *s, const float *rows, uint64_t n, bool subtract_cente
     return qg_launch_rot_gemm(&g, st);
 }
 
-// setCentroids for K = 1 (space_rabitq.h:2918-2936).
+// setCentroids (space_rabitq.h:2918-2936): the centroids, each one's
+// |c|^2 (FP64 sum, stored as float) and rotation, and for K > 1 the
+// pairwise terms of buildCentroidDeltaContexts (:2564-2610).
+int space_set_centroids(Space *s, const float *centroids_host, size_t count) {
+    if (!s || !centroids_host) return fail("null space or centroids");
+    const size_t K = s->centroid_count;
+    if (count != K) return fail("the space holds %zu centroids, not %zu", K, count);
+    s->h_center.assign(centroids_host, centroids_host + K * s->dim);
+    if (check_hip(hipMemcpy(s->d_center, centroids_host, K * s->dim * 4, hipMemcpyHostToDevice), "upload centroids"))
+        return -1;
+    double *d_norm = nullptr;
+    if (check_hip(hipMalloc(&d_norm, K * sizeof(double)), "hipMalloc")) return -1;
+    int rc = qg_launch_row_seed(s->d_center, K, (uint32_t)s->dim, nullptr, nullptr, d_norm, nullptr, 0);
+    if (!rc && s->rotation == 1) rc = qr_rotate(s, s->d_center, K, false, s->d_rot_center, 0);
+    else if (!rc) rc = qg_launch_rotate_rows(s->d_center, K, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0,
+                                             s->d_signs, s->d_rot_center, 0);
+    if (!rc && K > 1)
+        rc = qg_launch_centroid_pairs(s->d_center, s->d_rot_center, (uint32_t)K, (uint32_t)s->dim, (uint32_t)s->cd,
+                                      s->d_pair, 0);
+    std::vector<double> norm(K, 0.0);
+    if (!rc) rc = hipMemcpy(norm.data(), d_norm, K * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d_norm);
+    if (rc) return check_hip(rc, "set centroids");
+    for (size_t k = 0; k < K; ++k) s->h_cnorm[k] = static_cast<float>(norm[k]);
+    s->center_norm_sqr = s->h_cnorm[0];
+    return 0;
+}
+
+// setGlobalCenter: setCentroids with one centroid, which fails for K > 1.
 int space_set_center(Space *s, const float *center_host) {
     if (!s || !center_host) return fail("null space or center");
-    s->h_center.assign(center_host, center_host + s->dim);
-    if (check_hip(hipMemcpy(s->d_center, center_host, s->dim * 4, hipMemcpyHostToDevice), "upload center")) return -1;
-    double *d_norm = nullptr;
-    if (check_hip(hipMalloc(&d_norm, sizeof(double)), "hipMalloc")) return -1;
-    int rc = qg_launch_row_seed(s->d_center, 1, (uint32_t)s->dim, nullptr, nullptr, d_norm, nullptr, 0);
-    if (!rc && s->rotation == 1) rc = qr_rotate(s, s->d_center, 1, false, s->d_rot_center, 0);
-    else if (!rc) rc = qg_launch_rotate_rows(s->d_center, 1, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0,
-                                             s->d_signs, s->d_rot_center, 0);
-    double norm = 0.0;
-    if (!rc) rc = hipMemcpy(&norm, d_norm, sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d_norm);
-    if (rc) return check_hip(rc, "set center");
-    s->center_norm_sqr = static_cast<float>(norm);
-    return 0;
+    if (s->centroid_count != 1) return fail("the space holds %zu centroids: set them all", s->centroid_count);
+    return space_set_centroids(s, center_host, 1);
 }
 
 IndexView view_of(const qg_index_desc *d) {
@@ -181,6 +204,7 @@ IndexView view_of(const qg_index_desc *d) {
     v.rscale = d->rscale;
     v.rcodes = d->rcodes;
     v.n = d->n;
+    v.cid = d->cid;
     return v;
 }
 
@@ -205,6 +229,8 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
     if (!out->codes || !out->hdr || !out->sfac || !out->msb || !out->factors || !out->fvalid || !out->rscale ||
         !out->rcodes || !out->rfac)
         return fail("encode output view has a null array");
+    if (s->centroid_count > 1 && !out->cid) return fail("a space of several centroids encodes into a centroid-id array");
+    if (s->centroid_count > 1 && s->rotation != 0) return fail("the QR rotation holds one centroid");
     const size_t per = encode_row_ws_bytes(s);
     const uint64_t batch = ws_bytes / per;
     if (batch == 0 || !ws) return fail("encode workspace too small: need at least %zu bytes", per);
@@ -224,6 +250,8 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.precision = s->rotation == 1 ? 1 : 0;
         a.signs = s->d_signs;
         a.center = s->d_center;
+        a.centroid_count = (uint32_t)s->centroid_count;
+        a.cid = out->cid ? out->cid + b0 : nullptr;
         a.rot = reinterpret_cast<float *>(w);
         a.sa = reinterpret_cast<float *>(w + m * cd * 4);
         a.si = reinterpret_cast<uint16_t *>(w + m * cd * 8);
@@ -264,12 +292,21 @@ int qg_space_set_center(void *space, const float *center_host) {
     return space_set_center(static_cast<Space *>(space), center_host);
 }
 
+int qg_space_set_centroids(void *space, const float *centroids_host, size_t count) {
+    return space_set_centroids(static_cast<Space *>(space), centroids_host, count);
+}
+
+size_t qg_space_centroid_count(const void *space) {
+    return space ? static_cast<const Space *>(space)->centroid_count : 0;
+}
+
 int qg_space_set_rotation_qr(void *space, const double *gaussian, int bf16_parts) {
     Space *s = static_cast<Space *>(space);
     if (!s || !gaussian) return fail("null space or Gaussian matrix");
     if (bf16_parts != 2 && bf16_parts != 3) return fail("bf16_parts must be 2 or 3");
     const size_t cd = s->cd;
     if (cd < 128) return fail("the QR rotation needs a code dimension of at least 128 (dimension above 64)");
+    if (s->centroid_count != 1) return fail("the QR rotation holds one centroid");
     // A = G^T: row j of A is column j of G
     std::vector<double> at(cd * cd);
     for (size_t i = 0; i < cd; ++i)
@@ -387,6 +424,7 @@ int qg_unpack_records(const void *space, const uint8_t *compact, size_t compact_
     a.in_compact = compact;
     a.in_residual = residual;
     a.codes = out->codes;
+    a.cid_out = out->cid;
     a.hdr = reinterpret_cast<float2 *>(out->hdr);
     a.sfac = reinterpret_cast<float2 *>(out->sfac);
     a.rscale = out->rscale;
@@ -408,38 +446,47 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8 && codec != QG_CODEC_B1 && codec != QG_CODEC_B1_MAIN)
         return fail("unsupported query codec %d", codec);
     if (!out->q || !out->x8 || !out->x8p || !out->qs) return fail("query output view has a null array");
-    out->nq = nq;
+    // one context per centroid (the eager centroid queries of
+    // initializePreparedQuery, space_rabitq.h:3665-3698): rows k * nq + i
+    const size_t K = s->centroid_count;
+    out->nq = nq * K;
     if (nq == 0) return 0;
     char *w = static_cast<char *>(ws);
     uint64_t *seed = reinterpret_cast<uint64_t *>(w);
     double *nrm = reinterpret_cast<double *>(w + nq * 8);
     double *dot = reinterpret_cast<double *>(w + nq * 16);
     hipStream_t st = (hipStream_t)stream;
-    // the FNV-1a seed feeds only the INT8 dither
-    if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center, codec == QG_CODEC_INT8 ? seed : nullptr,
-                                     nrm, dot, st),
-                  "row seed"))
-        return -1;
-    PrepParams p;
-    p.raw = queries;
-    p.nq = nq;
-    p.dim = (uint32_t)s->dim;
-    p.cd = (uint32_t)s->cd;
-    p.codec = codec;
-    p.identity = s->seed == 0;
-    p.signs = s->d_signs;
-    p.rot_center = s->d_rot_center;
-    p.center_norm_sqr = s->center_norm_sqr;
-    p.seed = seed;
-    p.raw_norm = nrm;
-    p.raw_dot = dot;
-    p.q_out = out->q;
-    p.pre_rotated = s->rotation == 1 ? 1 : 0;
-    if (p.pre_rotated && check_hip(qr_rotate(s, queries, nq, false, out->q, st), "rotation GEMM launch")) return -1;
-    p.x8_out = out->x8;
-    p.x8p_out = out->x8p;
-    p.qs_out = reinterpret_cast<QScalars *>(out->qs);
-    return check_hip(qg_launch_query_prep(&p, st), "query prep");
+    const size_t cd = s->cd;
+    for (size_t k = 0; k < K; ++k) {
+        // the FNV-1a seed feeds only the INT8 dither; |q|^2 and <q, c_k> in FP64
+        if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center + k * s->dim,
+                                         codec == QG_CODEC_INT8 ? seed : nullptr, nrm, dot, st),
+                      "row seed"))
+            return -1;
+        PrepParams p;
+        p.raw = queries;
+        p.nq = nq;
+        p.dim = (uint32_t)s->dim;
+        p.cd = (uint32_t)cd;
+        p.codec = codec;
+        p.identity = s->seed == 0;
+        p.signs = s->d_signs;
+        p.rot_center = s->d_rot_center + k * cd;
+        p.center_norm_sqr = s->h_cnorm[k];
+        p.seed = seed;
+        p.centroid_id = (uint32_t)k;
+        p.raw_norm = nrm;
+        p.raw_dot = dot;
+        p.q_out = out->q + k * nq * cd;
+        p.pre_rotated = s->rotation == 1 ? 1 : 0;
+        if (p.pre_rotated && check_hip(qr_rotate(s, queries, nq, false, p.q_out, st), "rotation GEMM launch"))
+            return -1;
+        p.x8_out = out->x8 + k * nq * cd;
+        p.x8p_out = out->x8p + k * nq * cd;
+        p.qs_out = reinterpret_cast<QScalars *>(out->qs) + k * nq;
+        if (check_hip(qg_launch_query_prep(&p, st), "query prep")) return -1;
+    }
+    return 0;
 }
 
 int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, const qg_query_desc *qv,
@@ -448,7 +495,12 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
             RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !ix || !ids || (!qv && kernel != QG_SCAN_SYMMETRIC)) return fail("null argument to qg_scan");
-    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_B1) return fail("unknown scan kernel %d", kernel);
+    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_RESIDUAL_RECORD) return fail("unknown scan kernel %d", kernel);
+    const bool record_form = kernel == QG_SCAN_RESIDUAL_RECORD;
+    if (record_form) {
+        if (fixed) return fail("the per-record residual form scores pairs");
+        kernel = QG_SCAN_RESIDUAL;
+    }
     if (!fixed && !qidx) return fail("pair scans need qidx");
     if (fixed && kernel != QG_SCAN_INT8_FULL && kernel != QG_SCAN_ASYM && kernel != QG_SCAN_RESIDUAL)
         return fail("kernel %d has no fixed-list form", kernel);
@@ -474,6 +526,12 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
     a.out_est = reinterpret_cast<PaperEstimate *>(out_est);
     a.out_iv = reinterpret_cast<Interval *>(out_iv);
     a.out_dot = out_dot;
+    a.centroid_count = (uint32_t)s->centroid_count;
+    a.rot_centroids = s->d_rot_center;
+    a.pair_terms = s->d_pair;
+    a.record_form = record_form ? 1 : 0;
+    if (kernel == QG_SCAN_SYMMETRIC && s->centroid_count > 1 && !ix->cid)
+        return fail("symmetric distances of a space of several centroids need the index's centroid ids");
     return check_hip(qg_launch_scan(kernel, fixed, &a, (hipStream_t)stream), "scan launch");
 }
 
@@ -507,6 +565,7 @@ int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc
     if (!s || !ix || !qv || !degrees || !edges || !workspace || (!pool_only && (!out_ids || !out_dists)))
         return fail("null argument to qg_search");
     if (pool_only && !pool_ids) return fail("qg_search without outputs needs the pool outputs");
+    if (s->centroid_count != 1) return fail("the batched search holds one query context: spaces of one centroid only");
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
     // the search stages MSB rows in 16-byte pieces: cd / 8 bytes must fill one
     if (s->cd < 128) return fail("the batched search needs a code dimension of at least 128 (dimension above 64)");

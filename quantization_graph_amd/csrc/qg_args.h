@@ -16,6 +16,7 @@ struct PrepParams {
     const float *rot_center;   // [cd]
     float center_norm_sqr;
     const uint64_t *seed;      // [nq] FNV-1a of the raw bytes
+    uint32_t centroid_id;      // mixed into the INT8 dither seed (space_rabitq.h:3549-3558)
     const double *raw_norm;    // [nq]
     const double *raw_dot;     // [nq]
     float *q_out;              // [nq][cd]; holds the rotated raw rows on entry when pre_rotated
@@ -50,6 +51,12 @@ struct ScanArgs {
     PaperEstimate *out_est;     // gates
     Interval *out_iv;           // residual intervals
     int64_t *out_dot;           // raw integer dots (optional)
+    // symmetric distances of spaces with K > 1 centroids: the cross-centroid
+    // terms (space_rabitq.h:2634-2683)
+    uint32_t centroid_count;
+    const float *rot_centroids; // [K][cd] rotated centroids
+    const float2 *pair_terms;   // [K][K] {|c_a - c_b|^2, half sum of r_a - r_b}
+    int record_form;            // residual pairs: the per-record lane form (see k_residual)
 };
 
 struct SelectArgs {
@@ -81,7 +88,9 @@ struct EncArgs {
     int mse;
     int precision;             // QR rotation: compensated FP32 norms and factors, rot filled by the GEMM
     const float *signs;        // [cd]
-    const float *center;       // [dim]
+    const float *center;       // [K][dim] centroids; vector v uses row cid[v]
+    uint32_t centroid_count;   // K
+    uint8_t *cid;              // [n] assigned centroid (K > 1: written by k_enc_assign; may be null for K = 1)
     float *rot;                // ws [n][cd]
     float *sa;                 // ws [n][cd]  |o| sorted descending
     uint16_t *si;              // ws [n][cd]  matching dimension index
@@ -122,6 +131,7 @@ struct PackArgs {
     const uint8_t *in_compact; const uint8_t *in_residual;
     uint8_t *codes; float2 *hdr; float2 *sfac; uint16_t *rscale; uint8_t *rcodes; float2 *rfac_out;
     uint8_t *msb; float4 *factors; uint8_t *fvalid;
+    uint8_t *cid_out;          // unpack: centroid ids (optional)
 };
 
 struct SearchArgs {

@@ -87,8 +87,10 @@ bool stage_index(const Space *s, Arena *a, uint64_t n, qg_index_desc *d) {
     d->rscale = static_cast<uint16_t *>(a->take(n * 2 * s->block_count));
     d->rcodes = static_cast<uint8_t *>(a->take(n * cd / 2));
     d->rfac = static_cast<float *>(a->take(n * 8));
+    d->cid = static_cast<uint8_t *>(a->take(n));
     d->n = n;
-    return d->codes && d->hdr && d->sfac && d->msb && d->factors && d->fvalid && d->rscale && d->rcodes && d->rfac;
+    return d->codes && d->hdr && d->sfac && d->msb && d->factors && d->fvalid && d->rscale && d->rcodes && d->rfac &&
+           d->cid;
 }
 
 // Thread-local prepared query: device buffers for one query.
@@ -112,19 +114,22 @@ struct BridgeQuery {
         if (raw) (void)hipFree(raw);
         q = nullptr; x8 = nullptr; x8p = nullptr; qs = nullptr; ws = nullptr; raw = nullptr;
     }
+    size_t contexts = 0;  // one per centroid of the space
     bool ensure(const Space *s) {
-        if (owner == s && cd == s->cd && dim == s->dim) return true;
+        if (owner == s && cd == s->cd && dim == s->dim && contexts == s->centroid_count) return true;
         release();
         cd = s->cd;
         dim = s->dim;
+        contexts = s->centroid_count;
         owner = s;
-        return hipMalloc(&q, cd * 4) == hipSuccess && hipMalloc(&x8, cd) == hipSuccess &&
-               hipMalloc(&x8p, cd) == hipSuccess && hipMalloc(&qs, 48) == hipSuccess &&
+        const size_t K = contexts;
+        return hipMalloc(&q, K * cd * 4) == hipSuccess && hipMalloc(&x8, K * cd) == hipSuccess &&
+               hipMalloc(&x8p, K * cd) == hipSuccess && hipMalloc(&qs, K * 48) == hipSuccess &&
                hipMalloc(&ws, 256) == hipSuccess && hipMalloc(&raw, dim * 4) == hipSuccess;
     }
     qg_query_desc desc() const {
         qg_query_desc d;
-        d.q = q; d.x8 = x8; d.x8p = x8p; d.qs = qs; d.nq = 1;
+        d.q = q; d.x8 = x8; d.x8p = x8p; d.qs = qs; d.nq = contexts;
         return d;
     }
 };
@@ -172,6 +177,23 @@ uint32_t *upload_zeros_u32(Arena *a, size_t count) {
     return d;
 }
 
+// The query context of each staged record: its centroid id (the compact
+// record's last byte; queryForEncoded, space_rabitq.h:3404-3416), one
+// prepared context per centroid.
+uint32_t *upload_centroid_qidx(const Space *s, Arena *a, const std::vector<uint8_t> &comp, size_t count) {
+    std::vector<uint32_t> h(count);
+    for (size_t i = 0; i < count; ++i) h[i] = comp[i * s->compact_bytes + 16 + s->cd / 2];
+    uint32_t *d = static_cast<uint32_t *>(a->take(count * 4 + 4));
+    if (d && count) (void)hipMemcpy(d, h.data(), count * 4, hipMemcpyHostToDevice);
+    return d;
+}
+
+bool centroid_ids_valid(const Space *s, const std::vector<uint8_t> &comp, size_t count) {
+    for (size_t i = 0; i < count; ++i)
+        if (comp[i * s->compact_bytes + 16 + s->cd / 2] >= s->centroid_count) return false;
+    return true;
+}
+
 // Stage the compact records ids[] (strided host rows) and run one pair scan
 // of `kernel` for the prepared query; out_f / out_est copied back.
 bool scan_compact(const Space *s, const BridgeQuery *bq, int kernel, const uint32_t *ids, size_t count,
@@ -180,10 +202,11 @@ bool scan_compact(const Space *s, const BridgeQuery *bq, int kernel, const uint3
     gather(base, stride, s->compact_bytes, ids, count, &comp);
     Arena a;
     if (!arena_for(s, count * (s->compact_bytes + 2 * s->cd + 128) + 4096, &a)) return false;
+    if (!centroid_ids_valid(s, comp, count)) return false;
     qg_index_desc d;
     if (!upload_records(s, &a, comp, nullptr, count, &d)) return false;
     uint32_t *rows = upload_iota(&a, count);
-    uint32_t *qidx = upload_zeros_u32(&a, count);
+    uint32_t *qidx = upload_centroid_qidx(s, &a, comp, count);
     float *df = static_cast<float *>(a.take(count * 4 + 4));
     RabitqPaperEstimate *de = static_cast<RabitqPaperEstimate *>(a.take(count * 24 + 24));
     if (!rows || !qidx || !df || !de) return false;
@@ -254,7 +277,7 @@ bool rabitq_encode_full(const void *space, const float *raw, void *encoded_out) 
     std::memcpy(o + 16, r.data(), 8);
     std::memcpy(o + 24, r.data() + 8, sb);
     std::memcpy(o + 24 + sb, c.data() + 16, cb);
-    o[24 + sb + cb] = 0;
+    o[24 + sb + cb] = c[16 + cb];  // centroid id (assignCentroid, space_rabitq.h:2944-2960)
     std::memcpy(o + 25 + sb + cb, r.data() + 8 + sb, cb);
     return true;
 }
@@ -449,12 +472,13 @@ bool rabitq_query_distance_batch(const void *space, const void *prepared, const 
     if (count == 0) return true;
     std::vector<uint8_t> comp;
     gather(encoded_base, encoded_stride, s->compact_bytes, ids, count, &comp);
+    if (!centroid_ids_valid(s, comp, count)) return false;
     Arena a;
     if (!arena_for(s, count * (s->compact_bytes + 2 * s->cd + 96) + 4096, &a)) return false;
     qg_index_desc d;
     if (!upload_records(s, &a, comp, nullptr, count, &d)) return false;
     uint32_t *rows = upload_iota(&a, count);
-    uint32_t *qidx = upload_zeros_u32(&a, count);
+    uint32_t *qidx = upload_centroid_qidx(s, &a, comp, count);
     float *dsi = static_cast<float *>(a.take(count * 4 + 4));
     float *dout = static_cast<float *>(a.take(count * 4 + 4));
     if (!rows || !qidx || !dsi || !dout) return false;
@@ -541,9 +565,13 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
     return hipMemcpy(out, dout, count * 24, hipMemcpyDeviceToHost) == hipSuccess;
 }
 
-bool rabitq_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
-                                    const void *encoded_base, size_t encoded_stride, const void *residual_base,
-                                    size_t residual_stride, const float *long_distances, RabitqDistanceInterval *out) {
+}  // extern "C"
+
+namespace qg {
+namespace {
+bool residual_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                    const void *encoded_base, size_t encoded_stride, const void *residual_base,
+                    size_t residual_stride, const float *long_distances, RabitqDistanceInterval *out, bool single) {
     const Space *s = static_cast<const Space *>(space);
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s ||
@@ -553,32 +581,47 @@ bool rabitq_residual_distance_batch(const void *space, const void *prepared, con
     std::vector<uint8_t> comp, res;
     gather(encoded_base, encoded_stride, s->compact_bytes, ids, count, &comp);
     gather(residual_base, residual_stride, s->residual_bytes, ids, count, &res);
+    if (!centroid_ids_valid(s, comp, count)) return false;
     Arena a;
     if (!arena_for(s, count * (s->compact_bytes + s->residual_bytes + 2 * s->cd + 96) + 4096, &a)) return false;
     qg_index_desc d;
     if (!upload_records(s, &a, comp, &res, count, &d)) return false;
     uint32_t *rows = upload_iota(&a, count);
-    uint32_t *qidx = upload_zeros_u32(&a, count);
+    uint32_t *qidx = upload_centroid_qidx(s, &a, comp, count);
     float *dl = static_cast<float *>(a.take(count * 4));
     RabitqDistanceInterval *dout = static_cast<RabitqDistanceInterval *>(a.take(count * 12));
     if (!rows || !qidx || !dl || !dout) return false;
     (void)hipMemcpy(dl, long_distances, count * 4, hipMemcpyHostToDevice);
     const qg_query_desc qd = bq->desc();
-    if (qg_scan(s, QG_SCAN_RESIDUAL, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, dl, nullptr, nullptr, nullptr,
+    // the batch loop holds one centroid; otherwise the reference scores each
+    // record alone (space_rabitq.h:5446-5461)
+    const int kernel = (s->centroid_count > 1 || single) ? QG_SCAN_RESIDUAL_RECORD : QG_SCAN_RESIDUAL;
+    if (qg_scan(s, kernel, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, dl, nullptr, nullptr, nullptr,
                 dout, nullptr, nullptr) != 0) {
         for (size_t i = 0; i < count; ++i) out[i] = RabitqDistanceInterval{long_distances[i], long_distances[i], long_distances[i]};
         return true;
     }
     return hipMemcpy(out, dout, count * 12, hipMemcpyDeviceToHost) == hipSuccess;
 }
+}  // namespace
+}  // namespace qg
+
+extern "C" {
+
+bool rabitq_residual_distance_batch(const void *space, const void *prepared, const uint32_t *ids, size_t count,
+                                    const void *encoded_base, size_t encoded_stride, const void *residual_base,
+                                    size_t residual_stride, const float *long_distances, RabitqDistanceInterval *out) {
+    return residual_batch(space, prepared, ids, count, encoded_base, encoded_stride, residual_base, residual_stride,
+                          long_distances, out, false);
+}
 
 // ------------------------------------------- single-record and remaining ---
 
 bool rabitq_space_set_centroids(void *space, const float *centroids, size_t centroid_count) {
     // setCentroids (space_rabitq.h:2918-2936): the count must match the
-    // space's centroid count, which is 1 here.
-    if (!space || !centroids || centroid_count != 1) return false;
-    return rabitq_space_set_center(space, centroids);
+    // space's centroid count.
+    if (!space || !centroids) return false;
+    return space_set_centroids(static_cast<Space *>(space), centroids, centroid_count) == 0;
 }
 
 bool rabitq_train_centroids(const float *data, size_t count, size_t dim, size_t k, size_t sample_count,
@@ -722,8 +765,8 @@ RabitqDistanceInterval rabitq_residual_distance_from_record(const void *space, c
     const uint32_t id = 0;
     RabitqDistanceInterval iv{long_distance, long_distance, long_distance};
     if (!s || !encoded || !residual_record ||
-        !rabitq_residual_distance_batch(space, prepared, &id, 1, encoded, s->compact_bytes, residual_record,
-                                        s->residual_bytes, &long_distance, &iv))
+        !residual_batch(space, prepared, &id, 1, encoded, s->compact_bytes, residual_record, s->residual_bytes,
+                        &long_distance, &iv, true))
         return RabitqDistanceInterval{long_distance, long_distance, long_distance};
     return iv;
 }
@@ -744,8 +787,9 @@ bool rabitq_full_residual_distance_batch(const void *space, const void *prepared
     bool ok = arena_for(s, count * (s->compact_bytes + s->residual_bytes + 2 * s->cd + 128) + 4096, &a);
     qg_index_desc d;
     ok = ok && upload_records(s, &a, comp, &res, count, &d);
+    ok = ok && centroid_ids_valid(s, comp, count);
     uint32_t *rows = ok ? upload_iota(&a, count) : nullptr;
-    uint32_t *qidx = ok ? upload_zeros_u32(&a, count) : nullptr;
+    uint32_t *qidx = ok ? upload_centroid_qidx(s, &a, comp, count) : nullptr;
     float *dl = ok ? static_cast<float *>(a.take(count * 4)) : nullptr;
     RabitqDistanceInterval *dout = ok ? static_cast<RabitqDistanceInterval *>(a.take(count * 12)) : nullptr;
     ok = rows && qidx && dl && dout;
@@ -753,7 +797,8 @@ bool rabitq_full_residual_distance_batch(const void *space, const void *prepared
     // mode-0 long distance, then the residual interval from it, on the device
     ok = ok && qg_scan(s, QG_SCAN_ASYM, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, nullptr, dl, nullptr,
                        nullptr, nullptr, nullptr, nullptr) == 0;
-    ok = ok && qg_scan(s, QG_SCAN_RESIDUAL, 0, &d, &qd, rows, qidx, 0, count, 0.0f, nullptr, dl, nullptr, nullptr,
+    ok = ok && qg_scan(s, s->centroid_count > 1 ? QG_SCAN_RESIDUAL_RECORD : QG_SCAN_RESIDUAL, 0, &d, &qd, rows, qidx,
+                       0, count, 0.0f, nullptr, dl, nullptr, nullptr,
                        nullptr, dout, nullptr, nullptr) == 0;
     ok = ok && hipMemcpy(out, dout, count * 12, hipMemcpyDeviceToHost) == hipSuccess;
     if (!ok)

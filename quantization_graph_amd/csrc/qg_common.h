@@ -28,6 +28,7 @@ struct IndexView {
     const uint16_t *rscale;   // [n][cd/16]  residual block scales (fp16 bits)
     const uint8_t *rcodes;    // [n][cd/2]   residual 4-bit codes
     uint64_t n;
+    const uint8_t *cid;       // [n]         centroid id (null: all 0)
 };
 
 // Per-query scalars of the prepared query context (QueryContext,

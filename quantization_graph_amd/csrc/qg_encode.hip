@@ -89,6 +89,31 @@ __device__ __forceinline__ float comp_block(Comp v, float *red_s, float *red_c) 
 
 
 
+// assignCentroid (space_rabitq.h:2944-2960): the first centroid of least
+// float squared distance, the sum over dimensions in index order with the
+// reference compiler's fused multiply-add. One lane per vector.
+__global__ void k_enc_assign(EncArgs a) {
+    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.n) return;
+    const float *x = a.x + v * a.dim;
+    uint32_t best = 0;
+    float best_d = __builtin_huge_valf();
+    for (uint32_t k = 0; k < a.centroid_count; ++k) {
+        const float *c = a.center + (uint64_t)k * a.dim;
+        float d = 0.0f;
+        for (uint32_t i = 0; i < a.dim; ++i) {
+            const float diff = x[i] - c[i];
+            d = fmaf(diff, diff, d);
+        }
+        if (d < best_d) { best_d = d; best = k; }
+    }
+    a.cid[v] = (uint8_t)best;
+}
+
+__device__ __forceinline__ const float *center_of(const EncArgs &a, uint64_t v) {
+    return a.centroid_count > 1 ? a.center + (uint64_t)a.cid[v] * a.dim : a.center;
+}
+
 // ||x - c||^2 (float). The reference compiler vectorises this loop: the
 // products of its 16- and 8-wide blocks are rounded and added in index
 // order; only the scalar tail (D mod 8 elements) is fused.
@@ -96,17 +121,19 @@ __global__ void k_enc_norm(EncArgs a) {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= a.n) return;
     const float *x = a.x + v * a.dim;
+    const float *center = center_of(a, v);
     const uint32_t D = a.dim;
+    if (a.cid && a.centroid_count == 1) a.cid[v] = 0;
     if (a.precision) {  // compensated sum of squares
         Comp c;
-        for (uint32_t i = 0; i < D; ++i) c.add_sq(x[i] - a.center[i]);
+        for (uint32_t i = 0; i < D; ++i) c.add_sq(x[i] - center[i]);
         a.meta[v].rn = sqrtf(c.value());
         return;
     }
     const uint32_t vec_end = D >= 16 ? (D / 16) * 16 + ((D % 16) >= 8 ? 8 : 0) : 0;
     float s = 0.0f;
     for (uint32_t i = 0; i < D; ++i) {
-        const float r = x[i] - a.center[i];
+        const float r = x[i] - center[i];
         if (i < vec_end) { const float p = r * r; s = s + p; }
         else s = fmaf(r, r, s);
     }
@@ -152,10 +179,11 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_rotate(EncArgs a) {
     } else {
         const bool scale = rn > 0.0f;
         const float inv = scale ? 1.0f / rn : 1.0f;
+        const float *center = center_of(a, vi);
         for (uint32_t i = t; i < cd; i += blockDim.x) {
             float u = 0.0f;
             if (i < a.dim) {
-                u = x[i] - a.center[i];
+                u = x[i] - center[i];
                 if (scale) u = u * inv;
                 if (!a.identity) u = u * a.signs[i];
             }
@@ -500,7 +528,7 @@ __global__ void k_pack(PackArgs a) {
         if (i < 8) b = ((const uint8_t *)&a.ix.hdr[v])[i];
         else if (i < 16) b = ((const uint8_t *)&a.ix.sfac[v])[i - 8];
         else if (i < 16 + cd / 2) b = a.ix.codes[v * (cd / 2) + (i - 16)];
-        else b = 0;
+        else b = a.ix.cid ? a.ix.cid[v] : 0;  // centroid id
         c[i] = b;
     }
     const uint32_t sb = 2 * a.block_count;
@@ -522,6 +550,7 @@ __global__ void k_unpack(PackArgs a) {
     const uint8_t *c = a.in_compact + v * a.compact_stride;
     const uint8_t *r = a.in_residual + v * a.residual_stride;
     for (uint32_t i = threadIdx.x; i < cd / 2; i += blockDim.x) a.codes[v * (cd / 2) + i] = c[16 + i];
+    if (a.cid_out && threadIdx.x == 0) a.cid_out[v] = c[16 + cd / 2];
     for (uint32_t b = threadIdx.x; b < cd / 8; b += blockDim.x) {
         uint8_t m = 0;
         for (int j = 0; j < 4; ++j) {
@@ -583,6 +612,7 @@ extern "C" {
 int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream) {
     if (a->n == 0) return 0;
     const unsigned lane_blocks = (unsigned)((a->n + 63) / 64);
+    if (a->cid && a->centroid_count > 1) hipLaunchKernelGGL(qg::k_enc_assign, dim3(lane_blocks), dim3(64), 0, stream, *a);
     hipLaunchKernelGGL(qg::k_enc_norm, dim3(lane_blocks), dim3(64), 0, stream, *a);
     hipLaunchKernelGGL(qg::k_enc_rotate, dim3((unsigned)a->n), dim3(qg::kEncThreads), (size_t)a->cd * 12, stream, *a);
     hipLaunchKernelGGL(qg::k_enc_scan, dim3(lane_blocks), dim3(64), 0, stream, *a);

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
         scale = max_abs / 128.0f;
         if (t == 0) {
             uint64_t sd = p.seed[qi];
-            sd ^= 0ull + 0x9E3779B97F4A7C15ull + (sd << 6) + (sd >> 2);  // centroid id 0
+            sd ^= (uint64_t)p.centroid_id + 0x9E3779B97F4A7C15ull + (sd << 6) + (sd >> 2);
             mt[0] = sd;
             for (int i = 1; i < 312; ++i) mt[i] = 6364136223846793005ull * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
         }

@@ -352,7 +352,10 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
 // Residual-4 rerank (compute_residual_distance_intervals_batch_by_id,
 // space_rabitq.h:5432-5548): per 16-dim block, 8 lane products
 // fma(hi, odd, lo*even), lane sum in order, ips = fma(scale, block, ips);
-// blocks whose fp16 scale decodes to 0 or non-finite are skipped.
+// blocks whose fp16 scale decodes to 0 or non-finite are skipped. With
+// record_form (the per-record computeResidualDistanceIntervalFromRecord,
+// :1723-1735, which single-record calls and spaces of several centroids
+// take) the lane products are fma(lo, even, hi*odd).
 template <bool kFixed>
 __global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
     __shared__ float qsh[2048];
@@ -393,8 +396,12 @@ __global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
             int lo = (int)(byte & 0x0Fu), hi = (int)(byte >> 4);
             lo -= (lo > 7) ? 16 : 0;
             hi -= (hi > 7) ? 16 : 0;
-            const float pl = (float)lo * qv[2 * l];
-            bip = bip + fmaf((float)hi, qv[2 * l + 1], pl);
+            // the batch loop fuses the odd product into the even one; the
+            // per-record form (residualInt4IpAvx2, :2278-2325) the other way
+            float prod;
+            if (a.record_form) prod = fmaf((float)lo, qv[2 * l], (float)hi * qv[2 * l + 1]);
+            else prod = fmaf((float)hi, qv[2 * l + 1], (float)lo * qv[2 * l]);
+            bip = bip + prod;
         }
         ips = fmaf(scale, bip, ips);
         const double pde = 0.5 * fabs((double)scale);
@@ -607,9 +614,101 @@ __global__ void __launch_bounds__(kScanThreads) k_b1(ScanArgs a) {
     if (a.out_dot) a.out_dot[item] = acc;
 }
 
+// estimateTwiceResidualIp (space_rabitq.h:2612-2626) with the centroid
+// delta context r_a - r_b (buildCentroidDeltaContexts, :2564-2610) as the
+// query: long_scale * (8 short_ip + remaining_ip - 7 half_sum), the inner
+// products in asym_one's lane order, the delta taken on the fly.
+__device__ __forceinline__ float twice_residual_ip(const float *ra, const float *rb, float half_sum,
+                                                   const uint8_t *code_row, uint32_t cd, float2 h) {
+    if (!long_scale_ok(h.y)) return 0.0f;
+    const uint4 *row = (const uint4 *)code_row;
+    float sh[16], rm[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) { sh[j] = 0.0f; rm[j] = 0.0f; }
+    for (uint32_t c = 0; c < cd / 32; ++c) {
+        const uint4 w = row[c];
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * k + b;
+                const uint32_t i = 32 * c + 8 * k + 2 * b;  // dims i (low nibble), i + 1 (high)
+                const float qe = ra[i] - rb[i], qo = ra[i + 1] - rb[i + 1];
+                const uint32_t byte = (ws[k] >> (8 * b)) & 0xFFu;
+                const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
+                rm[j] = fmaf((float)(lo & 7u), qe, rm[j]);
+                rm[j] = fmaf((float)(hi & 7u), qo, rm[j]);
+                if (lo > 7u) sh[j] = sh[j] + qe;
+                if (hi > 7u) sh[j] = sh[j] + qo;
+            }
+        }
+    }
+    const float short_ip = reduce16(sh) - half_sum;
+    const float t = 8.0f * short_ip + reduce16(rm);
+    return h.y * fmaf(-7.0f, half_sum, t);
+}
+
+// The cross-centroid terms of distanceBetweenEncoded (:2634-2683) on top of
+// k_symmetric's distances, for pairs whose records hold different
+// centroids: |c_a - c_b|^2 enters the invalid-record sum, else the distance
+// gains |c_a - c_b|^2 + 2<d, r_l> - 2<d, r_r> estimated with d = r_a - r_b.
+__global__ void __launch_bounds__(kScanThreads) k_sym_cross(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint32_t l = a.qidx[item], r = a.ids[item];
+    const uint32_t K = a.centroid_count;
+    const uint32_t ca = a.ix.cid[l], cb = a.ix.cid[r];
+    if (!(ca < K && cb < K && ca != cb)) return;
+    const float2 pt = a.pair_terms[ca * K + cb];
+    const float2 lh = a.ix.hdr[l], rh = a.ix.hdr[r];
+    const double ln = sqrt((0.0 < (double)lh.x) ? (double)lh.x : 0.0);
+    const double rn = sqrt((0.0 < (double)rh.x) ? (double)rh.x : 0.0);
+    if (!(ln > 0.0) || !(rn > 0.0) || !(lh.y > 0.0f) || !(rh.y > 0.0f) || !__builtin_isfinite(lh.y) ||
+        !__builtin_isfinite(rh.y)) {
+        a.out_f[item] = (float)((double)lh.x + (double)rh.x + (double)pt.x);
+        return;
+    }
+    const float *ra = a.rot_centroids + (uint64_t)ca * a.cd, *rb = a.rot_centroids + (uint64_t)cb * a.cd;
+    const float tl = twice_residual_ip(ra, rb, pt.y, a.ix.codes + (uint64_t)l * (a.cd / 2), a.cd, lh);
+    const float tr = twice_residual_ip(ra, rb, pt.y, a.ix.codes + (uint64_t)r * (a.cd / 2), a.cd, rh);
+    a.out_f[item] = a.out_f[item] + ((pt.x + tl) - tr);
+}
+
+// Pair terms of buildCentroidDeltaContexts (:2564-2610), one lane per
+// (a, b): |c_a - c_b|^2 summed in FP64 in index order (stored as float),
+// and the half sum of r_a - r_b over coordinate pairs in float.
+__global__ void k_centroid_pairs(const float *c, const float *rc, uint32_t K, uint32_t dim, uint32_t cd,
+                                 float2 *pairs) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K * K) return;
+    const uint32_t ia = e / K, ib = e % K;
+    const float *ca = c + (uint64_t)ia * dim, *cb = c + (uint64_t)ib * dim;
+    double pn = 0.0;
+    for (uint32_t i = 0; i < dim; ++i) {
+        const double d = (double)ca[i] - (double)cb[i];
+        pn = fma(d, d, pn);
+    }
+    const float *ra = rc + (uint64_t)ia * cd, *rb = rc + (uint64_t)ib * cd;
+    float half = 0.0f;
+    for (uint32_t p = 0; p < cd / 2; ++p) {
+        const float ev = ra[2 * p] - rb[2 * p], od = ra[2 * p + 1] - rb[2 * p + 1];
+        half = half + (ev + od);
+    }
+    pairs[e] = make_float2((float)pn, half * 0.5f);
+}
+
 }  // namespace qg
 
 extern "C" {
+
+int qg_launch_centroid_pairs(const float *centroids, const float *rot_centroids, uint32_t K, uint32_t dim,
+                             uint32_t cd, float2 *pairs, hipStream_t st) {
+    const uint32_t total = K * K;
+    hipLaunchKernelGGL(qg::k_centroid_pairs, dim3((total + 63) / 64), dim3(64), 0, st, centroids, rot_centroids, K,
+                       dim, cd, pairs);
+    return (int)hipGetLastError();
+}
 
 enum { QG_K_INT8_FULL = 0, QG_K_ASYM = 1, QG_K_REM_FP32 = 2, QG_K_REM_INT8 = 3, QG_K_GATE_FP32 = 4,
        QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7, QG_K_PAPER_RECORD = 8, QG_K_B1 = 9 };
@@ -650,7 +749,10 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
             case QG_K_GATE_FP32: hipLaunchKernelGGL(qg::k_gate_fp32, grid, block, 0, stream, *a); break;
             case QG_K_GATE_INT8: hipLaunchKernelGGL(qg::k_gate_int8, grid, block, 0, stream, *a); break;
             case QG_K_RESIDUAL: hipLaunchKernelGGL(qg::k_residual<false>, grid, block, 0, stream, *a); break;
-            case QG_K_SYMMETRIC: hipLaunchKernelGGL(qg::k_symmetric, grid, block, 0, stream, *a); break;
+            case QG_K_SYMMETRIC:
+                hipLaunchKernelGGL(qg::k_symmetric, grid, block, 0, stream, *a);
+                if (a->centroid_count > 1) hipLaunchKernelGGL(qg::k_sym_cross, grid, block, 0, stream, *a);
+                break;
             case QG_K_PAPER_RECORD: hipLaunchKernelGGL(qg::k_paper_record, grid, block, 0, stream, *a); break;
             case QG_K_B1: hipLaunchKernelGGL(qg::k_b1, grid, block, 0, stream, *a); break;
             default: return (int)hipErrorInvalidValue;

@@ -17,11 +17,14 @@ struct Space {
     bool mse = true, fp16 = true;
     int codec = 0;  // query coarse codec selected through the bridge API
     size_t compact_bytes = 0, residual_bytes = 0, full_bytes = 0;
-    std::vector<float> h_signs, h_center;
+    std::vector<float> h_signs, h_center;  // h_center: [K][dim]
+    size_t centroid_count = 1;             // K
     float *d_signs = nullptr;
-    float *d_center = nullptr;
-    float *d_rot_center = nullptr;
-    float center_norm_sqr = 0.0f;
+    float *d_center = nullptr;             // [K][dim] centroids (row 0 = the global center when K = 1)
+    float *d_rot_center = nullptr;         // [K][cd] rotated centroids
+    float center_norm_sqr = 0.0f;          // |c_0|^2
+    std::vector<float> h_cnorm;            // [K] |c_k|^2
+    float2 *d_pair = nullptr;              // [K][K] {|c_a - c_b|^2, half sum of r_a - r_b}
     // rotation: 0 = sign-flipped Walsh-Hadamard (the reference's), 1 = the
     // QR rotation of the precision mode (qg_space_set_rotation_qr)
     int rotation = 0;
@@ -35,6 +38,7 @@ Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t res
                     bool mse, bool fp16);
 void space_destroy(Space *s);
 int space_set_center(Space *s, const float *center_host);
+int space_set_centroids(Space *s, const float *centroids_host, size_t count);
 int fail(const char *fmt, ...);
 int check_hip(int rc, const char *what);
 

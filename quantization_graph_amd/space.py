@@ -32,6 +32,10 @@ SCAN_REM_INT8 = 3
 SCAN_GATE_FP32 = 4
 SCAN_GATE_INT8 = 5
 SCAN_RESIDUAL = 6
+SCAN_SYMMETRIC = 7
+SCAN_PAPER_RECORD = 8
+SCAN_B1 = 9
+SCAN_RESIDUAL_RECORD = 10  # E1 in the per-record form (single records, spaces of several centroids)
 
 # RabitqPaperEstimate (rabitq_bridge.cpp:21-28) and RabitqDistanceInterval
 # (:15-19) as numpy record types.
@@ -83,6 +87,7 @@ class DeviceIndex:
     rscale  int16 [n, cd/16]  residual block scales (fp16 bit patterns)
     rcodes  uint8 [n, cd/2]   residual 4-bit two's-complement codes
     rfac    f32   [n, 2]      residual_norm_sqr, long_residual_ip
+    cid     uint8 [n]         centroid id (0 in a space of one centroid)
     """
 
     def __init__(self, space, n):
@@ -99,12 +104,13 @@ class DeviceIndex:
         self.rscale = torch.empty((n, cd // 16), dtype=torch.int16, device=dev)
         self.rcodes = torch.empty((n, cd // 2), dtype=torch.uint8, device=dev)
         self.rfac = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        self.cid = torch.zeros((n,), dtype=torch.uint8, device=dev)
 
     def desc(self):
         return IndexDesc(self.codes.data_ptr(), self.hdr.data_ptr(), self.sfac.data_ptr(),
                          self.msb.data_ptr(), self.factors.data_ptr(), self.fvalid.data_ptr(),
                          self.rscale.data_ptr(), self.rcodes.data_ptr(), self.rfac.data_ptr(),
-                         self.n)
+                         self.n, self.cid.data_ptr())
 
     def nbytes(self):
         return sum(t.numel() * t.element_size() for t in (
@@ -150,18 +156,21 @@ class DeviceQueries:
         self.space = space
         self.nq = nq
         self.codec = codec
-        self.q = torch.empty((nq, cd), dtype=torch.float32, device="cuda")
-        self.x8 = torch.empty((nq, cd), dtype=torch.int8, device="cuda")
-        self.x8p = torch.empty((nq, cd), dtype=torch.int8, device="cuda")
-        self.qs = torch.empty((nq, 48), dtype=torch.uint8, device="cuda")
+        # one context per centroid: row k * nq + i is query i's context for centroid k
+        rows = nq * space.centroid_count
+        self.rows = rows
+        self.q = torch.empty((rows, cd), dtype=torch.float32, device="cuda")
+        self.x8 = torch.empty((rows, cd), dtype=torch.int8, device="cuda")
+        self.x8p = torch.empty((rows, cd), dtype=torch.int8, device="cuda")
+        self.qs = torch.empty((rows, 48), dtype=torch.uint8, device="cuda")
 
     def desc(self):
         return QueryDesc(self.q.data_ptr(), self.x8.data_ptr(), self.x8p.data_ptr(),
-                         self.qs.data_ptr(), self.nq)
+                         self.qs.data_ptr(), self.rows)
 
     def scalars(self):
         """Per-query scalars as a numpy record array (QSCALAR_DTYPE)."""
-        return self.qs.cpu().numpy().view(QSCALAR_DTYPE).reshape(self.nq)
+        return self.qs.cpu().numpy().view(QSCALAR_DTYPE).reshape(self.rows)
 
 
 class RabitqSpace:
@@ -192,6 +201,7 @@ class RabitqSpace:
         self.residual_fp16 = bool(residual_fp16)
         self.codec = CODEC_INT8
         self.rotation = "fht"
+        self.centroid_count = L.qg_space_centroid_count(self.handle)
 
     @classmethod
     def new_with_centroids(cls, dim, seed, centroid_count, centroids):
@@ -218,6 +228,15 @@ class RabitqSpace:
         if c.size != self.dim:
             raise ValueError("center has %d values, expected %d" % (c.size, self.dim))
         check(_lib.lib().qg_space_set_center(self.handle, c.ctypes.data_as(C.c_void_p)))
+
+    def set_centroids(self, centroids):
+        """rabitq_space_set_centroids (rabitq_bridge.cpp:89-99): K x dim
+        centroids; records then carry their nearest centroid's id and queries
+        one context per centroid."""
+        c = np.ascontiguousarray(np.asarray(centroids, np.float32).reshape(-1))
+        if c.size != self.dim * self.centroid_count:
+            raise ValueError("centroids hold %d values, expected %d x %d" % (c.size, self.centroid_count, self.dim))
+        check(_lib.lib().qg_space_set_centroids(self.handle, c.ctypes.data_as(C.c_void_p), self.centroid_count))
 
     def set_qr_rotation(self, seed=None, gaussian=None, bf16_parts=3):
         """Precision mode: rotate with Q of the FP64 QR factorisation (on the
@@ -340,8 +359,9 @@ class RabitqSpace:
         out_est = torch.empty((n, 24), dtype=torch.uint8, device="cuda") if "est" in want else None
         out_iv = torch.empty((n, 12), dtype=torch.uint8, device="cuda") if "iv" in want else None
         out_dot = torch.empty(ids.shape, dtype=torch.int64, device="cuda") if "dot" in want else None
-        di, dq = index.desc(), queries.desc()
-        check(_lib.lib().qg_scan(self.handle, kernel, 1 if fixed else 0, C.byref(di), C.byref(dq),
+        di = index.desc()
+        dq = C.byref(queries.desc()) if queries is not None else None  # symmetric pairs take no query
+        check(_lib.lib().qg_scan(self.handle, kernel, 1 if fixed else 0, C.byref(di), dq,
                                  _ptr(ids), _ptr(qidx), per_query, n_pairs, C.c_float(eps0),
                                  _ptr(short_ips), _ptr(long_in), _ptr(out_f), _ptr(out_f2),
                                  _ptr(out_est), _ptr(out_iv), _ptr(out_dot), _stream()))
@@ -354,6 +374,20 @@ class RabitqSpace:
             raise ValueError("ids must be [nq, per_query]")
         if long_in is not None:
             long_in = _dev(long_in, torch.float32)
+        if self.centroid_count > 1:
+            # each record is scored with its centroid's context (queryForEncoded,
+            # space_rabitq.h:3404-3416); the 1-bit sidecar gates use centroid 0's
+            nq = queries.nq
+            if kernel == SCAN_RESIDUAL:  # the reference scores each record alone here
+                kernel = SCAN_RESIDUAL_RECORD
+            if kernel in (SCAN_GATE_FP32, SCAN_GATE_INT8):
+                qidx = torch.arange(nq, dtype=torch.int32, device="cuda")[:, None].expand_as(ids)
+            else:
+                qidx = index.cid[ids.long()].int() * nq + torch.arange(nq, dtype=torch.int32, device="cuda")[:, None]
+            out = self._scan(kernel, False, index, queries, ids.reshape(-1), qidx=qidx.contiguous().reshape(-1),
+                             n_pairs=ids.numel(), eps0=eps0,
+                             long_in=None if long_in is None else long_in.reshape(-1), want=want)
+            return tuple(None if t is None else (t.reshape(ids.shape) if t.dim() == 1 else t) for t in out)
         return self._scan(kernel, True, index, queries, ids, per_query=ids.shape[1], eps0=eps0,
                           long_in=long_in, want=want)
 

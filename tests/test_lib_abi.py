@@ -36,7 +36,7 @@ def test_errors_surface_without_a_device():
     import quantization_graph_amd as qg
     from quantization_graph_amd import _lib
     L = _lib.lib()
-    assert L.qg_space_new(960, 100, 2, 4, 16, True, True) is None
+    assert L.qg_space_new(960, 100, 257, 4, 16, True, True) is None
     assert b"centroid_count" in L.qg_last_error()
     assert L.qg_space_new(960, 100, 1, 8, 16, True, True) is None
     assert qg.code_dim_for(960) == 1024 and qg.code_dim_for(1536) == 2048 and qg.code_dim_for(100) == 128
