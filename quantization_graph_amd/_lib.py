@@ -80,6 +80,8 @@ def lib():
     L.qg_search_noalloc_scratch_bytes.argtypes = []
     L.qg_search_noalloc.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, vp,
                                     f32, u32, u32, u32, vp, vp, vp, vp, vp, vp]
+    L.qg_search_learned.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, vp,
+                                    f32, u32, u32, u32, vp, f32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
     L.qg_search_lds_bytes.restype = sz
     L.qg_search_lds_bytes.argtypes = [u32, u32]
     L.qg_kmeans_workspace_bytes.restype = sz

@@ -555,11 +555,20 @@ size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots) {
     return (size_t)slots * ((n + 32) / 32) * 4 + 256;
 }
 
-int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
-                 const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
-                 const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots, uint32_t waves,
-                 uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists,
-                 void *workspace, size_t workspace_bytes, void *stream) {
+struct LearnedDepth {
+    const float *mlp;
+    float kmult;
+    uint32_t min_depth;
+    const float *rstats, *rfac;
+    uint32_t *out_depth;
+};
+
+static int search_common(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                         const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
+                         uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots,
+                         uint32_t waves, uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids,
+                         float *pool_dists, void *workspace, size_t workspace_bytes, const LearnedDepth *learned,
+                         void *stream) {
     const Space *s = static_cast<const Space *>(space);
     const bool pool_only = out_ids == nullptr && out_dists == nullptr;
     if (!s || !ix || !qv || !degrees || !edges || !workspace || (!pool_only && (!out_ids || !out_dists)))
@@ -606,7 +615,47 @@ int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc
     a.pool_d = pool_dists;
     a.pool_only = pool_only ? 1 : 0;
     a.lq = l_per_query;
+    a.mlp = nullptr;
+    a.kmult = 0.0f;
+    a.min_depth = 0;
+    a.rstats = nullptr;
+    a.rfac = nullptr;
+    a.out_depth = nullptr;
+    if (learned) {
+        if (pool_only) return fail("the learned rerank depth needs the rerank outputs");
+        if ((rerank > k ? rerank : k) > 112) return fail("the learned rerank depth serves at most 112 candidates");
+        if (!learned->mlp || !learned->rstats || !learned->rfac) return fail("null learned-depth argument");
+        a.mlp = learned->mlp;
+        a.kmult = learned->kmult;
+        a.min_depth = learned->min_depth;
+        a.rstats = reinterpret_cast<const float4 *>(learned->rstats);
+        a.rfac = reinterpret_cast<const float2 *>(learned->rfac);
+        a.out_depth = learned->out_depth;
+    }
     return check_hip(qg_launch_search(&a, slots, waves, (hipStream_t)stream), "search launch");
+}
+
+int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                 const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,
+                 const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots, uint32_t waves,
+                 uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *pool_ids, float *pool_dists,
+                 void *workspace, size_t workspace_bytes, void *stream) {
+    return search_common(space, ix, qv, codec, degrees, edges, edge_stride, k, l_search, l_per_query, eps0, rerank,
+                         slots, waves, out_ids, out_dists, out_stats, pool_ids, pool_dists, workspace, workspace_bytes,
+                         nullptr, stream);
+}
+
+int qg_search_learned(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
+                      const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
+                      uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots,
+                      uint32_t waves, const float *mlp_params, float kmult, uint32_t min_depth, const float *rstats,
+                      uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *out_depth,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+    if (!ix) return fail("null argument to qg_search_learned");
+    const LearnedDepth ld{mlp_params, kmult, min_depth, rstats, ix->rfac, out_depth};
+    return search_common(space, ix, qv, codec, degrees, edges, edge_stride, k, l_search, l_per_query, eps0, rerank,
+                         slots, waves, out_ids, out_dists, out_stats, nullptr, nullptr, workspace, workspace_bytes, &ld,
+                         stream);
 }
 
 int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,

@@ -156,6 +156,13 @@ struct SearchArgs {
     int pool_only;             // skip the rerank (out_ids / out_d unused)
     float *pool_d;             // [nq][L]
     const uint32_t *lq;        // [nq] per-query search list sizes (optional; L is then the pool capacity)
+    // learned rerank depth fused after the traversal (optional: mlp null)
+    const float *mlp;          // [5825] MLP parameters
+    float kmult;               // error multiplier of the depth rule
+    uint32_t min_depth;
+    const float4 *rstats;      // [n] residual block stats
+    const float2 *rfac;        // [n] residual factors
+    uint32_t *out_depth;       // [nq] rerank depth taken (optional)
 };
 
 // Vamana build (qg_build.hip)

@@ -20,31 +20,11 @@
 // Adam on FP32 master weights.
 #include "qg_args.h"
 #include "qg_device.h"
+#include "qg_learned.h"
 
 namespace qg {
 
-constexpr int kF = 24, kFP = 32, kH = 64;
-constexpr int kOffW1 = 0, kOffB1 = kH * kF, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH,
-              kOffB3 = kOffW3 + kH, kParams = kOffB3 + 1;  // 5825
-constexpr int kTilesPerWave = 8;                           // training: 128 samples per wave
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef short short8_t __attribute__((ext_vector_type(8)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even, finite inputs
-    uint32_t b = __float_as_uint(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-__device__ __forceinline__ bf16x8_t pack8(const uint16_t *v) {
-    short8_t s;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) s[j] = (short)v[j];
-    return __builtin_bit_cast(bf16x8_t, s);
-}
+constexpr int kTilesPerWave = 8;  // training: 128 samples per wave
 
 // Forward of one 16-sample tile held by a wavefront. Lane l = (g = l/16,
 // c = l%16) ends with Z1 / Z2 for rows 4g + r, columns 16nt + c; H1 (bf16)
@@ -349,11 +329,6 @@ __global__ void k_residual_stats(const uint16_t *rscale, uint64_t n, uint32_t nb
     out[i] = make_float4((float)sqrt(s2 / nb), mx, (float)zero / (float)nb, 0.0f);
 }
 
-__device__ __forceinline__ float clampf(float v) {
-    if (!__builtin_isfinite(v)) return 0.0f;
-    return v < -1e4f ? -1e4f : (v > 1e4f ? 1e4f : v);
-}
-
 // features [nq][depth][32] (bf16) of the first min(len, depth) pool entries
 // (one thread per pool entry; len = the first empty slot within the depth)
 __global__ void __launch_bounds__(128) k_rerank_features(RerankArgs a) {
@@ -375,50 +350,9 @@ __global__ void __launch_bounds__(128) k_rerank_features(RerankArgs a) {
         for (int k = 0; k < kFP / 8; ++k) out[k] = make_uint4(0, 0, 0, 0);
         return;
     }
-    const QScalars s = a.qv.qs[qi];
-    const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
-    const float qn = s.query_norm;
     const uint32_t id = pid[i];
-    const float e = pd[i];
-    const float e0 = pd[0], e9 = pd[len > 9 ? 9 : len - 1];
-    const float enext = i + 1 < len ? pd[i + 1] : e, eprev = i > 0 ? pd[i - 1] : e;
-    const float elo = pd[i >= 5 ? i - 5 : 0], ehi = pd[i + 5 < len ? i + 5 : len - 1];
-    const float2 h = a.ix.hdr[id];
-    const float2 sf = a.ix.sfac[id];
-    const float2 rf = a.rfac[id];
-    const float4 rs = a.rstats[id];
-    const float sqcd = sqrtf((float)a.cd);
-    const float nbk = (float)(a.cd / 16);
-    const float bound = sf.y * qn;
-    float f[kF];
-    f[0] = e / S;
-    f[1] = (e - e0) / S;
-    f[2] = (e - e9) / S;
-    f[3] = (float)i / 100.0f;
-    f[4] = logf(1.0f + (float)i) / logf(101.0f);
-    f[5] = (enext - e) / S;
-    f[6] = (e - eprev) / S;
-    f[7] = bound / S;
-    f[8] = h.x / S;
-    f[9] = sqrtf(h.x > 0.0f ? h.x : 0.0f) * qn / S;
-    f[10] = h.y * qn / S;
-    f[11] = sf.x * qn / S;
-    f[12] = e0 / S;
-    f[13] = e9 / S;
-    f[14] = rf.x / S;
-    f[15] = rf.y / S;
-    f[16] = rs.x * qn * sqcd / S;
-    f[17] = rs.y * qn * sqcd / S;
-    f[18] = rs.z;
-    f[19] = 4.0f * qn * rs.x * sqrtf(nbk) / S;
-    f[20] = (float)i / (float)len;
-    f[21] = (ehi - elo) / S;
-    f[22] = e / (e9 > 0.0f ? e9 : 1.0f);
-    f[23] = (e - e9) / (bound + 1e-12f);
     uint32_t w[kFP / 2];
-#pragma unroll
-    for (int k = 0; k < kFP / 2; ++k)
-        w[k] = 2 * k < kF ? (uint32_t)f2bf(clampf(f[2 * k])) | ((uint32_t)f2bf(clampf(f[2 * k + 1])) << 16) : 0u;
+    learned_feature_row(i, len, pd, a.qv.qs[qi], a.ix.hdr[id], a.ix.sfac[id], a.rfac[id], a.rstats[id], a.cd, w);
 #pragma unroll
     for (int k = 0; k < kFP / 8; ++k) out[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
@@ -469,24 +403,9 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
         const uint32_t want = a.depth > a.k ? a.depth : a.k;
         const uint32_t rc = len < want ? len : want;
         uint32_t depth = rc;
-        if (a.preds && rc > 0) {
-            const float S = s.query_norm_sqr > 0.0f ? s.query_norm_sqr : 1.0f;
-            const uint32_t top = rc < a.min_depth ? rc : a.min_depth;
-            float ub = -__builtin_huge_valf();
-            for (uint32_t j = lane; j < top; j += 64) {
-                const float pr = a.preds[qi * a.depth + j];
-                ub = fmaxf(ub, pd[j] + a.kmult * (pr > 0.0f ? pr : 0.0f) * S);
-            }
-            for (int off = 32; off > 0; off >>= 1) ub = fmaxf(ub, __shfl_xor(ub, off));
-            uint32_t last = 0;
-            for (uint32_t j = lane; j < rc; j += 64) {
-                const float pr = a.preds[qi * a.depth + j];
-                const float lb = pd[j] - a.kmult * (pr > 0.0f ? pr : 0.0f) * S;
-                if (lb <= ub) last = j + 1 > last ? j + 1 : last;
-            }
-            for (int off = 32; off > 0; off >>= 1) last = max(last, (uint32_t)__shfl_xor((int)last, off));
-            depth = last < top ? top : last;
-        }
+        if (a.preds && rc > 0)
+            depth = learned_depth(rc, pd, [&](uint32_t j) { return a.preds[qi * a.depth + j]; }, s.query_norm_sqr,
+                                  a.kmult, a.min_depth);
         if (lane == 0) {
             s_depth = depth;
             if (a.out_depth) a.out_depth[qi] = depth;

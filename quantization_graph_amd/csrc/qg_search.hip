@@ -31,6 +31,7 @@
 
 #include "qg_args.h"
 #include "qg_device.h"
+#include "qg_learned.h"
 #include "qg_space.h"
 
 namespace qg {
@@ -73,8 +74,57 @@ __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's pieces are in LDS
 }
 
-// 4 waves per SIMD (<= 128 VGPRs): 1024 workgroups fit the chip at once.
+// The learned depth after a query's traversal (the LEARNED search kernel):
+// the feature rows of the rc pool entries, the MLP's predicted errors
+// (16-row tiles across the waves) and the depth rule, all in the stage,
+// which is idle after the traversal: features 112 x 64 B, each wave's hidden
+// layer 2 KB, 112 predictions. Returns the depth (at most rc).
 template <int W>
+__device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, uint32_t qi, uint32_t rc,
+                                                                   const float *pd, const uint32_t *pid,
+                                                                   const QScalars &s, uint4 *stage, uint32_t *s_out) {
+    constexpr uint32_t T = 64u * W;
+    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    uint16_t *feat = (uint16_t *)stage;
+    uint16_t *sh1 = (uint16_t *)((unsigned char *)stage + 7168) + wv * 16 * kH;
+    float *pred = (float *)((unsigned char *)stage + 7168 + 4 * 2048);
+    for (uint32_t i = t; i < rc; i += T) {
+        const uint32_t id = pid[i];
+        uint32_t w[kFP / 2];
+        learned_feature_row(i, rc, pd, s, a.ix.hdr[id], a.ix.sfac[id], a.rfac[id], a.rstats[id], a.cd, w);
+        uint4 *o = (uint4 *)(feat + i * kFP);
+#pragma unroll
+        for (int k = 0; k < kFP / 8; ++k) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    }
+    __syncthreads();
+    const uint32_t tiles = (rc + 15) / 16;
+    for (uint32_t tl = wv; tl < tiles; tl += W) {
+        float y[4];
+        const uint32_t rows = rc - 16 * tl < 16 ? rc - 16 * tl : 16;
+        mlp_forward_tile(feat + 16 * tl * kFP, rows, a.mlp, sh1, y);
+        if ((lane & 15u) == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t row = 16 * tl + 4 * (lane >> 4) + (uint32_t)r;
+                if (row < rc) pred[row] = y[r];
+            }
+        }
+    }
+    __syncthreads();
+    if (wv == 0) {
+        const uint32_t d = learned_depth(rc, pd, [&](uint32_t j) { return pred[j]; }, s.query_norm_sqr, a.kmult,
+                                         a.min_depth);
+        if (lane == 0) {
+            *s_out = d;
+            if (a.out_depth) a.out_depth[qi] = d;
+        }
+    }
+    __syncthreads();
+    return *s_out;
+}
+
+// 4 waves per SIMD (<= 128 VGPRs): 1024 workgroups fit the chip at once.
+template <int W, bool LEARNED>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_search(SearchArgs a) {
     constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -511,7 +561,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         if (!a.pool_only) {
             // rerank: the first rc pool entries; their estimates replace pd[0..rc)
             const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
-            const uint32_t rc = len < want ? len : want;
+            uint32_t rc = len < want ? len : want;
+            if (LEARNED && rc > 0) rc = search_learned_depth<W>(a, qi, rc, pd, pid, s, (uint4 *)stage, &s_len);
             if (a.codec == 3) {
                 // FP32 long distance (recomputed for the INT8 codec): one
                 // candidate per 16-lane group over its code row staged in LDS
@@ -612,16 +663,16 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 
 }  // namespace qg
 
-template <int W>
+template <int W, bool LEARNED>
 static int launch_search_w(const qg::SearchArgs *a, uint32_t slots, size_t smem, hipStream_t stream) {
     static size_t raised = 0;  // dynamic LDS above the 64 KB default needs the attribute once per size
     if (smem > raised) {
-        const hipError_t e = hipFuncSetAttribute((const void *)qg::k_search<W>,
+        const hipError_t e = hipFuncSetAttribute((const void *)qg::k_search<W, LEARNED>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return (int)e;
         raised = smem;
     }
-    hipLaunchKernelGGL(qg::k_search<W>, dim3(slots), dim3(64 * W), smem, stream, *a);
+    hipLaunchKernelGGL((qg::k_search<W, LEARNED>), dim3(slots), dim3(64 * W), smem, stream, *a);
     return (int)hipGetLastError();
 }
 
@@ -631,7 +682,13 @@ extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_
     if (a->qv.nq == 0) return 0;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
     const size_t smem = qg_search_lds_bytes(a->cd, a->L);
-    return waves == 2 ? launch_search_w<2>(a, slots, smem, stream) : launch_search_w<4>(a, slots, smem, stream);
+    // the learned-depth variant is a kernel of its own, so the plain search
+    // keeps its register allocation
+    if (a->mlp)
+        return waves == 2 ? launch_search_w<2, true>(a, slots, smem, stream)
+                          : launch_search_w<4, true>(a, slots, smem, stream);
+    return waves == 2 ? launch_search_w<2, false>(a, slots, smem, stream)
+                      : launch_search_w<4, false>(a, slots, smem, stream);
 }
 
 // ------------------------------------------ search without a workspace

@@ -242,11 +242,46 @@ class LearnedRerank:
         return self.mlp.forward(f)
 
     def search(self, sp, index, qv, degrees, edges, l_search, rstats, k=10, workspace=None, check_graph=True):
+        """Traversal, pool write-out, features, MLP and learned rerank as
+        four launches (the reference form the fused search is checked against)."""
         pid, pd, st = search_pool(sp, index, qv, degrees, edges, l_search, workspace=workspace,
                                   check_graph=check_graph)
         preds = self.predict(sp, index, qv, pid, pd, rstats)
         ids, dists, depths = rerank(sp, index, qv, pid, pd, preds, self.kmult, self.depth, self.min_depth, k)
         return ids, dists, depths, st
+
+    def search_fused(self, sp, index, qv, degrees, edges, l_search, rstats, k=10, eps0=1.9, slots=None,
+                     workspace=None, check_graph=True, events=None):
+        """One launch (qg_search_learned): each query's workgroup computes
+        its pool's features, the MLP's errors and the depth right after its
+        traversal and reranks that many; equal to search() bit for bit.
+        Returns (ids [nq, k], estimates, depths [nq], stats [nq, 6])."""
+        degrees = _dev(degrees, torch.int32).reshape(-1)
+        edges = _dev(edges, torch.int32)
+        nq = qv.nq
+        slots = slots or max(1, min(nq, 4096))
+        ids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        dists = torch.empty((nq, k), dtype=torch.float32, device="cuda")
+        depths = torch.empty((nq,), dtype=torch.int32, device="cuda")
+        stats = torch.zeros((nq, 6), dtype=torch.int64, device="cuda")
+        Lb = _lib.lib()
+        ws_bytes = Lb.qg_search_workspace_bytes(index.n, slots)
+        ws = workspace if workspace is not None else torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        if ws.numel() < ws_bytes:
+            raise ValueError("search workspace too small: need %d bytes" % ws_bytes)
+        di, dq = index.desc(), qv.desc()
+        if events is not None:
+            events[0].record(torch.cuda.current_stream())
+        check(Lb.qg_search_learned(sp.handle, C.byref(di), C.byref(dq), qv.codec, _ptr(degrees), _ptr(edges),
+                                   edges.shape[1], k, l_search, None, C.c_float(eps0), self.depth, slots, 4,
+                                   _ptr(self.mlp.params), C.c_float(self.kmult), self.min_depth, _ptr(rstats),
+                                   _ptr(ids), _ptr(dists), _ptr(stats), _ptr(depths), _ptr(ws), ws.numel(),
+                                   _stream()))
+        if events is not None:
+            events[1].record(torch.cuda.current_stream())
+        if check_graph:
+            S.search_error(ws)
+        return ids, dists, depths, stats
 
 
 CALIBRATION_GRID = (0.25, 0.35, 0.5, 0.6, 0.75, 1.0, 1.25, 1.5, 2.0, 3.0, 4.0, 6.0)

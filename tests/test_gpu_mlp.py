@@ -241,3 +241,28 @@ def test_learned_depth_rule_and_rerank(gpu, parity_case, codec):
     assert (depth2.cpu() == depth.cpu()).all()
     assert (ids.cpu() == ids2.cpu()).all()
     assert (dists.cpu().view(torch.int32) == dists2.cpu().view(torch.int32)).all()
+
+
+@pytest.mark.parametrize("codec", ["full", "int8"])
+@pytest.mark.parametrize("L", [64, 128])
+def test_fused_learned_search_equals_pipeline(gpu, parity_case, codec, L):
+    """qg_search_learned (features, MLP and depth rule inside the search
+    kernel, after each query's traversal) equals the four-launch pipeline
+    (traversal with pool write-out, qg_rerank_features, qg_mlp_forward,
+    qg_learned_rerank) bit for bit: ids, estimates, depths and counters."""
+    import torch
+    from quantization_graph_amd import learned
+    c = parity_case
+    sp, ix = c["sp"], c["ix"]
+    print("case: fused learned search, D %d, %s, L %d" % (c["dim"], codec, L), flush=True)
+    qv = sp.prepare(c["q"][:300], codec)
+    for seed, km in ((0, 0.5), (3, 2.0)):
+        lr = learned.LearnedRerank(learned.MLP(_params(seed)), km)
+        ids, d, depth, st = lr.search(sp, ix, qv, c["deg"], c["edges"], L, c["rstats"])
+        ids2, d2, depth2, st2 = lr.search_fused(sp, ix, qv, c["deg"], c["edges"], L, c["rstats"])
+        assert torch.equal(depth.cpu(), depth2.cpu().to(depth.dtype))
+        assert torch.equal(ids.cpu(), ids2.cpu())
+        assert torch.equal(d.cpu().view(torch.int32), d2.cpu().view(torch.int32))
+        assert torch.equal(st.cpu(), st2.cpu())
+        dd = depth2.cpu().numpy()
+        assert dd.min() >= 10 and dd.max() <= 100
