@@ -436,37 +436,68 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
     model = learned.LearnedRerank(mlp, kmult)
 
     ws = sp.search_workspace(index.n, max(1, min(nq, 4096)))
+    slots = max(1, min(nq, 4096))
 
-    def step(ev=None):
-        qv = sp.prepare(q, "full")
-        if ev is not None:
-            ev[0].record(stream)
-        out = model.search(sp, index, qv, deg, edges, L, rstats, workspace=ws, check_graph=False)
-        if ev is not None:
-            ev[1].record(stream)
-        return out
+    def timed(kind):
+        """ms per step (prepare + launch) and the mean search-kernel time of
+        one form: 'learned' (qg_search_learned, one launch), 'plain' (the
+        fused top-100 rerank), 'traversal' (pools only, no rerank) or
+        'pipeline' (traversal, features, MLP, learned rerank: four launches)."""
+        def step(ev=None):
+            qv = sp.prepare(q, "full")
+            if kind == "learned":
+                return model.search_fused(sp, index, qv, deg, edges, L, rstats, slots=slots, workspace=ws,
+                                          check_graph=False, events=ev)
+            if ev is not None:
+                ev[0].record(stream)
+            if kind == "plain":
+                out = sp.search(index, qv, deg, edges, k=10, l_search=L, slots=slots, workspace=ws,
+                                check_graph=False)
+                out = (out[0], out[1], None, out[2])
+            elif kind == "traversal":
+                out = learned.search_pool(sp, index, qv, deg, edges, L, slots=slots, workspace=ws, check_graph=False)
+                out = (None, None, None, out[2])
+            else:
+                out = model.search(sp, index, qv, deg, edges, L, rstats, workspace=ws, check_graph=False)
+            if ev is not None:
+                ev[1].record(stream)
+            return out
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ids, _, depths, _ = step(evs[i])
-    torch.cuda.synchronize()
-    ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            out = step(evs[i])
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+        return ms, float(np.mean([a.elapsed_time(b) for a, b in evs])), out
+
+    ms_t, k_t, _ = timed("traversal")
+    ms_p, k_p, (pids, _, _, _) = timed("plain")
+    ms_u, k_u, _ = timed("pipeline")
+    ms, k_l, (ids, _, depths, _) = timed("learned")
     from quantization_graph_amd.space import search_error
     search_error(ws)
     rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
-    res = {"qps": nq / (ms / 1000.0), "ms_per_step": ms,
-           "gpu_ms": float(np.mean([a.elapsed_time(b) for a, b in evs])), "recall_at_10": rec,
+    rec_p = (pids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
+    stage_plain, stage_learned = k_p - k_t, k_l - k_t
+    res = {"qps": nq / (ms / 1000.0), "ms_per_step": ms, "gpu_ms": k_l, "recall_at_10": rec,
            "mean_depth": float(depths.float().mean().item()), "kmult": kmult, "calibration": table,
            "training_pairs": int(args.train_queries * min(100, L)), "train_epochs": args.train_epochs,
            "loss_history": hist, "pairs_seconds": pairs_s, "train_seconds": train_s,
-           "plain_qps": head["qps"], "plain_recall_at_10": head["recall_at_10"],
-           "speedup_vs_plain": (nq / (ms / 1000.0)) / head["qps"]}
-    log("learned: %.3f ms/step  %.0f q/s  recall@10 %.4f  mean depth %.1f  (plain: %.0f q/s, recall %.4f)"
-        % (ms, res["qps"], rec, res["mean_depth"], head["qps"], head["recall_at_10"]))
+           "plain_qps": nq / (ms_p / 1000.0), "plain_ms_per_step": ms_p, "plain_gpu_ms": k_p,
+           "plain_recall_at_10": rec_p, "speedup_vs_plain": ms_p / ms,
+           "traversal_gpu_ms": k_t, "rerank_stage_ms": {"plain_top100": stage_plain, "learned": stage_learned},
+           "rerank_stage_speedup": stage_plain / stage_learned if stage_learned > 0 else None,
+           "pipeline_four_launches": {"ms_per_step": ms_u, "gpu_ms": k_u, "qps": nq / (ms_u / 1000.0)},
+           "form": "qg_search_learned: features, MLP and depth rule fused after each query's traversal"}
+    log("learned (fused): %.3f ms/step  %.0f q/s  recall@10 %.4f  mean depth %.1f | plain: %.3f ms/step %.0f q/s "
+        "recall %.4f | rerank stage %.3f -> %.3f ms (traversal %.3f ms) | four-launch pipeline %.3f ms/step"
+        % (ms, res["qps"], rec, res["mean_depth"], ms_p, res["plain_qps"], rec_p, stage_plain, stage_learned,
+           k_t, ms_u))
     return res
 
 
@@ -1114,7 +1145,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch", "signscan"), default="all")
+    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch", "signscan", "learned"), default="all")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=1536)
     ap.add_argument("--nq", type=int, default=1000)
@@ -1169,6 +1200,16 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy}))
         return
 
+    if args.workload == "learned":  # the learned-depth workload alone
+        r = run_learned_workload(args, torch, np, qg)
+        print(json.dumps({"metric": "queries/s, learned rerank depth (fused) vs the plain top-100 rerank",
+                          "value": r["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "learned rerank depth, %d x %d power-law (seed 4)"
+                                                 % (args.n, args.dim)},
+                          "roofline": None, "cpu_baseline": None, "learned": r}))
+        return
     graph = run_graph(args, torch, np, qg) if args.workload in ("all", "graph") else None
     if graph is not None and args.train_queries:
         graph["learned"] = run_learned_workload(args, torch, np, qg)

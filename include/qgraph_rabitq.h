@@ -259,17 +259,20 @@ size_t qg_search_lds_bytes(uint32_t cd, uint32_t L);
  * traversal (no counterpart in the reference; the plain top-100 rerank of
  * qg_search_ex is its fallback and judge): the feature rows of the first
  * min(|pool|, max(rerank, k)) pool entries, the MLP's predicted errors
- * (mlp_params: the qg_mlp_* parameter vector, qg_mlp_param_count() floats)
- * and the depth rule (kmult, min_depth) pick how many of them the
+ * (mlp_packed: qg_mlp_pack of the qg_mlp_* parameter vector, the weights
+ * rounded to bf16 in each lane's order, qg_mlp_packed_bytes() of device
+ * memory) and the depth rule (kmult, min_depth) pick how many of them the
  * residual-4 rerank scores; out_depth [nq] (optional) receives it. The
  * results equal qg_rerank_features + qg_mlp_forward + qg_learned_rerank on
  * the traversal's pools, bit for bit, without the pool write-out or the
  * extra launches. rstats: qg_residual_stats of the index. max(rerank, k)
  * <= 112. */
+size_t qg_mlp_packed_bytes(void);
+int qg_mlp_pack(const float *params, void *packed, void *stream);
 int qg_search_learned(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
                       const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
                       uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots,
-                      uint32_t waves, const float *mlp_params, float kmult, uint32_t min_depth, const float *rstats,
+                      uint32_t waves, const void *mlp_packed, float kmult, uint32_t min_depth, const float *rstats,
                       uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *out_depth,
                       void *workspace, size_t workspace_bytes, void *stream);
 

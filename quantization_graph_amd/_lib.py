@@ -80,6 +80,9 @@ def lib():
     L.qg_search_noalloc_scratch_bytes.argtypes = []
     L.qg_search_noalloc.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, vp,
                                     f32, u32, u32, u32, vp, vp, vp, vp, vp, vp]
+    L.qg_mlp_packed_bytes.restype = sz
+    L.qg_mlp_packed_bytes.argtypes = []
+    L.qg_mlp_pack.argtypes = [vp, vp, vp]
     L.qg_search_learned.argtypes = [vp, C.POINTER(IndexDesc), C.POINTER(QueryDesc), i32, vp, vp, sz, u32, u32, vp,
                                     f32, u32, u32, u32, vp, f32, u32, vp, vp, vp, vp, vp, vp, sz, vp]
     L.qg_search_lds_bytes.restype = sz

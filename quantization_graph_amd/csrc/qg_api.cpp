@@ -26,6 +26,7 @@ int qg_launch_encode(const qg::EncArgs *a, hipStream_t stream);
 int qg_launch_pack(const qg::PackArgs *a, uint64_t n, int unpack, hipStream_t stream);
 int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream);
 int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st);
+int qg_launch_mlp_pack(const float *P, void *packed, hipStream_t st);
 uint32_t qg_mlp_grad_waves(uint64_t n);
 int qg_launch_mlp_grad(const float *P, const uint16_t *X, const float *t, uint64_t n, float tau, float *partial,
                        float *partial_loss, hipStream_t st);
@@ -648,11 +649,11 @@ int qg_search_ex(const void *space, const qg_index_desc *ix, const qg_query_desc
 int qg_search_learned(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
                       const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k,
                       uint32_t l_search, const uint32_t *l_per_query, float eps0, uint32_t rerank, uint32_t slots,
-                      uint32_t waves, const float *mlp_params, float kmult, uint32_t min_depth, const float *rstats,
+                      uint32_t waves, const void *mlp_packed, float kmult, uint32_t min_depth, const float *rstats,
                       uint32_t *out_ids, float *out_dists, uint64_t *out_stats, uint32_t *out_depth,
                       void *workspace, size_t workspace_bytes, void *stream) {
     if (!ix) return fail("null argument to qg_search_learned");
-    const LearnedDepth ld{mlp_params, kmult, min_depth, rstats, ix->rfac, out_depth};
+    const LearnedDepth ld{static_cast<const float *>(mlp_packed), kmult, min_depth, rstats, ix->rfac, out_depth};
     return search_common(space, ix, qv, codec, degrees, edges, edge_stride, k, l_search, l_per_query, eps0, rerank,
                          slots, waves, out_ids, out_dists, out_stats, nullptr, nullptr, workspace, workspace_bytes, &ld,
                          stream);
@@ -672,6 +673,11 @@ size_t qg_mlp_param_count(void) { return 5825; }
 
 size_t qg_mlp_workspace_bytes(uint64_t n) {
     return ((size_t)qg_mlp_grad_waves(n) * (5825 + 1) + 64) * 4;
+}
+
+int qg_mlp_pack(const float *params, void *packed, void *stream) {
+    if (!params || !packed) return fail("null argument to qg_mlp_pack");
+    return check_hip(qg_launch_mlp_pack(params, packed, (hipStream_t)stream), "mlp pack launch");
 }
 
 int qg_mlp_forward(const float *params, const uint16_t *features, uint64_t n, float *out, void *stream) {

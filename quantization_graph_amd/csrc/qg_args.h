@@ -157,7 +157,7 @@ struct SearchArgs {
     float *pool_d;             // [nq][L]
     const uint32_t *lq;        // [nq] per-query search list sizes (optional; L is then the pool capacity)
     // learned rerank depth fused after the traversal (optional: mlp null)
-    const float *mlp;          // [5825] MLP parameters
+    const float *mlp;          // MLP weights packed per lane (qg_mlp_pack)
     float kmult;               // error multiplier of the depth rule
     uint32_t min_depth;
     const float4 *rstats;      // [n] residual block stats

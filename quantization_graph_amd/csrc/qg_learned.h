@@ -52,35 +52,72 @@ __device__ __forceinline__ void learned_feature_row(uint32_t i, uint32_t len, co
     const float sqcd = sqrtf((float)cd);
     const float nbk = (float)(cd / 16);
     const float bound = sf.y * qn;
-    float f[kF];
-    f[0] = e / S;
-    f[1] = (e - e0) / S;
-    f[2] = (e - e9) / S;
-    f[3] = (float)i / 100.0f;
-    f[4] = logf(1.0f + (float)i) / logf(101.0f);
-    f[5] = (enext - e) / S;
-    f[6] = (e - eprev) / S;
-    f[7] = bound / S;
-    f[8] = h.x / S;
-    f[9] = sqrtf(h.x > 0.0f ? h.x : 0.0f) * qn / S;
-    f[10] = h.y * qn / S;
-    f[11] = sf.x * qn / S;
-    f[12] = e0 / S;
-    f[13] = e9 / S;
-    f[14] = rf.x / S;
-    f[15] = rf.y / S;
-    f[16] = rs.x * qn * sqcd / S;
-    f[17] = rs.y * qn * sqcd / S;
-    f[18] = rs.z;
-    f[19] = 4.0f * qn * rs.x * sqrtf(nbk) / S;
-    f[20] = (float)i / (float)len;
-    f[21] = (ehi - elo) / S;
-    f[22] = e / (e9 > 0.0f ? e9 : 1.0f);
-    f[23] = (e - e9) / (bound + 1e-12f);
+    const float f[kF] = {
+        e / S, (e - e0) / S, (e - e9) / S, (float)i / 100.0f, logf(1.0f + (float)i) / logf(101.0f),
+        (enext - e) / S, (e - eprev) / S, bound / S, h.x / S, sqrtf(h.x > 0.0f ? h.x : 0.0f) * qn / S,
+        h.y * qn / S, sf.x * qn / S, e0 / S, e9 / S, rf.x / S, rf.y / S, rs.x * qn * sqcd / S,
+        rs.y * qn * sqcd / S, rs.z, 4.0f * qn * rs.x * sqrtf(nbk) / S, (float)i / (float)len, (ehi - elo) / S,
+        e / (e9 > 0.0f ? e9 : 1.0f), (e - e9) / (bound + 1e-12f)};
 #pragma unroll
     for (int k = 0; k < kFP / 2; ++k)
         w[k] = 2 * k < kF ? (uint32_t)f2bf(feat_clamp(f[2 * k])) | ((uint32_t)f2bf(feat_clamp(f[2 * k + 1])) << 16)
                           : 0u;
+}
+
+
+// The MLP weights a lane needs for its part of every 16-row tile, rounded
+// to bf16: lane l = (g = l / 16, c = l % 16) uses W1 rows 16 nt + c, inputs
+// 8g .. 8g + 7 (B1[nt]) and W2 rows 16 nt + c, inputs 32 ks + 8g .. + 7
+// (B2[nt][ks]); biases b1, b2 and W3 of rows 16 nt + c. The packed form
+// holds them per lane as 12 uint4 (B1[0..3], B2[0..3][0..1]) and 12 floats,
+// so a wavefront fetches its weights with a few wide loads.
+struct MlpLane {
+    bf16x8_t B1[4], B2[4][2];
+    float b1[4], b2[4], w3[4], b3;
+};
+
+constexpr uint32_t kMlpPackedU4 = 12, kMlpPackedF = 13;  // per lane
+constexpr size_t kMlpPackedBytes = 64 * (kMlpPackedU4 * 16 + 16 * 4);  // 16 floats per lane reserved
+
+__device__ __forceinline__ void mlp_lane_from_params(const float *P, int lane, MlpLane &m) {
+    const int g = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const int oc = 16 * nt + c;
+        uint16_t wb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            wb[j] = k < kF ? f2bf(P[kOffW1 + oc * kF + k]) : (uint16_t)0;
+        }
+        m.B1[nt] = pack8(wb);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wb[j] = f2bf(P[kOffW2 + oc * kH + 32 * ks + 8 * g + j]);
+            m.B2[nt][ks] = pack8(wb);
+        }
+        m.b1[nt] = P[kOffB1 + oc];
+        m.b2[nt] = P[kOffB2 + oc];
+        m.w3[nt] = P[kOffW3 + oc];
+    }
+    m.b3 = P[kOffB3];
+}
+
+// packed layout: uint4 [12][64] (lane-minor), then float [16][64]
+__device__ __forceinline__ void mlp_lane_store_packed(const MlpLane &m, int lane, void *packed) {
+    uint4 *u = (uint4 *)packed;
+    float *f = (float *)(u + kMlpPackedU4 * 64);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        u[nt * 64 + lane] = __builtin_bit_cast(uint4, m.B1[nt]);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) u[(4 + 2 * nt + ks) * 64 + lane] = __builtin_bit_cast(uint4, m.B2[nt][ks]);
+        f[nt * 64 + lane] = m.b1[nt];
+        f[(4 + nt) * 64 + lane] = m.b2[nt];
+        f[(8 + nt) * 64 + lane] = m.w3[nt];
+    }
+    f[12 * 64 + lane] = m.b3;
 }
 
 __device__ __forceinline__ void wave_lds_order() {
@@ -90,12 +127,11 @@ __device__ __forceinline__ void wave_lds_order() {
 }
 
 // MLP forward of one 16-row tile on the calling wavefront: rows x16 (16 rows
-// of kFP bf16, LDS or global, 16-byte aligned), weights P (FP32 master
-// copy, rounded to bf16 as loaded), sh1 the wave's own 16 x kH bf16 LDS
-// scratch. Lane l = (g = l / 16, c = l % 16) returns y of rows 4g + r in
-// y[r] (the same in every lane of group g). valid_rows rows are real; the
-// rest read as zero features.
-__device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t valid_rows, const float *P,
+// of kFP bf16, LDS or global, 16-byte aligned; valid_rows of them real, the
+// rest read as zero features), weights m, sh1 the wave's own 16 x kH bf16
+// LDS scratch. Lane l = (g = l / 16, c = l % 16) returns y of rows 4g + r in
+// y[r] (the same in every lane of group g).
+__device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t valid_rows, const MlpLane &m,
                                                  uint16_t *sh1, float *y) {
     const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     uint16_t xa[8];
@@ -108,20 +144,67 @@ __device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t v
     const bf16x8_t A1 = pack8(xa);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        const int oc = 16 * nt + c;
-        uint16_t wb[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = 8 * g + j;
-            wb[j] = k < kF ? f2bf(P[kOffW1 + oc * kF + k]) : (uint16_t)0;
-        }
         f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, pack8(wb), acc, 0, 0, 0);
-        const float b1 = P[kOffB1 + oc];
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, m.B1[nt], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float z = acc[r] + m.b1[nt];
+            sh1[(4 * g + r) * kH + 16 * nt + c] = f2bf(z > 0.0f ? z : 0.0f);
+        }
+    }
+    // sh1 belongs to this wave alone: a wave-level barrier orders the writes
+    // above before the reads below (and those reads before the next tile's writes)
+    wave_lds_order();
+    bf16x8_t A2[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) A2[ks] = pack8(sh1 + c * kH + 32 * ks + 8 * g);
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[ks], m.B2[nt][ks], acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float z = acc[r] + m.b2[nt];
+            part[r] = fmaf(z > 0.0f ? z : 0.0f, m.w3[nt], part[r]);
+        }
+    }
+    wave_lds_order();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float v = part[r];
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
+        y[r] = v + m.b3;
+    }
+}
+
+// mlp_forward_tile with the weights read from the packed per-lane layout
+// as each layer needs them (wide loads, few registers held): the same
+// operands in the same order, so the same y.
+__device__ __forceinline__ void mlp_forward_tile_packed(const uint16_t *x16, uint32_t valid_rows, const void *packed,
+                                                        uint16_t *sh1, float *y, int lane) {
+    const int g = lane >> 4, c = lane & 15;
+    const uint4 *u = (const uint4 *)packed;
+    const float *f = (const float *)(u + kMlpPackedU4 * 64);
+    uint16_t xa[8];
+    {
+        const uint4 v = (uint32_t)c < valid_rows ? *(const uint4 *)(x16 + c * kFP + 8 * g) : make_uint4(0, 0, 0, 0);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { xa[2 * j] = (uint16_t)(w[j] & 0xFFFF); xa[2 * j + 1] = (uint16_t)(w[j] >> 16); }
+    }
+    const bf16x8_t A1 = pack8(xa);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, __builtin_bit_cast(bf16x8_t, u[nt * 64 + lane]), acc, 0, 0,
+                                                      0);
+        const float b1 = f[nt * 64 + lane];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const float z = acc[r] + b1;
-            sh1[(4 * g + r) * kH + oc] = f2bf(z > 0.0f ? z : 0.0f);
+            sh1[(4 * g + r) * kH + 16 * nt + c] = f2bf(z > 0.0f ? z : 0.0f);
         }
     }
     wave_lds_order();
@@ -131,16 +214,12 @@ __device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t v
     float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-        const int oc = 16 * nt + c;
         f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            uint16_t wb[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) wb[j] = f2bf(P[kOffW2 + oc * kH + 32 * ks + 8 * g + j]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[ks], pack8(wb), acc, 0, 0, 0);
-        }
-        const float b2 = P[kOffB2 + oc], w3 = P[kOffW3 + oc];
+        for (int ks = 0; ks < 2; ++ks)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                A2[ks], __builtin_bit_cast(bf16x8_t, u[(4 + 2 * nt + ks) * 64 + lane]), acc, 0, 0, 0);
+        const float b2 = f[(4 + nt) * 64 + lane], w3 = f[(8 + nt) * 64 + lane];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const float z = acc[r] + b2;
@@ -148,11 +227,12 @@ __device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t v
         }
     }
     wave_lds_order();
+    const float b3 = f[12 * 64 + lane];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         float v = part[r];
         for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
-        y[r] = v + P[kOffB3];
+        y[r] = v + b3;
     }
 }
 
@@ -163,8 +243,7 @@ __device__ __forceinline__ void mlp_forward_tile(const uint16_t *x16, uint32_t v
 // min(rc, min_depth).
 template <typename PredAt>
 __device__ __forceinline__ uint32_t learned_depth(uint32_t rc, const float *pd, PredAt pr_at, float qn2, float kmult,
-                                                  uint32_t min_depth) {
-    const int lane = threadIdx.x & 63;
+                                                  uint32_t min_depth, uint32_t lane) {
     if (rc == 0) return 0;
     const float S = qn2 > 0.0f ? qn2 : 1.0f;
     const uint32_t top = rc < min_depth ? rc : min_depth;

@@ -105,87 +105,31 @@ __device__ __forceinline__ void tile_forward(const uint16_t *xrow, bool xvalid, 
 constexpr int kFwdTiles = 8;
 
 __global__ void __launch_bounds__(256) k_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y) {
-    __shared__ uint16_t sh1_all[4][16 * kH];
+    __shared__ __align__(16) uint16_t sh1_all[4][16 * kH];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     uint16_t *sh1 = sh1_all[wv];
-    bf16x8_t B1[4], B2[4][2];
-    float b1[4], b2[4], w3[4];
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-        const int oc = 16 * nt + c;
-        uint16_t wb[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = 8 * g + j;
-            wb[j] = k < kF ? f2bf(P[kOffW1 + oc * kF + k]) : (uint16_t)0;
-        }
-        B1[nt] = pack8(wb);
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) wb[j] = f2bf(P[kOffW2 + oc * kH + 32 * ks + 8 * g + j]);
-            B2[nt][ks] = pack8(wb);
-        }
-        b1[nt] = P[kOffB1 + oc];
-        b2[nt] = P[kOffB2 + oc];
-        w3[nt] = P[kOffW3 + oc];
-    }
-    const float b3 = P[kOffB3];
+    MlpLane m;
+    mlp_lane_from_params(P, lane, m);
     const uint64_t tile0 = ((uint64_t)blockIdx.x * 4 + wv) * kFwdTiles;
     for (int tt = 0; tt < kFwdTiles; ++tt) {
         const uint64_t tile = tile0 + tt;
         if (tile * 16 >= n) break;
-        const uint64_t row = tile * 16 + c;
-        uint16_t xa[8];
-        {
-            const uint4 v = row < n ? *(const uint4 *)(X + row * kFP + 8 * g) : make_uint4(0, 0, 0, 0);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) { xa[2 * j] = (uint16_t)(w[j] & 0xFFFF); xa[2 * j + 1] = (uint16_t)(w[j] >> 16); }
-        }
-        const bf16x8_t A1 = pack8(xa);
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1, B1[nt], acc, 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float z = acc[r] + b1[nt];
-                sh1[(4 * g + r) * kH + 16 * nt + c] = f2bf(z > 0.0f ? z : 0.0f);
-            }
-        }
-        // sh1 belongs to this wave alone: LDS operations of one wavefront
-        // complete in order, so a wave-level barrier orders the writes above
-        // before the reads below (and those reads before the next tile's writes)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        bf16x8_t A2[2];
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) A2[ks] = pack8(sh1 + c * kH + 32 * ks + 8 * g);
-        float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[ks], B2[nt][ks], acc, 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float z = acc[r] + b2[nt];
-                part[r] = fmaf(z > 0.0f ? z : 0.0f, w3[nt], part[r]);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint64_t left = n - tile * 16;
+        float out[4];
+        mlp_forward_tile(X + tile * 16 * kFP, left < 16 ? (uint32_t)left : 16u, m, sh1, out);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float v = part[r];
-            for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off);
             const uint64_t rr = tile * 16 + 4 * g + r;
-            if (c == 0 && rr < n) y[rr] = v + b3;
+            if (c == 0 && rr < n) y[rr] = out[r];
         }
     }
+}
+
+// The packed per-lane weights of the fused learned search (mlp_lane_*).
+__global__ void k_mlp_pack(const float *P, void *packed) {
+    MlpLane m;
+    mlp_lane_from_params(P, (int)threadIdx.x, m);
+    mlp_lane_store_packed(m, (int)threadIdx.x, packed);
 }
 
 // Forward + backward of the pinball loss over a batch; each wave writes its
@@ -405,7 +349,7 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
         uint32_t depth = rc;
         if (a.preds && rc > 0)
             depth = learned_depth(rc, pd, [&](uint32_t j) { return a.preds[qi * a.depth + j]; }, s.query_norm_sqr,
-                                  a.kmult, a.min_depth);
+                                  a.kmult, a.min_depth, lane);
         if (lane == 0) {
             s_depth = depth;
             if (a.out_depth) a.out_depth[qi] = depth;
@@ -485,6 +429,13 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
 }  // namespace qg
 
 extern "C" {
+
+size_t qg_mlp_packed_bytes(void) { return qg::kMlpPackedBytes; }
+
+int qg_launch_mlp_pack(const float *P, void *packed, hipStream_t st) {
+    hipLaunchKernelGGL(qg::k_mlp_pack, dim3(1), dim3(64), 0, st, P, packed);
+    return (int)hipGetLastError();
+}
 
 int qg_launch_mlp_forward(const float *P, const uint16_t *X, uint64_t n, float *y, hipStream_t st) {
     if (n == 0) return 0;

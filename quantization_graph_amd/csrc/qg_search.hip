@@ -84,7 +84,12 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
                                                                    const float *pd, const uint32_t *pid,
                                                                    const QScalars &s, uint4 *stage, uint32_t *s_out) {
     constexpr uint32_t T = 64u * W;
-    const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+    // the thread id passes through an opaque move, so that the per-lane
+    // addresses of this section are computed here and not hoisted out of the
+    // query loop, where they would hold registers through the traversal
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const uint32_t lane = t & 63u, wv = t >> 6;
     uint16_t *feat = (uint16_t *)stage;
     uint16_t *sh1 = (uint16_t *)((unsigned char *)stage + 7168) + wv * 16 * kH;
     float *pred = (float *)((unsigned char *)stage + 7168 + 4 * 2048);
@@ -101,7 +106,7 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
     for (uint32_t tl = wv; tl < tiles; tl += W) {
         float y[4];
         const uint32_t rows = rc - 16 * tl < 16 ? rc - 16 * tl : 16;
-        mlp_forward_tile(feat + 16 * tl * kFP, rows, a.mlp, sh1, y);
+        mlp_forward_tile_packed(feat + 16 * tl * kFP, rows, a.mlp, sh1, y, (int)lane);
         if ((lane & 15u) == 0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -113,7 +118,7 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
     __syncthreads();
     if (wv == 0) {
         const uint32_t d = learned_depth(rc, pd, [&](uint32_t j) { return pred[j]; }, s.query_norm_sqr, a.kmult,
-                                         a.min_depth);
+                                         a.min_depth, lane);
         if (lane == 0) {
             *s_out = d;
             if (a.out_depth) a.out_depth[qi] = d;

@@ -250,6 +250,17 @@ class LearnedRerank:
         ids, dists, depths = rerank(sp, index, qv, pid, pd, preds, self.kmult, self.depth, self.min_depth, k)
         return ids, dists, depths, st
 
+    def packed(self):
+        """The weights rounded to bf16 and laid out per lane (qg_mlp_pack),
+        packed again whenever the parameters change."""
+        key = (self.mlp.params.data_ptr(), self.mlp.params._version)
+        if getattr(self, "_packed_key", None) != key:
+            Lb = _lib.lib()
+            self._packed = torch.empty(Lb.qg_mlp_packed_bytes(), dtype=torch.uint8, device="cuda")
+            check(Lb.qg_mlp_pack(_ptr(self.mlp.params), _ptr(self._packed), _stream()))
+            self._packed_key = key
+        return self._packed
+
     def search_fused(self, sp, index, qv, degrees, edges, l_search, rstats, k=10, eps0=1.9, slots=None,
                      workspace=None, check_graph=True, events=None):
         """One launch (qg_search_learned): each query's workgroup computes
@@ -274,7 +285,7 @@ class LearnedRerank:
             events[0].record(torch.cuda.current_stream())
         check(Lb.qg_search_learned(sp.handle, C.byref(di), C.byref(dq), qv.codec, _ptr(degrees), _ptr(edges),
                                    edges.shape[1], k, l_search, None, C.c_float(eps0), self.depth, slots, 4,
-                                   _ptr(self.mlp.params), C.c_float(self.kmult), self.min_depth, _ptr(rstats),
+                                   _ptr(self.packed()), C.c_float(self.kmult), self.min_depth, _ptr(rstats),
                                    _ptr(ids), _ptr(dists), _ptr(stats), _ptr(depths), _ptr(ws), ws.numel(),
                                    _stream()))
         if events is not None:
