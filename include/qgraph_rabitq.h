@@ -59,14 +59,20 @@ typedef struct RabitqPaperFactors {
     uint8_t valid;
 } RabitqPaperFactors;
 
-/* Structure-of-arrays device index (row i = base vector i). */
+/* Structure-of-arrays device index (row i = base vector i).
+ *
+ * Gate rows hold what the 1-bit gate reads about a record in one 16-byte
+ * aligned row of qg_gate_row_bytes(cd) bytes, so that a gated neighbour
+ * costs the cache lines of its MSB plane and nothing more. With m = cd/8
+ * rounded up to 16: bytes [0, cd/8) the MSB plane (zero up to m), [m, m+16)
+ * the paper-prune factors as four floats {norm_sqr, data_norm, cross_scale,
+ * error_cross_scale}, [m+16, m+24) a copy of the header {norm_sqr,
+ * long_scale}, byte m+24 the sidecar validity, [m+25, m+32) zero. */
 typedef struct qg_index_desc {
     uint8_t *codes;     /* [n][cd/2] primary 4-bit codes (reference nibble layout)  */
     float *hdr;         /* [n][2]    norm_sqr, long_scale                           */
     float *sfac;        /* [n][2]    short_scale, error_scale                       */
-    uint8_t *msb;       /* [n][cd/8] MSB plane                                      */
-    float *factors;     /* [n][4]    norm_sqr, data_norm, cross_scale, err_cross    */
-    uint8_t *fvalid;    /* [n]       sidecar validity                               */
+    uint8_t *gate;      /* [n][qg_gate_row_bytes(cd)] gate rows, below              */
     uint16_t *rscale;   /* [n][cd/16] residual block scales, fp16 bits              */
     uint8_t *rcodes;    /* [n][cd/2] residual 4-bit codes                           */
     float *rfac;        /* [n][2]    residual_norm_sqr, long_residual_ip            */
@@ -131,6 +137,8 @@ void qg_space_delete(void *space);
 int qg_space_set_center(void *space, const float *center_host);
 size_t qg_space_dim(const void *space);
 size_t qg_space_code_dim(const void *space);
+/* Bytes per gate row of a space: cd/8 rounded up to 16, plus 32. */
+size_t qg_gate_row_bytes(const void *space);
 /* Device pointers to the rotation signs [cd] and the rotated center [cd]. */
 const float *qg_space_signs(const void *space);
 const float *qg_space_rotated_center(const void *space);

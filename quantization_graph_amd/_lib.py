@@ -21,8 +21,7 @@ class QGError(RuntimeError):
 class IndexDesc(C.Structure):
     """qg_index_desc (include/qgraph_rabitq.h)."""
     _fields_ = [("codes", C.c_void_p), ("hdr", C.c_void_p), ("sfac", C.c_void_p),
-                ("msb", C.c_void_p), ("factors", C.c_void_p), ("fvalid", C.c_void_p),
-                ("rscale", C.c_void_p), ("rcodes", C.c_void_p), ("rfac", C.c_void_p),
+                ("gate", C.c_void_p), ("rscale", C.c_void_p), ("rcodes", C.c_void_p), ("rfac", C.c_void_p),
                 ("n", C.c_uint64), ("cid", C.c_void_p)]
 
 
@@ -48,6 +47,8 @@ def lib():
     L.qg_space_set_center.argtypes = [vp, vp]
     L.qg_space_code_dim.restype = sz
     L.qg_space_code_dim.argtypes = [vp]
+    L.qg_gate_row_bytes.restype = sz
+    L.qg_gate_row_bytes.argtypes = [vp]
     L.qg_space_signs.restype = vp
     L.qg_space_signs.argtypes = [vp]
     L.qg_space_host_constants.argtypes = [vp, vp, vp]

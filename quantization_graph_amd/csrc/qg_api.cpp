@@ -199,9 +199,7 @@ IndexView view_of(const qg_index_desc *d) {
     v.codes = d->codes;
     v.hdr = reinterpret_cast<const float2 *>(d->hdr);
     v.sfac = reinterpret_cast<const float2 *>(d->sfac);
-    v.msb = d->msb;
-    v.factors = reinterpret_cast<const float4 *>(d->factors);
-    v.fvalid = d->fvalid;
+    v.gate = d->gate;
     v.rscale = d->rscale;
     v.rcodes = d->rcodes;
     v.n = d->n;
@@ -227,7 +225,7 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
            hipStream_t stream) {
     if (!s || (!x && n) || !out) return fail("null argument to encode");
     if (n == 0) return 0;
-    if (!out->codes || !out->hdr || !out->sfac || !out->msb || !out->factors || !out->fvalid || !out->rscale ||
+    if (!out->codes || !out->hdr || !out->sfac || !out->gate || !out->rscale ||
         !out->rcodes || !out->rfac)
         return fail("encode output view has a null array");
     if (s->centroid_count > 1 && !out->cid) return fail("a space of several centroids encodes into a centroid-id array");
@@ -260,9 +258,7 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.codes = out->codes + b0 * (cd / 2);
         a.hdr = reinterpret_cast<float2 *>(out->hdr) + b0;
         a.sfac = reinterpret_cast<float2 *>(out->sfac) + b0;
-        a.msb = out->msb + b0 * (cd / 8);
-        a.factors = reinterpret_cast<float4 *>(out->factors) + b0;
-        a.fvalid = out->fvalid + b0;
+        a.gate = out->gate + b0 * gate_row_bytes(cd);
         a.rscale = out->rscale + b0 * s->block_count;
         a.rcodes = out->rcodes + b0 * (cd / 2);
         a.rfac = reinterpret_cast<float2 *>(out->rfac) + b0;
@@ -368,6 +364,9 @@ int qg_rotate_rows(const void *space, const float *rows, uint64_t n, int subtrac
 
 size_t qg_space_dim(const void *space) { return space ? static_cast<const Space *>(space)->dim : 0; }
 size_t qg_space_code_dim(const void *space) { return space ? static_cast<const Space *>(space)->cd : 0; }
+size_t qg_gate_row_bytes(const void *space) {
+    return space ? (size_t)gate_row_bytes(static_cast<const Space *>(space)->cd) : 0;
+}
 const float *qg_space_signs(const void *space) { return space ? static_cast<const Space *>(space)->d_signs : nullptr; }
 int qg_space_host_constants(const void *space, float *signs_out, float *center_out) {
     const Space *s = static_cast<const Space *>(space);
@@ -431,9 +430,7 @@ int qg_unpack_records(const void *space, const uint8_t *compact, size_t compact_
     a.rscale = out->rscale;
     a.rcodes = out->rcodes;
     a.rfac_out = reinterpret_cast<float2 *>(out->rfac);
-    a.msb = out->msb;
-    a.factors = reinterpret_cast<float4 *>(out->factors);
-    a.fvalid = out->fvalid;
+    a.gate = out->gate;
     out->n = n;
     return check_hip(qg_launch_pack(&a, n, 1, (hipStream_t)stream), "unpack launch");
 }

@@ -99,9 +99,7 @@ struct EncArgs {
     uint8_t *codes;
     float2 *hdr;
     float2 *sfac;
-    uint8_t *msb;
-    float4 *factors;
-    uint8_t *fvalid;
+    uint8_t *gate;             // [n][cd/8 + 32] gate rows
     uint16_t *rscale;
     uint8_t *rcodes;
     float2 *rfac;
@@ -130,7 +128,7 @@ struct PackArgs {
     // unpack direction
     const uint8_t *in_compact; const uint8_t *in_residual;
     uint8_t *codes; float2 *hdr; float2 *sfac; uint16_t *rscale; uint8_t *rcodes; float2 *rfac_out;
-    uint8_t *msb; float4 *factors; uint8_t *fvalid;
+    uint8_t *gate;             // unpack: gate rows
     uint8_t *cid_out;          // unpack: centroid ids (optional)
 };
 

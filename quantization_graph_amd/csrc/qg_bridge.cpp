@@ -81,16 +81,34 @@ bool stage_index(const Space *s, Arena *a, uint64_t n, qg_index_desc *d) {
     d->codes = static_cast<uint8_t *>(a->take(n * cd / 2));
     d->hdr = static_cast<float *>(a->take(n * 8));
     d->sfac = static_cast<float *>(a->take(n * 8));
-    d->msb = static_cast<uint8_t *>(a->take(n * cd / 8));
-    d->factors = static_cast<float *>(a->take(n * 16));
-    d->fvalid = static_cast<uint8_t *>(a->take(n));
+    d->gate = static_cast<uint8_t *>(a->take(n * gate_row_bytes(cd)));
     d->rscale = static_cast<uint16_t *>(a->take(n * 2 * s->block_count));
     d->rcodes = static_cast<uint8_t *>(a->take(n * cd / 2));
     d->rfac = static_cast<float *>(a->take(n * 8));
     d->cid = static_cast<uint8_t *>(a->take(n));
     d->n = n;
-    return d->codes && d->hdr && d->sfac && d->msb && d->factors && d->fvalid && d->rscale && d->rcodes && d->rfac &&
-           d->cid;
+    return d->codes && d->hdr && d->sfac && d->gate && d->rscale && d->rcodes && d->rfac && d->cid;
+}
+
+// The sidecar of `count` device gate rows in the reference's form: MSB rows
+// of cd/8 bytes and PaperPruneFactors records.
+bool download_sidecar(const Space *s, const qg_index_desc &d, size_t count, uint8_t *msb_out, void *factors_out) {
+    const size_t gb = gate_row_bytes(s->cd), mb = s->cd / 8, m = gate_msb_span(s->cd);
+    std::vector<uint8_t> g(count * gb);
+    if (hipMemcpy(g.data(), d.gate, g.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    RabitqPaperFactors *o = static_cast<RabitqPaperFactors *>(factors_out);
+    for (size_t i = 0; i < count; ++i) {
+        const uint8_t *row = g.data() + i * gb;
+        std::memcpy(msb_out + i * mb, row, mb);
+        float f[4];
+        std::memcpy(f, row + m, 16);
+        o[i].norm_sqr = f[0];
+        o[i].data_norm = f[1];
+        o[i].cross_scale = f[2];
+        o[i].error_cross_scale = f[3];
+        o[i].valid = row[m + 24];
+    }
+    return true;
 }
 
 // Thread-local prepared query: device buffers for one query.
@@ -320,21 +338,7 @@ bool rabitq_export_paper_sidecar(const void *space, const void *compact_records_
     if (!arena_for(s, record_count * (s->compact_bytes + 2 * s->cd + 64), &a)) return false;
     qg_index_desc d;
     if (!upload_records(s, &a, comp, nullptr, record_count, &d)) return false;
-    const size_t mb = s->cd / 8;
-    std::vector<float> f(record_count * 4);
-    std::vector<uint8_t> v(record_count);
-    if (hipMemcpy(msb_out, d.msb, record_count * mb, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    if (hipMemcpy(f.data(), d.factors, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    if (hipMemcpy(v.data(), d.fvalid, v.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
-    RabitqPaperFactors *o = static_cast<RabitqPaperFactors *>(factors_out);
-    for (size_t i = 0; i < record_count; ++i) {
-        o[i].norm_sqr = f[4 * i];
-        o[i].data_norm = f[4 * i + 1];
-        o[i].cross_scale = f[4 * i + 2];
-        o[i].error_cross_scale = f[4 * i + 3];
-        o[i].valid = v[i];
-    }
-    return true;
+    return download_sidecar(s, d, record_count, msb_out, factors_out);
 }
 
 // rabitq_build_vamana_graph (rabitq_bridge.cpp:224-303): the records are
@@ -393,19 +397,8 @@ bool rabitq_build_vamana_graph(void *space, const void *compact_records_in, size
         if (hipMemcpy(hdeg.data(), deg, record_count * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
         if (hipMemcpy(hedges.data(), edges, record_count * r * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
     }
-    const size_t mb = s->cd / 8;
-    std::vector<float> f(record_count * 4);
-    std::vector<uint8_t> v(record_count);
-    if (hipMemcpy(paper_msb_out, d.msb, record_count * mb, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    if (hipMemcpy(f.data(), d.factors, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
-    if (hipMemcpy(v.data(), d.fvalid, v.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
-    RabitqPaperFactors *o = static_cast<RabitqPaperFactors *>(paper_factors_out);
+    if (!download_sidecar(s, d, record_count, paper_msb_out, paper_factors_out)) return false;
     for (size_t i = 0; i < record_count; ++i) {
-        o[i].norm_sqr = f[4 * i];
-        o[i].data_norm = f[4 * i + 1];
-        o[i].cross_scale = f[4 * i + 2];
-        o[i].error_cross_scale = f[4 * i + 3];
-        o[i].valid = v[i];
         degrees_out[i] = hdeg[i];
         std::memcpy(edges_out + i * edge_stride, hedges.data() + i * r, hdeg[i] * 4);
     }
@@ -527,36 +520,32 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
     // reads the dithered code, B1 and Int4 have gates of their own (not built
     // here), every other codec takes the FP32 gate.
     if (bq->codec == QG_CODEC_B1 || bq->codec == 2) return fail("query codec %d has no gate here", bq->codec) == 0;
-    const size_t mb = s->cd / 8;
-    std::vector<uint8_t> msb;
-    gather(msb_base, msb_stride, mb, ids, count, &msb);
-    std::vector<float> fac(count * 4);
-    std::vector<uint8_t> valid(count);
+    // the caller's sidecar rows as device gate rows (the header copy is not
+    // read by the gate and stays zero)
+    const size_t mb = s->cd / 8, gb = gate_row_bytes(s->cd), m = gate_msb_span(s->cd);
+    std::vector<uint8_t> rowsg(count * gb, 0);
+    const uint8_t *mbase = static_cast<const uint8_t *>(msb_base);
     const uint8_t *fb = static_cast<const uint8_t *>(factors_base);
     for (size_t i = 0; i < count; ++i) {
+        uint8_t *row = rowsg.data() + i * gb;
+        std::memcpy(row, mbase + static_cast<size_t>(ids[i]) * msb_stride, mb);
         RabitqPaperFactors f;
         std::memcpy(&f, fb + static_cast<size_t>(ids[i]) * factors_stride, 20);
-        fac[4 * i] = f.norm_sqr;
-        fac[4 * i + 1] = f.data_norm;
-        fac[4 * i + 2] = f.cross_scale;
-        fac[4 * i + 3] = f.error_cross_scale;
-        valid[i] = f.valid;
+        const float fv[4] = {f.norm_sqr, f.data_norm, f.cross_scale, f.error_cross_scale};
+        std::memcpy(row + m, fv, 16);
+        row[m + 24] = f.valid;
     }
     Arena a;
-    if (!arena_for(s, count * (mb + 16 + 1 + 64) + 4096, &a)) return false;
+    if (!arena_for(s, count * (gb + 64) + 4096, &a)) return false;
     qg_index_desc d;
     std::memset(&d, 0, sizeof(d));
-    d.msb = static_cast<uint8_t *>(a.take(count * mb));
-    d.factors = static_cast<float *>(a.take(count * 16));
-    d.fvalid = static_cast<uint8_t *>(a.take(count));
+    d.gate = static_cast<uint8_t *>(a.take(count * gb));
     d.n = count;
     uint32_t *rows = upload_iota(&a, count);
     uint32_t *qidx = upload_zeros_u32(&a, count);
     RabitqPaperEstimate *dout = static_cast<RabitqPaperEstimate *>(a.take(count * 24));
-    if (!d.msb || !d.factors || !d.fvalid || !rows || !qidx || !dout) return false;
-    (void)hipMemcpy(d.msb, msb.data(), msb.size(), hipMemcpyHostToDevice);
-    (void)hipMemcpy(d.factors, fac.data(), fac.size() * 4, hipMemcpyHostToDevice);
-    (void)hipMemcpy(d.fvalid, valid.data(), count, hipMemcpyHostToDevice);
+    if (!d.gate || !rows || !qidx || !dout) return false;
+    (void)hipMemcpy(d.gate, rowsg.data(), rowsg.size(), hipMemcpyHostToDevice);
     const qg_query_desc qd = bq->desc();
     const int k = bq->codec == QG_CODEC_INT8 ? QG_SCAN_GATE_INT8 : QG_SCAN_GATE_FP32;
     if (qg_scan(s, k, 0, &d, &qd, rows, qidx, 0, count, epsilon0, nullptr, nullptr, nullptr, nullptr, dout, nullptr,

@@ -22,14 +22,42 @@ struct IndexView {
     const uint8_t *codes;     // [n][cd/2]   primary 4-bit codes
     const float2 *hdr;        // [n]         {norm_sqr, long_scale}
     const float2 *sfac;       // [n]         {short_scale, error_scale}
-    const uint8_t *msb;       // [n][cd/8]   MSB plane (paper-prune sidecar)
-    const float4 *factors;    // [n]         {norm_sqr, data_norm, cross_scale, error_cross_scale}
-    const uint8_t *fvalid;    // [n]         sidecar validity
+    const uint8_t *gate;      // [n][cd/8 + 32] gate rows (below)
     const uint16_t *rscale;   // [n][cd/16]  residual block scales (fp16 bits)
     const uint8_t *rcodes;    // [n][cd/2]   residual 4-bit codes
     uint64_t n;
     const uint8_t *cid;       // [n]         centroid id (null: all 0)
 };
+
+// Gate rows: everything the 1-bit gate reads about one record, in one
+// 16-byte-aligned row, so that a gated neighbour costs the lines of its MSB
+// plane and nothing more. With m = cd/8 rounded up to 16 bytes:
+//   [0, cd/8)         MSB plane (the paper-prune sidecar's bit plane),
+//                     zero up to m
+//   [m, m + 16)       sidecar factors {norm_sqr, data_norm, cross_scale,
+//                     error_cross_scale}
+//   [m + 16, m + 24)  a copy of the header {norm_sqr, long_scale}
+//   m + 24            sidecar validity byte
+//   [m + 25, m + 32)  zero
+__host__ __device__ constexpr uint64_t gate_msb_span(uint64_t cd) { return (cd / 8 + 15) / 16 * 16; }
+__host__ __device__ constexpr uint64_t gate_row_bytes(uint64_t cd) { return gate_msb_span(cd) + 32; }
+__device__ __forceinline__ const uint8_t *gate_row(const uint8_t *gate, uint32_t cd, uint64_t id) {
+    return gate + id * gate_row_bytes(cd);
+}
+__device__ __forceinline__ float4 gate_factors(const uint8_t *row, uint32_t cd) {
+    return *(const float4 *)(row + gate_msb_span(cd));
+}
+__device__ __forceinline__ float2 gate_hdr(const uint8_t *row, uint32_t cd) {
+    return *(const float2 *)(row + gate_msb_span(cd) + 16);
+}
+__device__ __forceinline__ uint8_t gate_valid(const uint8_t *row, uint32_t cd) { return row[gate_msb_span(cd) + 24]; }
+// the 32-byte tail of a row as two 16-byte stores
+__device__ __forceinline__ void gate_store_tail(uint8_t *row, uint32_t cd, float4 f, float2 h, uint8_t valid) {
+    if (cd / 8 < gate_msb_span(cd)) *(uint2 *)(row + cd / 8) = make_uint2(0u, 0u);  // cd = 64
+    uint4 *t = (uint4 *)(row + gate_msb_span(cd));
+    t[0] = make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z), __float_as_uint(f.w));
+    t[1] = make_uint4(__float_as_uint(h.x), __float_as_uint(h.y), (uint32_t)valid, 0u);
+}
 
 // Per-query scalars of the prepared query context (QueryContext,
 // space_rabitq.h:101-132).

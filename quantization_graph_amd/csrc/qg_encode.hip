@@ -410,7 +410,7 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_codes(EncArgs a) {
     }
     __syncthreads();
     uint8_t *code_row = a.codes + vi * (cd / 2);
-    uint8_t *msb_row = a.msb + vi * (cd / 8);
+    uint8_t *msb_row = a.gate + vi * gate_row_bytes(cd);
     for (uint32_t p = t; p < cd / 2; p += blockDim.x)
         code_row[p] = (uint8_t)(nib[2 * p] | (nib[2 * p + 1] << 4));
     for (uint32_t b = t; b < cd / 8; b += blockDim.x) {
@@ -476,8 +476,7 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_codes(EncArgs a) {
                 valid = __builtin_isfinite(f.w) ? 1 : 0;
             }
         }
-        a.factors[vi] = f;
-        a.fvalid[vi] = valid;
+        gate_store_tail(a.gate + vi * gate_row_bytes(cd), cd, f, h, valid);
     }
 }
 
@@ -558,7 +557,7 @@ __global__ void k_unpack(PackArgs a) {
             m |= (uint8_t)(((byte >> 3) & 1u) << (2 * j));
             m |= (uint8_t)(((byte >> 7) & 1u) << (2 * j + 1));
         }
-        a.msb[v * (cd / 8) + b] = m;
+        a.gate[v * gate_row_bytes(cd) + b] = m;
     }
     const uint32_t sb = 2 * a.block_count;
     if (a.in_residual) {
@@ -600,8 +599,7 @@ __global__ void k_unpack(PackArgs a) {
                 valid = __builtin_isfinite(f.w) ? 1 : 0;
             }
         }
-        a.factors[v] = f;
-        a.fvalid[v] = valid;
+        gate_store_tail(a.gate + v * gate_row_bytes(cd), cd, f, make_float2(hv[0], hv[1]), valid);
     }
 }
 

@@ -286,7 +286,8 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_fp32(ScanArgs a) {
     const uint32_t id = a.ids[item];
     const QScalars s = a.qv.qs[qi];
     const float *qg = a.qv.q + qi * a.cd;
-    const uint2 *m = (const uint2 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    const uint8_t *grow = gate_row(a.ix.gate, a.cd, id);
+    const uint2 *m = (const uint2 *)grow;
     float acc[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
@@ -308,7 +309,8 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_fp32(ScanArgs a) {
         }
     }
     const float short_ip = reduce16(acc) - s.half_sum;
-    a.out_est[item] = finish_gate(short_ip, a.ix.factors[id], a.ix.fvalid[id], s.query_norm, s.query_norm_sqr, a.eps0);
+    a.out_est[item] = finish_gate(short_ip, gate_factors(grow, a.cd), gate_valid(grow, a.cd), s.query_norm,
+                                  s.query_norm_sqr, a.eps0);
 }
 
 // ---------------------------------------------------------------- C5 -----
@@ -320,14 +322,15 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
     const uint64_t qi = a.qidx[item];
     const uint32_t id = a.ids[item];
     const QScalars s = a.qv.qs[qi];
-    const float4 f = a.ix.factors[id];
-    const uint8_t fv = a.ix.fvalid[id];
+    const uint8_t *grow = gate_row(a.ix.gate, a.cd, id);
+    const float4 f = gate_factors(grow, a.cd);
+    const uint8_t fv = gate_valid(grow, a.cd);
     PaperEstimate r;
     r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
     r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
     if (!fv || s.query_norm <= 0.0f || f.z <= 0.0f || !__builtin_isfinite(f.z)) { a.out_est[item] = r; return; }
     const uint32_t *xg = (const uint32_t *)(a.qv.x8 + qi * a.cd);
-    const uint2 *m = (const uint2 *)(a.ix.msb + (uint64_t)id * (a.cd / 8));
+    const uint2 *m = (const uint2 *)grow;
     int sel = 0;
     for (uint32_t c = 0; c < a.cd / 64; ++c) {  // 8 bytes of MSB = 64 dims (code_dim >= 64)
         const uint2 w = m[c];

@@ -54,20 +54,27 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // Copy rows ids[0..count) (ids in LDS) of a row-major device table (row_u4
-// uint4 per row, a power of two) to LDS rows 0..count) with gfx950's direct
-// global-to-LDS loads, the W waves of the group taking interleaved 1 KB
-// pieces; every piece is in flight at once and no registers hold the data.
-// Lanes past the end re-read the last piece into stage slots below 1024 that
-// nobody reads. count * row_u4 <= 1024. The caller's barrier publishes the
-// rows to the other waves.
+// uint4 per row) to LDS rows 0..count) with gfx950's direct global-to-LDS
+// loads, the W waves of the group taking interleaved 1 KB pieces; every
+// piece is in flight at once and no registers hold the data. A piece's row
+// is e / row_u4 by a multiply-high with magic = 2^32 / row_u4 rounded up
+// (exact for e < 2^32 / row_u4). Lanes past the end re-read the last piece
+// into stage slots below the 64-piece round-up of count * row_u4 that nobody
+// reads; that round-up is at most kStageU4. The caller's barrier publishes
+// the rows to the other waves.
+constexpr uint32_t kStageU4 = 1152;  // 64 gate rows of cd 2048 (18 uint4 each)
+
+__host__ __device__ constexpr uint32_t div_magic(uint32_t d) { return 0xFFFFFFFFu / d + 1u; }
+
 template <int W>
 __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
-                                           const uint4 *__restrict__ src, uint32_t wv, uint32_t lane) {
+                                           uint32_t magic, const uint4 *__restrict__ src, uint32_t wv,
+                                           uint32_t lane) {
     const uint32_t total = count * row_u4;
-    const uint32_t sh = (uint32_t)__builtin_ctz(row_u4);
     for (uint32_t k0 = 64u * wv; k0 < total; k0 += 64u * W) {
         const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
-        const uint4 *g = src + (uint64_t)ids[e >> sh] * row_u4 + (e & (row_u4 - 1u));
+        const uint32_t r = __umulhi(e, magic);
+        const uint4 *g = src + (uint64_t)ids[r] * row_u4 + (e - r * row_u4);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
                                          (__attribute__((address_space(3))) void *)(dst + k0), 16, 0, 0);
     }
@@ -143,12 +150,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     float *pd = (float *)(smem + 8 * cd);                       // a.L
     uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * a.L);      // a.L
     uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * a.L);       // a.L (+pad)
-    __shared__ uint4 stage[1024];  // 16 KB: MSB / code rows of a batch; rerank block sums after the traversal
+    // gate rows / code rows of a batch; rerank block sums after the traversal
+    __shared__ uint4 stage[kStageU4];
     __shared__ uint32_t fresh[64], surv[64], sid[64];
-    __shared__ float4 f_fac[64];
     __shared__ float f_lb[64], f_sip[64], s_d[64];
-    __shared__ float2 s_h[64];
-    __shared__ uint8_t f_fv[64], f_valid[64];
+    __shared__ float2 f_h[64];  // header of fresh neighbour c (from its gate row)
+    __shared__ uint8_t f_valid[64];
     __shared__ uint32_t s_qi, s_nf, s_ns, s_len;
     // next hop's neighbour list and visited words, loaded during this hop's
     // insertion for the predicted next frontier s_sp_node
@@ -158,8 +165,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ float s_spa_d;
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6, grp = t >> 4;
     const int j16 = (int)(t & 15u);
-    const uint32_t mch = cd / 128;  // uint4 per MSB row
+    const uint32_t gch = (uint32_t)gate_row_bytes(cd) / 16;  // uint4 per gate row
+    const uint32_t gmagic = div_magic(gch);
     const uint32_t cch = cd / 32;   // uint4 per code row
+    const uint32_t cmagic = div_magic(cch);
     const uint32_t per = 1024u / cch;
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
 
@@ -201,7 +210,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         if (a.n > 0 && L > 0 && wv == 0) {
             if (lane == 0) sid[0] = 0u;
             wave_sync_lds();
-            stage_rows<1>(stage, sid, 1, cch, (const uint4 *)a.ix.codes, 0u, lane);
+            stage_rows<1>(stage, sid, 1, cch, cmagic, (const uint4 *)a.ix.codes, 0u, lane);
             wave_sync_lds();
             if (grp == 0) {
                 const float d0 = asym_group(q, (const uint8_t *)stage, cd, a.ix.hdr[0], s, j16);
@@ -275,16 +284,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             const uint32_t nf = s_nf;
             if (nf == 0) continue;
             gates += nf;
-            // gate: factors and MSB rows of the fresh neighbours in one round trip
-            // (already done during the last insertion when the frontier was
-            // predicted)
+            // gate: the fresh neighbours' gate rows (MSB plane, factors, header,
+            // validity) in one round trip (already done during the last
+            // insertion when the frontier was predicted)
             if (!s_hit) {
-                if (t < nf) {
-                    const uint32_t id = fresh[t];
-                    f_fac[t] = a.ix.factors[id];
-                    f_fv[t] = a.ix.fvalid[id];
-                }
-                stage_rows<W>(stage, fresh, nf, mch, (const uint4 *)a.ix.msb, wv, lane);
+                stage_rows<W>(stage, fresh, nf, gch, gmagic, (const uint4 *)a.ix.gate, wv, lane);
                 __syncthreads();
             }
             // each group scores candidates grp, grp + G, grp + 2G, grp + 3G (FP32:
@@ -296,18 +300,20 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     cc[k] = c0 + (uint32_t)k * G;
-                    rr[k] = stage + (cc[k] < nf ? cc[k] : c0) * mch;
+                    rr[k] = stage + (cc[k] < nf ? cc[k] : c0) * gch;
                 }
                 if (a.codec == 3) {
                     // integer sums: one candidate at a time, unrolled
                     for (int k = 0; k < 4; ++k) {
                         if (cc[k] >= nf) continue;
-                        const PaperEstimate e = gate_int8_group(x8, (const uint8_t *)rr[k], cd, f_fac[cc[k]],
-                                                                f_fv[cc[k]], s, a.eps0, j16);
+                        const uint8_t *row = (const uint8_t *)rr[k];
+                        const PaperEstimate e = gate_int8_group(x8, row, cd, gate_factors(row, cd),
+                                                                gate_valid(row, cd), s, a.eps0, j16);
                         if (j16 == 0) {
                             f_lb[cc[k]] = e.lower_bound;
                             f_sip[cc[k]] = e.short_ip;
                             f_valid[cc[k]] = e.valid;
+                            f_h[cc[k]] = gate_hdr(row, cd);
                         }
                     }
                 } else {
@@ -316,12 +322,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (cc[k] >= nf) continue;
-                        const PaperEstimate e = finish_gate(group_reduce16(acc[k]) - s.half_sum, f_fac[cc[k]],
-                                                            f_fv[cc[k]], s.query_norm, s.query_norm_sqr, a.eps0);
+                        const uint8_t *row = (const uint8_t *)rr[k];
+                        const PaperEstimate e = finish_gate(group_reduce16(acc[k]) - s.half_sum, gate_factors(row, cd),
+                                                            gate_valid(row, cd), s.query_norm, s.query_norm_sqr,
+                                                            a.eps0);
                         if (j16 == 0) {
                             f_lb[cc[k]] = e.lower_bound;
                             f_sip[cc[k]] = e.short_ip;
                             f_valid[cc[k]] = e.valid;
+                            f_h[cc[k]] = gate_hdr(row, cd);
                         }
                     }
                 }
@@ -358,19 +367,17 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             __syncthreads();
             const uint32_t ns = s_ns;
             pruned += nf - ns;
-            // survivors' distances: headers and code rows in one round trip per
-            // batch of `per` rows, one 16-lane group per candidate
-            float2 hreg = make_float2(0.0f, 0.0f);
-            if (t < ns) hreg = a.ix.hdr[sid[t]];
+            // survivors' distances: code rows in one round trip per batch of
+            // `per` rows (the headers came with the gate rows), one 16-lane
+            // group per candidate
             for (uint32_t b0 = 0; b0 < ns; b0 += per) {
                 const uint32_t nb_rows = ns - b0 < per ? ns - b0 : per;
-                stage_rows<W>(stage, sid + b0, nb_rows, cch, (const uint4 *)a.ix.codes, wv, lane);
-                if (b0 == 0 && t < ns) s_h[t] = hreg;
+                stage_rows<W>(stage, sid + b0, nb_rows, cch, cmagic, (const uint4 *)a.ix.codes, wv, lane);
                 __syncthreads();
                 for (uint32_t b = grp; b < nb_rows; b += G) {
                     const uint32_t slot = surv[b0 + b];
                     const uint8_t *row8 = (const uint8_t *)(stage + b * cch);
-                    const float2 h = s_h[b0 + b];
+                    const float2 h = f_h[slot];
                     float d;
                     if (!f_valid[slot]) d = asym_group(q, row8, cd, h, s, j16);
                     else if (a.codec == 3) d = rem_int8_group(x8p, row8, cd, h, s, f_sip[slot], j16);
@@ -432,17 +439,13 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     const uint64_t fm2 = __ballot(fresh2);
                     const uint64_t bm2 = __ballot(bad2);
                     sp_nb[lane] = nb2;
-                    // the next hop's gate inputs: fresh list, factors and MSB rows
-                    // (fresh[], f_fac, f_fv and the stage are idle during the
-                    // insertion; the filter above is done with the stage)
-                    if (fresh2) {
-                        const uint32_t r = (uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull));
-                        fresh[r] = nb2;
-                        f_fac[r] = a.ix.factors[nb2];
-                        f_fv[r] = a.ix.fvalid[nb2];
-                    }
+                    // the next hop's gate inputs: fresh list and gate rows (fresh[]
+                    // and the stage are idle during the insertion; the filter
+                    // above is done with the stage)
+                    if (fresh2) fresh[(uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull))] = nb2;
                     wave_sync_lds();
-                    stage_rows<1>(stage, fresh, (uint32_t)__popcll(fm2), mch, (const uint4 *)a.ix.msb, 0u, lane);
+                    stage_rows<1>(stage, fresh, (uint32_t)__popcll(fm2), gch, gmagic, (const uint4 *)a.ix.gate, 0u,
+                                  lane);
                     if (lane == 0) { s_sp_node = nxt; s_sp_fresh = fm2; s_sp_bad = bm2 != 0ull ? 1u : 0u; }
                 } else if (lane == 0) {
                     s_sp_node = 0xFFFFFFFFu;

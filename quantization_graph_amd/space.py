@@ -81,9 +81,11 @@ class DeviceIndex:
     codes   uint8 [n, cd/2]   primary 4-bit codes (reference nibble layout)
     hdr     f32   [n, 2]      norm_sqr, long_scale
     sfac    f32   [n, 2]      short_scale, error_scale
-    msb     uint8 [n, cd/8]   MSB plane of the codes (paper-prune sidecar)
-    factors f32   [n, 4]      norm_sqr, data_norm, cross_scale, error_cross_scale
-    fvalid  uint8 [n]         sidecar validity
+    gate    uint8 [n, G]      gate rows (G = cd/8 rounded up to 16, plus 32):
+                              the MSB plane of the codes, the paper-prune
+                              factors, a copy of hdr and the sidecar validity,
+                              read by the gate as one row; the views msb,
+                              factors and fvalid below
     rscale  int16 [n, cd/16]  residual block scales (fp16 bit patterns)
     rcodes  uint8 [n, cd/2]   residual 4-bit two's-complement codes
     rfac    f32   [n, 2]      residual_norm_sqr, long_residual_ip
@@ -98,24 +100,39 @@ class DeviceIndex:
         self.codes = torch.empty((n, cd // 2), dtype=torch.uint8, device=dev)
         self.hdr = torch.empty((n, 2), dtype=torch.float32, device=dev)
         self.sfac = torch.empty((n, 2), dtype=torch.float32, device=dev)
-        self.msb = torch.empty((n, cd // 8), dtype=torch.uint8, device=dev)
-        self.factors = torch.empty((n, 4), dtype=torch.float32, device=dev)
-        self.fvalid = torch.empty((n,), dtype=torch.uint8, device=dev)
+        self.gate = torch.zeros((n, _lib.lib().qg_gate_row_bytes(space.handle)), dtype=torch.uint8, device=dev)
         self.rscale = torch.empty((n, cd // 16), dtype=torch.int16, device=dev)
         self.rcodes = torch.empty((n, cd // 2), dtype=torch.uint8, device=dev)
         self.rfac = torch.empty((n, 2), dtype=torch.float32, device=dev)
         self.cid = torch.zeros((n,), dtype=torch.uint8, device=dev)
 
+    @property
+    def _span(self):
+        return self.gate.shape[1] - 32
+
+    @property
+    def msb(self):
+        """MSB plane [n, cd/8] (a view of the gate rows)."""
+        return self.gate[:, :self.space.code_dim // 8]
+
+    @property
+    def factors(self):
+        """Paper-prune factors [n, 4] (a copy out of the gate rows)."""
+        return self.gate[:, self._span:self._span + 16].contiguous().view(torch.float32)
+
+    @property
+    def fvalid(self):
+        """Sidecar validity [n] (a view of the gate rows)."""
+        return self.gate[:, self._span + 24]
+
     def desc(self):
         return IndexDesc(self.codes.data_ptr(), self.hdr.data_ptr(), self.sfac.data_ptr(),
-                         self.msb.data_ptr(), self.factors.data_ptr(), self.fvalid.data_ptr(),
-                         self.rscale.data_ptr(), self.rcodes.data_ptr(), self.rfac.data_ptr(),
-                         self.n, self.cid.data_ptr())
+                         self.gate.data_ptr(), self.rscale.data_ptr(), self.rcodes.data_ptr(),
+                         self.rfac.data_ptr(), self.n, self.cid.data_ptr())
 
     def nbytes(self):
         return sum(t.numel() * t.element_size() for t in (
-            self.codes, self.hdr, self.sfac, self.msb, self.factors, self.fvalid, self.rscale,
-            self.rcodes, self.rfac))
+            self.codes, self.hdr, self.sfac, self.gate, self.rscale, self.rcodes, self.rfac))
 
     def to_records(self):
         """Reference AoS records: compact [n, 16+cd/2+1] and residual
