@@ -217,9 +217,9 @@ QueryView view_of(const qg_query_desc *d) {
     return v;
 }
 
-size_t encode_row_ws_bytes(const Space *s) {
-    return s->cd * 4 /*rot*/ + s->cd * 4 /*sa*/ + s->cd * 2 /*si*/ + sizeof(EncMeta);
-}
+// The encode keeps every intermediate in LDS; only the QR rotation's GEMM
+// output passes through the workspace.
+size_t encode_row_ws_bytes(const Space *s) { return s->rotation == 1 ? s->cd * 4 : 16; }
 
 int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void *ws, size_t ws_bytes,
            hipStream_t stream) {
@@ -231,8 +231,9 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
     if (s->centroid_count > 1 && !out->cid) return fail("a space of several centroids encodes into a centroid-id array");
     if (s->centroid_count > 1 && s->rotation != 0) return fail("the QR rotation holds one centroid");
     const size_t per = encode_row_ws_bytes(s);
-    const uint64_t batch = ws_bytes / per;
+    uint64_t batch = ws_bytes / per;
     if (batch == 0 || !ws) return fail("encode workspace too small: need at least %zu bytes", per);
+    if (batch > (1u << 24)) batch = 1u << 24;  // one launch's grid
     const uint32_t cd = (uint32_t)s->cd;
     for (uint64_t b0 = 0; b0 < n; b0 += batch) {
         const uint64_t m = (n - b0 < batch) ? n - b0 : batch;
@@ -251,10 +252,8 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.center = s->d_center;
         a.centroid_count = (uint32_t)s->centroid_count;
         a.cid = out->cid ? out->cid + b0 : nullptr;
-        a.rot = reinterpret_cast<float *>(w);
-        a.sa = reinterpret_cast<float *>(w + m * cd * 4);
-        a.si = reinterpret_cast<uint16_t *>(w + m * cd * 8);
-        a.meta = reinterpret_cast<EncMeta *>(w + ((m * cd * 10 + 15) / 16) * 16);
+        float *rot = reinterpret_cast<float *>(w);
+        a.rot = rot;
         a.codes = out->codes + b0 * (cd / 2);
         a.hdr = reinterpret_cast<float2 *>(out->hdr) + b0;
         a.sfac = reinterpret_cast<float2 *>(out->sfac) + b0;
@@ -264,7 +263,7 @@ int encode(const Space *s, const float *x, uint64_t n, qg_index_desc *out, void 
         a.rfac = reinterpret_cast<float2 *>(out->rfac) + b0;
         // the QR rotation of (x - c) on the matrix cores fills rot; the encode
         // kernels then normalise it
-        if (a.precision && check_hip(qr_rotate(s, a.x, m, true, a.rot, stream), "rotation GEMM launch")) return -1;
+        if (a.precision && check_hip(qr_rotate(s, a.x, m, true, rot, stream), "rotation GEMM launch")) return -1;
         if (check_hip(qg_launch_encode(&a, stream), "encode launch")) return -1;
     }
     return 0;

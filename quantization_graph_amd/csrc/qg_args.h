@@ -70,16 +70,6 @@ struct SelectArgs {
     uint32_t *out_pos;       // [nq][k] position in the input list (optional)
 };
 
-struct EncMeta {
-    double max_o;
-    double best_t;
-    float rn;
-    float ip_norm;
-    uint16_t lv_start[7];
-    uint16_t lv_end[7];
-    uint32_t pad;
-};
-
 struct EncArgs {
     const float *x;            // [n][dim]
     uint64_t n;                // vectors in this batch
@@ -91,10 +81,7 @@ struct EncArgs {
     const float *center;       // [K][dim] centroids; vector v uses row cid[v]
     uint32_t centroid_count;   // K
     uint8_t *cid;              // [n] assigned centroid (K > 1: written by k_enc_assign; may be null for K = 1)
-    float *rot;                // ws [n][cd]
-    float *sa;                 // ws [n][cd]  |o| sorted descending
-    uint16_t *si;              // ws [n][cd]  matching dimension index
-    EncMeta *meta;             // ws [n]
+    const float *rot;          // precision mode: ws [n][cd], the rotation GEMM's Q (x - c)
     // outputs (SoA rows for this batch)
     uint8_t *codes;
     float2 *hdr;
