@@ -84,12 +84,17 @@ typedef struct qg_index_desc {
 /* Prepared queries on the device (QueryContext, space_rabitq.h:101-132). */
 typedef struct qg_query_desc {
     float *q;           /* [nq][cd] rotated residual                              */
-    int8_t *x8;         /* [nq][cd] INT8 code                                     */
-    int8_t *x8p;        /* [nq][cd] INT8 code regrouped: per 8 dims, 4 even, 4 odd */
+    int8_t *x8;         /* [nq][cd] integer code: INT8 or INT4 (dithered), or the
+                         * +-1 sign code of B1 / B1Main                           */
+    int8_t *x8p;        /* [nq][cd] the same code regrouped: per 8 dims, 4 even, 4 odd */
     void *qs;           /* [nq] 48-byte scalar records (qg_query_scalars)          */
     uint64_t nq;
 } qg_query_desc;
 
+/* int8_scale / int8_error_norm / int8_code_sum describe the prepared
+ * integer code: INT8 or INT4 scale, error norm and code sum; for B1 and
+ * B1Main the sign code's short scale, error scale and signed sum, with
+ * b1_alpha its alpha (space_rabitq.h:3493-3524). */
 typedef struct qg_query_scalars {
     float query_norm;
     float query_norm_sqr;
@@ -100,11 +105,12 @@ typedef struct qg_query_scalars {
     float int8_error_norm;
     int32_t max_abs_bits;
     int64_t int8_code_sum;
-    int64_t pad;
+    float b1_alpha;
+    uint32_t pad;
 } qg_query_scalars;
 
 /* Query coarse codecs (QueryCoarseCodec, space_rabitq.h:36-44). */
-enum { QG_CODEC_FULL = 0, QG_CODEC_B1 = 1, QG_CODEC_INT8 = 3, QG_CODEC_B1_MAIN = 4 };
+enum { QG_CODEC_FULL = 0, QG_CODEC_B1 = 1, QG_CODEC_INT4 = 2, QG_CODEC_INT8 = 3, QG_CODEC_B1_MAIN = 4 };
 
 /* Estimator kernels for qg_scan. */
 enum {
@@ -120,9 +126,12 @@ enum {
                                  space_rabitq.h:3926-3959) -> RabitqPaperEstimate              */
     QG_SCAN_B1 = 9,        /* 4-bit code x B1 sign code (query_distance_b1, space_rabitq.h:
                               3742-3755); queries prepared with QG_CODEC_B1                    */
-    QG_SCAN_RESIDUAL_RECORD = 10 /* E1 in the per-record form (computeResidualDistanceInterval
+    QG_SCAN_RESIDUAL_RECORD = 10, /* E1 in the per-record form (computeResidualDistanceInterval
                               FromRecord, :1723-1735): what single-record calls and spaces of
                               several centroids take; pairs only                              */
+    QG_SCAN_GATE_B1 = 11   /* the B1 codec's symmetric 1-bit gate (compute_b1_paper_prune_
+                              estimate_batch_by_id, space_rabitq.h:4607-4753); pairs only;
+                              INT4 queries take QG_SCAN_GATE_INT8                             */
 };
 
 /* -------------------------------------------------------- device batch API */

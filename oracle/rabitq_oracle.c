@@ -630,12 +630,36 @@ void ro_prepare_query_rotated(const ro_space *s, const float *raw, const float *
     q->int8_scale = 1.0f;
     q->int8_error_norm = 0.0f;
     q->int8_code_sum = 0;
+    q->b1_short_scale = q->b1_error_scale = q->b1_alpha = 0.0f;
     memset(q->x8, 0, cd);
-    if (codec == 3 && mx > 0.0f) {
-        q->int8_scale = mx / 128.0f;
+    if (codec == 1 || codec == 4) {
+        /* the sign code and its scale factors (:3493-3524): |q| summed in
+         * float in index order */
+        float abs_sum = 0.0f;
+        for (size_t i = 0; i < cd; ++i) {
+            const float r = q->q[i];
+            abs_sum += fabsf(r);
+            q->x8[i] = r > 0.0f ? 1 : -1;
+            q->int8_code_sum += q->x8[i];
+        }
+        if (q->query_norm > 0.0f && abs_sum > 0.0f && cd > 1) {
+            const float a_q = abs_sum / q->query_norm;
+            float alpha_q = a_q / sqrtf((float)cd);
+            alpha_q = (1.0f < alpha_q) ? 1.0f : alpha_q;   /* std::min(1.0f, .) */
+            const float alpha2 = alpha_q * alpha_q;
+            q->b1_alpha = alpha_q;
+            q->b1_short_scale = 4.0f * q->query_norm / a_q;
+            float r = (1.0f - alpha2) / alpha2;
+            r = (0.0f < r) ? r : 0.0f;                      /* std::max(0.0f, .) */
+            q->b1_error_scale = 2.0f * q->query_norm * sqrtf(r) / sqrtf((float)(cd - 1));
+        }
+    }
+    if ((codec == 3 || codec == 2) && mx > 0.0f) {
+        const int int8 = codec == 3;
+        q->int8_scale = int8 ? mx / 128.0f : mx / 8.0f;
         /* FNV-1a over the raw query bytes, mixed with the centroid id
          * (:3549-3558), seeds mt19937_64; two uniform draws per dimension,
-         * the first for INT8 (:3585-3590, 3628-3631). */
+         * the first for INT8, the second for INT4 (:3585-3590, 3628-3652). */
         uint64_t seed = 1469598103934665603ull;
         const uint8_t *bytes = (const uint8_t *)raw;
         for (size_t i = 0; i < D * 4; ++i) { seed ^= bytes[i]; seed *= 1099511628211ull; }
@@ -643,12 +667,13 @@ void ro_prepare_query_rotated(const ro_space *s, const float *raw, const float *
         mt64 g;
         mt64_seed(&g, seed);
         double err = 0.0;
+        const int32_t lo = int8 ? -128 : -8, hi = int8 ? 127 : 7;
         for (size_t i = 0; i < cd; ++i) {
             const double u8 = u01_from(mt64_next(&g));
-            (void)mt64_next(&g);
-            int32_t x = (int32_t)floor((double)q->q[i] / (double)q->int8_scale + u8);
-            if (x > 127) x = 127;
-            if (x < -128) x = -128;
+            const double u4 = u01_from(mt64_next(&g));
+            int32_t x = (int32_t)floor((double)q->q[i] / (double)q->int8_scale + (int8 ? u8 : u4));
+            if (x > hi) x = hi;
+            if (x < lo) x = lo;
             const double e = (double)q->q[i] - (double)q->int8_scale * (double)x;
             err += e * e;
             q->x8[i] = (int8_t)x;
@@ -775,6 +800,45 @@ ro_estimate ro_gate_int8(const ro_space *s, const ro_query *q, const uint8_t *ms
     const int64_t sel = ro_masked_sum(s, q, msb);
     const float short_ip = 0.5f * q->int8_scale * (float)(2 * sel - q->int8_code_sum);
     return finish_gate(short_ip, f, q, eps0);
+}
+
+/* compute_b1_paper_prune_estimate_batch_by_id (:4607-4753): mismatches of
+ * the query's sign code against the MSB plane, signed_dot = cd - 2
+ * mismatches, and finish_b1_paper_prune_estimate (:4724-4753), whose bound
+ * adds the query side's error to the database side's. */
+ro_estimate ro_gate_b1(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0) {
+    ro_estimate r;
+    memset(&r, 0, sizeof(r));
+    const int query_valid = q->b1_short_scale > 0.0f && isfinite(q->b1_short_scale) && q->query_norm > 0.0f;
+    if (!query_valid || !f->valid || f->cross_scale <= 0.0f || !isfinite(f->cross_scale) || f->data_norm <= 0.0f)
+        return r;
+    uint64_t mism = 0;
+    for (size_t i = 0; i < s->code_dim; ++i) {
+        const unsigned qb = q->q[i] > 0.0f ? 1u : 0u;
+        mism += qb ^ ((msb[i >> 3] >> (i & 7u)) & 1u);
+    }
+    const float signed_dot = (float)s->code_dim - 2.0f * (float)mism;
+    const float short_ip = signed_dot * q->b1_short_scale / 8.0f;
+    const float cross = short_ip * f->cross_scale;
+    const float max_cross = 2.0f * f->data_norm * q->query_norm;
+    float sum;
+    if (SITE(RO_FMA_B1_SUM_DB)) sum = fmaf(f->error_cross_scale, q->query_norm, q->b1_error_scale * f->data_norm);
+    else if (SITE(RO_FMA_B1_SUM_Q)) sum = fmaf(q->b1_error_scale, f->data_norm, f->error_cross_scale * q->query_norm);
+    else {
+        const float db_err = f->error_cross_scale * q->query_norm;
+        const float query_err = q->b1_error_scale * f->data_norm;
+        sum = db_err + query_err;
+    }
+    float up = SITE(RO_FMA_B1_UPPER) ? fmaf(eps0, sum, cross) : cross + eps0 * sum;
+    up = (up < max_cross) ? up : max_cross;     /* std::min(max_cross, up) */
+    up = (-max_cross < up) ? up : -max_cross;   /* std::max(-max_cross, up) */
+    r.lower_bound = f->norm_sqr + q->query_norm_sqr - up;
+    r.short_ip = short_ip;
+    r.alpha = q->b1_alpha;
+    r.ip_hat = max_cross > 0.0f ? cross / max_cross : 0.0f;
+    r.error_bound = max_cross > 0.0f ? eps0 * sum / max_cross : 0.0f;
+    r.valid = isfinite(r.lower_bound) ? 1 : 0;
+    return r;
 }
 
 /* quantizedCodeDotDispatch (:5099-5155): exact integer dot. */

@@ -48,11 +48,15 @@ extern "C" {
 #define RO_FMA_SS_INTCONST (1u << 16) /* sign scan query: cd + (scale*scale)*code_norm2   */
 #define RO_FMA_SS_B1DIST   (1u << 17) /* sign scan B1: constant - (2*scale)*signed_dot     */
 #define RO_FMA_SS_INTDIST  (1u << 18) /* sign scan INT4/INT8: constant - (2*scale)*dot     */
+#define RO_FMA_B1_SUM_DB   (1u << 19) /* B1 gate: fma(ecs, qn, b1_error_scale*dn)          */
+#define RO_FMA_B1_SUM_Q    (1u << 20) /* B1 gate: fma(b1_error_scale, dn, ecs*qn)          */
+#define RO_FMA_B1_UPPER    (1u << 21) /* B1 gate: cross + eps*(db_err + query_err)          */
 /* The pattern the reference compiler produces (g++ 11, -O3, AVX-512). */
 #define RO_FMA_REFERENCE (RO_FMA_FQA_T | RO_FMA_ENC_OOBAR | RO_FMA_ENC_DECNORM | RO_FMA_ENC_X4IP | \
                           RO_FMA_C1_LANE | RO_FMA_C1_SUB7 | RO_FMA_C1_EST | RO_FMA_RES_HI | \
                           RO_FMA_RES_ACC | RO_FMA_PAPER_LOWER | RO_FMA_KMEANS_DIST | \
-                          RO_FMA_SS_B1CONST | RO_FMA_SS_INTCONST | RO_FMA_SS_B1DIST | RO_FMA_SS_INTDIST)
+                          RO_FMA_SS_B1CONST | RO_FMA_SS_INTCONST | RO_FMA_SS_B1DIST | RO_FMA_SS_INTDIST | \
+                          RO_FMA_B1_SUM_DB)
 
 extern unsigned ro_fma_mask;
 void ro_set_fma_mask(unsigned mask);
@@ -125,13 +129,18 @@ typedef struct ro_query {
     float half_sum;
     float sum;
     float positive_sum;
-    float int8_scale;
-    float int8_error_norm;
-    int64_t int8_code_sum;
+    float int8_scale;      /* INT8 or INT4 scale of the prepared integer code */
+    float int8_error_norm; /* its error norm */
+    int64_t int8_code_sum; /* its code sum (B1: the signed sum of the sign code) */
+    float b1_short_scale;  /* B1 / B1Main: the sign code's scale factors */
+    float b1_error_scale;
+    float b1_alpha;
 } ro_query;
 void ro_query_alloc(const ro_space *s, ro_query *q);
 void ro_query_free(ro_query *q);
-/* codec: 0 = Full, 3 = Int8 (QueryCoarseCodec, space_rabitq.h:36-44). */
+/* codec: 0 = Full, 1 = B1, 2 = Int4, 3 = Int8, 4 = B1Main (QueryCoarseCodec,
+ * space_rabitq.h:36-44). Int4 and Int8 fill x8 with the dithered code, B1
+ * and B1Main with the +-1 sign code. */
 void ro_prepare_query(const ro_space *s, const float *raw, int codec, ro_query *q);
 void ro_prepare_query_rotated(const ro_space *s, const float *raw, const float *rq, const float *rc, int codec,
                               ro_query *q);
@@ -157,6 +166,9 @@ double ro_asym_distance_f64(const ro_space *s, const ro_query *q, const uint8_t 
 float ro_remaining_distance_fp32(const ro_space *s, const ro_query *q, const uint8_t *compact, float short_ip);
 ro_estimate ro_gate_fp32(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0);
 ro_estimate ro_gate_int8(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0);
+/* compute_b1_paper_prune_estimate_batch_by_id (space_rabitq.h:4607-4753):
+ * the symmetric 1-bit gate of a B1 query. */
+ro_estimate ro_gate_b1(const ro_space *s, const ro_query *q, const uint8_t *msb, const ro_factors *f, float eps0);
 int64_t ro_masked_sum(const ro_space *s, const ro_query *q, const uint8_t *msb);
 int64_t ro_remaining_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact);
 int64_t ro_full_dot_int8(const ro_space *s, const ro_query *q, const uint8_t *compact);

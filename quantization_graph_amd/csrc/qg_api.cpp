@@ -440,7 +440,8 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
                        void *ws, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || (!queries && nq) || !out || (!ws && nq)) return fail("null argument to qg_prepare_queries");
-    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8 && codec != QG_CODEC_B1 && codec != QG_CODEC_B1_MAIN)
+    if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8 && codec != QG_CODEC_INT4 && codec != QG_CODEC_B1 &&
+        codec != QG_CODEC_B1_MAIN)
         return fail("unsupported query codec %d", codec);
     if (!out->q || !out->x8 || !out->x8p || !out->qs) return fail("query output view has a null array");
     // one context per centroid (the eager centroid queries of
@@ -455,9 +456,10 @@ int qg_prepare_queries(const void *space, const float *queries, uint64_t nq, int
     hipStream_t st = (hipStream_t)stream;
     const size_t cd = s->cd;
     for (size_t k = 0; k < K; ++k) {
-        // the FNV-1a seed feeds only the INT8 dither; |q|^2 and <q, c_k> in FP64
+        // the FNV-1a seed feeds only the INT8 / INT4 dither; |q|^2 and <q, c_k> in FP64
+        const bool dither = codec == QG_CODEC_INT8 || codec == QG_CODEC_INT4;
         if (check_hip(qg_launch_row_seed(queries, nq, (uint32_t)s->dim, s->d_center + k * s->dim,
-                                         codec == QG_CODEC_INT8 ? seed : nullptr, nrm, dot, st),
+                                         dither ? seed : nullptr, nrm, dot, st),
                       "row seed"))
             return -1;
         PrepParams p;
@@ -492,7 +494,7 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
             RabitqDistanceInterval *out_iv, int64_t *out_dot, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !ix || !ids || (!qv && kernel != QG_SCAN_SYMMETRIC)) return fail("null argument to qg_scan");
-    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_RESIDUAL_RECORD) return fail("unknown scan kernel %d", kernel);
+    if (kernel < QG_SCAN_INT8_FULL || kernel > QG_SCAN_GATE_B1) return fail("unknown scan kernel %d", kernel);
     const bool record_form = kernel == QG_SCAN_RESIDUAL_RECORD;
     if (record_form) {
         if (fixed) return fail("the per-record residual form scores pairs");
@@ -501,7 +503,8 @@ int qg_scan(const void *space, int kernel, int fixed, const qg_index_desc *ix, c
     if (!fixed && !qidx) return fail("pair scans need qidx");
     if (fixed && kernel != QG_SCAN_INT8_FULL && kernel != QG_SCAN_ASYM && kernel != QG_SCAN_RESIDUAL)
         return fail("kernel %d has no fixed-list form", kernel);
-    const bool gate = kernel == QG_SCAN_GATE_FP32 || kernel == QG_SCAN_GATE_INT8 || kernel == QG_SCAN_PAPER_RECORD;
+    const bool gate = kernel == QG_SCAN_GATE_FP32 || kernel == QG_SCAN_GATE_INT8 || kernel == QG_SCAN_PAPER_RECORD ||
+                      kernel == QG_SCAN_GATE_B1;
     if (gate && !out_est) return fail("gate kernels need out_est");
     if (kernel == QG_SCAN_RESIDUAL && (!out_iv || !long_in)) return fail("residual kernel needs long_in and out_iv");
     if (!gate && kernel != QG_SCAN_RESIDUAL && !out_f) return fail("kernel needs out_f");

@@ -437,9 +437,10 @@ const void *rabitq_prepare_query(const void *space, const float *query) {
     bq.codec = s->codec;
     if (hipMemcpy(bq.raw, query, s->dim * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     qg_query_desc d = bq.desc();
-    // The codecs this library prepares: Int8 (dithered code), B1 / B1Main
-    // (sign code); every other codec reads the FP32 residual only.
-    const int codec = (s->codec == QG_CODEC_INT8 || s->codec == QG_CODEC_B1 || s->codec == QG_CODEC_B1_MAIN)
+    // The codecs this library prepares: Int8 / Int4 (dithered codes), B1 /
+    // B1Main (sign code); every other codec reads the FP32 residual only.
+    const int codec = (s->codec == QG_CODEC_INT8 || s->codec == QG_CODEC_INT4 || s->codec == QG_CODEC_B1 ||
+                       s->codec == QG_CODEC_B1_MAIN)
                           ? s->codec : QG_CODEC_FULL;
     if (qg_prepare_queries(s, bq.raw, 1, codec, &d, bq.ws, nullptr) != 0) return nullptr;
     if (hipDeviceSynchronize() != hipSuccess) return nullptr;
@@ -516,10 +517,9 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
     const BridgeQuery *bq = static_cast<const BridgeQuery *>(prepared);
     if (!s || !bq || bq->owner != s || (count && (!ids || !msb_base || !factors_base || !out))) return false;
     if (count == 0) return true;
-    // compute_paper_prune_estimate_sidecar (space_rabitq.h:4519-4529): Int8
-    // reads the dithered code, B1 and Int4 have gates of their own (not built
-    // here), every other codec takes the FP32 gate.
-    if (bq->codec == QG_CODEC_B1 || bq->codec == 2) return fail("query codec %d has no gate here", bq->codec) == 0;
+    // rabitq_paper_estimate_batch_sidecar (rabitq_bridge.cpp:702-883): B1
+    // takes its symmetric sign-code gate, Int4 and Int8 the integer gate on
+    // their dithered codes, every other codec the FP32 gate.
     // the caller's sidecar rows as device gate rows (the header copy is not
     // read by the gate and stays zero)
     const size_t mb = s->cd / 8, gb = gate_row_bytes(s->cd), m = gate_msb_span(s->cd);
@@ -547,7 +547,8 @@ bool rabitq_paper_estimate_batch_sidecar(const void *space, const void *prepared
     if (!d.gate || !rows || !qidx || !dout) return false;
     (void)hipMemcpy(d.gate, rowsg.data(), rowsg.size(), hipMemcpyHostToDevice);
     const qg_query_desc qd = bq->desc();
-    const int k = bq->codec == QG_CODEC_INT8 ? QG_SCAN_GATE_INT8 : QG_SCAN_GATE_FP32;
+    const int k = (bq->codec == QG_CODEC_INT8 || bq->codec == QG_CODEC_INT4) ? QG_SCAN_GATE_INT8
+                  : bq->codec == QG_CODEC_B1 ? QG_SCAN_GATE_B1 : QG_SCAN_GATE_FP32;
     if (qg_scan(s, k, 0, &d, &qd, rows, qidx, 0, count, epsilon0, nullptr, nullptr, nullptr, nullptr, dout, nullptr,
                 nullptr, nullptr) != 0)
         return false;

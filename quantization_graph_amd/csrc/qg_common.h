@@ -60,7 +60,9 @@ __device__ __forceinline__ void gate_store_tail(uint8_t *row, uint32_t cd, float
 }
 
 // Per-query scalars of the prepared query context (QueryContext,
-// space_rabitq.h:101-132).
+// space_rabitq.h:101-132). The int8_* fields describe the prepared integer
+// code: INT8 or INT4 scale, error norm and code sum; for the B1 codecs the
+// sign code's short scale, error scale and signed sum (with b1_alpha).
 struct QScalars {
     float query_norm;
     float query_norm_sqr;
@@ -71,7 +73,8 @@ struct QScalars {
     float int8_error_norm;
     int32_t max_abs_bits;
     int64_t int8_code_sum;
-    int64_t pad;
+    float b1_alpha;
+    uint32_t pad;
 };
 static_assert(sizeof(QScalars) == 48, "QScalars layout");
 
@@ -164,6 +167,29 @@ __device__ __forceinline__ PaperEstimate finish_gate(float short_ip, float4 f, u
     r.ip_hat = max_cross > 0.0f ? cross / max_cross : 0.0f;
     r.error_bound = max_cross > 0.0f ? eq / max_cross : 0.0f;
     r.valid = __builtin_isfinite(r.lower_bound) ? 1 : 0;
+    return r;
+}
+
+// finish_b1_paper_prune_estimate (space_rabitq.h:4724-4753): the bound adds
+// the query side's error to the database side's; the reference compiler
+// fuses db_err into the sum (fma(ecs, |q|, b1_error_scale * data_norm)).
+__device__ __forceinline__ PaperEstimate finish_gate_b1(float short_ip, float4 f, float qn, float qn2, float b1e,
+                                                        float b1_alpha, float eps0) {
+    PaperEstimate r;
+    const float cross = short_ip * f.z;
+    const float max_cross = 2.0f * f.y * qn;
+    const float sum = fmaf(f.w, qn, b1e * f.y);
+    const float es = eps0 * sum;
+    float up = cross + es;
+    up = (up < max_cross) ? up : max_cross;
+    up = (-max_cross < up) ? up : -max_cross;
+    r.lower_bound = f.x + qn2 - up;
+    r.short_ip = short_ip;
+    r.alpha = b1_alpha;
+    r.ip_hat = max_cross > 0.0f ? cross / max_cross : 0.0f;
+    r.error_bound = max_cross > 0.0f ? es / max_cross : 0.0f;
+    r.valid = __builtin_isfinite(r.lower_bound) ? 1 : 0;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
     return r;
 }
 

@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
     const uint32_t cd = p.cd;
     float *v = (float *)smem;                          // cd floats
     uint64_t *mt = (uint64_t *)(smem + 4 * cd);        // 312 words
-    __shared__ float s_sum, s_pos, s_max;
+    __shared__ float s_sum, s_pos, s_max, s_abs;
     __shared__ long long s_csum;
     const uint64_t qi = blockIdx.x;
     const float *raw = p.raw + qi * p.dim;
@@ -210,6 +210,11 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
             s += pe + po;
         }
         s_pos = s;
+    } else if (t == 128 && (p.codec == 1 || p.codec == 4)) {
+        // the sign code's |q| sum in index order (:3497-3499)
+        float s = 0.0f;
+        for (uint32_t i = 0; i < cd; ++i) s += fabsf(v[i]);
+        s_abs = s;
     }
     float m = 0.0f;
     for (uint32_t i = t; i < cd; i += blockDim.x) { const float a = fabsf(v[i]); m = (m < a) ? a : m; }
@@ -223,9 +228,12 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
     const float qn = sqrtf(qn2);
     const float max_abs = s_max;
     float scale = 1.0f;
-    const bool int8 = (p.codec == 3) && (max_abs > 0.0f);
+    // INT8 or INT4 (:3536-3652): scale max|q| / 128 or / 8, the first or the
+    // second of each dimension's two dither draws, clamp to the code's range
+    const bool int8 = (p.codec == 3 || p.codec == 2) && (max_abs > 0.0f);
+    const bool is4 = p.codec == 2;
     if (int8) {
-        scale = max_abs / 128.0f;
+        scale = is4 ? max_abs / 8.0f : max_abs / 128.0f;
         if (t == 0) {
             uint64_t sd = p.seed[qi];
             sd ^= (uint64_t)p.centroid_id + 0x9E3779B97F4A7C15ull + (sd << 6) + (sd >> 2);
@@ -239,12 +247,13 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
             mt64_twist(mt);
             for (uint32_t k = t; k < 312; k += blockDim.x) {
                 const uint32_t idx = base + k;
-                if (idx < draws && (idx & 1u) == 0) {
+                if (idx < draws && (idx & 1u) == (is4 ? 1u : 0u)) {
                     const uint32_t i = idx >> 1;
                     double u = (double)mt64_temper(mt[k]) * 0x1p-64;
                     if (u >= 1.0) u = 0x1.fffffffffffffp-1;  // nextafter(1.0, 0.0)
                     int x = (int)floor((double)v[i] / (double)scale + u);
-                    x = x > 127 ? 127 : (x < -128 ? -128 : x);
+                    const int hi = is4 ? 7 : 127, lo = is4 ? -8 : -128;
+                    x = x > hi ? hi : (x < lo ? lo : x);
                     p.x8_out[qi * cd + i] = (int8_t)x;
                     // regroup: per 8 dims, 4 even dims then 4 odd dims
                     const uint32_t g = i >> 3, w = i & 7u;
@@ -282,10 +291,27 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
         s.sum = s_sum;
         s.positive_sum = s_pos;
         s.int8_scale = scale;
-        s.int8_error_norm = 0.0f;  // filled by k_query_err for INT8 queries
+        s.int8_error_norm = 0.0f;  // filled by k_query_err for INT8 / INT4 queries
         s.max_abs_bits = __float_as_int(max_abs);
         s.int8_code_sum = (int8 || p.codec == 1 || p.codec == 4) ? (int64_t)s_csum : 0;
+        s.b1_alpha = 0.0f;
         s.pad = 0;
+        if (p.codec == 1 || p.codec == 4) {
+            // the sign code's scale factors (:3512-3524)
+            const float abs_sum = s_abs;
+            s.int8_scale = 0.0f;
+            if (qn > 0.0f && abs_sum > 0.0f && cd > 1) {
+                const float a_q = abs_sum / qn;
+                float alpha_q = a_q / sqrtf((float)cd);
+                alpha_q = (1.0f < alpha_q) ? 1.0f : alpha_q;  // std::min(1.0f, .)
+                const float alpha2 = alpha_q * alpha_q;
+                s.b1_alpha = alpha_q;
+                s.int8_scale = 4.0f * qn / a_q;
+                float r = (1.0f - alpha2) / alpha2;
+                r = (0.0f < r) ? r : 0.0f;  // std::max(0.0f, .)
+                s.int8_error_norm = 2.0f * qn * sqrtf(r) / sqrtf((float)(cd - 1));
+            }
+        }
         p.qs_out[qi] = s;
     }
 }
@@ -370,7 +396,7 @@ int qg_launch_query_prep(const qg::PrepParams *p, hipStream_t stream) {
     if (p->nq == 0) return 0;
     const size_t smem = (size_t)p->cd * 4 + 312 * 8;
     hipLaunchKernelGGL(qg::k_query_prep, dim3((unsigned)p->nq), dim3(qg::kPrepThreads), smem, stream, *p);
-    if (p->codec == 3)
+    if (p->codec == 3 || p->codec == 2)
         hipLaunchKernelGGL(qg::k_query_err, dim3((unsigned)((p->nq + 63) / 64)), dim3(64), 0, stream, p->q_out,
                            p->x8_out, p->nq, p->cd, p->qs_out);
     return (int)hipGetLastError();

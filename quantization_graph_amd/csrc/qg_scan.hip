@@ -351,6 +351,45 @@ __global__ void __launch_bounds__(kScanThreads) k_gate_int8(ScanArgs a) {
     a.out_est[item] = finish_gate(short_ip, f, fv, s.query_norm, s.query_norm_sqr, a.eps0);
 }
 
+// The B1 codec's symmetric 1-bit gate (compute_b1_paper_prune_estimate_
+// batch_by_id, space_rabitq.h:4607-4753). With the +-1 sign code x, the
+// masked sum sel = sum of x_i over set MSB bits gives the reference's
+// cd - 2 mismatches as 2 sel - sum(x), exactly.
+__global__ void __launch_bounds__(kScanThreads) k_gate_b1(ScanArgs a) {
+    const uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.n_pairs) return;
+    const uint64_t qi = a.qidx[item];
+    const uint32_t id = a.ids[item];
+    const QScalars s = a.qv.qs[qi];
+    const uint8_t *grow = gate_row(a.ix.gate, a.cd, id);
+    const float4 f = gate_factors(grow, a.cd);
+    const uint8_t fv = gate_valid(grow, a.cd);
+    PaperEstimate r;
+    r.lower_bound = r.short_ip = r.alpha = r.ip_hat = r.error_bound = 0.0f;
+    r.valid = 0; r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    const bool query_valid = s.int8_scale > 0.0f && __builtin_isfinite(s.int8_scale) && s.query_norm > 0.0f;
+    if (!query_valid || !fv || f.z <= 0.0f || !__builtin_isfinite(f.z) || f.y <= 0.0f) { a.out_est[item] = r; return; }
+    const uint32_t *xg = (const uint32_t *)(a.qv.x8 + qi * a.cd);
+    const uint2 *m = (const uint2 *)grow;
+    int sel = 0;
+    for (uint32_t c = 0; c < a.cd / 64; ++c) {
+        const uint2 w = m[c];
+        const uint32_t ws[2] = {w.x, w.y};
+#pragma unroll
+        for (int k2 = 0; k2 < 2; ++k2) {
+#pragma unroll
+            for (int nib = 0; nib < 8; ++nib) {
+                const uint32_t mask = nibble_bytes((ws[k2] >> (4 * nib)) & 0xFu);
+                const uint32_t d4 = 64 * c + 32 * k2 + 4 * nib;
+                sel = __builtin_amdgcn_sdot4((int)mask, (int)xg[d4 / 4], sel, false);
+            }
+        }
+    }
+    const float signed_dot = (float)((int64_t)2 * (int64_t)sel - s.int8_code_sum);
+    const float short_ip = signed_dot * s.int8_scale / 8.0f;
+    a.out_est[item] = finish_gate_b1(short_ip, f, s.query_norm, s.query_norm_sqr, s.int8_error_norm, s.b1_alpha, a.eps0);
+}
+
 // ---------------------------------------------------------------- E1 -----
 // Residual-4 rerank (compute_residual_distance_intervals_batch_by_id,
 // space_rabitq.h:5432-5548): per 16-dim block, 8 lane products
@@ -714,7 +753,8 @@ int qg_launch_centroid_pairs(const float *centroids, const float *rot_centroids,
 }
 
 enum { QG_K_INT8_FULL = 0, QG_K_ASYM = 1, QG_K_REM_FP32 = 2, QG_K_REM_INT8 = 3, QG_K_GATE_FP32 = 4,
-       QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7, QG_K_PAPER_RECORD = 8, QG_K_B1 = 9 };
+       QG_K_GATE_INT8 = 5, QG_K_RESIDUAL = 6, QG_K_SYMMETRIC = 7, QG_K_PAPER_RECORD = 8, QG_K_B1 = 9,
+       QG_K_GATE_B1 = 11 };
 
 int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t stream) {
     const dim3 block(qg::kScanThreads);
@@ -758,6 +798,7 @@ int qg_launch_scan(int kernel, int fixed, const qg::ScanArgs *a, hipStream_t str
                 break;
             case QG_K_PAPER_RECORD: hipLaunchKernelGGL(qg::k_paper_record, grid, block, 0, stream, *a); break;
             case QG_K_B1: hipLaunchKernelGGL(qg::k_b1, grid, block, 0, stream, *a); break;
+            case QG_K_GATE_B1: hipLaunchKernelGGL(qg::k_gate_b1, grid, block, 0, stream, *a); break;
             default: return (int)hipErrorInvalidValue;
         }
     }

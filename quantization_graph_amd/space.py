@@ -21,9 +21,15 @@ import torch
 from . import _lib
 from ._lib import IndexDesc, QueryDesc, check
 
+# QueryCoarseCodec (space_rabitq.h:36-44): the formal codecs Full and Int8, and
+# the ablation codecs B1 (sign code), Int4 and B1Main
 CODEC_FULL = 0
+CODEC_B1 = 1
+CODEC_INT4 = 2
 CODEC_INT8 = 3
-_CODECS = {"full": CODEC_FULL, "int8": CODEC_INT8, CODEC_FULL: CODEC_FULL, CODEC_INT8: CODEC_INT8}
+CODEC_B1_MAIN = 4
+_CODECS = {"full": CODEC_FULL, "b1": CODEC_B1, "int4": CODEC_INT4, "int8": CODEC_INT8, "b1main": CODEC_B1_MAIN}
+_CODECS.update({v: v for v in list(_CODECS.values())})
 
 SCAN_INT8_FULL = 0
 SCAN_ASYM = 1
@@ -36,6 +42,7 @@ SCAN_SYMMETRIC = 7
 SCAN_PAPER_RECORD = 8
 SCAN_B1 = 9
 SCAN_RESIDUAL_RECORD = 10  # E1 in the per-record form (single records, spaces of several centroids)
+SCAN_GATE_B1 = 11  # the B1 codec's sign-code gate (pairs; INT4 queries take SCAN_GATE_INT8)
 
 # RabitqPaperEstimate (rabitq_bridge.cpp:21-28) and RabitqDistanceInterval
 # (:15-19) as numpy record types.
@@ -46,7 +53,7 @@ INTERVAL_DTYPE = np.dtype([("estimate", "<f4"), ("lower_bound", "<f4"), ("upper_
 QSCALAR_DTYPE = np.dtype([("query_norm", "<f4"), ("query_norm_sqr", "<f4"), ("half_sum", "<f4"),
                           ("sum", "<f4"), ("positive_sum", "<f4"), ("int8_scale", "<f4"),
                           ("int8_error_norm", "<f4"), ("max_abs_bits", "<i4"),
-                          ("int8_code_sum", "<i8"), ("pad", "<i8")])
+                          ("int8_code_sum", "<i8"), ("b1_alpha", "<f4"), ("pad", "<u4")])
 
 
 def _stream():
@@ -397,7 +404,7 @@ class RabitqSpace:
             nq = queries.nq
             if kernel == SCAN_RESIDUAL:  # the reference scores each record alone here
                 kernel = SCAN_RESIDUAL_RECORD
-            if kernel in (SCAN_GATE_FP32, SCAN_GATE_INT8):
+            if kernel in (SCAN_GATE_FP32, SCAN_GATE_INT8, SCAN_GATE_B1):
                 qidx = torch.arange(nq, dtype=torch.int32, device="cuda")[:, None].expand_as(ids)
             else:
                 qidx = index.cid[ids.long()].int() * nq + torch.arange(nq, dtype=torch.int32, device="cuda")[:, None]

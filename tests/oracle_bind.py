@@ -24,7 +24,8 @@ FMA = dict(ENC_NORM=1 << 0, FQA_T=1 << 1, ENC_OOBAR=1 << 2, ENC_DECNORM=1 << 3,
            ENC_X4IP=1 << 4, C1_LANE=1 << 5, C1_SUB7=1 << 6, C1_EST=1 << 7,
            GATE_UPPER=1 << 8, RES_HI=1 << 9, RES_LO=1 << 10, RES_ACC=1 << 11,
            PAPER_LOWER=1 << 12, KMEANS_DIST=1 << 13, SS_NORM2=1 << 14, SS_B1CONST=1 << 15,
-           SS_INTCONST=1 << 16, SS_B1DIST=1 << 17, SS_INTDIST=1 << 18)
+           SS_INTCONST=1 << 16, SS_B1DIST=1 << 17, SS_INTDIST=1 << 18, B1_SUM_DB=1 << 19,
+           B1_SUM_Q=1 << 20, B1_UPPER=1 << 21)
 
 
 class ROSpace(C.Structure):
@@ -43,7 +44,8 @@ class ROQuery(C.Structure):
                 ("query_norm", C.c_float), ("query_norm_sqr", C.c_float),
                 ("half_sum", C.c_float), ("sum", C.c_float), ("positive_sum", C.c_float),
                 ("int8_scale", C.c_float), ("int8_error_norm", C.c_float),
-                ("int8_code_sum", C.c_int64)]
+                ("int8_code_sum", C.c_int64), ("b1_short_scale", C.c_float),
+                ("b1_error_scale", C.c_float), ("b1_alpha", C.c_float)]
 
 
 class SSQuery(C.Structure):
@@ -139,6 +141,8 @@ class Oracle:
         L.ro_gate_fp32.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.ro_gate_int8.restype = Estimate
         L.ro_gate_int8.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
+        L.ro_gate_b1.restype = Estimate
+        L.ro_gate_b1.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float]
         L.ro_int8_full_estimate.restype = C.c_float
         L.ro_int8_full_estimate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                                             C.POINTER(C.c_float)]
@@ -215,6 +219,11 @@ class Oracle:
 
     def set_fma(self, mask):
         self.lib.ro_set_fma_mask(mask)
+
+    def get_fma(self):
+        """The library-wide contraction mask (shared by every Oracle)."""
+        self.lib.ro_get_fma_mask.restype = C.c_uint
+        return int(self.lib.ro_get_fma_mask())
 
     def signs(self):
         return np.ctypeslib.as_array(self.s.signs, (self.cd,)).copy()
@@ -365,7 +374,7 @@ class Oracle:
                          for i in range(comp.shape[0])], np.float32)
 
     def gate(self, q, msb, fac, codec, eps0=1.9):
-        fn = self.lib.ro_gate_int8 if codec == 3 else self.lib.ro_gate_fp32
+        fn = self.lib.ro_gate_int8 if codec in (2, 3) else (self.lib.ro_gate_b1 if codec == 1 else self.lib.ro_gate_fp32)
         out = np.zeros(msb.shape[0], ESTIMATE_DTYPE)
         for i in range(msb.shape[0]):
             e = fn(self.sp, C.byref(q), _p(msb[i]), C.c_void_p(fac.ctypes.data + 20 * i), C.c_float(eps0))
