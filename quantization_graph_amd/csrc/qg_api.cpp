@@ -89,7 +89,10 @@ Space *space_create(size_t dim, uint32_t seed, size_t centroid_count, size_t res
     Space *s = new Space();
     s->dim = dim;
     s->cd = round_up64_pow2(dim);
-    if (s->cd > 2048) { delete s; fail("code dimension %zu exceeds 2048", round_up64_pow2(dim)); return nullptr; }
+    // encode, query preparation, every estimator and the bridge take code
+    // dimensions up to 4096; the graph build and search, the learned rerank
+    // and the 1-bit sign scan up to 2048 (each refuses larger spaces)
+    if (s->cd > kMaxCodeDim) { delete s; fail("code dimension %zu exceeds %u", round_up64_pow2(dim), kMaxCodeDim); return nullptr; }
     s->seed = seed;
     s->block = block;
     s->block_count = (s->cd + block - 1) / block;
@@ -302,6 +305,7 @@ int qg_space_set_rotation_qr(void *space, const double *gaussian, int bf16_parts
     if (bf16_parts != 2 && bf16_parts != 3) return fail("bf16_parts must be 2 or 3");
     const size_t cd = s->cd;
     if (cd < 128) return fail("the QR rotation needs a code dimension of at least 128 (dimension above 64)");
+    if (cd > kMaxGraphCodeDim) return fail("the QR rotation takes code dimensions up to %u", kMaxGraphCodeDim);
     if (s->centroid_count != 1) return fail("the QR rotation holds one centroid");
     // A = G^T: row j of A is column j of G
     std::vector<double> at(cd * cd);
@@ -577,7 +581,7 @@ static int search_common(const void *space, const qg_index_desc *ix, const qg_qu
     if (s->centroid_count != 1) return fail("the batched search holds one query context: spaces of one centroid only");
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
     // the search stages MSB rows in 16-byte pieces: cd / 8 bytes must fill one
-    if (s->cd < 128) return fail("the batched search needs a code dimension of at least 128 (dimension above 64)");
+    if (s->cd > kMaxGraphCodeDim) return fail("the batched search takes code dimensions up to %u", kMaxGraphCodeDim);
     if (k == 0) return fail("k must be positive");
     if (edge_stride == 0 || edge_stride > 64) return fail("graph degree bound must be in [1, 64]");
     if (ix->n == 0 || ix->n > 0xFFFFFFFEull) return fail("index must hold 1 .. 2^32-2 points");
@@ -752,6 +756,7 @@ int qg_rerank_features(const void *space, const qg_index_desc *ix, const qg_quer
                        void *stream) {
     qg::RerankArgs a;
     if (rerank_args(static_cast<const Space *>(space), ix, qv, pool_ids, pool_dists, L, depth, &a)) return -1;
+    if (a.cd > kMaxGraphCodeDim) return fail("the rerank features take code dimensions up to %u", kMaxGraphCodeDim);
     if (!rstats || !features) return fail("null residual stats or feature output");
     if (reinterpret_cast<uintptr_t>(features) & 15u) return fail("feature output must be 16-byte aligned");
     a.rstats = reinterpret_cast<const float4 *>(rstats);
@@ -765,6 +770,7 @@ int qg_learned_rerank(const void *space, const qg_index_desc *ix, const qg_query
                       float *out_dists, uint32_t *out_depth, void *stream) {
     qg::RerankArgs a;
     if (rerank_args(static_cast<const Space *>(space), ix, qv, pool_ids, pool_dists, L, depth, &a)) return -1;
+    if (a.cd > kMaxGraphCodeDim) return fail("the workgroup rerank takes code dimensions up to %u", kMaxGraphCodeDim);
     if (codec != QG_CODEC_FULL && codec != QG_CODEC_INT8) return fail("unsupported query codec %d", codec);
     if (k == 0 || k > 128) return fail("k must be in [1, 128]");
     if (!out_ids || !out_dists) return fail("null rerank output");
@@ -799,6 +805,7 @@ static_assert(sizeof(qg_signscan_query) == sizeof(SsQuery), "qg_signscan_query l
 int qg_signscan_encode(const void *space, const float *x, uint64_t n, uint64_t *db, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !x || !db) return fail("null argument to qg_signscan_encode");
+    if (s->cd > kMaxGraphCodeDim) return fail("the 1-bit sign scan takes code dimensions up to %u", kMaxGraphCodeDim);
     if (s->rotation != 0) return fail("the 1-bit sign-plane scan uses the Walsh-Hadamard rotation, not a QR one");
     if (n % QG_SIGNSCAN_BATCH != 0) return fail("sign-scan databases hold whole blocks of 32 vectors (n = %llu)",
                                                 (unsigned long long)n);
@@ -812,6 +819,7 @@ int qg_signscan_prepare(const void *space, const float *queries, uint64_t nq, fl
                         qg_signscan_query *scalars, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !queries || !q || !bits || !scalars) return fail("null argument to qg_signscan_prepare");
+    if (s->cd > kMaxGraphCodeDim) return fail("the 1-bit sign scan takes code dimensions up to %u", kMaxGraphCodeDim);
     if (s->rotation != 0) return fail("the 1-bit sign-plane scan uses the Walsh-Hadamard rotation, not a QR one");
     if (nq > 0x7FFFFFFFull) return fail("nq too large for one launch");
     return check_hip(qg_launch_ss_prepare(queries, nq, (uint32_t)s->dim, (uint32_t)s->cd, s->seed == 0, s->d_signs,
@@ -823,6 +831,7 @@ int qg_signscan(const void *space, int codec, const uint64_t *db, uint64_t n, co
                 const qg_signscan_query *scalars, uint64_t nq, float *out, void *stream) {
     const Space *s = static_cast<const Space *>(space);
     if (!s || !db || !scalars || !out) return fail("null argument to qg_signscan");
+    if (s->cd > kMaxGraphCodeDim) return fail("the 1-bit sign scan takes code dimensions up to %u", kMaxGraphCodeDim);
     if (codec < QG_SIGNSCAN_FP32 || codec > QG_SIGNSCAN_INT8) return fail("unknown sign-scan codec %d", codec);
     if (codec == QG_SIGNSCAN_FP32 ? !q : !bits) return fail("null query rows for sign-scan codec %d", codec);
     if (n % QG_SIGNSCAN_BATCH != 0) return fail("sign-scan databases hold whole blocks of 32 vectors (n = %llu)",

@@ -1320,6 +1320,7 @@ int qg_build_graph(const void *space, const qg_index_desc *ix, uint32_t r, uint3
     const Space *s = static_cast<const Space *>(space);
     if (!s || !ix || !degrees || !edges || !workspace) return fail("null argument to qg_build_graph");
     if (s->centroid_count != 1) return fail("the device build has no cross-centroid terms: spaces of one centroid only");
+    if (s->cd > kMaxGraphCodeDim) return fail("the graph build takes code dimensions up to %u", kMaxGraphCodeDim);
     if (r == 0 || r > 64) return fail("graph degree R must be in [1, 64]");
     if (l_build == 0 || l_build > 1024) return fail("build list size must be in [1, 1024]");
     if (!(alpha >= 1.0f) || !std::isfinite(alpha)) return fail("Vamana alpha must be finite and >= 1.0");

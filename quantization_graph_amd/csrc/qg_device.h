@@ -259,14 +259,14 @@ __device__ __forceinline__ void stage_row_group(uint4 *dst, const uint8_t *src, 
 }
 
 // C1 on a staged code row: accumulator j sums dims 2(16c + j), 2(16c + j) + 1
-// (uniform trip count, unrolled so the LDS reads of 8 steps are in flight
-// together)
+// (uniform trip count; whole chunks of 8 steps, unrolled so that their LDS
+// reads are in flight together: the mask's asm barrier counts as a convergent
+// instruction, which rules out a remainder loop)
 __device__ __forceinline__ float asym_group(const float *q, const uint8_t *row_lds, uint32_t cd, float2 h,
                                            const QScalars &s, int j) {
     if (!long_scale_ok(h.y)) return h.x + s.query_norm_sqr;
     float sh = 0.0f, rm = 0.0f;
-#pragma unroll 8
-    for (uint32_t c = 0; c < cd / 32; ++c) {
+    auto step = [&](uint32_t c) {
         const uint32_t byte = row_lds[16 * c + j];
         const float2 qq = *(const float2 *)(q + 32 * c + 2 * j);
         const uint32_t lo = byte & 0x0Fu, hi = byte >> 4;
@@ -274,6 +274,15 @@ __device__ __forceinline__ float asym_group(const float *q, const uint8_t *row_l
         rm = fmaf((float)(hi & 7u), qq.y, rm);
         sh = sh + masked_q(byte, 3, qq.x);
         sh = sh + masked_q(byte, 7, qq.y);
+    };
+    const uint32_t n = cd / 32;
+    if ((n & 7u) == 0) {
+        for (uint32_t c0 = 0; c0 < n; c0 += 8) {
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) step(c0 + u);
+        }
+    } else {
+        for (uint32_t c = 0; c < n; ++c) step(c);
     }
     const float short_ip = group_reduce16(sh) - s.half_sum;
     return finish_long(h.x, h.y, s.query_norm_sqr, s.half_sum, short_ip, group_reduce16(rm));
@@ -304,6 +313,13 @@ __device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t
 #pragma unroll 8
     for (uint32_t i = 0; i < cd / 128; ++i) {
         const uint32_t d = (uint32_t)j + 16u * i;
+        const uint32_t w = row[d];
+        const uint2 x = xg[d];
+        acc = __builtin_amdgcn_sdot4((int)(w & 0x07070707u), (int)x.x, acc, false);
+        acc = __builtin_amdgcn_sdot4((int)((w >> 4) & 0x07070707u), (int)x.y, acc, false);
+    }
+    if ((cd & 64u) && j < 8) {  // the last 64 dims (code dimension 64): 8 dwords
+        const uint32_t d = 16u * (cd / 128) + (uint32_t)j;
         const uint32_t w = row[d];
         const uint2 x = xg[d];
         acc = __builtin_amdgcn_sdot4((int)(w & 0x07070707u), (int)x.x, acc, false);
@@ -345,8 +361,20 @@ __device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 
         QG_GATE_STEP(z, j2, q2.y)
         QG_GATE_STEP(w, j, q3.x)
         QG_GATE_STEP(w, j2, q3.y)
-#undef QG_GATE_STEP
     }
+    if (cd & 64u) {
+        // code dimension 64 (or an odd multiple): the last 64 dims, the low
+        // half of a 16-byte MSB step (the gate row's MSB span is zero-padded
+        // to 16 bytes)
+        const uint32_t u = cd / 128;
+        const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
+        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
+        QG_GATE_STEP(x, j, q0.x)
+        QG_GATE_STEP(x, j2, q0.y)
+        QG_GATE_STEP(y, j, q1.x)
+        QG_GATE_STEP(y, j2, q1.y)
+    }
+#undef QG_GATE_STEP
     acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;
 }
 

@@ -35,9 +35,11 @@ __global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
     if (kFixed) {
         live = fixed_item(a, qi, pos);
         item = qi * a.per_query + pos;
-        const uint32_t *src = (const uint32_t *)(a.qv.x8p + qi * a.cd);
-        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) xs[i] = src[i];
-        __syncthreads();
+        if (a.cd <= 2048) {
+            const uint32_t *src = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+            for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) xs[i] = src[i];
+            __syncthreads();
+        }
     } else {
         item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         live = item < a.n_pairs;
@@ -47,6 +49,7 @@ __global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
     const uint32_t id = a.ids[item];
     const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
     const uint32_t *xg = (const uint32_t *)(a.qv.x8p + qi * a.cd);
+    const bool staged = kFixed && a.cd <= 2048;  // above, the query is read from global memory
     int acc = 0;
     const uint32_t chunks = a.cd / 32;  // 16 code bytes = 32 dims per chunk
     for (uint32_t c = 0; c < chunks; ++c) {
@@ -55,8 +58,8 @@ __global__ void __launch_bounds__(kScanThreads) k_int8_full(ScanArgs a) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t d = 4 * c + k;  // code dword: dims 8d..8d+7
-            const uint32_t xl = kFixed ? xs[2 * d] : xg[2 * d];
-            const uint32_t xh = kFixed ? xs[2 * d + 1] : xg[2 * d + 1];
+            const uint32_t xl = staged ? xs[2 * d] : xg[2 * d];
+            const uint32_t xh = staged ? xs[2 * d + 1] : xg[2 * d + 1];
             acc = __builtin_amdgcn_sdot4((int)(ws[k] & 0x0F0F0F0Fu), (int)xl, acc, false);
             acc = __builtin_amdgcn_sdot4((int)((ws[k] >> 4) & 0x0F0F0F0Fu), (int)xh, acc, false);
         }
@@ -165,9 +168,11 @@ __global__ void __launch_bounds__(kScanThreads) k_asym(ScanArgs a) {
     if (kFixed) {
         live = fixed_item(a, qi, pos);
         item = qi * a.per_query + pos;
-        const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
-        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
-        __syncthreads();
+        if (a.cd <= 2048) {
+            const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
+            for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
+            __syncthreads();
+        }
     } else {
         item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         live = item < a.n_pairs;
@@ -178,7 +183,7 @@ __global__ void __launch_bounds__(kScanThreads) k_asym(ScanArgs a) {
     const QScalars s = a.qv.qs[qi];
     const float2 h = a.ix.hdr[id];
     if (!long_scale_ok(h.y)) { a.out_f[item] = h.x + s.query_norm_sqr; return; }
-    const float *qg = kFixed ? qsh : a.qv.q + qi * a.cd;
+    const float *qg = (kFixed && a.cd <= 2048) ? qsh : a.qv.q + qi * a.cd;  // above 2048 from global memory
     const uint4 *row = (const uint4 *)(a.ix.codes + (uint64_t)id * (a.cd / 2));
     float sh[16], rm[16];
 #pragma unroll
@@ -406,9 +411,11 @@ __global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
     if (kFixed) {
         live = fixed_item(a, qi, pos);
         item = qi * a.per_query + pos;
-        const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
-        for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
-        __syncthreads();
+        if (a.cd <= 2048) {
+            const float4 *src = (const float4 *)(a.qv.q + qi * a.cd);
+            for (uint32_t i = threadIdx.x; i < a.cd / 4; i += blockDim.x) ((float4 *)qsh)[i] = src[i];
+            __syncthreads();
+        }
     } else {
         item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
         live = item < a.n_pairs;
@@ -417,7 +424,7 @@ __global__ void __launch_bounds__(kScanThreads) k_residual(ScanArgs a) {
     if (!live) return;
     const uint32_t id = a.ids[item];
     const QScalars s = a.qv.qs[qi];
-    const float *qg = kFixed ? qsh : a.qv.q + qi * a.cd;
+    const float *qg = (kFixed && a.cd <= 2048) ? qsh : a.qv.q + qi * a.cd;  // above 2048 from global memory
     const uint32_t blocks = a.cd / 16;
     const uint16_t *sc = a.ix.rscale + (uint64_t)id * blocks;
     const uint2 *rc = (const uint2 *)(a.ix.rcodes + (uint64_t)id * (a.cd / 2));

@@ -11,6 +11,12 @@
 
 namespace qg {
 
+// Code dimensions a space may have (encode, query preparation, estimators,
+// bridge), and those the graph build and search, the learned rerank, the
+// QR rotation and the 1-bit sign scan take.
+constexpr uint32_t kMaxCodeDim = 4096;
+constexpr uint32_t kMaxGraphCodeDim = 2048;
+
 struct Space {
     size_t dim = 0, cd = 0, block = 16, block_count = 0;
     uint32_t seed = 100;

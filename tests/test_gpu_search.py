@@ -135,21 +135,6 @@ def test_search_rejects_bad_neighbour_past_the_start(gpu, graph_case, codec):
     sp.search(ix, qv, g["deg"].astype(np.int32), g["edges"].astype(np.int32), k=10, l_search=32)
 
 
-def test_search_rejects_small_code_dim(gpu):
-    """Dimension <= 64 gives code dimension 64, whose 8-byte MSB rows are
-    below the search's 16-byte staging unit: the call fails with an error."""
-    import quantization_graph_amd as qg
-    x, q = gaussian_mixture(300, 48, 4, 0.3, 2, nq=3)
-    sp = qg.RabitqSpace(48, 100)
-    assert sp.code_dim == 64
-    ix = sp.encode(gpu.from_numpy(x).cuda())
-    qv = sp.prepare(gpu.from_numpy(q).cuda(), "full")
-    deg = np.full(300, 4, np.int32)
-    edges = np.stack([(np.arange(300) + i) % 300 for i in range(1, 5)], 1).astype(np.int32)
-    with pytest.raises(qg._lib.QGError, match="code dimension"):
-        sp.search(ix, qv, deg, edges, k=3, l_search=16)
-
-
 PER_QUERY_L = [10, 16, 37, 64, 100, 200, 600, 1200, 2500, 4096, 23, 77]
 
 
