@@ -19,6 +19,14 @@ A second section repeats the fixed-candidate form on 1,000,000 x 960
 candidates, INT8 x 4-bit scoring, residual-4 rerank; it carries the
 code-scan kernel's roofline.
 
+Further sections: the plain method at its small size (100,000 x 960, 200
+queries x 1,000 candidates, FP32 scoring, residual-4 top-100, checked
+against the oracle and timed beside the reference CPU path); the batching
+form (20,000 queries with uneven search lists in one HIP-graph-captured
+persistent launch, with its roofline and the reference's per-query loop on
+the host); the 1-bit sign-plane scan. `host` records the host CPUs the
+baselines ran on.
+
 Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on
 one GPU.
 """
@@ -37,6 +45,32 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix-core peak
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host the CPU baselines ran on: logical CPUs this process may use
+    (sched_getaffinity, what `nproc` prints), the machine's count, the CPU
+    model, and the thread count of the all-cores legs. That count is the
+    host's CPU share as OMP_NUM_THREADS states it when set (a GPU host may
+    give one GPU's jobs a share of its cores), else every usable CPU."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = os.environ.get("OMP_NUM_THREADS")
+    threads = min(usable, int(share)) if share and share.isdigit() and int(share) > 0 else usable
+    return {"nproc": usable, "machine_cpus": os.cpu_count(), "model": model, "all_cores_threads": threads,
+            "omp_num_threads": share}
+
+
+def host_threads():
+    return host_cpu()["all_cores_threads"]
 
 
 def make_data(torch, n, dim, nq, clusters, std, seed, qstd):
@@ -73,66 +107,46 @@ def make_candidates(torch, top2000, per, g):
     return torch.gather(cand, 1, perm).contiguous()
 
 
-def _ref_query(ref, comp, res, ids, q, depth):
-    import numpy as np
-    zeros = np.zeros(ids.size, np.uint8)
-    pq = ref.prepare(q)
-    d = ref.distance_batch(pq, comp, ids, zeros)
-    order = np.lexsort((ids, d))[:depth]
-    top = ids[order]
-    iv = ref.residual_batch(pq, comp, res, top, d[order])
-    return top[np.lexsort((top, iv["estimate"]))[:10]]
-
-
 def cpu_reference_baseline(comp, res, cand, gt, q_host, center, dim, depth, sample, repeats, threads):
     """The reference's own fixed-candidate path (mode-0 FP32 4-bit scoring,
     top-`depth` by (distance, id), residual-4 rerank, top-10) through its C-ABI
-    bridge compiled from source (oracle/_ref). Protocol of the reference
-    runner: one warm-up pass, then `repeats` timed passes single-threaded,
-    median reported; plus one pass over all host cores (one bridge space per
-    thread; ctypes releases the GIL)."""
+    bridge compiled from source (oracle/_ref), driven per query from C
+    (oracle/ref_search.c, rs_fixed_range) as the reference's C++ harness
+    drives it. Protocol of the reference runner: one warm-up pass, then
+    `repeats` timed passes single-threaded, median reported; plus one pass
+    over `threads` host threads (contiguous query ranges; the bridge's
+    prepared query is thread-local)."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import Ref
-    if not Ref.available():
+    from oracle_bind import Ref, RefSearch
+    if not RefSearch.available():
         return None, None
     ref = Ref(dim, 100)
     ref.set_center(center)
     ref.set_codec(0)
+    rs = RefSearch(ref)
     ns = min(sample, cand.shape[0])
-    ids = cand[:ns].astype(np.uint32)
-    found = np.zeros((ns, 10), np.int64)
-    for i in range(min(ns, 64)):
-        _ref_query(ref, comp, res, ids[i], q_host[i], depth)
+    ids = np.ascontiguousarray(cand[:ns], np.uint32)
+    q = np.ascontiguousarray(q_host[:ns], np.float32)
+    found = np.zeros((ns, 10), np.uint32)
+    rs.fixed(q, comp, res, ids, depth=depth, q1=min(ns, 64), out=found)
     times = []
     for _ in range(repeats):
         t0 = time.perf_counter()
-        for i in range(ns):
-            found[i] = _ref_query(ref, comp, res, ids[i], q_host[i], depth)
+        rs.fixed(q, comp, res, ids, depth=depth, out=found)
         times.append(time.perf_counter() - t0)
     single = ns / float(np.median(times))
-
-    spaces = []
-    for _ in range(threads):
-        r = Ref(dim, 100)
-        r.set_center(center)
-        r.set_codec(0)
-        spaces.append(r)
-
-    def work(t):
-        for i in range(t, ns, threads):
-            _ref_query(spaces[t], comp, res, ids[i], q_host[i], depth)
-
+    multi_out = np.zeros((ns, 10), np.uint32)
+    bounds = [(ns * t // threads, ns * (t + 1) // threads) for t in range(threads)]
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        list(ex.map(work, range(threads)))
+        list(ex.map(lambda b: rs.fixed(q, comp, res, ids, depth=depth, q0=b[0], q1=b[1], out=multi_out), bounds))
     multi = ns / (time.perf_counter() - t0)
-    hits = [len(set(found[i]) & set(gt[i, :10])) for i in range(ns)]
+    hits = [len(set(found[i].tolist()) & set(gt[i, :10].tolist())) for i in range(ns)]
     return {"qps": single, "qps_all_cores": multi, "threads": threads, "queries": ns,
-            "seconds": float(np.sum(times)), "recall_at_10": float(np.mean(hits)) / 10.0}, found
-
-
+            "seconds": float(np.sum(times)), "recall_at_10": float(np.mean(hits)) / 10.0,
+            "all_cores_equal_single": bool(np.array_equal(found, multi_out))}, found
 
 
 def make_powerlaw(torch, n, dim, nq, seed, exponent=0.8, qnoise=0.1, chunk=131072):
@@ -212,6 +226,48 @@ def cpu_graph_baseline(sp_dim, center, comp, res, deg, edges, q_host, gt_host, L
     hits = [len(set(out[0][i].tolist()) & set(gt_host[i].tolist())) for i in range(nq)]
     return {"qps": single, "qps_all_cores": multi, "threads": threads, "queries": nq,
             "seconds": float(np.sum(times)), "recall_at_10": float(np.mean(hits)) / 10.0}, out[0]
+
+
+def cpu_batch_baseline(sp_dim, center, comp, res, deg, edges, q_host, lists, gt_host, threads):
+    """The reference's per-query loop (diskann_runner.rs:360-439: one search
+    per query with that query's own list size) over a sample of the batching
+    workload: its compiled bridge kernels driven by the C restatement of its
+    Rust search loop, FP32 codec, rerank 100. One warm-up pass over 16
+    queries, one timed single-thread pass, one pass over `threads` host
+    threads (contiguous query ranges; the visited tags persist across a
+    thread's queries, as the reference's scratch does)."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Ref, RefSearch
+    if not RefSearch.available():
+        return None, None
+    ref = Ref(sp_dim, 100)
+    ref.set_center(center)
+    ref.set_codec(0)
+    msb, fac = ref.sidecar(comp)
+    rs = RefSearch(ref)
+    nq = q_host.shape[0]
+    lists = np.ascontiguousarray(lists, np.uint32)
+
+    def run(a, b, out):
+        rs.search_lists(q_host, 0, comp, res, msb, fac, deg, edges, lists, k=10, q0=a, q1=b, out=out)
+
+    out = (np.zeros((nq, 10), np.uint32), np.zeros((nq, 10), np.float32))
+    run(0, min(nq, 16), out)
+    t0 = time.perf_counter()
+    run(0, nq, out)
+    single_s = time.perf_counter() - t0
+    multi_out = (np.zeros((nq, 10), np.uint32), np.zeros((nq, 10), np.float32))
+    bounds = [(nq * t // threads, nq * (t + 1) // threads) for t in range(threads)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda b: run(b[0], b[1], multi_out), bounds))
+    multi = nq / (time.perf_counter() - t0)
+    hits = [len(set(out[0][i].tolist()) & set(gt_host[i].tolist())) for i in range(nq)]
+    return {"qps": nq / single_s, "qps_all_cores": multi, "threads": threads, "queries": nq, "seconds": single_s,
+            "recall_at_10": float(np.mean(hits)) / 10.0,
+            "all_cores_equal_single": bool(np.array_equal(out[0], multi_out[0]))}, out[0]
 
 
 def cpu_build_baseline(dim, center, comp, sub, R, L, alpha, refine, cap):
@@ -313,7 +369,7 @@ def run_graph(args, torch, np, qg):
         del comp, res
         deg_h = deg.cpu().numpy().astype(np.uint32)
         edges_h = edges.cpu().numpy().astype(np.uint32)
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         base, ref_ids = cpu_graph_baseline(dim, center, comp_h, res_h, deg_h, edges_h, q.cpu().numpy(),
                                            gt.cpu().numpy(), head["L"], args.cpu_repeats, threads)
         if base is not None:
@@ -513,7 +569,8 @@ def run_batch(args, torch, np, qg):
     n, dim, nq = args.batch_n, 1024, args.batch_nq
     x, q = make_powerlaw(torch, n, dim, nq, 3)
     sp = qg.RabitqSpace(dim, 100)
-    sp.set_center(x.double().mean(0).float().cpu().numpy())
+    sp_center = x.double().mean(0).float().cpu().numpy()
+    sp.set_center(sp_center)
     index = sp.encode(x)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -584,6 +641,48 @@ def run_batch(args, torch, np, qg):
     qg.space.search_error(ws1)
     single_qps = ns / single_s
     log("batch: per-query launches: %.0f q/s over %d queries (%.2f ms each)" % (single_qps, ns, 1000 * single_s / ns))
+
+    # the search launch alone (clear + persistent kernel) on its stream, for
+    # the roofline: algorithmic bytes from the per-query counters
+    qv = sp.prepare(qx, "full")
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.batch_steps)]
+    for a, b in evs:
+        a.record(stream)
+        _, _, st = sp.search(index, qv, deg, edges, k=10, l_search=lmax, l_per_query=lqx, rerank=100,
+                             slots=best["slots"], waves=best["waves"], workspace=ws, check_graph=False)
+        b.record(stream)
+    torch.cuda.synchronize()
+    qg.space.search_error(ws)
+    kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    mean = st.double().mean(0).tolist()
+    reranked = float(torch.clamp(lqx.float(), max=100.0).mean().item())
+    bpq = search_bytes_per_query(sp.code_dim, mean[1], mean[2], reranked, False)
+    achieved = nq * bpq / (kernel_ms / 1000.0) / 1e9
+    log("batch: search launch %.2f ms, %.0f algorithmic bytes per query, %.1f GB/s (%.3f of HBM peak)"
+        % (kernel_ms, bpq, achieved, achieved / HBM_PEAK_GBS))
+
+    cpu, parity = None, None
+    if not args.no_cpu and args.batch_cpu_queries:
+        comp, res = index.to_records()
+        comp_h, res_h = comp.cpu().numpy(), res.cpu().numpy()
+        del comp, res
+        ncpu = min(args.batch_cpu_queries, nq)
+        base, ref_ids = cpu_batch_baseline(dim, sp_center, comp_h, res_h, deg.cpu().numpy().astype(np.uint32),
+                                           edges.cpu().numpy().astype(np.uint32), q[:ncpu].cpu().numpy(),
+                                           lq[:ncpu].cpu().numpy(), gt[:ncpu].cpu().numpy(), host_threads())
+        if base is not None:
+            parity = float((ids[:ncpu].cpu().numpy().astype(np.uint32) == ref_ids).all(1).mean())
+            cpu = {"value": base["qps"], "unit": "queries/s", "cores": 1, "kind": "reference",
+                   "sample": "the first %d queries of this workload with their own search lists, one search per "
+                             "query (the reference runner's per-query loop), warm-up + one timed pass, single "
+                             "thread; the reference bridge's compiled AVX-512 kernels driven by the C restatement "
+                             "of its Rust search loop, on the same GPU-built graph and records" % ncpu,
+                   "value_all_cores": base["qps_all_cores"], "all_cores_threads": base["threads"],
+                   "recall_at_10": base["recall_at_10"]}
+            log("batch: cpu reference %.1f q/s (1 thread), %.1f q/s (%d threads); GPU vs reference top-10 id "
+                "parity %.4f" % (base["qps"], base["qps_all_cores"], base["threads"], parity))
     v = lq.float()
     return {
         "config": {"workload": "batched search of %d queries with lognormal search lists (median 400, log-sigma "
@@ -598,6 +697,12 @@ def run_batch(args, torch, np, qg):
         "per_query_launch_qps": single_qps, "per_query_launch_sample": ns,
         "speedup_vs_per_query_launches": (nq / (ms / 1000.0)) / single_qps,
         "build_seconds": build_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "qg::k_search (persistent batched graph search, per-query lists)",
+                     "kernel_ms": kernel_ms, "bytes_per_unit": bpq, "unit_of_work": "query (mean)",
+                     "units_per_launch": nq},
+        "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity,
     }
 
 
@@ -877,6 +982,128 @@ def precision_form(args, torch, np, qg, x, q, cand, gt, center, depth, fht_ms, f
     return res
 
 
+def run_c1(args, torch, np, qg):
+    """The plain method at the small, CPU-runnable size (the fixed-candidate
+    harness of experiments/01_quantizer_fair/faiss_quantizer_smoke.cpp:
+    1312-1426): 100,000 x 960 vectors from a 64-component Gaussian mixture
+    (centers N(0, 1), per-cluster std 0.3, seed 0); 200 queries = base points
+    plus N(0, 0.05) noise; ExRaBitQ4 block16 encode (seed-100 FHT rotation,
+    MSE fp16 residual scales, mean center); per query 1,000 candidates = the
+    exact top-10 plus 990 drawn from exact ranks 11..2000, shuffled. One step
+    = prepare the 200 queries (FP32 codec) + score all 200,000 pairs with the
+    asymmetric FP32 x 4-bit estimator + residual-4 rerank of the top 100 +
+    top-10; Recall@10 against brute-force fp32 over all 100,000 vectors.
+
+    Beside the timing: the GPU records and first-stage estimates bit for bit
+    against the oracle on the whole workload (records on a sample), the
+    estimates' largest relative error against the FP64 form of the same
+    formula, and the reference bridge compiled from source timed on the host
+    with its top-10 ids compared with the GPU's."""
+    from quantization_graph_amd import fixed
+    n, dim, nq, per, depth = 100_000, 960, 200, 1000, 100
+    x, q, g = make_data(torch, n, dim, nq, 64, 0.3, 0, 0.05)
+    center = x.double().mean(0).float().cpu().numpy()
+    top2000 = exact_topk(torch, x, q, 2000)
+    gt = top2000[:, :10].contiguous()
+    cand = make_candidates(torch, top2000, per, g)
+    del top2000
+    sp = qg.RabitqSpace(dim, 100)
+    sp.set_center(center)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    index = sp.encode(x)
+    torch.cuda.synchronize()
+    encode_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+
+    def step(events=None):
+        qv = sp.prepare(q, "full")
+        if events is not None:
+            events[0].record(stream)
+        first = fixed.score(sp, index, qv, cand, codec="full")
+        if events is not None:
+            events[1].record(stream)
+        return fixed.rerank_topk(sp, index, qv, cand, first, k=10, depth=depth)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    steps = max(args.steps, 10)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ids, _ = step(ev[i])
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    kms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    recall = fixed.recall_at_k(ids, gt, 10)
+    cd = sp.code_dim
+    unit_bytes = cd // 2 + 8 + 4 + 4  # codes + {norm_sqr, long_scale} + id + estimate
+    units = nq * per
+    achieved = units * unit_bytes / (kms / 1000.0) / 1e9
+    log("C1 100k x 960: %.3f ms/step (%.0f q/s), scoring kernel %.4f ms (%.1f GB/s), recall@10 %.4f, encode "
+        "%.3f s" % (ms, nq / (ms / 1000.0), kms, achieved, recall, encode_s))
+
+    oracle_check, cpu, parity = None, None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Oracle
+        comp, res = index.to_records()
+        comp_h, res_h = comp.cpu().numpy(), res.cpu().numpy()
+        del comp, res
+        orc = Oracle(dim, 100)
+        orc.set_center(center)
+        sample = np.linspace(0, n - 1, 2000).astype(np.int64)
+        ocomp, ores = orc.split(orc.encode_full(x[torch.from_numpy(sample).cuda()].cpu().numpy()))
+        rec_equal = bool(np.array_equal(ocomp, comp_h[sample]) and np.array_equal(ores, res_h[sample]))
+        first = fixed.score(sp, index, sp.prepare(q, "full"), cand, codec="full").cpu().numpy()
+        cand_h, q_h = cand.cpu().numpy(), q.cpu().numpy()
+        est_equal, max_rel = 0, 0.0
+        for i in range(nq):
+            oq = orc.prepare(q_h[i], 0)
+            rows = comp_h[cand_h[i]]
+            est_equal += int(np.array_equal(first[i].view(np.uint32), orc.asym(oq, rows).view(np.uint32)))
+            exact = orc.asym_f64(oq, rows)
+            scale = np.abs(exact) + float(np.dot(q_h[i] - center, q_h[i] - center))
+            max_rel = max(max_rel, float((np.abs(first[i].astype(np.float64) - exact) / scale).max()))
+            orc.free_query(oq)
+        oracle_check = {"records_bit_exact": rec_equal, "records_sample": int(sample.size),
+                        "estimates_bit_exact": est_equal == nq, "estimate_pairs": nq * per,
+                        "max_rel_err_vs_fp64": max_rel,
+                        "rel_err_denominator": "|d_fp64| + |q - center|^2"}
+        log("C1 oracle: records bit-exact %s (%d sampled), all %d estimates bit-exact %s, max relative error vs "
+            "FP64 %.3g" % (rec_equal, sample.size, nq * per, est_equal == nq, max_rel))
+        threads = host_threads()
+        base, ref_found = cpu_reference_baseline(comp_h, res_h, cand_h, gt.cpu().numpy(), q_h, center, dim, depth,
+                                                 nq, args.cpu_repeats, threads)
+        if base is not None:
+            parity = float((ids.cpu().numpy() == ref_found).all(1).mean())
+            cpu = {"value": base["qps"], "unit": "queries/s", "cores": 1, "kind": "reference",
+                   "sample": "all %d queries x %d candidates of this workload, warm-up + %d repeats (median), "
+                             "mode-0 FP32 scoring + residual-4 top-%d rerank through the reference bridge compiled "
+                             "from source" % (nq, per, args.cpu_repeats, depth),
+                   "value_all_cores": base["qps_all_cores"], "all_cores_threads": base["threads"],
+                   "recall_at_10": base["recall_at_10"]}
+            log("C1 cpu reference: %.1f q/s (1 thread), %.1f q/s (%d threads); GPU vs reference top-10 id parity "
+                "%.4f" % (base["qps"], base["qps_all_cores"], base["threads"], parity))
+    del index, x, q, cand, gt
+    torch.cuda.empty_cache()
+    return {
+        "config": {"workload": "plain method, fixed candidates: %d x %d GMM(64, std 0.3, seed 0), %d queries x %d "
+                               "candidates (exact top-10 + ranks 11..2000), FP32 x 4-bit asymmetric scoring, "
+                               "residual-4 top-%d rerank" % (n, dim, nq, per, depth), "codec": "full"},
+        "qps": nq / (ms / 1000.0), "ms_per_step": ms, "steps": steps, "recall_at_10": recall,
+        "encode_seconds": encode_s,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "qg::k_asym (FP32 query x 4-bit code, fixed lists)", "kernel_ms": kms,
+                     "bytes_per_unit": unit_bytes, "units_per_launch": units,
+                     "note": "200,000 pairs (105 MB) per launch: the launch is too small to fill the chip's "
+                             "latency; the code-scan roofline is carried by the 1M x 960 fixed-candidate section"},
+        "oracle": oracle_check, "cpu_baseline": cpu, "gpu_vs_reference_id_parity": parity,
+    }
+
+
 def run_fixed(args, torch, np, qg):
     from quantization_graph_amd import fixed
     t_setup = time.perf_counter()
@@ -970,7 +1197,7 @@ def run_fixed(args, torch, np, qg):
         cand_h = cand[:ns].cpu().numpy()
         q_h = q[:ns].cpu().numpy()
         gt_h = gt[:ns].cpu().numpy()
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         base, ref_found = cpu_reference_baseline(comp_h, res_h, cand_h, gt_h, q_h, center, args.fixed_dim, depth,
                                                  ns, args.cpu_repeats, threads)
         if base is not None:
@@ -1145,7 +1372,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch", "signscan", "learned"), default="all")
+    ap.add_argument("--workload", choices=("all", "graph", "fixed", "batch", "signscan", "learned", "c1"), default="all")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--dim", type=int, default=1536)
     ap.add_argument("--nq", type=int, default=1000)
@@ -1173,6 +1400,8 @@ def main():
     ap.add_argument("--batch-nq", type=int, default=20_000)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-single", type=int, default=256, help="queries timed with one launch each")
+    ap.add_argument("--batch-cpu-queries", type=int, default=1000,
+                    help="queries of the batching workload searched by the reference CPU path")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching workload in --workload all")
     ap.add_argument("--scan-n", type=int, default=1_000_000, help="vectors of the 1-bit sign-plane scan workload")
     ap.add_argument("--scan-rounds", type=int, default=20, help="timed rounds per sign-scan configuration")
@@ -1210,6 +1439,16 @@ def main():
                                                  % (args.n, args.dim)},
                           "roofline": None, "cpu_baseline": None, "learned": r}))
         return
+    if args.workload == "c1":  # the small plain-method workload alone
+        r = run_c1(args, torch, np, qg)
+        print(json.dumps({"metric": "queries/s, fixed candidates, FP32 x 4-bit scoring + residual-4 top-100 rerank",
+                          "value": r["qps"], "unit": "queries/s", "n_gpus": 1, "steps": r["steps"],
+                          "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": r["config"], "recall_at_10": r["recall_at_10"], "roofline": r["roofline"],
+                          "cpu_baseline": r["cpu_baseline"], "host": host_cpu(), "c1": r}))
+        return
+    c1_res = run_c1(args, torch, np, qg) if args.workload == "all" else None
     graph = run_graph(args, torch, np, qg) if args.workload in ("all", "graph") else None
     if graph is not None and args.train_queries:
         graph["learned"] = run_learned_workload(args, torch, np, qg)
@@ -1266,7 +1505,8 @@ def main():
             "unit": "queries/s", "n_gpus": 1, "steps": args.batch_steps, "warmup": args.warmup,
             "ms_per_step": b["batched_ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": b["config"],
-            "recall_at_10": b["recall_at_10"], "roofline": None, "cpu_baseline": None, "batching": b,
+            "recall_at_10": b["recall_at_10"], "roofline": b["roofline"], "cpu_baseline": b["cpu_baseline"],
+            "batching": b,
         }
     else:
         f = fixed_res
@@ -1277,6 +1517,8 @@ def main():
             "recall_at_10": f["recall_at_10"], "roofline": f["roofline"], "cpu_baseline": f["cpu_baseline"],
             "fixed_candidates": f, "batching": batch_res,
         }
+    out["host"] = host_cpu()
+    out["c1"] = c1_res
     print(json.dumps(out), flush=True)
 
 

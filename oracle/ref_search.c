@@ -201,3 +201,66 @@ int rs_search_range(const void *space, const float *queries, size_t dim, size_t 
     free(tags);
     return rc;
 }
+
+/* rs_search_range with each query's own list size lists[qi] (the per-query
+ * loop of experiments/02_diskann_fair/src/diskann_runner.rs:360-439, whose
+ * visited-tag scratch persists across queries, ours_diskann.rs:1130-1151). */
+int rs_search_lists(const void *space, const float *queries, size_t dim, size_t q0, size_t q1, int codec,
+                    const uint8_t *comp, size_t cstride, const uint8_t *res, size_t rstride, const uint8_t *msb,
+                    size_t mstride, const void *factors, size_t fstride, const uint32_t *degrees,
+                    const uint32_t *edges, size_t estride, size_t n, size_t k, const uint32_t *lists, float eps0,
+                    size_t rerank, uint32_t *out_ids, float *out_d) {
+    uint32_t *tags = (uint32_t *)calloc(n + 1, sizeof(uint32_t));
+    if (!tags) return -1;
+    int rc = 0;
+    for (size_t qi = q0; qi < q1 && rc == 0; ++qi) {
+        for (size_t j = 0; j < k; ++j) { out_ids[qi * k + j] = 0xFFFFFFFFu; out_d[qi * k + j] = 0.0f; }
+        const long got = rs_search(space, queries + qi * dim, codec, comp, cstride, res, rstride, msb, mstride,
+                                   factors, fstride, degrees, edges, estride, n, k, lists[qi], eps0, rerank,
+                                   out_ids + qi * k, out_d + qi * k, tags, (uint32_t)(qi - q0 + 1));
+        if (got < 0) rc = -1;
+    }
+    free(tags);
+    return rc;
+}
+
+/* The fixed-candidate pipeline of the plain method (the harness of
+ * experiments/01_quantizer_fair/faiss_quantizer_smoke.cpp:1312-1426) through
+ * the bridge, for queries q0 .. q1-1: mode-0 distances of the query's `per`
+ * candidate ids (cands [nq][per]), the first `depth` by (distance, id),
+ * rabitq_residual_distance_batch on them, and the top `k` by (estimate, id)
+ * into out_ids [nq][k]. Returns 0, or -1 on a bridge failure. */
+int rs_fixed_range(const void *space, const float *queries, size_t dim, size_t q0, size_t q1, const uint8_t *comp,
+                   size_t cstride, const uint8_t *res, size_t rstride, const uint32_t *cands, size_t per, size_t depth,
+                   size_t k, uint32_t *out_ids) {
+    if (depth > per) depth = per;
+    if (k > depth) k = depth;
+    float *d = (float *)malloc((per + 1) * sizeof(float));
+    float *sips = (float *)calloc(per + 1, sizeof(float));
+    uint8_t *modes = (uint8_t *)calloc(per + 1, 1);
+    cand *pool = (cand *)malloc((depth + 1) * sizeof(cand));
+    uint32_t *rid = (uint32_t *)malloc((depth + 1) * sizeof(uint32_t));
+    float *rlong = (float *)malloc((depth + 1) * sizeof(float));
+    rs_interval *iv = (rs_interval *)malloc((depth + 1) * sizeof(rs_interval));
+    int rc = (d && sips && modes && pool && rid && rlong && iv) ? 0 : -1;
+    for (size_t qi = q0; qi < q1 && rc == 0; ++qi) {
+        const uint32_t *ids = cands + qi * per;
+        const void *pq = p_prepare(space, queries + qi * dim);
+        if (!pq || !p_distance(space, pq, ids, modes, per, comp, cstride, sips, d)) { rc = -1; break; }
+        size_t len = 0;
+        for (size_t i = 0; i < per; ++i) insert(pool, &len, depth, d[i], ids[i]);
+        for (size_t i = 0; i < len; ++i) { rid[i] = pool[i].id; rlong[i] = pool[i].d; }
+        if (len && !p_residual(space, pq, rid, len, comp, cstride, res, rstride, rlong, iv)) { rc = -1; break; }
+        for (size_t i = 1; i < len; ++i) {  /* sort by (estimate, id) */
+            const float e = iv[i].estimate;
+            const uint32_t id = rid[i];
+            size_t j = i;
+            while (j > 0 && less(e, id, iv[j - 1].estimate, rid[j - 1])) { iv[j] = iv[j - 1]; rid[j] = rid[j - 1]; --j; }
+            iv[j].estimate = e;
+            rid[j] = id;
+        }
+        for (size_t i = 0; i < k; ++i) out_ids[qi * k + i] = i < len ? rid[i] : 0xFFFFFFFFu;
+    }
+    free(d); free(sips); free(modes); free(pool); free(rid); free(rlong); free(iv);
+    return rc;
+}

@@ -766,6 +766,15 @@ class RefSearch:
                                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                           C.c_size_t, C.c_size_t, C.c_float, C.c_size_t, C.c_void_p, C.c_void_p]
+            L.rs_search_lists.restype = C.c_int
+            L.rs_search_lists.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int,
+                                          C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                          C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_void_p, C.c_float, C.c_size_t, C.c_void_p, C.c_void_p]
+            L.rs_fixed_range.restype = C.c_int
+            L.rs_fixed_range.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
+                                         C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                                         C.c_size_t, C.c_void_p]
             RefSearch._lib = L
         self.ref = ref
 
@@ -781,6 +790,40 @@ class RefSearch:
         rc = RefSearch._lib.rs_search_range(self.ref.h, _p(queries), dim, q0, q1, codec, _p(comp), comp.shape[1],
                                             _p(res), res.shape[1], _p(msb), msb.shape[1], _p(fac), fac.itemsize,
                                             _p(degrees), _p(edges), edges.shape[1], comp.shape[0], k, L,
+                                            C.c_float(eps0), rerank, _p(out[0]), _p(out[1]))
+        if rc != 0:
+            raise RuntimeError("reference search failed")
+        return out
+
+    def fixed(self, queries, comp, res, cand, depth=100, k=10, q0=0, q1=None, out=None):
+        """The plain fixed-candidate pipeline (mode-0 distances of each
+        query's candidates, top-`depth` by (distance, id), residual rerank,
+        top-k by (estimate, id)) for queries[q0:q1] into out [nq, k]; the
+        space's codec must be Full."""
+        queries = np.ascontiguousarray(queries, np.float32)
+        cand = np.ascontiguousarray(cand, np.uint32)
+        nq, dim = queries.shape
+        q1 = nq if q1 is None else q1
+        if out is None:
+            out = np.zeros((nq, k), np.uint32)
+        rc = RefSearch._lib.rs_fixed_range(self.ref.h, _p(queries), dim, q0, q1, _p(comp), comp.shape[1], _p(res),
+                                           res.shape[1], _p(cand), cand.shape[1], depth, k, _p(out))
+        if rc != 0:
+            raise RuntimeError("reference fixed-candidate pipeline failed")
+        return out
+
+    def search_lists(self, queries, codec, comp, res, msb, fac, degrees, edges, lists, k=10, eps0=1.9, rerank=100,
+                     q0=0, q1=None, out=None):
+        """search() with query i's own list size lists[i] (uint32 [nq])."""
+        queries = np.ascontiguousarray(queries, np.float32)
+        lists = np.ascontiguousarray(lists, np.uint32)
+        nq, dim = queries.shape
+        q1 = nq if q1 is None else q1
+        if out is None:
+            out = (np.zeros((nq, k), np.uint32), np.zeros((nq, k), np.float32))
+        rc = RefSearch._lib.rs_search_lists(self.ref.h, _p(queries), dim, q0, q1, codec, _p(comp), comp.shape[1],
+                                            _p(res), res.shape[1], _p(msb), msb.shape[1], _p(fac), fac.itemsize,
+                                            _p(degrees), _p(edges), edges.shape[1], comp.shape[0], k, _p(lists),
                                             C.c_float(eps0), rerank, _p(out[0]), _p(out[1]))
         if rc != 0:
             raise RuntimeError("reference search failed")

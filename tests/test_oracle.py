@@ -325,6 +325,43 @@ def test_search_matches_reference_kernels():
             assert np.array_equal(ids[i, :len(o_ids)], o_ids), (codec, i)
             assert np.array_equal(ds[i, :len(o_d)].view(np.uint32), o_d.view(np.uint32))
             o.free_query(oq)
+    # per-query list sizes in one call (the visited tags carry over between
+    # queries) equal one call per query at that query's list size
+    r.set_codec(0)
+    lists = np.array([16, 48, 200, 10, 64, 33, 48, 128], np.uint32)
+    ids_l, ds_l = rs.search_lists(q, 0, comp, res, msb, fac, deg, edges, lists, k=10, rerank=100)
+    for i in range(q.shape[0]):
+        one = rs.search(q, 0, comp, res, msb, fac, deg, edges, k=10, L=int(lists[i]), rerank=100, q0=i, q1=i + 1)
+        assert np.array_equal(ids_l[i], one[0][i]) and np.array_equal(ds_l[i].view(np.uint32),
+                                                                      one[1][i].view(np.uint32)), i
+
+
+@pytest.mark.skipif(not RefSearch.available(), reason="reference bridge not built (oracle/_ref)")
+def test_fixed_candidate_driver_matches_bridge_calls():
+    """The C driver of the plain fixed-candidate pipeline (the CPU baseline's
+    loop) against the same steps spelled out over the bridge's batch calls:
+    mode-0 distances, the first 100 by (distance, id), residual intervals of
+    those, top-10 by (estimate, id)."""
+    n, dim, per, depth = 800, 128, 300, 100
+    x, q = gaussian_mixture(n, dim, 8, 0.3, 23, nq=6)
+    c = x.mean(0).astype(np.float32)
+    o = Oracle(dim, 100)
+    o.set_center(c)
+    comp, res = o.split(o.encode_full(x))
+    r = Ref(dim, 100)
+    r.set_center(c)
+    r.set_codec(0)
+    rng = np.random.default_rng(5)
+    cand = np.stack([rng.choice(n, per, replace=False) for _ in range(q.shape[0])]).astype(np.uint32)
+    got = RefSearch(r).fixed(q, comp, res, cand, depth=depth)
+    for i in range(q.shape[0]):
+        pq = r.prepare(q[i])
+        d = r.distance_batch(pq, comp, cand[i], np.zeros(per, np.uint8))
+        order = np.lexsort((cand[i], d))[:depth]
+        top = cand[i][order]
+        est = r.residual_batch(pq, comp, res, top, d[order])["estimate"]
+        want = top[np.lexsort((top, est))[:10]]
+        assert np.array_equal(got[i], want), i
 
 
 @pytest.mark.parametrize("dim", [960, 1536])
