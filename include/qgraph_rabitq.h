@@ -243,10 +243,10 @@ int qg_rotate_rows(const void *space, const float *rows, uint64_t n, int subtrac
  * distance_computations = distances - re-check prunes, paper_checked =
  * prunes - re-check prunes + re-checks, paper_would_prune = prunes.
  * Workspace:
- * qg_search_workspace_bytes(n, slots) of device memory (per-slot visited
- * bitsets and the work-queue head); slots = concurrent queries.
- * max(l_search, k) <= 4096, max(rerank, k) <= 1024 and code_dim >= 128
- * (dimension above 64). */
+ * qg_search_workspace_bytes(n, slots) of device memory, 16-byte aligned
+ * (per-slot visited bitsets and the work-queue head); slots = concurrent
+ * queries. max(l_search, k) <= 4096, max(rerank, k) <= 1024 and
+ * code_dim <= 2048. */
 size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots);
 int qg_search(const void *space, const qg_index_desc *ix, const qg_query_desc *qv, int codec,
               const uint32_t *degrees, const uint32_t *edges, size_t edge_stride, uint32_t k, uint32_t l_search,

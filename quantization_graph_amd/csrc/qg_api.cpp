@@ -555,8 +555,12 @@ int qg_topk(const float *values, const uint32_t *ids, uint64_t nq, uint32_t per_
     return check_hip(qg_launch_topk(&a, (hipStream_t)stream), "topk launch");
 }
 
+// one visited bit per point (and one spare), each slot's bitset a whole
+// number of 16-byte words, which the search clears with 16-byte stores
+static uint64_t visited_words_of(uint64_t n) { return ((n + 32) / 32 + 3) & ~uint64_t(3); }
+
 size_t qg_search_workspace_bytes(uint64_t n, uint32_t slots) {
-    return (size_t)slots * ((n + 32) / 32) * 4 + 256;
+    return (size_t)slots * visited_words_of(n) * 4 + 256;
 }
 
 struct LearnedDepth {
@@ -593,6 +597,7 @@ static int search_common(const void *space, const qg_index_desc *ix, const qg_qu
     if (slots == 0) slots = 1;
     if (workspace_bytes < qg_search_workspace_bytes(ix->n, slots))
         return fail("search workspace too small: need %zu bytes", qg_search_workspace_bytes(ix->n, slots));
+    if (reinterpret_cast<uintptr_t>(workspace) % 16 != 0) return fail("search workspace must be 16-byte aligned");
     if (!qv->nq) return 0;
     SearchArgs a;
     a.ix = view_of(ix);
@@ -611,7 +616,7 @@ static int search_common(const void *space, const qg_index_desc *ix, const qg_qu
     a.counter = reinterpret_cast<uint32_t *>(w);
     a.err = reinterpret_cast<uint32_t *>(w + 4);
     a.visited = reinterpret_cast<uint32_t *>(w + 256);
-    a.visited_words = (ix->n + 32) / 32;
+    a.visited_words = visited_words_of(ix->n);
     a.out_ids = out_ids;
     a.out_d = out_dists;
     a.stats = out_stats;

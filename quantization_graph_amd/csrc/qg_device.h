@@ -333,34 +333,34 @@ __device__ __forceinline__ float rem_int8_group(const int8_t *x8p, const uint8_t
 
 // C4 on staged MSB rows, four candidates per 16-lane group in one pass (rows
 // r0..r3 in LDS): lane j's accumulator for each candidate sums dims 16m + j
-// in order, as the reference's 16-lane loop does; qp2 holds the query
-// pair-interleaved, qp2[16w + j] = (q[32w + j], q[32w + 16 + j]), the two
-// dims of MSB word w that lane j owns. The four chains are independent and
-// share the query loads. Returns the four lanes' partial sums; the caller
-// reduces (reduce16 order) and finishes each.
-__device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 *r0, const uint4 *r1,
+// in order, as the reference's 16-lane loop does; MSB word w holds dims
+// 32w .. 32w + 31, of which lane j owns 32w + j (bit j) and 32w + 16 + j
+// (bit 16 + j). The four chains are independent and share the query loads.
+// Returns the four lanes' partial sums; the caller reduces (reduce16 order)
+// and finishes each.
+__device__ __forceinline__ void gate_fp32_group4(const float *q, const uint4 *r0, const uint4 *r1,
                                                  const uint4 *r2, const uint4 *r3, uint32_t cd, int j,
                                                  float (&acc)[4]) {
     const int j2 = 16 + j;
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    const float *qj = q + j;
 #pragma unroll 1
     for (uint32_t u = 0; u < cd / 128; ++u) {
         const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
-        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
-        const float2 q2 = qp2[64 * u + 32 + j], q3 = qp2[64 * u + 48 + j];
+        const float *qu = qj + 128 * u;
         // the four chains advance in turn (masked_q's register barrier keeps
         // source order, so the interleaving has to be written out)
 #define QG_GATE_STEP(W, JJ, QV)                                                    \
         a0 = a0 + masked_q(m0.W, JJ, QV); a1 = a1 + masked_q(m1.W, JJ, QV);        \
         a2 = a2 + masked_q(m2.W, JJ, QV); a3 = a3 + masked_q(m3.W, JJ, QV);
-        QG_GATE_STEP(x, j, q0.x)
-        QG_GATE_STEP(x, j2, q0.y)
-        QG_GATE_STEP(y, j, q1.x)
-        QG_GATE_STEP(y, j2, q1.y)
-        QG_GATE_STEP(z, j, q2.x)
-        QG_GATE_STEP(z, j2, q2.y)
-        QG_GATE_STEP(w, j, q3.x)
-        QG_GATE_STEP(w, j2, q3.y)
+        QG_GATE_STEP(x, j, qu[0])
+        QG_GATE_STEP(x, j2, qu[16])
+        QG_GATE_STEP(y, j, qu[32])
+        QG_GATE_STEP(y, j2, qu[48])
+        QG_GATE_STEP(z, j, qu[64])
+        QG_GATE_STEP(z, j2, qu[80])
+        QG_GATE_STEP(w, j, qu[96])
+        QG_GATE_STEP(w, j2, qu[112])
     }
     if (cd & 64u) {
         // code dimension 64 (or an odd multiple): the last 64 dims, the low
@@ -368,14 +368,51 @@ __device__ __forceinline__ void gate_fp32_group4(const float2 *qp2, const uint4 
         // to 16 bytes)
         const uint32_t u = cd / 128;
         const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
-        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
-        QG_GATE_STEP(x, j, q0.x)
-        QG_GATE_STEP(x, j2, q0.y)
-        QG_GATE_STEP(y, j, q1.x)
-        QG_GATE_STEP(y, j2, q1.y)
+        const float *qu = qj + 128 * u;
+        QG_GATE_STEP(x, j, qu[0])
+        QG_GATE_STEP(x, j2, qu[16])
+        QG_GATE_STEP(y, j, qu[32])
+        QG_GATE_STEP(y, j2, qu[48])
     }
 #undef QG_GATE_STEP
     acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;
+}
+
+// gate_fp32_group4 for one candidate (the same chain, alone)
+__device__ __forceinline__ float gate_fp32_group1(const float *q, const uint4 *r0, uint32_t cd, int j) {
+    const int j2 = 16 + j;
+    float a0 = 0.0f;
+    const float *qj = q + j;
+    auto step = [&](uint32_t u) {
+        const uint4 m0 = r0[u];
+        const float *qu = qj + 128 * u;
+        a0 = a0 + masked_q(m0.x, j, qu[0]);
+        a0 = a0 + masked_q(m0.x, j2, qu[16]);
+        a0 = a0 + masked_q(m0.y, j, qu[32]);
+        a0 = a0 + masked_q(m0.y, j2, qu[48]);
+        a0 = a0 + masked_q(m0.z, j, qu[64]);
+        a0 = a0 + masked_q(m0.z, j2, qu[80]);
+        a0 = a0 + masked_q(m0.w, j, qu[96]);
+        a0 = a0 + masked_q(m0.w, j2, qu[112]);
+    };
+    // two MSB steps per trip (their loads issue together), an odd last one alone
+    const uint32_t n = cd / 128;
+#pragma unroll 1
+    for (uint32_t u = 0; u + 1 < n; u += 2) {
+        step(u);
+        step(u + 1);
+    }
+    if (n & 1u) step(n - 1);
+    if (cd & 64u) {
+        const uint32_t u = cd / 128;
+        const uint4 m0 = r0[u];
+        const float *qu = qj + 128 * u;
+        a0 = a0 + masked_q(m0.x, j, qu[0]);
+        a0 = a0 + masked_q(m0.x, j2, qu[16]);
+        a0 = a0 + masked_q(m0.y, j, qu[32]);
+        a0 = a0 + masked_q(m0.y, j2, qu[48]);
+    }
+    return a0;
 }
 
 // C5 on a staged MSB row: 4-dim groups g4 = j + 16i (uniform trip count)

@@ -81,6 +81,157 @@ __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's pieces are in LDS
 }
 
+// ---------------------------------------------------------------- gate filter
+// The FP32 gate's float sum S_f (the query summed over a candidate's MSB
+// bits: 16 lane chains of cd/16 adds, then the reduce16 tree) lies within w
+// of sc * X, where X = sum m_i x_i is an exact integer sum over the
+// fixed-point query x_i = rint(q_i / sc), sc = max|q| / 16256:
+//   |S_f - sc X| <= sum |q_i - sc x_i| + 134 u sum |q_i| = w
+// (u = 2^-24; each term passes at most cd/16 + 4 <= 132 roundings). A wave
+// forms X for 16 candidates at once on the int8 matrix cores (x = 128 xh +
+// xl, the halves as B columns 0 and 1, the MSB bits as 0/1 bytes in A).
+// finish_gate's lower bound is monotone in short_ip = S_f - half_sum, so its
+// values at the interval's ends bracket the exact one: when both are finite
+// and above the pool's back (the pool full, the factors valid), the
+// candidate is pruned whatever S_f is, and its float sum is not needed.
+// Every other candidate gets the exact sum, so decisions, estimates and
+// counters are those of the plain gate.
+constexpr double kFilterX = 16256.0;  // |x| <= 127 * 128
+typedef int v4i_gf __attribute__((ext_vector_type(4)));
+
+// index of set bit number p (from 0) of m, for p < popcount(m)
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t p) {
+    uint32_t base = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = m & ((1ull << w) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(low);
+        const bool up = p >= c;
+        p = up ? p - c : p;
+        m = up ? (m >> w) : low;
+        base += up ? w : 0u;
+    }
+    return base;
+}
+
+// The query's fixed-point form (xh, xl in LDS) and per-wave partial sums of
+// |q_i - sc x_i| and |q_i|; s_qmx holds each wave's largest |q| bits. Every
+// thread of the group calls it after the barrier that publishes q and s_qmx.
+template <int W>
+__device__ void gate_filter_query(const float *q, int8_t *xh, int8_t *xl, uint32_t cd, const uint32_t *s_qmx,
+                                  double *s_qe, double *s_qa, uint32_t t) {
+    constexpr uint32_t T = 64u * W;
+    uint32_t mb = 0u;
+    for (int k = 0; k < W; ++k) mb = max(mb, s_qmx[k]);
+    const bool ok = mb > 0u && mb < 0x7F800000u;
+    const double sc = ok ? (double)__uint_as_float(mb) / kFilterX : 1.0;
+    double e = 0.0, qa = 0.0;
+    for (uint32_t i = t; i < cd; i += T) {
+        const double v = (double)q[i];
+        const double xr = ok ? rint(v / sc) : 0.0;
+        const int x = (int)xr;
+        const int h = (x + 64) >> 7;  // floor((x + 64) / 128): xh in [-127, 127], xl in [-64, 63]
+        xh[i] = (int8_t)h;
+        xl[i] = (int8_t)(x - 128 * h);
+        e += fabs(v - sc * xr);
+        qa += fabs(v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        qa += __shfl_xor(qa, off);
+    }
+    if ((t & 63u) == 0u) {
+        s_qe[t >> 6] = e;
+        s_qa[t >> 6] = qa;
+    }
+}
+
+// The scale and the half-width w from the partials (one thread, after the
+// barrier that publishes them); the inflations cover the double sums' own
+// rounding.
+template <int W>
+__device__ void gate_filter_constants(const uint32_t *s_qmx, const double *s_qe, const double *s_qa, double *fsc,
+                                      double *fw, uint32_t *fok) {
+    uint32_t mb = 0u;
+    double e = 0.0, qa = 0.0;
+    for (int k = 0; k < W; ++k) {
+        mb = max(mb, s_qmx[k]);
+        e += s_qe[k];
+        qa += s_qa[k];
+    }
+    const double mx = (double)__uint_as_float(mb);
+    const bool ok = mb > 0u && mb < 0x7F800000u && __builtin_isfinite(e) && __builtin_isfinite(qa);
+    *fsc = ok ? mx / kFilterX : 1.0;
+    *fw = e * (1.0 + 0x1p-30) + qa * (1.0 + 0x1p-30) * (134.0 * 0x1p-24) + mx * 0x1p-38;
+    *fok = ok ? 1u : 0u;
+}
+
+// The filter over the hop's nf fresh neighbours (rows staged, frow[c] their
+// stage rows): wave wv takes candidates 16 wv .. 16 wv + 15, then every
+// 16 W-th tile. Lane (g, r) supplies dims 64 k + 16 g .. + 15 of candidate
+// c0 + r (A) and of xh (r = 0) or xl (r = 1) (B); accumulator j holds
+// candidate c0 + 4 g + j against column r. Writes f_sure[c] for every c <
+// nf, and for a sure prune f_lb[c] = +inf, f_valid[c] = 1 (the prune test
+// then removes it).
+template <int W>
+__device__ __forceinline__ void gate_filter_tiles(const uint4 *stage, const uint8_t *frow, uint32_t gch,
+                                                  uint32_t cd, uint32_t nf, const int8_t *xh, const int8_t *xl,
+                                                  double sc, double w, float back, const QScalars &s, float eps0,
+                                                  uint8_t *f_sure, float *f_lb, uint8_t *f_valid, uint32_t wv,
+                                                  uint32_t lane) {
+    const uint32_t g = lane >> 4, r = lane & 15u;
+    const uint32_t nk = cd / 64;
+    for (uint32_t c0 = 16u * wv; c0 < nf; c0 += 16u * W) {
+        const uint32_t ca = c0 + r < nf ? c0 + r : c0;
+        const uint16_t *abits = (const uint16_t *)((const uint8_t *)(stage + (uint32_t)frow[ca] * gch) + 2u * g);
+        const int8_t *bsrc = (r == 0u ? xh : xl) + 16u * g;
+        v4i_gf acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, bq = {0, 0, 0, 0};
+        auto step = [&](uint32_t kb, v4i_gf &acc) {
+            const uint32_t bits = abits[4u * kb];
+            v4i_gf av;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) av[i] = (int)nibble_bytes(__builtin_amdgcn_ubfe(bits, 4u * i, 4u));
+            if (r < 2u) bq = *(const v4i_gf *)(bsrc + 64u * kb);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bq, acc, 0, 0, 0);
+        };
+#pragma unroll 1
+        for (uint32_t kb = 0; kb + 1 < nk; kb += 2) {
+            step(kb, acc0);
+            step(kb + 1, acc1);
+        }
+        if (nk & 1u) step(nk - 1, acc0);
+        const v4i_gf acc = acc0 + acc1;
+        int hx = 0, lx = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int hj = __shfl(acc[j], (int)(lane & ~15u));
+            const int lj = __shfl(acc[j], (int)((lane & ~15u) | 1u));
+            if (r == (uint32_t)j) {
+                hx = hj;
+                lx = lj;
+            }
+        }
+        const uint32_t c = c0 + 4u * g + r;
+        if (r < 4u && c < nf) {
+            const uint8_t *row = (const uint8_t *)(stage + (uint32_t)frow[c] * gch);
+            const float4 f = gate_factors(row, cd);
+            const uint8_t fv = gate_valid(row, cd);
+            const double A = sc * (double)(128 * hx + lx);
+            const double slack = w + fabs(A) * 0x1p-50;
+            const float lo = __double2float_rd(A - slack), hi = __double2float_ru(A + slack);
+            const PaperEstimate e0 = finish_gate(lo - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
+            const PaperEstimate e1 = finish_gate(hi - s.half_sum, f, fv, s.query_norm, s.query_norm_sqr, eps0);
+            const bool sure = fv != 0 && __builtin_isfinite(f.z) && e0.valid && e1.valid &&
+                              fminf(e0.lower_bound, e1.lower_bound) > back;
+            f_sure[c] = sure ? 1 : 0;
+            if (sure) {
+                f_lb[c] = __builtin_huge_valf();
+                f_valid[c] = 1;
+            }
+        }
+    }
+}
+
 // The learned depth after a query's traversal (the LEARNED search kernel):
 // the feature rows of the rc pool entries, the MLP's predicted errors
 // (16-row tiles across the waves) and the depth rule, all in the stage,
@@ -141,18 +292,27 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd;
-    // query: FP32 residual, then one codec-specific region of 4 cd bytes:
-    // the pair-interleaved FP32 query of the gate, or the INT8 codes
+    // query: FP32 residual, then one codec-specific region of 2 cd bytes:
+    // the gate filter's fixed-point query (FP32 codec) or the INT8 codes
     float *q = (float *)smem;                                   // cd
-    float2 *qp2 = (float2 *)(smem + 4 * cd);                    // cd / 2 (FP32 codec)
+    int8_t *xh = (int8_t *)(smem + 4 * cd);                     // cd     (FP32 codec)
+    int8_t *xl = (int8_t *)(smem + 5 * cd);                     // cd     (FP32 codec)
     int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd     (INT8 codec)
     int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd     (INT8 codec)
-    float *pd = (float *)(smem + 8 * cd);                       // a.L
-    uint32_t *pid = (uint32_t *)(smem + 8 * cd + 4 * a.L);      // a.L
-    uint8_t *pexp = (uint8_t *)(smem + 8 * cd + 8 * a.L);       // a.L (+pad)
+    float *pd = (float *)(smem + 6 * cd);                       // a.L
+    uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * a.L);      // a.L
+    uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * a.L);       // a.L (+pad)
     // gate rows / code rows of a batch; rerank block sums after the traversal
     __shared__ uint4 stage[kStageU4];
     __shared__ uint32_t fresh[64], surv[64], sid[64];
+    __shared__ uint8_t frow[64];  // stage row of fresh neighbour c
+    __shared__ uint8_t f_sure[64];  // the filter proved fresh neighbour c pruned
+    // the gate filter's query constants: per-wave partials, then the scale
+    // and the interval half-width (FP32 codec)
+    __shared__ uint32_t s_qmx[W];
+    __shared__ double s_qe[W], s_qa[W];
+    __shared__ double s_fsc, s_fw;
+    __shared__ uint32_t s_fok;
     __shared__ float f_lb[64], f_sip[64], s_d[64];
     __shared__ float2 f_h[64];  // header of fresh neighbour c (from its gate row)
     __shared__ uint8_t f_valid[64];
@@ -184,25 +344,31 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             const uint32_t l = a.lq[qi];
             L = l < a.k ? a.k : (l > a.L ? a.L : l);
         }
-        // stage the prepared query, clear this slot's visited bits
-        for (uint32_t i = t; i < cd / 4; i += T)
-            ((float4 *)q)[i] = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
+        // stage the prepared query (and its largest |q| bits), clear this
+        // slot's visited bits
+        uint32_t qmx = 0u;
+        for (uint32_t i = t; i < cd / 4; i += T) {
+            const float4 v = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
+            ((float4 *)q)[i] = v;
+            const uint32_t m0 = __float_as_uint(v.x) & 0x7FFFFFFFu, m1 = __float_as_uint(v.y) & 0x7FFFFFFFu;
+            const uint32_t m2 = __float_as_uint(v.z) & 0x7FFFFFFFu, m3 = __float_as_uint(v.w) & 0x7FFFFFFFu;
+            qmx = max(qmx, max(max(m0, m1), max(m2, m3)));
+        }
         if (a.codec == 3) {
             for (uint32_t i = t; i < cd / 4; i += T) {
                 ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
                 ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
             }
         } else {
-            for (uint32_t i = t; i < cd / 2; i += T) {
-                const float *qq = a.qv.q + (uint64_t)qi * cd + 32 * (i >> 4) + (i & 15u);
-                qp2[i] = make_float2(qq[0], qq[16]);
-            }
+            for (int off = 32; off > 0; off >>= 1) qmx = max(qmx, (uint32_t)__shfl_xor((int)qmx, off));
+            if (lane == 0) s_qmx[wv] = qmx;
         }
-        for (uint32_t i = t; i < a.visited_words; i += T) visited[i] = 0u;
+        for (uint32_t i = t; i < a.visited_words / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
         if (t == 0) s_sp_node = 0xFFFFFFFFu;
         const QScalars s = a.qv.qs[qi];
         __threadfence();
         __syncthreads();
+        if (a.codec != 3) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
 
         uint32_t len = 0;
         uint64_t hops = 0, gates = 0, dists = 1, pruned = 0, rchecked = 0, rpruned = 0;
@@ -224,6 +390,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         }
         len = (a.n > 0 && L > 0) ? 1 : 0;
         __syncthreads();
+        if (a.codec != 3 && t == 0) gate_filter_constants<W>(s_qmx, s_qe, s_qa, &s_fsc, &s_fw, &s_fok);
 
         for (;;) {
             // first unexpanded entry (every wave reads the same pool)
@@ -276,7 +443,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 }
                 const uint64_t fm = __ballot(is_fresh);
                 const bool hit = s_sp_node == node;  // fresh list, factors and MSB rows already staged
-                if (!hit && is_fresh) fresh[(uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = nb;
+                if (!hit && is_fresh) {
+                    const uint32_t r = (uint32_t)__popcll(fm & ((1ull << lane) - 1ull));
+                    fresh[r] = nb;
+                    frow[r] = (uint8_t)r;
+                }
                 if (lane == 0) { s_nf = (uint32_t)__popcll(fm); s_hit = hit ? 1u : 0u; }
                 __builtin_amdgcn_s_setprio(0);
             }
@@ -285,22 +456,56 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             if (nf == 0) continue;
             gates += nf;
             // gate: the fresh neighbours' gate rows (MSB plane, factors, header,
-            // validity) in one round trip (already done during the last
-            // insertion when the frontier was predicted)
+            // validity) in one round trip; when the frontier was predicted,
+            // every listed neighbour's row was staged during the last
+            // insertion and frow[] places the fresh ones
             if (!s_hit) {
                 stage_rows<W>(stage, fresh, nf, gch, gmagic, (const uint4 *)a.ix.gate, wv, lane);
                 __syncthreads();
             }
-            // each group scores candidates grp, grp + G, grp + 2G, grp + 3G (FP32:
-            // in one pass, four independent chains; slots past nf re-read a
-            // live row and are dropped)
-            for (uint32_t c0 = grp; c0 < nf; c0 += 4u * G) {
+            // FP32 with the pool full: the filter first (its sure prunes need
+            // no float sum), then the exact sums of the candidates it left open
+            const bool full_h = len >= L;
+            const bool filter = a.codec != 3 && full_h && s_fok != 0u;
+            uint32_t nx = nf;        // candidates scored exactly
+            uint64_t open = ~0ull;   // which ones (bit c: fresh neighbour c)
+            if (filter) {
+                gate_filter_tiles<W>(stage, frow, gch, cd, nf, xh, xl, s_fsc, s_fw, pd[len - 1u], s, a.eps0, f_sure,
+                                     f_lb, f_valid, wv, lane);
+                __syncthreads();
+                open = __ballot(lane < nf && f_sure[lane] == 0);
+                nx = (uint32_t)__popcll(open);
+            }
+            if (filter && nx <= G) {
+                // one open candidate per group: a single chain
+                if (grp < nx) {
+                    const uint32_t c = nth_set_bit(open, grp);
+                    const uint8_t *row = (const uint8_t *)(stage + (uint32_t)frow[c] * gch);
+                    const float acc1 = gate_fp32_group1(q, (const uint4 *)row, cd, j16);
+                    const PaperEstimate e = finish_gate(group_reduce16(acc1) - s.half_sum, gate_factors(row, cd),
+                                                        gate_valid(row, cd), s.query_norm, s.query_norm_sqr, a.eps0);
+                    if (j16 == 0) {
+                        f_lb[c] = e.lower_bound;
+                        f_sip[c] = e.short_ip;
+                        f_valid[c] = e.valid;
+                        f_h[c] = gate_hdr(row, cd);
+                    }
+                }
+                nx = 0;  // done
+            }
+            // each group scores open candidates grp, grp + G, grp + 2G, grp + 3G
+            // (FP32: in one pass, four independent chains; slots past the
+            // count re-read a live row and are dropped)
+            for (uint32_t p0 = grp; p0 < nx; p0 += 4u * G) {
                 uint32_t cc[4];
                 const uint4 *rr[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    cc[k] = c0 + (uint32_t)k * G;
-                    rr[k] = stage + (cc[k] < nf ? cc[k] : c0) * gch;
+                    const uint32_t p = p0 + (uint32_t)k * G;
+                    const uint32_t pc = p < nx ? p : p0;
+                    const uint32_t c = filter ? nth_set_bit(open, pc) : pc;
+                    cc[k] = p < nx ? c : nf;
+                    rr[k] = stage + (uint32_t)frow[c] * gch;
                 }
                 if (a.codec == 3) {
                     // integer sums: one candidate at a time, unrolled
@@ -318,7 +523,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     }
                 } else {
                     float acc[4];
-                    gate_fp32_group4(qp2, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
+                    gate_fp32_group4(q, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (cc[k] >= nf) continue;
@@ -434,18 +639,25 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                             }
                         }
                     }
+                    // the next hop's gate rows: the row of every listed
+                    // neighbour, in list order, in flight together with the
+                    // visited words (the stage is idle during the insertion and
+                    // the filter above is done with it; an out-of-range id
+                    // loads row 0 and is flagged below)
+                    sp_nb[lane] = nb2 < a.n ? nb2 : 0u;
+                    wave_sync_lds();
+                    stage_rows<1>(stage, sp_nb, deg2 < 64u ? deg2 : 64u, gch, gmagic, (const uint4 *)a.ix.gate, 0u,
+                                  lane);
                     const bool bad2 = first2 && nb2 >= a.n;
                     const bool fresh2 = first2 && !bad2 && (w2 & (1u << (nb2 & 31u))) == 0u;
                     const uint64_t fm2 = __ballot(fresh2);
                     const uint64_t bm2 = __ballot(bad2);
-                    sp_nb[lane] = nb2;
-                    // the next hop's gate inputs: fresh list and gate rows (fresh[]
-                    // and the stage are idle during the insertion; the filter
-                    // above is done with the stage)
-                    if (fresh2) fresh[(uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull))] = nb2;
-                    wave_sync_lds();
-                    stage_rows<1>(stage, fresh, (uint32_t)__popcll(fm2), gch, gmagic, (const uint4 *)a.ix.gate, 0u,
-                                  lane);
+                    // the fresh list and where each fresh neighbour's row sits
+                    if (fresh2) {
+                        const uint32_t r = (uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull));
+                        fresh[r] = nb2;
+                        frow[r] = (uint8_t)lane;
+                    }
                     if (lane == 0) { s_sp_node = nxt; s_sp_fresh = fm2; s_sp_bad = bm2 != 0ull ? 1u : 0u; }
                 } else if (lane == 0) {
                     s_sp_node = 0xFFFFFFFFu;
@@ -684,7 +896,7 @@ static int launch_search_w(const qg::SearchArgs *a, uint32_t slots, size_t smem,
     return (int)hipGetLastError();
 }
 
-extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)cd * 8 + (size_t)L * 9 + 16; }
+extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)cd * 6 + (size_t)L * 9 + 16; }
 
 extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream) {
     if (a->qv.nq == 0) return 0;
