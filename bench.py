@@ -149,6 +149,33 @@ def cpu_reference_baseline(comp, res, cand, gt, q_host, center, dim, depth, samp
             "all_cores_equal_single": bool(np.array_equal(found, multi_out))}, found
 
 
+def timed_encode(torch, sp, x, batch):
+    """Encode x (one warm-up launch on 256 rows first) with the wall clock
+    and HIP events on the current stream around the whole encode. Returns
+    (index, report): algorithmic bytes per vector are the fp32 input read
+    plus the reference's compact and residual records written (16 + cd/2 + 1
+    and 8 + cd/8 + cd/2 bytes: 5,017 B at D=960, 8,473 B at D=1536)."""
+    n, dim = x.shape
+    cd = sp.code_dim
+    sp.encode(x[:256].contiguous())
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    index = sp.encode(x, batch=batch)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    per = 4 * dim + (16 + cd // 2 + 1) + (8 + cd // 8 + cd // 2)
+    rep = {"vectors": n, "dim": dim, "seconds": wall, "gpu_ms": gpu_ms, "bytes_per_vector": per,
+           "gb_per_s": n * per / (gpu_ms / 1e3) / 1e9, "mvec_per_s": n / (gpu_ms / 1e3) / 1e6,
+           "hbm_frac": n * per / (gpu_ms / 1e3) / 1e9 / HBM_PEAK_GBS, "batch": batch}
+    log("encode %d x %d: %.1f ms on the GPU (%.2f s wall), %.2f M vectors/s, %.1f GB/s algorithmic (%d B/vector)"
+        % (n, dim, gpu_ms, wall, rep["mvec_per_s"], rep["gb_per_s"], per))
+    return index, rep
+
+
 def make_powerlaw(torch, n, dim, nq, seed, exponent=0.8, qnoise=0.1, chunk=131072):
     """Power-law spectrum vectors: z ~ N(0, diag(i^-exponent)), rotated by a
     seeded orthogonal basis (QR of a Gaussian matrix), L2-normalised; queries
@@ -295,12 +322,8 @@ def run_graph(args, torch, np, qg):
     log("graph data ready", tuple(x.shape), tuple(q.shape))
     sp = qg.RabitqSpace(dim, 100)
     sp.set_center(center)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    index = sp.encode(x, batch=131072)
-    torch.cuda.synchronize()
-    encode_s = time.perf_counter() - t0
-    log("encoded %d x %d in %.2f s" % (n, dim, encode_s))
+    index, enc = timed_encode(torch, sp, x, 131072)
+    encode_s = enc["seconds"]
     t0 = time.perf_counter()
     deg, edges, evals = sp.build_graph(index, r=args.R, l_build=args.l_build, alpha=1.2, batch=4096,
                                        refine_passes=args.refine, prune_cap=args.prune_cap)
@@ -392,7 +415,7 @@ def run_graph(args, torch, np, qg):
     del index, deg, edges
     torch.cuda.empty_cache()
     return {
-        "head": head, "sweep": sweep, "encode_seconds": encode_s, "build_seconds": build_s,
+        "head": head, "sweep": sweep, "encode_seconds": encode_s, "encode": enc, "build_seconds": build_s,
         "build_distance_evaluations": evals, "build_evals_per_s": evals / build_s,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -1118,12 +1141,8 @@ def run_fixed(args, torch, np, qg):
 
     sp = qg.RabitqSpace(args.fixed_dim, 100)
     sp.set_center(center)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    index = sp.encode(x, batch=262144)
-    torch.cuda.synchronize()
-    encode_s = time.perf_counter() - t0
-    log("encoded %d vectors in %.2f s" % (args.fixed_n, encode_s))
+    index, enc = timed_encode(torch, sp, x, 262144)
+    encode_s = enc["seconds"]
 
     nq, per = args.fixed_nq, args.per
     stream = torch.cuda.current_stream()
@@ -1221,7 +1240,7 @@ def run_fixed(args, torch, np, qg):
                                "seed 2), %d queries x %d candidates" % (args.fixed_n, args.fixed_dim, nq, per),
                    "rerank_depth": depth, "codec": "int8"},
         "gdist_per_s": units / (kernel_ms / 1000.0) / 1e9,
-        "encode_seconds": encode_s, "encode_mvec_per_s": args.fixed_n / encode_s / 1e6,
+        "encode_seconds": encode_s, "encode_mvec_per_s": args.fixed_n / encode_s / 1e6, "encode": enc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "qg::k_int8_full_grouped<1024> (INT8 query x 4-bit code, fixed lists)",
@@ -1478,7 +1497,8 @@ def main():
             "recall_at_10": h["recall_at_10"], "recall_target_met": h["recall_at_10"] >= args.target_recall,
             "rerank": "plain top-100 residual4 (learned adaptive depth under 'learned')",
             "search_sweep": graph["sweep"],
-            "encode_seconds": graph["encode_seconds"], "build_seconds": graph["build_seconds"],
+            "encode_seconds": graph["encode_seconds"], "encode": graph["encode"],
+            "build_seconds": graph["build_seconds"],
             "build_distance_evaluations": graph["build_distance_evaluations"],
             "build_evals_per_s": graph["build_evals_per_s"],
             "roofline": graph["roofline"], "cpu_baseline": graph["cpu_baseline"],
