@@ -378,6 +378,43 @@ __device__ __forceinline__ void gate_fp32_group4(const float *q, const uint4 *r0
     acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;
 }
 
+// gate_fp32_group4 over the pair-interleaved query, qp2[16w + j] =
+// (q[32w + j], q[32w + 16 + j]): one 8-byte LDS read per MSB word and lane
+__device__ __forceinline__ void gate_fp32_group4_pairs(const float2 *qp2, const uint4 *r0, const uint4 *r1,
+                                                       const uint4 *r2, const uint4 *r3, uint32_t cd, int j,
+                                                       float (&acc)[4]) {
+    const int j2 = 16 + j;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll 1
+    for (uint32_t u = 0; u < cd / 128; ++u) {
+        const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
+        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
+        const float2 q2 = qp2[64 * u + 32 + j], q3 = qp2[64 * u + 48 + j];
+#define QG_GATE_STEP(W, JJ, QV)                                                    \
+        a0 = a0 + masked_q(m0.W, JJ, QV); a1 = a1 + masked_q(m1.W, JJ, QV);        \
+        a2 = a2 + masked_q(m2.W, JJ, QV); a3 = a3 + masked_q(m3.W, JJ, QV);
+        QG_GATE_STEP(x, j, q0.x)
+        QG_GATE_STEP(x, j2, q0.y)
+        QG_GATE_STEP(y, j, q1.x)
+        QG_GATE_STEP(y, j2, q1.y)
+        QG_GATE_STEP(z, j, q2.x)
+        QG_GATE_STEP(z, j2, q2.y)
+        QG_GATE_STEP(w, j, q3.x)
+        QG_GATE_STEP(w, j2, q3.y)
+    }
+    if (cd & 64u) {
+        const uint32_t u = cd / 128;
+        const uint4 m0 = r0[u], m1 = r1[u], m2 = r2[u], m3 = r3[u];
+        const float2 q0 = qp2[64 * u + j], q1 = qp2[64 * u + 16 + j];
+        QG_GATE_STEP(x, j, q0.x)
+        QG_GATE_STEP(x, j2, q0.y)
+        QG_GATE_STEP(y, j, q1.x)
+        QG_GATE_STEP(y, j2, q1.y)
+    }
+#undef QG_GATE_STEP
+    acc[0] = a0; acc[1] = a1; acc[2] = a2; acc[3] = a3;
+}
+
 // gate_fp32_group4 for one candidate (the same chain, alone)
 __device__ __forceinline__ float gate_fp32_group1(const float *q, const uint4 *r0, uint32_t cd, int j) {
     const int j2 = 16 + j;

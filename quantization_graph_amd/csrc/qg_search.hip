@@ -42,6 +42,16 @@ __device__ __forceinline__ bool cand_less(float ad, uint32_t aid, float bd, uint
     return ad < bd || (ad == bd && aid < bid);
 }
 
+// float_order_key of a number (not NaN): keys of -inf .. +inf fill
+// [0x007FFFFF, 0xFF800000]; NaNs lie outside
+__device__ __forceinline__ bool key_is_num(uint32_t k) { return k - 0x007FFFFFu <= 0xFF800000u - 0x007FFFFFu; }
+
+// cand_less of two entries given as order keys (float_order_key): false
+// when either distance is NaN, as the float comparisons are
+__device__ __forceinline__ bool key_less(uint32_t ak, uint32_t aid, uint32_t bk, uint32_t bid) {
+    return key_is_num(ak) && key_is_num(bk) && (ak < bk || (ak == bk && aid < bid));
+}
+
 // the same order without short-circuit evaluation (no branches on uniform values)
 __device__ __forceinline__ bool cand_less_nb(float ad, uint32_t aid, float bd, uint32_t bid) {
     return (ad < bd) | ((ad == bd) & (aid < bid));
@@ -287,21 +297,25 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
 }
 
 // 4 waves per SIMD (<= 128 VGPRs): 1024 workgroups fit the chip at once.
-template <int W, bool LEARNED>
+template <int W, bool LEARNED, bool FILTER>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_search(SearchArgs a) {
     constexpr uint32_t T = 64u * W, G = 4u * W;
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t cd = a.cd;
-    // query: FP32 residual, then one codec-specific region of 2 cd bytes:
-    // the gate filter's fixed-point query (FP32 codec) or the INT8 codes
+    // query: FP32 residual, then one codec-specific region: with the gate
+    // filter (FILTER: FP32 codec, code dimension 2048 and up) its
+    // fixed-point query in 2 cd bytes; otherwise 4 cd bytes holding the
+    // pair-interleaved FP32 query of the gate, or the INT8 codes
+    constexpr uint32_t kQ = FILTER ? 6u : 8u;                   // query bytes per dimension
     float *q = (float *)smem;                                   // cd
-    int8_t *xh = (int8_t *)(smem + 4 * cd);                     // cd     (FP32 codec)
-    int8_t *xl = (int8_t *)(smem + 5 * cd);                     // cd     (FP32 codec)
+    int8_t *xh = (int8_t *)(smem + 4 * cd);                     // cd     (FILTER)
+    int8_t *xl = (int8_t *)(smem + 5 * cd);                     // cd     (FILTER)
+    float2 *qp2 = (float2 *)(smem + 4 * cd);                    // cd / 2 (FP32 codec, no FILTER)
     int8_t *x8 = (int8_t *)(smem + 4 * cd);                     // cd     (INT8 codec)
     int8_t *x8p = (int8_t *)(smem + 5 * cd);                    // cd     (INT8 codec)
-    float *pd = (float *)(smem + 6 * cd);                       // a.L
-    uint32_t *pid = (uint32_t *)(smem + 6 * cd + 4 * a.L);      // a.L
-    uint8_t *pexp = (uint8_t *)(smem + 6 * cd + 8 * a.L);       // a.L (+pad)
+    float *pd = (float *)(smem + kQ * cd);                      // a.L
+    uint32_t *pid = (uint32_t *)(smem + kQ * cd + 4 * a.L);     // a.L
+    uint8_t *pexp = (uint8_t *)(smem + kQ * cd + 8 * a.L);      // a.L (+pad)
     // gate rows / code rows of a batch; rerank block sums after the traversal
     __shared__ uint4 stage[kStageU4];
     __shared__ uint32_t fresh[64], surv[64], sid[64];
@@ -331,6 +345,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t cmagic = div_magic(cch);
     const uint32_t per = 1024u / cch;
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
+    constexpr bool use_filter = FILTER;  // the FP32 gate's filter (below)
 
     for (;;) {
         if (t == 0) s_qi = atomicAdd(a.counter, 1u);
@@ -359,16 +374,21 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
                 ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
             }
-        } else {
+        } else if (use_filter) {
             for (int off = 32; off > 0; off >>= 1) qmx = max(qmx, (uint32_t)__shfl_xor((int)qmx, off));
             if (lane == 0) s_qmx[wv] = qmx;
+        } else {
+            for (uint32_t i = t; i < cd / 2; i += T) {
+                const float *qq = a.qv.q + (uint64_t)qi * cd + 32 * (i >> 4) + (i & 15u);
+                qp2[i] = make_float2(qq[0], qq[16]);
+            }
         }
         for (uint32_t i = t; i < a.visited_words / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
         if (t == 0) s_sp_node = 0xFFFFFFFFu;
         const QScalars s = a.qv.qs[qi];
         __threadfence();
         __syncthreads();
-        if (a.codec != 3) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
+        if (use_filter) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
 
         uint32_t len = 0;
         uint64_t hops = 0, gates = 0, dists = 1, pruned = 0, rchecked = 0, rpruned = 0;
@@ -390,7 +410,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         }
         len = (a.n > 0 && L > 0) ? 1 : 0;
         __syncthreads();
-        if (a.codec != 3 && t == 0) gate_filter_constants<W>(s_qmx, s_qe, s_qa, &s_fsc, &s_fw, &s_fok);
+        if (use_filter && t == 0) gate_filter_constants<W>(s_qmx, s_qe, s_qa, &s_fsc, &s_fw, &s_fok);
 
         for (;;) {
             // first unexpanded entry (every wave reads the same pool)
@@ -463,10 +483,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 stage_rows<W>(stage, fresh, nf, gch, gmagic, (const uint4 *)a.ix.gate, wv, lane);
                 __syncthreads();
             }
-            // FP32 with the pool full: the filter first (its sure prunes need
-            // no float sum), then the exact sums of the candidates it left open
+            // FILTER with the pool full: the filter first (its sure prunes
+            // need no float sum), then the exact sums of the candidates it
+            // left open. At code dimensions below 2048 the exact sums are
+            // short and the filter's extra barrier costs more than it saves
+            // (1M x 1024, 20k queries with lists of median 400: 270 ms
+            // without, 285 ms with), so that kernel is built without it.
             const bool full_h = len >= L;
-            const bool filter = a.codec != 3 && full_h && s_fok != 0u;
+            const bool filter = use_filter && full_h && s_fok != 0u;
             uint32_t nx = nf;        // candidates scored exactly
             uint64_t open = ~0ull;   // which ones (bit c: fresh neighbour c)
             if (filter) {
@@ -523,7 +547,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     }
                 } else {
                     float acc[4];
-                    gate_fp32_group4(q, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
+                    if constexpr (FILTER) gate_fp32_group4(q, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
+                    else gate_fp32_group4_pairs(qp2, rr[0], rr[1], rr[2], rr[3], cd, j16, acc);
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
                         if (cc[k] >= nf) continue;
@@ -645,9 +670,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     // the filter above is done with it; an out-of-range id
                     // loads row 0 and is flagged below)
                     sp_nb[lane] = nb2 < a.n ? nb2 : 0u;
-                    wave_sync_lds();
-                    stage_rows<1>(stage, sp_nb, deg2 < 64u ? deg2 : 64u, gch, gmagic, (const uint4 *)a.ix.gate, 0u,
-                                  lane);
+                    if constexpr (FILTER) {
+                        wave_sync_lds();
+                        stage_rows<1>(stage, sp_nb, deg2 < 64u ? deg2 : 64u, gch, gmagic, (const uint4 *)a.ix.gate,
+                                      0u, lane);
+                    }
                     const bool bad2 = first2 && nb2 >= a.n;
                     const bool fresh2 = first2 && !bad2 && (w2 & (1u << (nb2 & 31u))) == 0u;
                     const uint64_t fm2 = __ballot(fresh2);
@@ -656,7 +683,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     if (fresh2) {
                         const uint32_t r = (uint32_t)__popcll(fm2 & ((1ull << lane) - 1ull));
                         fresh[r] = nb2;
-                        frow[r] = (uint8_t)lane;
+                        frow[r] = (uint8_t)(FILTER ? lane : r);
+                    }
+                    if constexpr (!FILTER) {
+                        // the fresh rows only, compacted, once the visited
+                        // words are in
+                        wave_sync_lds();
+                        stage_rows<1>(stage, fresh, (uint32_t)__popcll(fm2), gch, gmagic, (const uint4 *)a.ix.gate,
+                                      0u, lane);
                     }
                     if (lane == 0) { s_sp_node = nxt; s_sp_fresh = fm2; s_sp_bad = bm2 != 0ull ? 1u : 0u; }
                 } else if (lane == 0) {
@@ -689,37 +723,53 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 // last retained pool entry (window lane e - 1 - tb) and the
                 // last accepted survivor (lane na - 1).
                 uint32_t npruned = 0, nchecked = 0;
-                for (uint32_t b = 0; b < ns; ++b) {
-                    const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_d), (int)b));
-                    const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)my_id, (int)b);
-                    const float lb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_lb), (int)b));
-                    const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)my_v, (int)b);
-                    const int pi = e > 0 ? (int)(e - 1 - tb) : 0;
-                    const int ai = na > 0 ? (int)(na - 1) : 0;
-                    const float pbd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bt_d), pi));
-                    const uint32_t pbi = (uint32_t)__builtin_amdgcn_readlane((int)bt_id, pi);
-                    const float abd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a_d), ai));
-                    const uint32_t abi = (uint32_t)__builtin_amdgcn_readlane((int)a_id, ai);
-                    const bool from_a = (na > 0) & ((e == 0) | cand_less_nb(pbd, pbi, abd, abi));
-                    const float bk_d = from_a ? abd : pbd;
-                    const uint32_t bk_id = from_a ? abi : pbi;
-                    const bool full = e + na >= L;
-                    const bool prune = full & (v != 0u) & (lb > bk_d);
-                    const bool take = !full | (!prune & cand_less_nb(d, id, bk_d, bk_id));
-                    npruned += prune ? 1u : 0u;
-                    nchecked += (full & (v != 0u)) ? 1u : 0u;
-                    e -= (full & take & !from_a) ? 1u : 0u;
-                    const uint32_t nna = na - ((full & take & from_a) ? 1u : 0u);
-                    // sorted insert into the accepted list: lanes past the slot take
-                    // their left neighbour (DPP wave_shr:1)
-                    const uint32_t pos = (uint32_t)__popcll(__ballot((lane < nna) & cand_less_nb(a_d, a_id, d, id)));
-                    const int sd = __builtin_amdgcn_update_dpp(__float_as_int(a_d), __float_as_int(a_d), 0x138, 0xF, 0xF, false);
-                    const int si = __builtin_amdgcn_update_dpp((int)a_id, (int)a_id, 0x138, 0xF, 0xF, false);
-                    const bool shift = take & (lane > pos) & (lane <= nna);
-                    const bool put = take & (lane == pos);
-                    a_d = put ? d : (shift ? __int_as_float(sd) : a_d);
-                    a_id = put ? id : (shift ? (uint32_t)si : a_id);
-                    na = nna + (take ? 1u : 0u);
+                // The decisions on order keys (unsigned integers in float
+                // order, both zeros equal, NaN outside the range of the
+                // others and comparing false as a float would): every
+                // comparison and the loop state are scalar, and the vector
+                // part (the accepted list's sorted insert) runs only for a
+                // taken survivor.
+                {
+                    const uint32_t my_k = float_order_key(my_d), my_lbk = float_order_key(my_lb);
+                    const uint32_t bt_k = float_order_key(bt_d);
+                    uint32_t a_k = 0u;
+                    for (uint32_t b = 0; b < ns; ++b) {
+                        const uint32_t kd = (uint32_t)__builtin_amdgcn_readlane((int)my_k, (int)b);
+                        const uint32_t id = (uint32_t)__builtin_amdgcn_readlane((int)my_id, (int)b);
+                        const uint32_t klb = (uint32_t)__builtin_amdgcn_readlane((int)my_lbk, (int)b);
+                        const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)my_v, (int)b);
+                        const int pi = e > 0 ? (int)(e - 1 - tb) : 0;
+                        const int ai = na > 0 ? (int)(na - 1) : 0;
+                        const uint32_t pbk = (uint32_t)__builtin_amdgcn_readlane((int)bt_k, pi);
+                        const uint32_t pbi = (uint32_t)__builtin_amdgcn_readlane((int)bt_id, pi);
+                        const uint32_t abk = (uint32_t)__builtin_amdgcn_readlane((int)a_k, ai);
+                        const uint32_t abi = (uint32_t)__builtin_amdgcn_readlane((int)a_id, ai);
+                        const bool from_a = na > 0 && (e == 0 || key_less(pbk, pbi, abk, abi));
+                        const uint32_t bk = from_a ? abk : pbk;
+                        const uint32_t bk_id = from_a ? abi : pbi;
+                        const bool full = e + na >= L;
+                        const bool prune = full && v != 0u && key_is_num(bk) && klb > bk;
+                        const bool take = !full || (!prune && key_less(kd, id, bk, bk_id));
+                        npruned += prune ? 1u : 0u;
+                        nchecked += (full && v != 0u) ? 1u : 0u;
+                        e -= (full && take && !from_a) ? 1u : 0u;
+                        const uint32_t nna = na - ((full && take && from_a) ? 1u : 0u);
+                        if (take) {
+                            // sorted insert into the accepted list: lanes past the
+                            // slot take their left neighbour (DPP wave_shr:1)
+                            const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_d), (int)b));
+                            const uint32_t pos = (uint32_t)__popcll(__ballot(lane < nna && key_less(a_k, a_id, kd, id)));
+                            const int sd = __builtin_amdgcn_update_dpp(__float_as_int(a_d), __float_as_int(a_d), 0x138, 0xF, 0xF, false);
+                            const int sk = __builtin_amdgcn_update_dpp((int)a_k, (int)a_k, 0x138, 0xF, 0xF, false);
+                            const int si = __builtin_amdgcn_update_dpp((int)a_id, (int)a_id, 0x138, 0xF, 0xF, false);
+                            const bool shift = lane > pos && lane <= nna;
+                            const bool put = lane == pos;
+                            a_d = put ? d : (shift ? __int_as_float(sd) : a_d);
+                            a_k = put ? kd : (shift ? (uint32_t)sk : a_k);
+                            a_id = put ? id : (shift ? (uint32_t)si : a_id);
+                        }
+                        na = nna + (take ? 1u : 0u);
+                    }
                 }
                 pruned += npruned;
                 rpruned += npruned;
@@ -883,32 +933,40 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
 
 }  // namespace qg
 
-template <int W, bool LEARNED>
+template <int W, bool LEARNED, bool FILTER>
 static int launch_search_w(const qg::SearchArgs *a, uint32_t slots, size_t smem, hipStream_t stream) {
     static size_t raised = 0;  // dynamic LDS above the 64 KB default needs the attribute once per size
     if (smem > raised) {
-        const hipError_t e = hipFuncSetAttribute((const void *)qg::k_search<W, LEARNED>,
+        const hipError_t e = hipFuncSetAttribute((const void *)qg::k_search<W, LEARNED, FILTER>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
         if (e != hipSuccess) return (int)e;
         raised = smem;
     }
-    hipLaunchKernelGGL((qg::k_search<W, LEARNED>), dim3(slots), dim3(64 * W), smem, stream, *a);
+    hipLaunchKernelGGL((qg::k_search<W, LEARNED, FILTER>), dim3(slots), dim3(64 * W), smem, stream, *a);
     return (int)hipGetLastError();
 }
 
-extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)cd * 6 + (size_t)L * 9 + 16; }
+// dynamic LDS of the larger variant (the one without the gate filter)
+extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)cd * 8 + (size_t)L * 9 + 16; }
 
 extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream) {
     if (a->qv.nq == 0) return 0;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
-    const size_t smem = qg_search_lds_bytes(a->cd, a->L);
+    // the FP32 gate's filter from code dimension 2048 (its query takes 2 cd
+    // bytes of LDS where the pair-interleaved query takes 4 cd)
+    const bool filter = a->codec != 3 && a->cd >= 2048;
+    const size_t smem = (size_t)a->cd * (filter ? 6 : 8) + (size_t)a->L * 9 + 16;
     // the learned-depth variant is a kernel of its own, so the plain search
     // keeps its register allocation
     if (a->mlp)
-        return waves == 2 ? launch_search_w<2, true>(a, slots, smem, stream)
-                          : launch_search_w<4, true>(a, slots, smem, stream);
-    return waves == 2 ? launch_search_w<2, false>(a, slots, smem, stream)
-                      : launch_search_w<4, false>(a, slots, smem, stream);
+        return filter ? (waves == 2 ? launch_search_w<2, true, true>(a, slots, smem, stream)
+                                    : launch_search_w<4, true, true>(a, slots, smem, stream))
+                      : (waves == 2 ? launch_search_w<2, true, false>(a, slots, smem, stream)
+                                    : launch_search_w<4, true, false>(a, slots, smem, stream));
+    return filter ? (waves == 2 ? launch_search_w<2, false, true>(a, slots, smem, stream)
+                                : launch_search_w<4, false, true>(a, slots, smem, stream))
+                  : (waves == 2 ? launch_search_w<2, false, false>(a, slots, smem, stream)
+                                : launch_search_w<4, false, false>(a, slots, smem, stream));
 }
 
 // ------------------------------------------ search without a workspace
