@@ -35,10 +35,11 @@ int iio_save_state(std::size_t dim, std::uint32_t seed, bool mse, bool fp16, con
 }
 
 // loadState of the file at in_path into a space of the same configuration
-// (made with another seed), then saveState to out_path. -1 when the
-// reference's loader rejects the file.
-int iio_load_save_state(std::size_t dim, bool mse, bool fp16, const char *in_path, const char *out_path) {
-    void *space = rabitq_space_new(dim, 12345u, 1, 4, 16, mse, fp16);
+// (made with another seed; centroid_count centroids), then saveState to
+// out_path. -1 when the reference's loader rejects the file.
+int iio_load_save_state_k(std::size_t dim, std::size_t centroid_count, bool mse, bool fp16, const char *in_path,
+                          const char *out_path) {
+    void *space = rabitq_space_new(dim, 12345u, centroid_count, 4, 16, mse, fp16);
     if (space == nullptr) return -1;
     int rc = 0;
     try {
@@ -53,6 +54,10 @@ int iio_load_save_state(std::size_t dim, bool mse, bool fp16, const char *in_pat
     }
     rabitq_space_delete(space);
     return rc;
+}
+
+int iio_load_save_state(std::size_t dim, bool mse, bool fp16, const char *in_path, const char *out_path) {
+    return iio_load_save_state_k(dim, 1, mse, fp16, in_path, out_path);
 }
 
 // buildEncodedSymmetricBulk (+ refineGraphSymmetric) over n compact records

@@ -252,6 +252,7 @@ class RabitqSpace:
         if c.size != self.dim:
             raise ValueError("center has %d values, expected %d" % (c.size, self.dim))
         check(_lib.lib().qg_space_set_center(self.handle, c.ctypes.data_as(C.c_void_p)))
+        self._centroids = c.reshape(1, self.dim).copy()
 
     def set_centroids(self, centroids):
         """rabitq_space_set_centroids (rabitq_bridge.cpp:89-99): K x dim
@@ -261,6 +262,7 @@ class RabitqSpace:
         if c.size != self.dim * self.centroid_count:
             raise ValueError("centroids hold %d values, expected %d x %d" % (c.size, self.centroid_count, self.dim))
         check(_lib.lib().qg_space_set_centroids(self.handle, c.ctypes.data_as(C.c_void_p), self.centroid_count))
+        self._centroids = c.reshape(self.centroid_count, self.dim).copy()
 
     def set_qr_rotation(self, seed=None, gaussian=None, bf16_parts=3):
         """Precision mode: rotate with Q of the FP64 QR factorisation (on the
@@ -309,29 +311,36 @@ class RabitqSpace:
         file the reference's loadState reads."""
         from . import formats
         signs, center = self.host_constants()
-        formats.write_space_state(path, self.dim, self.seed, center, signs, self.residual_block_size,
+        centroids = center.reshape(1, self.dim)
+        if self.centroid_count > 1:
+            centroids = getattr(self, "_centroids", None)
+            if centroids is None or centroids.shape[0] != self.centroid_count:
+                raise ValueError("save_state: the %d centroids are not set" % self.centroid_count)
+        formats.write_space_state(path, self.dim, self.seed, centroids, signs, self.residual_block_size,
                                   self.residual_mse, self.residual_fp16)
 
     @classmethod
     def load_state(cls, path):
         """RaBitQSpace::loadState (space_rabitq.h:3200-3345) of an EXRBTQ31
-        file: a space of the stored dimension, seed, residual settings and
-        center. Raises ValueError when the file holds more than one centroid
-        (this build holds K = 1) or rotation signs other than its seed's."""
+        file: a space of the stored dimension, seed, centroid count, residual
+        settings and centroids (the center when K = 1). Raises ValueError on
+        rotation signs other than its seed's."""
         from . import formats
         head, seed, centroids, signs = formats.read_space_state(path)
         flags = head["flags"]
-        if head["centroid_count"] != 1:
-            raise ValueError("%s: %d centroids; this build holds K = 1" % (path, head["centroid_count"]))
-        sp = cls(head["dim"], seed, 1, head["residual_bits"], head["residual_block_size"], bool(flags & 4),
+        k = head["centroid_count"]
+        sp = cls(head["dim"], seed, k, head["residual_bits"], head["residual_block_size"], bool(flags & 4),
                  bool(flags & 8))
-        want = formats.space_state_fields(head["dim"], 1, head["residual_block_size"], bool(flags & 4),
+        want = formats.space_state_fields(head["dim"], k, head["residual_block_size"], bool(flags & 4),
                                           bool(flags & 8))
         if head != want:
             raise ValueError("%s: record layout differs from this space's" % path)
         if not np.array_equal(sp.host_constants()[0], signs):
             raise ValueError("%s: rotation signs are not those of seed %d" % (path, seed))
-        sp.set_center(centroids[0])
+        if k == 1:
+            sp.set_center(centroids[0])
+        else:
+            sp.set_centroids(centroids)
         return sp
 
     def set_query_coarse_codec(self, codec):

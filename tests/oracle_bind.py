@@ -930,6 +930,7 @@ class RefIndexIO:
         L = C.CDLL(INDEXIO_REF_SO)
         L.iio_save_state.argtypes = [C.c_size_t, C.c_uint32, C.c_bool, C.c_bool, C.c_void_p, C.c_char_p]
         L.iio_load_save_state.argtypes = [C.c_size_t, C.c_bool, C.c_bool, C.c_char_p, C.c_char_p]
+        L.iio_load_save_state_k.argtypes = [C.c_size_t, C.c_size_t, C.c_bool, C.c_bool, C.c_char_p, C.c_char_p]
         L.iio_build_save_index.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_float,
                                            C.c_size_t, C.c_size_t, C.c_size_t, C.c_char_p]
         L.iio_load_save_index.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
@@ -949,8 +950,8 @@ class RefIndexIO:
         c = None if center is None else np.ascontiguousarray(center, np.float32)
         return self.lib.iio_save_state(dim, seed, mse, fp16, _p(c) if c is not None else None, str(path).encode())
 
-    def load_save_state(self, dim, src, dst, mse=True, fp16=True):
-        return self.lib.iio_load_save_state(dim, mse, fp16, str(src).encode(), str(dst).encode())
+    def load_save_state(self, dim, src, dst, mse=True, fp16=True, centroid_count=1):
+        return self.lib.iio_load_save_state_k(dim, centroid_count, mse, fp16, str(src).encode(), str(dst).encode())
 
     def space(self, dim, seed, center):
         h = self.lib.rabitq_space_new(dim, seed, 1, 4, 16, True, True)

@@ -65,6 +65,30 @@ def test_index_and_state_round_trip(gpu):
         assert np.array_equal(u.cpu().numpy(), v.cpu().numpy())
 
 
+def test_state_round_trip_with_centroids(gpu):
+    """A space of 4 centroids: its EXRBTQ31 file holds the 4 centroids, the
+    reference's loadState / saveState (a space made with K = 4) writes it
+    back byte for byte, and the space loaded here encodes the same records."""
+    import quantization_graph_amd as qg
+    x, _ = gaussian_mixture(600, 200, 4, 0.3, 7)
+    cents = np.stack([x[i::4].mean(0) for i in range(4)]).astype(np.float32)
+    sp = qg.RabitqSpace(200, 100, 4)
+    sp.set_centroids(cents)
+    with tempfile.TemporaryDirectory() as td:
+        st = os.path.join(td, "space4.bin")
+        sp.save_state(st)
+        if RefIndexIO.available():
+            back = os.path.join(td, "back4.bin")
+            assert RefIndexIO().load_save_state(200, st, back, centroid_count=4) == 0
+            assert _bytes(back) == _bytes(st)
+        sp2 = qg.RabitqSpace.load_state(st)
+    assert sp2.centroid_count == 4
+    xg = gpu.from_numpy(x).cuda()
+    a, _ = sp.encode(xg).to_records()
+    b, _ = sp2.encode(xg).to_records()
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 def test_load_state_rejects_foreign_signs(gpu):
     import quantization_graph_amd as qg
     from quantization_graph_amd import formats
