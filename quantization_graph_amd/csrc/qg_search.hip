@@ -77,9 +77,9 @@ constexpr uint32_t kStageU4 = 1152;  // 64 gate rows of cd 2048 (18 uint4 each)
 __host__ __device__ constexpr uint32_t div_magic(uint32_t d) { return 0xFFFFFFFFu / d + 1u; }
 
 template <int W>
-__device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
-                                           uint32_t magic, const uint4 *__restrict__ src, uint32_t wv,
-                                           uint32_t lane) {
+__device__ __forceinline__ void stage_rows_issue(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
+                                                 uint32_t magic, const uint4 *__restrict__ src, uint32_t wv,
+                                                 uint32_t lane) {
     const uint32_t total = count * row_u4;
     for (uint32_t k0 = 64u * wv; k0 < total; k0 += 64u * W) {
         const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
@@ -88,7 +88,65 @@ __device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
                                          (__attribute__((address_space(3))) void *)(dst + k0), 16, 0, 0);
     }
+}
+
+template <int W>
+__device__ __forceinline__ void stage_rows(uint4 *dst, const uint32_t *ids, uint32_t count, uint32_t row_u4,
+                                           uint32_t magic, const uint4 *__restrict__ src, uint32_t wv,
+                                           uint32_t lane) {
+    stage_rows_issue<W>(dst, ids, count, row_u4, magic, src, wv, lane);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's pieces are in LDS
+}
+
+// The next frontier is the best survivor of this hop or the first
+// unexpanded pool entry. Their adjacency rows and degrees go to LDS together
+// with the survivors' code rows, so the prediction of the next frontier
+// starts from its neighbour list instead of loading it. They live in the
+// stage's last 2 KB (uint4 kAdjAt onwards), which a batch of code rows never
+// reaches (at most 1024 / cch rows of cch uint4, rounded up to 64 pieces):
+// the degrees in the first 32 bytes, then row r at 8 + r * stride words.
+// Row r belongs to the id in lane r of idl, r < cnt (cnt <= adj_rows_fit).
+// 16-byte global-to-LDS pieces when a row is a whole number of them, else
+// 4-byte ones; lanes past the end stay idle. The issuing wave's vmcnt(0)
+// and the caller's barrier publish them.
+constexpr uint32_t kAdjAt = 1024, kAdjMax = 7;  // survivors' rows: kAdjMax - 1, then the pool entry's
+
+__device__ __forceinline__ uint32_t adj_rows_fit(uint64_t stride) {
+    const uint64_t fit = ((kStageU4 - kAdjAt) * 16u - 32u) / (4u * (stride ? stride : 1u));
+    return fit < kAdjMax ? (uint32_t)fit : kAdjMax;
+}
+
+__device__ __forceinline__ void stage_adjacency(const SearchArgs &a, uint32_t idl, uint32_t cnt, uint4 *stage,
+                                                uint32_t lane) {
+    const uint32_t stride = (uint32_t)a.edge_stride;
+    uint32_t *adj = (uint32_t *)(stage + kAdjAt);
+    if ((stride & 3u) == 0u) {
+        const uint32_t row_u4 = stride / 4u, total = cnt * row_u4;
+        for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+            const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
+            const uint32_t r = e / row_u4;
+            const uint32_t id = (uint32_t)__shfl((int)idl, (int)r);
+            const uint4 *g = (const uint4 *)(a.edges + (uint64_t)id * stride) + (e - r * row_u4);
+            if (k0 + lane < total)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                                 (__attribute__((address_space(3))) void *)(adj + 8u + 4u * k0), 16,
+                                                 0, 0);
+        }
+    } else {
+        const uint32_t total = cnt * stride;
+        for (uint32_t k0 = 0; k0 < total; k0 += 64u) {
+            const uint32_t e = k0 + lane < total ? k0 + lane : total - 1u;
+            const uint32_t r = e / stride;
+            const uint32_t id = (uint32_t)__shfl((int)idl, (int)r);
+            const uint32_t *g = a.edges + (uint64_t)id * stride + (e - r * stride);
+            if (k0 + lane < total)
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                                 (__attribute__((address_space(3))) void *)(adj + 8u + k0), 4, 0, 0);
+        }
+    }
+    if (lane < cnt)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(a.degrees + idl),
+                                         (__attribute__((address_space(3))) void *)adj, 4, 0, 0);
 }
 
 // ---------------------------------------------------------------- gate filter
@@ -344,6 +402,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t cch = cd / 32;   // uint4 per code row
     const uint32_t cmagic = div_magic(cch);
     const uint32_t per = 1024u / cch;
+    const uint32_t arows = adj_rows_fit(a.edge_stride);  // adjacency rows the stage's tail holds
     uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
     constexpr bool use_filter = FILTER;  // the FP32 gate's filter (below)
 
@@ -600,9 +659,19 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             // survivors' distances: code rows in one round trip per batch of
             // `per` rows (the headers came with the gate rows), one 16-lane
             // group per candidate
+            // (with the first batch, the last wave stages the adjacency rows of
+            // the first survivors, rows 0 .. nadj - 1, and of the first
+            // unexpanded pool entry, row nadj)
+            const uint32_t nadj = arows == 0u ? 0u : (ns < arows - 1u ? ns : arows - 1u);
+            const bool adj_spa = ns > 0 && arows > 0 && s_spa_id != 0xFFFFFFFFu;
             for (uint32_t b0 = 0; b0 < ns; b0 += per) {
                 const uint32_t nb_rows = ns - b0 < per ? ns - b0 : per;
-                stage_rows<W>(stage, sid + b0, nb_rows, cch, cmagic, (const uint4 *)a.ix.codes, wv, lane);
+                stage_rows_issue<W>(stage, sid + b0, nb_rows, cch, cmagic, (const uint4 *)a.ix.codes, wv, lane);
+                if (b0 == 0 && wv == W - 1 && arows > 0) {
+                    const uint32_t idl = lane < nadj ? sid[lane] : s_spa_id;
+                    stage_adjacency(a, idl, nadj + (adj_spa ? 1u : 0u), stage, lane);
+                }
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
                 __syncthreads();
                 for (uint32_t b = grp; b < nb_rows; b += G) {
                     const uint32_t slot = surv[b0 + b];
@@ -624,22 +693,38 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             if (wv == 1) {
                 float bd = lane < ns ? s_d[lane] : 3.402823466e38f;
                 uint32_t bid = lane < ns ? sid[lane] : 0xFFFFFFFFu;
+                uint32_t bslot = lane;  // the best survivor's index in sid[]
                 bool bv = lane < ns;
                 for (int off = 32; off > 0; off >>= 1) {
                     const float od = __shfl_xor(bd, off);
                     const uint32_t oid = (uint32_t)__shfl_xor((int)bid, off);
+                    const uint32_t oslot = (uint32_t)__shfl_xor((int)bslot, off);
                     const bool ov = __shfl_xor((int)bv, off) != 0;
-                    if (ov && (!bv || cand_less(od, oid, bd, bid))) { bd = od; bid = oid; bv = true; }
+                    if (ov && (!bv || cand_less(od, oid, bd, bid))) { bd = od; bid = oid; bslot = oslot; bv = true; }
                 }
                 const uint32_t sp_aid = s_spa_id;
                 const bool sp_a = sp_aid != 0xFFFFFFFFu;
                 uint32_t nxt = 0xFFFFFFFFu;
-                if (bv && (!sp_a || cand_less(bd, bid, s_spa_d, sp_aid))) nxt = bid;
-                else if (sp_a) nxt = sp_aid;
+                // row of the next frontier's adjacency in adj[] (staged with the
+                // survivors' code rows when ns > 0), or none
+                uint32_t arow = 0xFFFFFFFFu;
+                if (bv && (!sp_a || cand_less(bd, bid, s_spa_d, sp_aid))) {
+                    nxt = bid;
+                    if (bslot < nadj) arow = bslot;
+                } else if (sp_a) {
+                    nxt = sp_aid;
+                    if (adj_spa) arow = nadj;
+                }
                 if (nxt != 0xFFFFFFFFu && nxt < a.n) {
-                    const uint32_t deg2 = a.degrees[nxt];
-                    const uint32_t raw2 = lane < a.edge_stride ? a.edges[(uint64_t)nxt * a.edge_stride + lane]
-                                                               : 0xFFFFFFFFu;
+                    uint32_t deg2, raw2 = 0xFFFFFFFFu;
+                    if (arow != 0xFFFFFFFFu) {
+                        const uint32_t *adj = (const uint32_t *)(stage + kAdjAt);
+                        deg2 = adj[arow];
+                        if (lane < a.edge_stride) raw2 = adj[8u + arow * (uint32_t)a.edge_stride + lane];
+                    } else {
+                        deg2 = a.degrees[nxt];
+                        if (lane < a.edge_stride) raw2 = a.edges[(uint64_t)nxt * a.edge_stride + lane];
+                    }
                     const uint32_t nb2 = lane < deg2 ? raw2 : 0xFFFFFFFFu;
                     uint32_t w2 = 0u;
                     if (nb2 < a.n)
