@@ -57,6 +57,11 @@ __device__ __forceinline__ bool cand_less_nb(float ad, uint32_t aid, float bd, u
     return (ad < bd) | ((ad == bd) & (aid < bid));
 }
 
+// a value every lane holds alike (read from LDS, say), moved to a scalar
+// register so the compiler keeps it and what derives from it out of VGPRs
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -409,7 +414,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     for (;;) {
         if (t == 0) s_qi = atomicAdd(a.counter, 1u);
         __syncthreads();
-        const uint32_t qi = s_qi;
+        const uint32_t qi = uni(s_qi);
         __syncthreads();
         if (qi >= a.qv.nq) break;
         // this query's search list size: its own (at least k) or the launch's
@@ -480,7 +485,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 if (m) { f = c0 + (uint32_t)__builtin_ctzll(m); break; }
             }
             if (f == 0xFFFFFFFFu) break;
-            const uint32_t node = pid[f];
+            const uint32_t node = uni(pid[f]);
             __syncthreads();
             ++hops;
             if (wv == 0) {
@@ -531,7 +536,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
-            const uint32_t nf = s_nf;
+            const uint32_t nf = uni(s_nf);
             if (nf == 0) continue;
             gates += nf;
             // gate: the fresh neighbours' gate rows (MSB plane, factors, header,
@@ -654,7 +659,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 if (lane == 0) s_ns = (uint32_t)__popcll(sm);
             }
             __syncthreads();
-            const uint32_t ns = s_ns;
+            const uint32_t ns = uni(s_ns);
             pruned += nf - ns;
             // survivors' distances: code rows in one round trip per batch of
             // `per` rows (the headers came with the gate rows), one 16-lane
@@ -902,7 +907,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
-            len = s_len;
+            len = uni(s_len);
         }
 
         // the traversal pool goes out before the rerank reuses its arrays
