@@ -524,20 +524,22 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
         'pipeline' (traversal, features, MLP, learned rerank: four launches)."""
         def step(ev=None):
             qv = sp.prepare(q, "full")
+            # the single-launch forms record their events around the launch
+            # alone; the pipeline's span its four launches
             if kind == "learned":
                 return model.search_fused(sp, index, qv, deg, edges, L, rstats, slots=slots, workspace=ws,
                                           check_graph=False, events=ev)
-            if ev is not None:
-                ev[0].record(stream)
             if kind == "plain":
                 out = sp.search(index, qv, deg, edges, k=10, l_search=L, slots=slots, workspace=ws,
-                                check_graph=False)
-                out = (out[0], out[1], None, out[2])
-            elif kind == "traversal":
-                out = learned.search_pool(sp, index, qv, deg, edges, L, slots=slots, workspace=ws, check_graph=False)
-                out = (None, None, None, out[2])
-            else:
-                out = model.search(sp, index, qv, deg, edges, L, rstats, workspace=ws, check_graph=False)
+                                check_graph=False, events=ev)
+                return (out[0], out[1], None, out[2])
+            if kind == "traversal":
+                out = learned.search_pool(sp, index, qv, deg, edges, L, slots=slots, workspace=ws, check_graph=False,
+                                          events=ev)
+                return (None, None, None, out[2])
+            if ev is not None:
+                ev[0].record(stream)
+            out = model.search(sp, index, qv, deg, edges, L, rstats, workspace=ws, check_graph=False)
             if ev is not None:
                 ev[1].record(stream)
             return out
@@ -562,21 +564,76 @@ def run_learned(args, torch, np, sp, index, deg, edges, x, q, gt, head, stream):
     search_error(ws)
     rec = (ids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
     rec_p = (pids.unsqueeze(2) == gt.unsqueeze(1)).any(2).float().sum(1).mean().item() / 10.0
-    stage_plain, stage_learned = k_p - k_t, k_l - k_t
+
+    # The rerank stage inside the fused kernels: traversal-only, plain and
+    # learned launches interleaved round by round, the median of each
+    # round's difference (slow drift of the clock cancels within a round).
+    def one(kind):
+        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        qv_ = sp.prepare(q, "full")
+        if kind == "traversal":
+            learned.search_pool(sp, index, qv_, deg, edges, L, slots=slots, workspace=ws, check_graph=False,
+                                events=ev)
+        elif kind == "plain":
+            sp.search(index, qv_, deg, edges, k=10, l_search=L, slots=slots, workspace=ws, check_graph=False,
+                      events=ev)
+        else:
+            model.search_fused(sp, index, qv_, deg, edges, L, rstats, slots=slots, workspace=ws, check_graph=False,
+                               events=ev)
+        return ev
+
+    rounds = []
+    for _ in range(max(10, 2 * args.steps)):
+        evs = [one(k) for k in ("traversal", "plain", "learned")]
+        rounds.append(evs)
+    torch.cuda.synchronize()
+    tms = np.array([[a.elapsed_time(b) for a, b in r] for r in rounds])
+    stage_plain = float(np.median(tms[:, 1] - tms[:, 0]))
+    stage_learned = float(np.median(tms[:, 2] - tms[:, 0]))
+
+    # The stage on its own, as separate launches over the same pools: the
+    # plain top-100 rerank against features + MLP + the learned-depth rerank.
+    pid0, pd0, _ = learned.search_pool(sp, index, sp.prepare(q, "full"), deg, edges, L, slots=slots, workspace=ws,
+                                       check_graph=False)
+    qv0 = sp.prepare(q, "full")
+    sep = []
+    for _ in range(max(10, 2 * args.steps)):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record(stream)
+        learned.rerank(sp, index, qv0, pid0, pd0, None, 1.0, 100, 10, 10)
+        e[1].record(stream)
+        preds0 = model.predict(sp, index, qv0, pid0, pd0, rstats)
+        e[2].record(stream)
+        learned.rerank(sp, index, qv0, pid0, pd0, preds0, model.kmult, model.depth, model.min_depth, 10)
+        e[3].record(stream)
+        sep.append(e)
+    torch.cuda.synchronize()
+    sms = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), e[2].elapsed_time(e[3])] for e in sep])
+    sep_plain = float(np.median(sms[:, 0]))
+    sep_fm, sep_rr = float(np.median(sms[:, 1])), float(np.median(sms[:, 2]))
     res = {"qps": nq / (ms / 1000.0), "ms_per_step": ms, "gpu_ms": k_l, "recall_at_10": rec,
            "mean_depth": float(depths.float().mean().item()), "kmult": kmult, "calibration": table,
            "training_pairs": int(args.train_queries * min(100, L)), "train_epochs": args.train_epochs,
            "loss_history": hist, "pairs_seconds": pairs_s, "train_seconds": train_s,
            "plain_qps": nq / (ms_p / 1000.0), "plain_ms_per_step": ms_p, "plain_gpu_ms": k_p,
            "plain_recall_at_10": rec_p, "speedup_vs_plain": ms_p / ms,
-           "traversal_gpu_ms": k_t, "rerank_stage_ms": {"plain_top100": stage_plain, "learned": stage_learned},
+           "traversal_gpu_ms": k_t,
+           "rerank_stage_ms": {"plain_top100": stage_plain, "learned": stage_learned,
+                               "method": "fused kernels: median over %d rounds of (plain or learned launch) - "
+                                         "(traversal-only launch), the three launches interleaved" % len(rounds)},
            "rerank_stage_speedup": stage_plain / stage_learned if stage_learned > 0 else None,
+           "separate_launches_ms": {"plain_top100_rerank": sep_plain, "features_and_mlp": sep_fm,
+                                    "learned_depth_rerank": sep_rr, "learned_total": sep_fm + sep_rr,
+                                    "speedup": sep_plain / (sep_fm + sep_rr),
+                                    "method": "the same pools, each launch timed with events, median of %d"
+                                              % len(sep)},
            "pipeline_four_launches": {"ms_per_step": ms_u, "gpu_ms": k_u, "qps": nq / (ms_u / 1000.0)},
            "form": "qg_search_learned: features, MLP and depth rule fused after each query's traversal"}
     log("learned (fused): %.3f ms/step  %.0f q/s  recall@10 %.4f  mean depth %.1f | plain: %.3f ms/step %.0f q/s "
-        "recall %.4f | rerank stage %.3f -> %.3f ms (traversal %.3f ms) | four-launch pipeline %.3f ms/step"
+        "recall %.4f | rerank stage in the fused kernels %.3f -> %.3f ms (traversal %.3f ms) | as separate launches "
+        "%.3f -> %.3f + %.3f ms | four-launch pipeline %.3f ms/step"
         % (ms, res["qps"], rec, res["mean_depth"], ms_p, res["plain_qps"], rec_p, stage_plain, stage_learned,
-           k_t, ms_u))
+           k_t, sep_plain, sep_fm, sep_rr, ms_u))
     return res
 
 

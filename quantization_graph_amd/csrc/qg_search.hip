@@ -310,6 +310,21 @@ __device__ __forceinline__ void gate_filter_tiles(const uint4 *stage, const uint
 // (16-row tiles across the waves) and the depth rule, all in the stage,
 // which is idle after the traversal: features 112 x 64 B, each wave's hidden
 // layer 2 KB, 112 predictions. Returns the depth (at most rc).
+// Kernel-argument fields the hop loop does not use (outputs, per-query
+// inputs, the rerank's and the learned depth's tables), read from the
+// kernarg segment where they are needed through a pointer the compiler
+// cannot hoist, so they do not occupy scalar registers across the traversal.
+// The segment holds the SearchArgs record at offset 0.
+__device__ __forceinline__ const __attribute__((address_space(4))) unsigned char *cold_kernarg() {
+    const __attribute__((address_space(4))) unsigned char *p =
+        (const __attribute__((address_space(4))) unsigned char *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return p;
+}
+#define COLD(field) \
+    (*(const __attribute__((address_space(4))) decltype(SearchArgs::field) *)(cold_kernarg() + \
+                                                                             offsetof(SearchArgs, field)))
+
 template <int W>
 __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, uint32_t qi, uint32_t rc,
                                                                    const float *pd, const uint32_t *pid,
@@ -327,7 +342,7 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
     for (uint32_t i = t; i < rc; i += T) {
         const uint32_t id = pid[i];
         uint32_t w[kFP / 2];
-        learned_feature_row(i, rc, pd, s, a.ix.hdr[id], a.ix.sfac[id], a.rfac[id], a.rstats[id], a.cd, w);
+        learned_feature_row(i, rc, pd, s, COLD(ix.hdr)[id], COLD(ix.sfac)[id], COLD(rfac)[id], COLD(rstats)[id], a.cd, w);
         uint4 *o = (uint4 *)(feat + i * kFP);
 #pragma unroll
         for (int k = 0; k < kFP / 8; ++k) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
@@ -337,7 +352,7 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
     for (uint32_t tl = wv; tl < tiles; tl += W) {
         float y[4];
         const uint32_t rows = rc - 16 * tl < 16 ? rc - 16 * tl : 16;
-        mlp_forward_tile_packed(feat + 16 * tl * kFP, rows, a.mlp, sh1, y, (int)lane);
+        mlp_forward_tile_packed(feat + 16 * tl * kFP, rows, COLD(mlp), sh1, y, (int)lane);
         if ((lane & 15u) == 0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -348,11 +363,11 @@ __device__ __forceinline__ uint32_t search_learned_depth(const SearchArgs &a, ui
     }
     __syncthreads();
     if (wv == 0) {
-        const uint32_t d = learned_depth(rc, pd, [&](uint32_t j) { return pred[j]; }, s.query_norm_sqr, a.kmult,
-                                         a.min_depth, lane);
+        const uint32_t d = learned_depth(rc, pd, [&](uint32_t j) { return pred[j]; }, s.query_norm_sqr, COLD(kmult),
+                                         COLD(min_depth), lane);
         if (lane == 0) {
             *s_out = d;
-            if (a.out_depth) a.out_depth[qi] = d;
+            if (COLD(out_depth)) COLD(out_depth)[qi] = d;
         }
     }
     __syncthreads();
@@ -408,26 +423,26 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t cmagic = div_magic(cch);
     const uint32_t per = 1024u / cch;
     const uint32_t arows = adj_rows_fit(a.edge_stride);  // adjacency rows the stage's tail holds
-    uint32_t *visited = a.visited + (uint64_t)blockIdx.x * a.visited_words;
+    uint32_t *visited = COLD(visited) + (uint64_t)blockIdx.x * COLD(visited_words);
     constexpr bool use_filter = FILTER;  // the FP32 gate's filter (below)
 
     for (;;) {
-        if (t == 0) s_qi = atomicAdd(a.counter, 1u);
+        if (t == 0) s_qi = atomicAdd(COLD(counter), 1u);
         __syncthreads();
         const uint32_t qi = uni(s_qi);
         __syncthreads();
-        if (qi >= a.qv.nq) break;
+        if (qi >= COLD(qv.nq)) break;
         // this query's search list size: its own (at least k) or the launch's
-        uint32_t L = a.L;
-        if (a.lq) {
-            const uint32_t l = a.lq[qi];
-            L = l < a.k ? a.k : (l > a.L ? a.L : l);
+        uint32_t L = COLD(L);
+        if (COLD(lq)) {
+            const uint32_t l = COLD(lq)[qi];
+            L = l < COLD(k) ? COLD(k) : (l > COLD(L) ? COLD(L) : l);
         }
         // stage the prepared query (and its largest |q| bits), clear this
         // slot's visited bits
         uint32_t qmx = 0u;
         for (uint32_t i = t; i < cd / 4; i += T) {
-            const float4 v = ((const float4 *)(a.qv.q + (uint64_t)qi * cd))[i];
+            const float4 v = ((const float4 *)(COLD(qv.q) + (uint64_t)qi * cd))[i];
             ((float4 *)q)[i] = v;
             const uint32_t m0 = __float_as_uint(v.x) & 0x7FFFFFFFu, m1 = __float_as_uint(v.y) & 0x7FFFFFFFu;
             const uint32_t m2 = __float_as_uint(v.z) & 0x7FFFFFFFu, m3 = __float_as_uint(v.w) & 0x7FFFFFFFu;
@@ -435,21 +450,21 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         }
         if (a.codec == 3) {
             for (uint32_t i = t; i < cd / 4; i += T) {
-                ((uint32_t *)x8)[i] = ((const uint32_t *)(a.qv.x8 + (uint64_t)qi * cd))[i];
-                ((uint32_t *)x8p)[i] = ((const uint32_t *)(a.qv.x8p + (uint64_t)qi * cd))[i];
+                ((uint32_t *)x8)[i] = ((const uint32_t *)(COLD(qv.x8) + (uint64_t)qi * cd))[i];
+                ((uint32_t *)x8p)[i] = ((const uint32_t *)(COLD(qv.x8p) + (uint64_t)qi * cd))[i];
             }
         } else if (use_filter) {
             for (int off = 32; off > 0; off >>= 1) qmx = max(qmx, (uint32_t)__shfl_xor((int)qmx, off));
             if (lane == 0) s_qmx[wv] = qmx;
         } else {
             for (uint32_t i = t; i < cd / 2; i += T) {
-                const float *qq = a.qv.q + (uint64_t)qi * cd + 32 * (i >> 4) + (i & 15u);
+                const float *qq = COLD(qv.q) + (uint64_t)qi * cd + 32 * (i >> 4) + (i & 15u);
                 qp2[i] = make_float2(qq[0], qq[16]);
             }
         }
-        for (uint32_t i = t; i < a.visited_words / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t i = t; i < COLD(visited_words) / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
         if (t == 0) s_sp_node = 0xFFFFFFFFu;
-        const QScalars s = a.qv.qs[qi];
+        const QScalars s = COLD(qv.qs)[qi];
         __threadfence();
         __syncthreads();
         if (use_filter) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
@@ -463,7 +478,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             stage_rows<1>(stage, sid, 1, cch, cmagic, (const uint4 *)a.ix.codes, 0u, lane);
             wave_sync_lds();
             if (grp == 0) {
-                const float d0 = asym_group(q, (const uint8_t *)stage, cd, a.ix.hdr[0], s, j16);
+                const float d0 = asym_group(q, (const uint8_t *)stage, cd, COLD(ix.hdr)[0], s, j16);
                 if (j16 == 0) {
                     atomicOr(&visited[0], 1u);
                     pd[0] = d0;
@@ -911,16 +926,16 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         }
 
         // the traversal pool goes out before the rerank reuses its arrays
-        if (a.pool_ids) {
-            for (uint32_t i = t; i < a.L; i += T) {
-                a.pool_ids[(uint64_t)qi * a.L + i] = i < len ? pid[i] : 0xFFFFFFFFu;
-                a.pool_d[(uint64_t)qi * a.L + i] = i < len ? pd[i] : __builtin_huge_valf();
+        if (COLD(pool_ids)) {
+            for (uint32_t i = t; i < COLD(L); i += T) {
+                COLD(pool_ids)[(uint64_t)qi * COLD(L) + i] = i < len ? pid[i] : 0xFFFFFFFFu;
+                COLD(pool_d)[(uint64_t)qi * COLD(L) + i] = i < len ? pd[i] : __builtin_huge_valf();
             }
             __syncthreads();
         }
-        if (!a.pool_only) {
+        if (!COLD(pool_only)) {
             // rerank: the first rc pool entries; their estimates replace pd[0..rc)
-            const uint32_t want = a.rerank > a.k ? a.rerank : a.k;
+            const uint32_t want = COLD(rerank) > COLD(k) ? COLD(rerank) : COLD(k);
             uint32_t rc = len < want ? len : want;
             if (LEARNED && rc > 0) rc = search_learned_depth<W>(a, qi, rc, pd, pid, s, (uint4 *)stage, &s_len);
             if (a.codec == 3) {
@@ -939,7 +954,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         const uint32_t i = (uint32_t)j16 + 16u * k;
                         buf[k] = i < cch ? src[i] : make_uint4(0u, 0u, 0u, 0u);
                     }
-                    h = a.ix.hdr[id];
+                    h = COLD(ix.hdr)[id];
                 };
                 if (grp < rc) fetch(grp);
                 for (uint32_t c = grp; c < rc; c += G) {
@@ -966,23 +981,23 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 const uint32_t pp = 4096u / stride;
                 for (uint32_t c0 = 0; c0 < rc; c0 += pp) {
                     const uint32_t cnt = rc - c0 < pp ? rc - c0 : pp;
-                    residual_block_sums<T, 4>(work, stride, q, a.ix.rcodes, cd, cnt,
+                    residual_block_sums<T, 4>(work, stride, q, COLD(ix.rcodes), cd, cnt,
                                            [&](uint32_t c) { return pid[c0 + c]; }, t);
                     __syncthreads();
                     for (uint32_t c = t; c < cnt; c += T)
-                        pd[c0 + c] = residual_chain(a.ix.rscale + (uint64_t)pid[c0 + c] * nbk, work + c * stride,
+                        pd[c0 + c] = residual_chain(COLD(ix.rscale) + (uint64_t)pid[c0 + c] * nbk, work + c * stride,
                                                     nbk, pd[c0 + c]);
                     __syncthreads();
                 }
             }
             // top-k by (estimate, id): k rounds of a wave-wide argmin on wave 0
             if (wv == 0) {
-                const uint32_t kout = rc < a.k ? rc : a.k;
-                for (uint32_t r = 0; r < a.k; ++r) {
+                const uint32_t kout = rc < COLD(k) ? rc : COLD(k);
+                for (uint32_t r = 0; r < COLD(k); ++r) {
                     if (r >= kout) {
                         if (lane == 0) {
-                            a.out_ids[(uint64_t)qi * a.k + r] = 0xFFFFFFFFu;
-                            a.out_d[(uint64_t)qi * a.k + r] = __builtin_huge_valf();
+                            COLD(out_ids)[(uint64_t)qi * COLD(k) + r] = 0xFFFFFFFFu;
+                            COLD(out_d)[(uint64_t)qi * COLD(k) + r] = __builtin_huge_valf();
                         }
                         continue;
                     }
@@ -1000,16 +1015,16 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         if (take) { bd = od; bid = oid; bpos = opos; }
                     }
                     if (lane == 0) {
-                        a.out_ids[(uint64_t)qi * a.k + r] = bid;
-                        a.out_d[(uint64_t)qi * a.k + r] = bd;
+                        COLD(out_ids)[(uint64_t)qi * COLD(k) + r] = bid;
+                        COLD(out_d)[(uint64_t)qi * COLD(k) + r] = bd;
                         pid[bpos] = 0xFFFFFFFFu;
                     }
                     wave_sync_lds();
                 }
             }
         }
-        if (t == 0 && a.stats) {
-            uint64_t *st = a.stats + (uint64_t)qi * 6;
+        if (t == 0 && COLD(stats)) {
+            uint64_t *st = COLD(stats) + (uint64_t)qi * 6;
             st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned; st[4] = rchecked; st[5] = rpruned;
         }
         __syncthreads();

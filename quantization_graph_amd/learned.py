@@ -112,11 +112,12 @@ def residual_stats(sp, index):
 
 
 def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=None, workspace=None,
-                check_graph=True):
+                check_graph=True, events=None):
     """Traversal only (qg_search without outputs): pools [nq, L] and stats.
     With check_graph=False and a preallocated `workspace` (sized for `slots`)
     nothing synchronises; read the error flag later with
-    space.search_error(workspace)."""
+    space.search_error(workspace). `events` = (start, end) CUDA events
+    recorded around the launch alone."""
     degrees = _dev(degrees, torch.int32).reshape(-1)
     edges = _dev(edges, torch.int32)
     nq = queries.nq
@@ -131,9 +132,13 @@ def search_pool(sp, index, queries, degrees, edges, l_search, eps0=1.9, slots=No
     if ws.numel() < ws_bytes:
         raise ValueError("search workspace too small: need %d bytes" % ws_bytes)
     di, dq = index.desc(), queries.desc()
+    if events is not None:
+        events[0].record(torch.cuda.current_stream())
     check(Lb.qg_search(sp.handle, C.byref(di), C.byref(dq), queries.codec, _ptr(degrees), _ptr(edges),
                        edges.shape[1], 10, L, C.c_float(eps0), 100, slots, None, None, _ptr(stats), _ptr(pool_ids),
                        _ptr(pool_d), _ptr(ws), ws.numel(), _stream()))
+    if events is not None:
+        events[1].record(torch.cuda.current_stream())
     if check_graph:
         S.search_error(ws)
     return pool_ids, pool_d, stats
