@@ -423,6 +423,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t cmagic = div_magic(cch);
     const uint32_t per = 1024u / cch;
     const uint32_t arows = adj_rows_fit(a.edge_stride);  // adjacency rows the stage's tail holds
+    // the point count and the adjacency row stride fit 32 bits (the launcher
+    // takes at most 2^32 - 2 points)
+    const uint32_t npts = (uint32_t)a.n, estr = (uint32_t)a.edge_stride;
     uint32_t *visited = COLD(visited) + (uint64_t)blockIdx.x * COLD(visited_words);
     constexpr bool use_filter = FILTER;  // the FP32 gate's filter (below)
 
@@ -470,9 +473,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         if (use_filter) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
 
         uint32_t len = 0;
-        uint64_t hops = 0, gates = 0, dists = 1, pruned = 0, rchecked = 0, rpruned = 0;
+        uint32_t hops = 0, gates = 0, dists = 1, pruned = 0, rchecked = 0, rpruned = 0;
         // start node 0 with its full FP32 distance
-        if (a.n > 0 && L > 0 && wv == 0) {
+        if (npts > 0 && L > 0 && wv == 0) {
             if (lane == 0) sid[0] = 0u;
             wave_sync_lds();
             stage_rows<1>(stage, sid, 1, cch, cmagic, (const uint4 *)a.ix.codes, 0u, lane);
@@ -487,11 +490,17 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 }
             }
         }
-        len = (a.n > 0 && L > 0) ? 1 : 0;
+        len = (npts > 0 && L > 0) ? 1 : 0;
         __syncthreads();
         if (use_filter && t == 0) gate_filter_constants<W>(s_qmx, s_qe, s_qa, &s_fsc, &s_fw, &s_fok);
 
         for (;;) {
+            // the thread id through an opaque move: addresses derived from the
+            // lane are formed inside the hop, not kept (and spilled) across it
+            uint32_t t_hop = threadIdx.x;
+            asm volatile("" : "+v"(t_hop));
+            const uint32_t t = t_hop, lane = t_hop & 63u, grp = t_hop >> 4;
+            const int j16 = (int)(t_hop & 15u);
             // first unexpanded entry (every wave reads the same pool)
             uint32_t f = 0xFFFFFFFFu;
             for (uint32_t c0 = 0; c0 < len; c0 += 64u) {
@@ -521,7 +530,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     if (is_fresh) (void)atomicOr(&visited[nb >> 5], 1u << (nb & 31u));  // no reply needed
                 } else {
                     const uint32_t deg = a.degrees[node];
-                    const uint32_t raw_nb = lane < a.edge_stride ? a.edges[(uint64_t)node * a.edge_stride + lane]
+                    const uint32_t raw_nb = lane < estr ? a.edges[(uint64_t)node * estr + lane]
                                                                  : 0xFFFFFFFFu;
                     nb = lane < deg ? raw_nb : 0xFFFFFFFFu;
                     // first occurrence within the list, then test-and-set visited
@@ -530,7 +539,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         const uint32_t other = (uint32_t)__builtin_amdgcn_readlane((int)nb, jj);
                         if ((uint32_t)jj < lane && other == nb) first_occ = false;
                     }
-                    if (first_occ && nb >= a.n) {  // out-of-range neighbour: the reference stops with an error
+                    if (first_occ && nb >= npts) {  // out-of-range neighbour: the reference stops with an error
                         atomicOr(a.err, 1u);
                         first_occ = false;
                     }
@@ -735,19 +744,19 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     nxt = sp_aid;
                     if (adj_spa) arow = nadj;
                 }
-                if (nxt != 0xFFFFFFFFu && nxt < a.n) {
+                if (nxt != 0xFFFFFFFFu && nxt < npts) {
                     uint32_t deg2, raw2 = 0xFFFFFFFFu;
                     if (arow != 0xFFFFFFFFu) {
                         const uint32_t *adj = (const uint32_t *)(stage + kAdjAt);
                         deg2 = adj[arow];
-                        if (lane < a.edge_stride) raw2 = adj[8u + arow * (uint32_t)a.edge_stride + lane];
+                        if (lane < estr) raw2 = adj[8u + arow * (uint32_t)estr + lane];
                     } else {
                         deg2 = a.degrees[nxt];
-                        if (lane < a.edge_stride) raw2 = a.edges[(uint64_t)nxt * a.edge_stride + lane];
+                        if (lane < estr) raw2 = a.edges[(uint64_t)nxt * estr + lane];
                     }
                     const uint32_t nb2 = lane < deg2 ? raw2 : 0xFFFFFFFFu;
                     uint32_t w2 = 0u;
-                    if (nb2 < a.n)
+                    if (nb2 < npts)
                         w2 = __hip_atomic_load(&visited[nb2 >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     // first occurrence within the list (while the loads fly): a
                     // 2^17-bit hash filter in the stage (idle during the
@@ -774,13 +783,13 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     // visited words (the stage is idle during the insertion and
                     // the filter above is done with it; an out-of-range id
                     // loads row 0 and is flagged below)
-                    sp_nb[lane] = nb2 < a.n ? nb2 : 0u;
+                    sp_nb[lane] = nb2 < npts ? nb2 : 0u;
                     if constexpr (FILTER) {
                         wave_sync_lds();
                         stage_rows<1>(stage, sp_nb, deg2 < 64u ? deg2 : 64u, gch, gmagic, (const uint4 *)a.ix.gate,
                                       0u, lane);
                     }
-                    const bool bad2 = first2 && nb2 >= a.n;
+                    const bool bad2 = first2 && nb2 >= npts;
                     const bool fresh2 = first2 && !bad2 && (w2 & (1u << (nb2 & 31u))) == 0u;
                     const uint64_t fm2 = __ballot(fresh2);
                     const uint64_t bm2 = __ballot(bad2);
