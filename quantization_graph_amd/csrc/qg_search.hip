@@ -720,6 +720,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             // insertion prunes by its bound makes this a miss, handled by
             // loading again); load its adjacency row and visited words
             if (wv == 1) {
+                // the prediction runs beside the insertion on the hop's
+                // critical path: the same issue priority as wave 0's steps
+                __builtin_amdgcn_s_setprio(3);
                 float bd = lane < ns ? s_d[lane] : 3.402823466e38f;
                 uint32_t bid = lane < ns ? sid[lane] : 0xFFFFFFFFu;
                 uint32_t bslot = lane;  // the best survivor's index in sid[]
@@ -810,6 +813,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 } else if (lane == 0) {
                     s_sp_node = 0xFFFFFFFFu;
                 }
+                __builtin_amdgcn_s_setprio(0);
             }
             // ordered insertion with the re-check against the current back
             // (insert_candidate_sorted in survivor order), on wave 0. The
