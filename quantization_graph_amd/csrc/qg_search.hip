@@ -513,8 +513,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             __syncthreads();
             ++hops;
             if (wv == 0) {
-                // the serial steps of a hop hold up the whole group: let this
-                // wave issue ahead of the other groups' waves on its SIMD
+                // wave 0 runs each hop's serial steps (neighbour step, prune,
+                // insertion), which hold up the whole group: from its first
+                // hop on it issues ahead of the other queries' waves on its
+                // SIMD. It keeps the priority between its steps (dropping it
+                // let other queries' raised waves hold it back); raising it
+                // for the query setup as well measured slower.
                 __builtin_amdgcn_s_setprio(3);
                 if (lane == 0) pexp[f] = 1;
                 // the neighbour list with its first occurrences and their visited
@@ -557,7 +561,6 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     frow[r] = (uint8_t)r;
                 }
                 if (lane == 0) { s_nf = (uint32_t)__popcll(fm); s_hit = hit ? 1u : 0u; }
-                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
             const uint32_t nf = uni(s_nf);
@@ -822,7 +825,6 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             // pool's last 64 entries and the accepted survivors (sorted) sit in
             // lane registers, and the pool is rewritten once at the end.
             if (wv == 0) {
-                __builtin_amdgcn_s_setprio(3);
                 const uint32_t tb = len > 64u ? len - 64u : 0u;  // pool tail window [tb, len)
                 const float bt_d = tb + lane < len ? pd[tb + lane] : 0.0f;
                 const uint32_t bt_id = tb + lane < len ? pid[tb + lane] : 0u;
@@ -932,7 +934,6 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 }
                 len = e + na;
                 if (lane == 0) s_len = len;
-                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
             len = uni(s_len);
