@@ -1041,6 +1041,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             uint64_t *st = COLD(stats) + (uint64_t)qi * 6;
             st[0] = hops; st[1] = gates; st[2] = dists; st[3] = pruned; st[4] = rchecked; st[5] = rpruned;
         }
+        if (wv == 0) __builtin_amdgcn_s_setprio(0);  // the next query's setup runs at the default priority
         __syncthreads();
     }
 }
