@@ -102,6 +102,22 @@ def test_search_duplicate_neighbours_match_oracle(gpu, graph_case, codec):
     _compare_with_oracle(gpu, g, deg, edges, codec, 64)
 
 
+@pytest.mark.parametrize("codec", [3, 0])
+def test_search_odd_row_stride_matches_oracle(gpu, graph_case, codec):
+    """Adjacency tables whose row stride is not a multiple of four words (33
+    for the degree-32 graph, 63 for the degree-64 one, its lists cut to 63):
+    the kernel stages the candidate next frontiers' rows word by word
+    instead of in 16-byte pieces; results still equal the oracle's."""
+    g = graph_case
+    deg, edges = g["deg"], g["edges"]
+    stride = 33 if edges.shape[1] <= 32 else 63
+    keep = min(edges.shape[1], stride)
+    e2 = np.full((edges.shape[0], stride), 0xFFFFFFFF, np.uint32)
+    e2[:, :keep] = edges[:, :keep]
+    d2 = np.minimum(deg, stride).astype(np.uint32)
+    _compare_with_oracle(gpu, g, d2, e2, codec, 64)
+
+
 def test_search_rejects_bad_graph(gpu, graph_case):
     import quantization_graph_amd as qg
     g = graph_case
