@@ -21,79 +21,62 @@ namespace qg {
 
 constexpr int kPrepThreads = 256;
 
-// One thread per row: FNV-1a over the row's bytes (only when seed_out is
-// given: the INT8 dither is its one consumer), ||row||^2 and <row, c> in
-// double, accumulated in index order exactly as the reference loops do.
-// The recurrences are serial per row, so rows go one per lane; each lane
-// walks its row in chunks of kSeedChunk values and issues the next chunk's
-// loads before it folds the current one in.
-constexpr uint32_t kSeedChunk = 32;
-
-template <bool HASH>
-__device__ __forceinline__ void seed_fold(float v, float c, uint64_t &h, double &nrm, double &dot) {
-    if (HASH) {
-        const uint32_t b = __float_as_uint(v);
+// FNV-1a over a row's bytes (the INT8 dither is its one consumer), folded
+// in the reference's byte order, one value's four bytes at a time.
+__device__ __forceinline__ void fnv_fold(float v, uint64_t &h) {
+    const uint32_t b = __float_as_uint(v);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            h ^= (uint64_t)((b >> (8 * k)) & 0xFFu);
-            h *= 1099511628211ull;
-        }
+    for (int k = 0; k < 4; ++k) {
+        h ^= (uint64_t)((b >> (8 * k)) & 0xFFu);
+        h *= 1099511628211ull;
     }
-    const double dv = (double)v;
-    nrm += dv * dv;  // exact product: ordering of the add is what matters
-    dot += dv * (double)c;
 }
 
-template <bool HASH>
-__global__ void __launch_bounds__(64) k_row_seed(const float *__restrict__ rows, uint64_t n, uint32_t dim,
-                                                 const float *__restrict__ center, uint64_t *__restrict__ seed_out,
-                                                 double *__restrict__ norm_out, double *__restrict__ dot_out) {
-    const uint64_t r = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    const bool live = r < n;
-    const float *x = rows + (live ? r : 0) * dim;  // idle lanes walk row 0
-    uint64_t h = 1469598103934665603ull;
-    double nrm = 0.0, dot = 0.0;
-    const uint32_t full = dim / kSeedChunk * kSeedChunk;
-    float cur[kSeedChunk], nxt[kSeedChunk];
-    if (full) {
-#pragma unroll
-        for (uint32_t u = 0; u < kSeedChunk; ++u) cur[u] = x[u];
-    }
-    for (uint32_t base = 0; base < full; base += kSeedChunk) {
-        if (base + kSeedChunk < full) {
-#pragma unroll
-            for (uint32_t u = 0; u < kSeedChunk; ++u) nxt[u] = x[base + kSeedChunk + u];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kSeedChunk; ++u) seed_fold<HASH>(cur[u], center ? center[base + u] : 0.0f, h, nrm, dot);
-#pragma unroll
-        for (uint32_t u = 0; u < kSeedChunk; ++u) cur[u] = nxt[u];
-    }
-    for (uint32_t i = full; i < dim; ++i) seed_fold<HASH>(x[i], center ? center[i] : 0.0f, h, nrm, dot);
-    if (!live) return;
-    if (HASH) seed_out[r] = h;
-    if (norm_out) norm_out[r] = nrm;
-    if (dot_out) dot_out[r] = center ? dot : 0.0;
-}
-
-// Without the hash (every codec but INT8) the two recurrences of a row are
-// independent FP64 add chains. They go to two wavefronts: wave 0 sums
-// ||row||^2 and wave 1 sums <row, c> for the same 64 rows, one row per
-// lane, each in index order. The center is widened to double in LDS once
-// per block. This halves each lane's FP64 issue (FP64 runs at half the FP32
-// rate here), and 64-value chunks keep the next loads in flight meanwhile.
+// A row's serial recurrences are independent chains, and each goes to a
+// wavefront of its own, one row per lane, in index order: wave 0 sums
+// ||row||^2 and wave 1 <row, c> (FP64 add chains), and with HASH wave 2 runs
+// the row's FNV-1a (only the INT8 dither needs it). The center is widened to
+// double in LDS once per block. Splitting the chains divides each lane's
+// issue between waves on different SIMDs, and 64-value chunks keep the next
+// loads in flight meanwhile.
 constexpr uint32_t kNormChunk = 64;
 
-__global__ void __launch_bounds__(128) k_row_norms(const float *__restrict__ rows, uint64_t n, uint32_t dim,
-                                                   const float *__restrict__ center, double *__restrict__ norm_out,
-                                                   double *__restrict__ dot_out) {
+template <bool HASH>
+__global__ void __launch_bounds__(HASH ? 192 : 128) k_row_norms(const float *__restrict__ rows, uint64_t n,
+                                                                uint32_t dim, const float *__restrict__ center,
+                                                                uint64_t *__restrict__ seed_out,
+                                                                double *__restrict__ norm_out,
+                                                                double *__restrict__ dot_out) {
     extern __shared__ __align__(16) double cs[];  // [dim] when center is given
     const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
     if (center)
-        for (uint32_t i = t; i < dim; i += 128) cs[i] = (double)center[i];
+        for (uint32_t i = t; i < dim; i += blockDim.x) cs[i] = (double)center[i];
     __syncthreads();
     const uint64_t r = (uint64_t)blockIdx.x * 64 + lane;
     const bool live = r < n;
+    if (HASH && wv == 2) {
+        const float *x = rows + (live ? r : 0) * dim;  // idle lanes walk row 0
+        uint64_t h = 1469598103934665603ull;
+        const uint32_t full = dim / kNormChunk * kNormChunk;
+        float cur[kNormChunk], nxt[kNormChunk];
+        if (full) {
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) cur[u] = x[u];
+        }
+        for (uint32_t base = 0; base < full; base += kNormChunk) {
+            if (base + kNormChunk < full) {
+#pragma unroll
+                for (uint32_t u = 0; u < kNormChunk; ++u) nxt[u] = x[base + kNormChunk + u];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) fnv_fold(cur[u], h);
+#pragma unroll
+            for (uint32_t u = 0; u < kNormChunk; ++u) cur[u] = nxt[u];
+        }
+        for (uint32_t i = full; i < dim; ++i) fnv_fold(x[i], h);
+        if (live) seed_out[r] = h;
+        return;
+    }
     if (wv == 1 && !center) {
         if (live && dot_out) dot_out[r] = 0.0;
         return;
@@ -318,46 +301,28 @@ __global__ void __launch_bounds__(kPrepThreads) k_query_prep(PrepParams p) {
 
 // The INT8 error norm (:3592-3600): sqrt of sum_i e_i^2 in index order as
 // one FP64 fma chain, e_i = q_i - scale x_i as the dither computed it. One
-// lane per query, so the 2 cd-long chain of every query runs at once instead
-// of on one lane of its preparation workgroup.
+// 64-thread workgroup per query: its lanes form the e_i into LDS, then lane 0
+// runs the chain, one LDS value and one fused multiply-add per dimension (the
+// e_i are formed off the chain; the product scale * x_i is exact in double).
 __global__ void __launch_bounds__(64) k_query_err(const float *q, const int8_t *x8, uint64_t nq, uint32_t cd,
                                                   QScalars *qs) {
-    const uint64_t qi = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    if (qi >= nq) return;
+    extern __shared__ __align__(16) double es[];  // [cd]
+    const uint64_t qi = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
     QScalars &s = qs[qi];
     const float max_abs = __int_as_float(s.max_abs_bits);
-    if (!(max_abs > 0.0f)) return;  // no INT8 code: the norm stays 0
+    if (!(max_abs > 0.0f)) return;  // no INT8 code: the norm stays 0 (the whole block returns)
     const double scale = (double)s.int8_scale;
-    const float4 *qr = (const float4 *)(q + qi * cd);
-    const uint4 *xr = (const uint4 *)(x8 + qi * cd);
-    // 16 dimensions per step; the next step's rows are loaded while this one
-    // is folded in (cd is a multiple of 64)
-    float4 qc[4], qn[4];
-    uint4 xc, xn;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) qc[k] = qr[k];
-    xc = xr[0];
+    const float *qr = q + qi * cd;
+    const int8_t *xr = x8 + qi * cd;
+    for (uint32_t i = lane; i < cd; i += 64) es[i] = (double)qr[i] - scale * (double)xr[i];
+    __syncthreads();
+    if (lane != 0) return;
     double a = 0.0;
-    for (uint32_t c = 0; c < cd / 16; ++c) {
-        if (c + 1 < cd / 16) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) qn[k] = qr[4 * (c + 1) + k];
-            xn = xr[c + 1];
-        }
-        const uint32_t xw[4] = {xc.x, xc.y, xc.z, xc.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float qv[4] = {qc[k].x, qc[k].y, qc[k].z, qc[k].w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int x = (int)(int8_t)(xw[k] >> (8 * j));
-                const double e = (double)qv[j] - scale * (double)x;
-                a = fma(e, e, a);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) qc[k] = qn[k];
-        xc = xn;
+    for (uint32_t i = 0; i < cd; i += 2) {  // cd is a multiple of 64
+        const double2 e = *(const double2 *)(es + i);
+        a = fma(e.x, e.x, a);
+        a = fma(e.y, e.y, a);
     }
     s.int8_error_norm = (float)sqrt(a);
 }
@@ -384,11 +349,13 @@ int qg_launch_row_seed(const float *rows, uint64_t n, uint32_t dim, const float 
                        uint64_t *seed, double *norm, double *dot, hipStream_t stream) {
     if (n == 0) return 0;
     const dim3 grid((unsigned)((n + 63) / 64));
+    const size_t smem = center ? (size_t)dim * 8 : 0;
     if (seed)
-        hipLaunchKernelGGL(qg::k_row_seed<true>, grid, dim3(64), 0, stream, rows, n, dim, center, seed, norm, dot);
+        hipLaunchKernelGGL(qg::k_row_norms<true>, grid, dim3(192), smem, stream, rows, n, dim, center, seed, norm,
+                           dot);
     else
-        hipLaunchKernelGGL(qg::k_row_norms, grid, dim3(128), center ? (size_t)dim * 8 : 0, stream, rows, n, dim,
-                           center, norm, dot);
+        hipLaunchKernelGGL(qg::k_row_norms<false>, grid, dim3(128), smem, stream, rows, n, dim, center, nullptr,
+                           norm, dot);
     return (int)hipGetLastError();
 }
 
@@ -397,7 +364,7 @@ int qg_launch_query_prep(const qg::PrepParams *p, hipStream_t stream) {
     const size_t smem = (size_t)p->cd * 4 + 312 * 8;
     hipLaunchKernelGGL(qg::k_query_prep, dim3((unsigned)p->nq), dim3(qg::kPrepThreads), smem, stream, *p);
     if (p->codec == 3 || p->codec == 2)
-        hipLaunchKernelGGL(qg::k_query_err, dim3((unsigned)((p->nq + 63) / 64)), dim3(64), 0, stream, p->q_out,
+        hipLaunchKernelGGL(qg::k_query_err, dim3((unsigned)p->nq), dim3(64), (size_t)p->cd * 8, stream, p->q_out,
                            p->x8_out, p->nq, p->cd, p->qs_out);
     return (int)hipGetLastError();
 }
