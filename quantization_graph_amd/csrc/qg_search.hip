@@ -427,6 +427,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     // takes at most 2^32 - 2 points)
     const uint32_t npts = (uint32_t)a.n, estr = (uint32_t)a.edge_stride;
     uint32_t *visited = COLD(visited) + (uint64_t)blockIdx.x * COLD(visited_words);
+    bool first_query = true;
     constexpr bool use_filter = FILTER;  // the FP32 gate's filter (below)
 
     for (;;) {
@@ -465,7 +466,11 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 qp2[i] = make_float2(qq[0], qq[16]);
             }
         }
-        for (uint32_t i = t; i < COLD(visited_words) / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
+        // the launch zeroed every slot's bitset: only a slot's later queries
+        // clear the marks of the one before
+        if (!first_query)
+            for (uint32_t i = t; i < COLD(visited_words) / 4; i += T) ((uint4 *)visited)[i] = make_uint4(0u, 0u, 0u, 0u);
+        first_query = false;
         if (t == 0) s_sp_node = 0xFFFFFFFFu;
         const QScalars s = COLD(qv.qs)[qi];
         __threadfence();
@@ -1051,6 +1056,12 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     if (i < n) p[i] = 0u;
 }
 
+// zero n4 uint4 (grid-stride): the visited bitsets of a launch's slots
+__global__ void __launch_bounds__(256) k_zero_u4(uint4 *p, uint64_t n4) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * 256)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 }  // namespace qg
 
 template <int W, bool LEARNED, bool FILTER>
@@ -1071,7 +1082,16 @@ extern "C" size_t qg_search_lds_bytes(uint32_t cd, uint32_t L) { return (size_t)
 
 extern "C" int qg_launch_search(const qg::SearchArgs *a, uint32_t slots, uint32_t waves, hipStream_t stream) {
     if (a->qv.nq == 0) return 0;
+    // no more workgroups than queries; every slot's bitset starts at zero
+    // (a fill at full memory bandwidth, so a slot's first query skips its
+    // own clear; later queries on the slot clear theirs)
+    if (slots > a->qv.nq) slots = (uint32_t)a->qv.nq;
     hipLaunchKernelGGL(qg::k_zero_u32, dim3(1), dim3(64), 0, stream, a->counter, 2u);  // head + error flag
+    {
+        const uint64_t n4 = (uint64_t)slots * a->visited_words / 4;
+        const uint64_t blocks = (n4 + 255) / 256 < 8192 ? (n4 + 255) / 256 : 8192;
+        hipLaunchKernelGGL(qg::k_zero_u4, dim3((unsigned)blocks), dim3(256), 0, stream, (uint4 *)a->visited, n4);
+    }
     // the FP32 gate's filter from code dimension 2048 (its query takes 2 cd
     // bytes of LDS where the pair-interleaved query takes 4 cd)
     const bool filter = a->codec != 3 && a->cd >= 2048;
