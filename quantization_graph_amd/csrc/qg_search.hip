@@ -473,7 +473,10 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         first_query = false;
         if (t == 0) s_sp_node = 0xFFFFFFFFu;
         const QScalars s = COLD(qv.qs)[qi];
-        __threadfence();
+        // the bitset belongs to this workgroup alone: the barrier's
+        // workgroup-scope release (vmcnt(0)) puts the clear in L2 before any
+        // of the workgroup's atomics on it; a device-scope fence here would
+        // write back and invalidate the XCD's L2 once per query
         __syncthreads();
         if (use_filter) gate_filter_query<W>(q, xh, xl, cd, s_qmx, s_qe, s_qa, t);
 
