@@ -219,6 +219,54 @@ __device__ __forceinline__ float residual_chain(const uint16_t *scale_row, const
     return long_d - 2.0f * ips;
 }
 
+// residual_chain for up to T / 8 candidates at once, eight lanes per
+// candidate (lane group c = t / 8; nb <= 128, a power of two >= 4). Lane j
+// of a group owns the blocks [j * per, (j + 1) * per) of its candidate's
+// chain. residual_chain_scales issues the loads of its fp16 scales (at most
+// 16 blocks: four 8-byte loads), so that they fly together with the block
+// sums' loads that follow; residual_chains_x8 then runs the eight spans in
+// block order, each lane continuing from the running sum of the lane before
+// it, so every step is residual_chain's step. out[c] = long_d[c] - 2 * ips.
+__device__ __forceinline__ uint32_t residual_chain_per(uint32_t nb) { return (nb + 31u) / 32u * 4u; }
+
+template <typename IdOf>
+__device__ __forceinline__ void residual_chain_scales(uint2 (&sv)[4], const uint16_t *__restrict__ rscale,
+                                                      uint32_t nb, uint32_t cnt, IdOf id_of, uint32_t t) {
+    const uint32_t c = t >> 3, per = residual_chain_per(nb), b0 = (t & 7u) * per;
+    const bool live = c < cnt && b0 < nb;
+    const uint2 *srow = (const uint2 *)(rscale + (uint64_t)(live ? id_of(c) : 0u) * nb) + b0 / 4u;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) sv[u] = (live && 4u * u < per) ? srow[u] : make_uint2(0u, 0u);
+}
+
+__device__ __forceinline__ void residual_chains_x8(const uint2 (&sv)[4], const float *work, uint32_t stride,
+                                                   uint32_t nb, uint32_t cnt, float *out, uint32_t t) {
+    const uint32_t c = t >> 3, j = t & 7u, per = residual_chain_per(nb), b0 = j * per;
+    const bool live = c < cnt;
+    const uint32_t own = (live && b0 < nb) ? per : 0u;
+    const float *bip = work + (live ? c : 0u) * stride + (b0 < nb ? b0 : 0u);
+    const uint32_t base = (t & 63u) & ~7u;
+    float ips = 0.0f;
+    for (uint32_t r = 0; r < 8; ++r) {
+        if (j == r) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t sw[2] = {sv[u].x, sv[u].y};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float scale =
+                        (float)__builtin_bit_cast(_Float16, (uint16_t)(sw[e >> 1] >> (16 * (e & 1))));
+                    const bool use = 4u * u + e < own && scale != 0.0f && __builtin_isfinite(scale);
+                    const float next = fmaf(scale, 4u * u + e < own ? bip[4 * u + e] : 0.0f, ips);
+                    ips = use ? next : ips;
+                }
+            }
+        }
+        ips = __shfl(ips, (int)(base + r));  // lane r's running sum, to the whole group
+    }
+    if (live && j == 0) out[c] = out[c] - 2.0f * ips;
+}
+
 // ---------------------------------------------------------------------------
 // 16-lane group forms: lane j (= lane % 16) of an aligned group of 16 owns
 // accumulator j of the reference's 16-lane vector and runs its chain in the

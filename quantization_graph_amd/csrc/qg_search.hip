@@ -996,19 +996,21 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             }
             // residual-4: each (candidate, block) product sum on its own
             // thread into the stage (coalesced loads of consecutive blocks),
-            // then one thread per candidate runs the block recurrence
+            // then eight lanes per candidate run the block recurrence; the
+            // recurrence's scales are loaded first and fly with the block
+            // loads, so a pass of up to T / 8 candidates waits on memory once
             {
                 float *work = (float *)stage;
                 const uint32_t nbk = cd / 16, stride = nbk + 1;
-                const uint32_t pp = 4096u / stride;
+                const uint32_t pp = 4096u / stride < T / 8u ? 4096u / stride : T / 8u;
                 for (uint32_t c0 = 0; c0 < rc; c0 += pp) {
                     const uint32_t cnt = rc - c0 < pp ? rc - c0 : pp;
-                    residual_block_sums<T, 4>(work, stride, q, COLD(ix.rcodes), cd, cnt,
-                                           [&](uint32_t c) { return pid[c0 + c]; }, t);
+                    uint2 sv[4];
+                    residual_chain_scales(sv, COLD(ix.rscale), nbk, cnt, [&](uint32_t c) { return pid[c0 + c]; }, t);
+                    residual_block_sums<T, 8>(work, stride, q, COLD(ix.rcodes), cd, cnt,
+                                               [&](uint32_t c) { return pid[c0 + c]; }, t);
                     __syncthreads();
-                    for (uint32_t c = t; c < cnt; c += T)
-                        pd[c0 + c] = residual_chain(COLD(ix.rscale) + (uint64_t)pid[c0 + c] * nbk, work + c * stride,
-                                                    nbk, pd[c0 + c]);
+                    residual_chains_x8(sv, work, stride, nbk, cnt, pd + c0, t);
                     __syncthreads();
                 }
             }
