@@ -240,7 +240,8 @@ __device__ __forceinline__ void residual_chain_scales(uint2 (&sv)[4], const uint
 }
 
 __device__ __forceinline__ void residual_chains_x8(const uint2 (&sv)[4], const float *work, uint32_t stride,
-                                                   uint32_t nb, uint32_t cnt, float *out, uint32_t t) {
+                                                   uint32_t nb, uint32_t cnt, const float *long_d, float *out,
+                                                   uint32_t t) {
     const uint32_t c = t >> 3, j = t & 7u, per = residual_chain_per(nb), b0 = j * per;
     const bool live = c < cnt;
     const uint32_t own = (live && b0 < nb) ? per : 0u;
@@ -264,7 +265,7 @@ __device__ __forceinline__ void residual_chains_x8(const uint2 (&sv)[4], const f
         }
         ips = __shfl(ips, (int)(base + r));  // lane r's running sum, to the whole group
     }
-    if (live && j == 0) out[c] = out[c] - 2.0f * ips;
+    if (live && j == 0) out[c] = long_d[c] - 2.0f * ips;
 }
 
 // ---------------------------------------------------------------------------

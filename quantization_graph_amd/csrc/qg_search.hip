@@ -1010,7 +1010,7 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     residual_block_sums<T, 8>(work, stride, q, COLD(ix.rcodes), cd, cnt,
                                                [&](uint32_t c) { return pid[c0 + c]; }, t);
                     __syncthreads();
-                    residual_chains_x8(sv, work, stride, nbk, cnt, pd + c0, t);
+                    residual_chains_x8(sv, work, stride, nbk, cnt, pd + c0, pd + c0, t);
                     __syncthreads();
                 }
             }
