@@ -394,17 +394,15 @@ __global__ void __launch_bounds__(kRrThreads) k_learned_rerank(RerankArgs a) {
     }
     __syncthreads();
     const uint32_t stride = nb + 1;
-    // eight lanes per candidate run the recurrence; its scales fly with the
-    // block sums' loads
-    const uint32_t per = kRrWork / stride < kRrThreads / 8 ? kRrWork / stride : kRrThreads / 8;
+    const uint32_t per = kRrWork / stride;
     for (uint32_t c0 = 0; c0 < depth; c0 += per) {
         const uint32_t cnt = depth - c0 < per ? depth - c0 : per;
-        uint2 sv[4];
-        residual_chain_scales(sv, a.ix.rscale, nb, cnt, [&](uint32_t c) { return r_id[c0 + c]; }, t);
         residual_block_sums<kRrThreads, 8>(work, stride, qs, a.ix.rcodes, cd, cnt,
                                         [&](uint32_t c) { return r_id[c0 + c]; }, t);
         __syncthreads();
-        residual_chains_x8(sv, work, stride, nb, cnt, r_ld + c0, r_est + c0, t);
+        for (uint32_t c = t; c < cnt; c += kRrThreads)
+            r_est[c0 + c] = residual_chain(a.ix.rscale + (uint64_t)r_id[c0 + c] * nb, work + c * stride, nb,
+                                           r_ld[c0 + c]);
         __syncthreads();
     }
     const uint32_t kout = depth < a.k ? depth : a.k;
